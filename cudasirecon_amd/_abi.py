@@ -1,0 +1,151 @@
+"""ctypes mirrors of the POD structs in include/simr.h.
+
+The field order must match include/simr.h exactly; tests/test_abi.py checks
+sizeof against the C side.
+"""
+import ctypes as C
+
+SIMR_MAX_DIRS = 8
+SIMR_MAX_ORDERS = 3
+SIMR_MAX_PHASES = 5
+
+
+class SimrParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("ndirs", C.c_int32), ("nphases", C.c_int32), ("norders_output", C.c_int32),
+        ("dxy", C.c_float), ("dz", C.c_float),
+        ("wave", C.c_int32),
+        ("ntimes", C.c_int32),
+        ("k0startangle", C.c_float), ("linespacing", C.c_float), ("na", C.c_float), ("nimm", C.c_float),
+        ("nk0angles", C.c_int32),
+        ("k0angles", C.c_float * SIMR_MAX_DIRS),
+        ("otfcutoff", C.c_float), ("zoomfact", C.c_float),
+        ("z_zoom", C.c_int32),
+        ("explodefact", C.c_float),
+        ("bFilteroverlaps", C.c_int32), ("recalcarrays", C.c_int32), ("napodize", C.c_int32),
+        ("bSearchforvector", C.c_int32), ("bUseTime0k0", C.c_int32),
+        ("apodizeoutput", C.c_int32),
+        ("apoGamma", C.c_float),
+        ("bSuppress_singularities", C.c_int32), ("suppression_radius", C.c_int32),
+        ("bDampenOrder0", C.c_int32), ("bFitallphases", C.c_int32),
+        ("do_rescale", C.c_int32), ("equalizez", C.c_int32), ("equalizet", C.c_int32), ("bNoKz0", C.c_int32),
+        ("wiener", C.c_float),
+        ("nforceamp", C.c_int32),
+        ("forceamp", C.c_float * SIMR_MAX_ORDERS),
+        ("bRadAvgOTF", C.c_int32), ("bOneOTFperAngle", C.c_int32),
+        ("nxotf", C.c_int32), ("nyotf", C.c_int32), ("nzotf", C.c_int32),
+        ("dkrotf", C.c_float), ("dkzotf", C.c_float),
+        ("constbkgd", C.c_float),
+        ("bFastSIM", C.c_int32),
+        ("fit_fp64", C.c_int32),
+        ("reserved", C.c_int32 * 15),
+    ]
+
+    def copy(self):
+        p = SimrParams()
+        C.memmove(C.byref(p), C.byref(self), C.sizeof(SimrParams))
+        return p
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            if hasattr(v, "__len__"):
+                v = list(v)
+            out[name] = v
+        return out
+
+
+class SimrFit(C.Structure):
+    _fields_ = [
+        ("k0x", C.c_float), ("k0y", C.c_float),
+        ("k0guess_x", C.c_float), ("k0guess_y", C.c_float),
+        ("amp_re", C.c_float * SIMR_MAX_ORDERS), ("amp_im", C.c_float * SIMR_MAX_ORDERS),
+        ("corr", C.c_float * SIMR_MAX_ORDERS),
+        ("n_angle_evals", C.c_int32), ("n_mag_evals", C.c_int32), ("n_reductions", C.c_int32),
+    ]
+
+    def amps(self, norders):
+        return [complex(self.amp_re[i], self.amp_im[i]) for i in range(norders)]
+
+
+def default_params():
+    """Python restatement of simr_params_default() (the cudasirecon CLI defaults).
+
+    Mirrors SetDefaultParams (src/cudaSirecon/cudaSirecon.cpp:16-72) overlaid with the
+    program-option defaults (cudaSirecon.cpp:918-1005); apodizeoutput is 2 because
+    gammaApo always has a value (cudaSirecon.cpp:1040-1045).
+    """
+    p = SimrParams()
+    p.nx = p.ny = p.nz = 0
+    p.ndirs = 3
+    p.nphases = 5
+    p.norders_output = 0
+    p.dxy = 0.1
+    p.dz = 0.2
+    p.wave = 530
+    p.ntimes = 1
+    p.k0startangle = 1.648
+    p.linespacing = 0.172
+    p.na = 1.2
+    p.nimm = 1.33
+    p.nk0angles = 0
+    p.otfcutoff = 0.006
+    p.zoomfact = 2.0
+    p.z_zoom = 1
+    p.explodefact = 1.0
+    p.bFilteroverlaps = 1
+    p.recalcarrays = 1
+    p.napodize = 10
+    p.bSearchforvector = 1
+    p.bUseTime0k0 = 1
+    p.apodizeoutput = 2
+    p.apoGamma = 1.0
+    p.bSuppress_singularities = 1
+    p.suppression_radius = 10
+    p.bDampenOrder0 = 0
+    p.bFitallphases = 1
+    p.do_rescale = 1
+    p.equalizez = 0
+    p.equalizet = 0
+    p.bNoKz0 = 0
+    p.wiener = 0.01
+    p.nforceamp = 0
+    p.bRadAvgOTF = 0
+    p.bOneOTFperAngle = 0
+    p.nxotf = p.nyotf = p.nzotf = 0
+    p.dkrotf = p.dkzotf = 0.0
+    p.constbkgd = 0.0
+    p.bFastSIM = 0
+    p.fit_fp64 = 1
+    return p
+
+
+def set_params(p, **kw):
+    for k, v in kw.items():
+        if k == "k0angles":
+            p.nk0angles = len(v)
+            for i, a in enumerate(v):
+                p.k0angles[i] = a
+        elif k == "forceamp":
+            p.nforceamp = len(v)
+            for i, a in enumerate(v):
+                p.forceamp[i] = a
+        else:
+            if not hasattr(p, k):
+                raise AttributeError(k)
+            setattr(p, k, v)
+    return p
+
+
+def norders_of(p):
+    return p.norders_output if p.norders_output else p.nphases // 2 + 1
+
+
+def output_shape(p):
+    """(z, y, x) shape of the reconstructed volume."""
+    import math
+    zx = int(round(p.zoomfact * p.nx))
+    zy = int(round(p.zoomfact * p.ny))
+    return (p.z_zoom * p.nz, zy, zx)

@@ -1,0 +1,215 @@
+// extern "C" stage-level ABI (include/simr.h).  Nothing throws across it:
+// every entry point maps exceptions to a simr_status and a thread-local message.
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/simr.h"
+#include "engine.hpp"
+#include "fft_api.hpp"
+
+struct simr_ctx {
+  simr::Engine* eng;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return SIMR_OK;
+  } catch (const simr::ArgError& e) {
+    g_err = e.what();
+    return SIMR_ERR_ARG;
+  } catch (const simr::NoMemError& e) {
+    g_err = e.what();
+    return SIMR_ERR_NOMEM;
+  } catch (const simr::HipError& e) {
+    g_err = e.what();
+    return SIMR_ERR_HIP;
+  } catch (const simr::StateError& e) {
+    g_err = e.what();
+    return SIMR_ERR_STATE;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return SIMR_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return SIMR_ERR_ARG;
+  }
+}
+
+simr::Engine& E(simr_ctx* c) {
+  if (!c || !c->eng) throw simr::ArgError("null context");
+  return *c->eng;
+}
+}  // namespace
+
+extern "C" {
+
+void simr_set_last_error(const char* msg) { g_err = msg ? msg : ""; }
+
+const char* simr_last_error(void) { return g_err.c_str(); }
+
+const char* simr_version(void) { return "1.2.0"; }
+size_t simr_params_size(void) { return sizeof(simr_params); }
+size_t simr_fit_size(void) { return sizeof(simr_fit); }
+
+void simr_params_default(simr_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  // SetDefaultParams (cudaSirecon.cpp:16-72) overlaid with the option defaults (918-1005)
+  p->ndirs = 3;
+  p->nphases = 5;
+  p->norders_output = 0;
+  p->dxy = 0.1f;
+  p->dz = 0.2f;
+  p->wave = 530;
+  p->ntimes = 1;
+  p->k0startangle = 1.648f;
+  p->linespacing = 0.172f;
+  p->na = 1.2f;
+  p->nimm = 1.33f;
+  p->otfcutoff = 0.006f;
+  p->zoomfact = 2.0f;
+  p->z_zoom = 1;
+  p->explodefact = 1.0f;
+  p->bFilteroverlaps = 1;
+  p->recalcarrays = 1;
+  p->napodize = 10;
+  p->bSearchforvector = 1;
+  p->bUseTime0k0 = 1;
+  p->apodizeoutput = 2;  // gammaApo always has a value (cudaSirecon.cpp:1040-1045)
+  p->apoGamma = 1.0f;
+  p->bSuppress_singularities = 1;
+  p->suppression_radius = 10;
+  p->bDampenOrder0 = 0;
+  p->bFitallphases = 1;
+  p->do_rescale = 1;
+  p->wiener = 0.01f;
+  p->fit_fp64 = 1;
+}
+
+size_t simr_otf_size(const simr_params* p) {
+  if (!p) return 0;
+  return (size_t)p->nzotf * p->nxotf * p->nyotf;
+}
+
+size_t simr_footprint(const simr_params* p) {
+  size_t n = 0;
+  int rc = guard([&] { n = simr::Layout::of(*p).bytes_total; });
+  return rc ? 0 : n;
+}
+
+int simr_create(const simr_params* p, const float* otf, int device, simr_ctx** out) {
+  return guard([&] {
+    if (!p || !out) throw simr::ArgError("null argument");
+    *out = nullptr;
+    simr_ctx* c = new simr_ctx{nullptr};
+    try {
+      c->eng = new simr::Engine(*p, otf, device);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int simr_destroy(simr_ctx* ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    delete ctx->eng;
+    delete ctx;
+  });
+}
+
+int simr_get_params(const simr_ctx* ctx, simr_params* out) {
+  return guard([&] { *out = E(const_cast<simr_ctx*>(ctx)).params(); });
+}
+
+int simr_set_raw_f32_host(simr_ctx* ctx, const float* raw) { return guard([&] { E(ctx).set_raw(raw, false, false); }); }
+int simr_set_raw_u16_host(simr_ctx* ctx, const uint16_t* raw) { return guard([&] { E(ctx).set_raw(raw, true, false); }); }
+int simr_set_raw_f32_dev(simr_ctx* ctx, const float* raw) { return guard([&] { E(ctx).set_raw(raw, false, true); }); }
+int simr_set_raw_u16_dev(simr_ctx* ctx, const uint16_t* raw) { return guard([&] { E(ctx).set_raw(raw, true, true); }); }
+int simr_set_flatfield(simr_ctx* ctx, const float* bg, const float* slope) {
+  return guard([&] { E(ctx).set_flatfield(bg, slope); });
+}
+
+int simr_load_and_rescale(simr_ctx* ctx, int it) { return guard([&] { E(ctx).load_and_rescale(it); }); }
+int simr_set_time_index(simr_ctx* ctx, int it) { return guard([&] { E(ctx).set_time(it); }); }
+int simr_find_modulation(simr_ctx* ctx) { return guard([&] { E(ctx).find_modulation(); }); }
+int simr_filter_assemble(simr_ctx* ctx) { return guard([&] { E(ctx).filter_assemble(); }); }
+int simr_process_volume(simr_ctx* ctx) {
+  return guard([&] {
+    E(ctx).find_modulation();
+    E(ctx).filter_assemble();
+  });
+}
+
+size_t simr_result_elems(const simr_ctx* ctx) { return ctx && ctx->eng ? ctx->eng->result_elems() : 0; }
+int simr_get_result_host(simr_ctx* ctx, float* out) { return guard([&] { E(ctx).get_result(out); }); }
+const float* simr_result_dev(simr_ctx* ctx) { return ctx && ctx->eng ? ctx->eng->result_dev() : nullptr; }
+int simr_get_fit(const simr_ctx* ctx, int dir, simr_fit* out) {
+  return guard([&] { *out = E(const_cast<simr_ctx*>(ctx)).fit(dir); });
+}
+int simr_set_fit(simr_ctx* ctx, int dir, const simr_fit* in) { return guard([&] { E(ctx).set_fit(dir, *in); }); }
+int simr_get_band_host(simr_ctx* ctx, int dir, int band, float* out) {
+  return guard([&] { E(ctx).get_band(dir, band, out); });
+}
+int simr_set_band_host(simr_ctx* ctx, int dir, int band, const float* in) {
+  return guard([&] { E(ctx).set_band(dir, band, in); });
+}
+int simr_filter_dir(simr_ctx* ctx, int dir) {
+  return guard([&] {
+    E(ctx).filter_dir(dir);
+    E(ctx).sync();
+  });
+}
+int simr_debug_overlaps(simr_ctx* ctx, int dir, int o1, int o2, float k0x, float k0y, double* P, double* sx,
+                        double* sy) {
+  return guard([&] { E(ctx).debug_overlaps(dir, o1, o2, k0x, k0y, P, sx, sy); });
+}
+
+int simr_sync(simr_ctx* ctx) { return guard([&] { E(ctx).sync(); }); }
+void* simr_stream(simr_ctx* ctx) { return ctx && ctx->eng ? (void*)ctx->eng->stream() : nullptr; }
+int simr_enable_timing(simr_ctx* ctx, int on) { return guard([&] { E(ctx).enable_timing(on != 0); }); }
+int simr_stage_timings(simr_ctx* ctx, const char** names, float* ms, int cap) {
+  int n = 0;
+  int rc = guard([&] { n = E(ctx).timings(names, ms, cap); });
+  return rc ? rc : n;
+}
+
+int simr_fft_size_supported(int n) { return simr::fft_size_supported(n) ? 1 : 0; }
+
+int simr_fft3d_dev(float* data, int nx, int ny, int nz, int inverse, void* stream) {
+  return guard([&] {
+    if (!simr::fft_size_supported(nx) || !simr::fft_size_supported(ny) || !simr::fft_size_supported(nz))
+      throw simr::ArgError("unsupported FFT size");
+    simr::fft3d((float2*)data, nx, ny, nz, inverse != 0, (hipStream_t)stream);
+  });
+}
+
+int simr_fft3d_host(float* data, int nx, int ny, int nz, int inverse) {
+  return guard([&] {
+    if (!simr::fft_size_supported(nx) || !simr::fft_size_supported(ny) || !simr::fft_size_supported(nz))
+      throw simr::ArgError("unsupported FFT size");
+    const size_t bytes = (size_t)nx * ny * nz * 8;
+    float2* d = nullptr;
+    SIMR_HIP(hipMalloc(&d, bytes));
+    try {
+      SIMR_HIP(hipMemcpy(d, data, bytes, hipMemcpyHostToDevice));
+      simr::fft3d(d, nx, ny, nz, inverse != 0, nullptr);
+      SIMR_HIP(hipMemcpy(data, d, bytes, hipMemcpyDeviceToHost));
+    } catch (...) {
+      (void)hipFree(d);
+      throw;
+    }
+    SIMR_HIP(hipFree(d));
+  });
+}
+
+}  // extern "C"

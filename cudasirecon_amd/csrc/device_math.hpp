@@ -1,0 +1,65 @@
+// Device helpers shared by the stage kernels.
+#pragma once
+#include "common.hpp"
+#include "stages.hpp"
+
+namespace simr {
+
+// Bilinear (kr, kz) lookup in a rotationally averaged OTF table, following
+// dev_otfinterpolate (src/cudaSirecon/gpuFunctionsImpl.cu:1677-1715): kz wraps
+// (negative kz index + nzotf; the last kz row interpolates towards row 0).
+// Indices are clamped to the table, so a support wider than the table reads
+// the table edge instead of memory past it.
+__device__ __forceinline__ float2 otf_interp(const OtfRA& o, float kx, float ky, int kz, float kzscale) {
+#pragma clang fp contract(off)
+  const int nzotf = o.nzotf;
+  float krindex = sqrtf(kx * kx + ky * ky) / o.dkrotf;
+  float kzindex = (float)kz * kzscale;
+  if (kzindex < 0) kzindex += (float)nzotf;
+  int irindex = (int)floorf(krindex);
+  int izindex = (int)floorf(kzindex);
+  float ar = krindex - (float)irindex;
+  float az = kzindex - (float)izindex;
+  if (irindex > o.nxotf - 2) {
+    irindex = o.nxotf - 2;
+    ar = 1.f;
+  }
+  if (irindex < 0) {
+    irindex = 0;
+    ar = 0.f;
+  }
+  if (izindex > nzotf - 1) izindex = nzotf - 1;
+  if (izindex < 0) izindex = 0;
+  const int iz1 = (izindex == nzotf - 1) ? 0 : izindex + 1;
+  const float2 a00 = o.t[irindex * nzotf + izindex];
+  const float2 a01 = o.t[irindex * nzotf + iz1];
+  const float2 a10 = o.t[(irindex + 1) * nzotf + izindex];
+  const float2 a11 = o.t[(irindex + 1) * nzotf + iz1];
+  float2 v;
+  v.x = (1 - ar) * (a00.x * (1 - az) + a01.x * az) + ar * (a10.x * (1 - az) + a11.x * az);
+  v.y = (1 - ar) * (a00.y * (1 - az) + a01.y * az) + ar * (a10.y * (1 - az) + a11.y * az);
+  return v;
+}
+
+__device__ __forceinline__ float2 cmul_nc(float2 a, float2 b) {
+#pragma clang fp contract(off)
+  float re = a.x * b.x - a.y * b.y;
+  float im = a.x * b.y + a.y * b.x;
+  return make_float2(re, im);
+}
+
+template <int BS>
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace simr

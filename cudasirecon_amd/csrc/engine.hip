@@ -1,0 +1,955 @@
+// Host orchestration of one reconstruction context.  See engine.hpp.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "engine.hpp"
+#include "fft_api.hpp"
+#include "sizes.hpp"
+
+namespace simr {
+
+namespace {
+constexpr double kPI = 3.14159265358979323846;
+constexpr float K0_WARNING_THRESH = 0.05f;  // cudaSireconImpl.h:54
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int norders_of(const simr_params& p) { return p.norders_output != 0 ? p.norders_output : p.nphases / 2 + 1; }
+
+// makeoverlaps' axial support (gpuFunctionsImpl.cu:508-548), float semantics.
+int overlap_zd(const simr_params& p) {
+  const int nz = p.nz;
+  const float dz = p.dz;
+  const float dkz = dz > 0 ? 1.0f / ((float)nz * dz) : p.dkzotf;
+  const float lambdaem = (float)((double)((float)p.wave / p.nimm) / 1.0e3);
+  const float alpha = asinf(p.na / p.nimm);
+  float zd = (float)(int)std::ceil(((1.0 - (double)cosf(alpha)) / (double)lambdaem) / (double)dkz);
+  if (zd > (float)(nz / 2)) zd = (float)(((nz / 2 - 1) > 0) ? (nz / 2 - 1) : 0);
+  return (int)zd;
+}
+
+#define SIMR_CASE(N) case N:
+bool in_list_xy(int n) {
+  switch (n) {
+    SIMR_XY_SIZES(SIMR_CASE)
+    return true;
+    default:
+      return false;
+  }
+}
+bool in_list_zoom(int n) {
+  switch (n) {
+    SIMR_ZOOM_SIZES(SIMR_CASE)
+    return true;
+    default:
+      return false;
+  }
+}
+bool in_list_z(int n) {
+  switch (n) {
+    SIMR_Z_SIZES(SIMR_CASE)
+    return true;
+    default:
+      return false;
+  }
+}
+#undef SIMR_CASE
+}  // namespace
+
+// ------------------------------------------------------------------- arena
+void Arena::reserve(size_t bytes) {
+  release();
+  if (bytes == 0) return;
+  hipError_t e = hipMalloc(&base_, bytes);
+  if (e != hipSuccess) {
+    base_ = nullptr;
+    (void)hipGetLastError();
+    throw NoMemError("device arena of " + std::to_string(bytes >> 20) + " MiB could not be allocated: " +
+                     hipGetErrorString(e));
+  }
+  cap_ = bytes;
+  off_ = 0;
+}
+
+void* Arena::take(size_t bytes) {
+  bytes = align256(bytes);
+  if (off_ + bytes > cap_) throw NoMemError("device arena exhausted");
+  void* p = base_ + off_;
+  off_ += bytes;
+  return p;
+}
+
+void Arena::release() {
+  if (base_) (void)hipFree(base_);
+  base_ = nullptr;
+  cap_ = off_ = 0;
+}
+
+// ------------------------------------------------------------------- layout
+Layout Layout::of(const simr_params& p) {
+  Layout L{};
+  SIMR_REQUIRE(p.nx > 0 && p.ny > 0 && p.nz > 0, "image dimensions must be positive");
+  SIMR_REQUIRE(p.ndirs >= 1 && p.ndirs <= SIMR_MAX_DIRS, "ndirs must be in 1..8");
+  L.norders = norders_of(p);
+  SIMR_REQUIRE(L.norders >= 1 && L.norders <= SIMR_MAX_ORDERS, "norders must be in 1..3");
+  SIMR_REQUIRE(2 * L.norders - 1 == p.nphases, "nphases must equal 2*norders-1");
+  SIMR_REQUIRE(p.bRadAvgOTF, "only rotationally averaged OTFs are supported (otfRA)");
+  SIMR_REQUIRE(p.nxotf >= 2 && p.nzotf >= 1 && p.nyotf == 1, "invalid radially averaged OTF geometry");
+  SIMR_REQUIRE(p.dkrotf > 0 && p.dkzotf > 0, "OTF pixel sizes must be positive");
+  SIMR_REQUIRE(p.dxy > 0, "dxy must be positive");
+  SIMR_REQUIRE(p.zoomfact >= 1 && p.z_zoom >= 1, "zoom factors must be >= 1");
+  L.nx = p.nx;
+  L.ny = p.ny;
+  L.nz = p.nz;
+  L.nxh = p.nx / 2 + 1;
+  L.ndirs = p.ndirs;
+  L.nphases = p.nphases;
+  L.nbands = 2 * L.norders - 1;
+  L.nimg = p.nz * p.ndirs * p.nphases;
+  L.nDirsOTF = p.bOneOTFperAngle ? p.ndirs : 1;
+  L.H = (long long)L.nxh * p.ny * p.nz;
+  L.nxy = (long long)p.nx * p.ny;
+  L.xdim = (int)std::rint((double)p.zoomfact * p.nx);
+  L.ydim = (int)std::rint((double)p.zoomfact * p.ny);
+  L.zdim = p.z_zoom * p.nz;
+  L.nasm = 1 + p.ndirs * (L.norders - 1);
+  L.zd_ov_max = overlap_zd(p);
+  L.otfsize = (size_t)p.nzotf * p.nxotf * p.nyotf;
+  SIMR_REQUIRE(in_list_xy(p.nx), "image width " + std::to_string(p.nx) + " is not a supported size");
+  SIMR_REQUIRE(fft_size_supported(p.ny) && p.ny <= 4096, "image height " + std::to_string(p.ny) + " is not a supported size");
+  SIMR_REQUIRE(in_list_z(p.nz), "stack depth " + std::to_string(p.nz) + " is not a supported size");
+  SIMR_REQUIRE(in_list_zoom(L.xdim) && in_list_zoom(L.ydim), "zoomed lateral size is not a supported size");
+  SIMR_REQUIRE(in_list_z(L.zdim), "zoomed stack depth is not a supported size");
+  SIMR_REQUIRE(2 * L.zd_ov_max + 1 <= p.nz, "overlap z support wider than the stack");
+  const int ncand = 64;
+  size_t b = 0;
+  b += align256((size_t)L.nimg * L.nxy * 4);            // raw (float or uint16)
+  b += 2 * align256((size_t)L.nxy * 4);                  // bg, slope
+  b += align256((size_t)L.nimg * 8) + align256((size_t)L.nimg * 4) + align256((size_t)p.nz * 8) + align256(256);
+  b += align256((size_t)L.ndirs * L.nbands * L.H * 8);   // bands
+  b += align256((size_t)L.nDirsOTF * L.norders * L.otfsize * 8);
+  b += align256((size_t)2 * (2 * L.zd_ov_max + 1) * L.nxy * 8);  // overlap x/y stage
+  b += align256((size_t)L.nxy * 16) + 2 * align256((size_t)L.nxy * 8) + align256((size_t)L.nxy * 8);
+  b += 4 * align256(4096) + align256((size_t)ncand * 8);
+  b += align256((size_t)ncand * modamp_partials_fp64(p.nx, p.ny) * 16);
+  b += align256((size_t)ncand * modamp_partials_fp32(p.nx, p.ny) * 16);
+  b += align256((size_t)2 * 128 * 8);
+  b += align256((size_t)L.nasm * L.zdim * L.nxy * 8);                       // stageZ
+  b += align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * p.nx * 8);       // stageY
+  b += align256((size_t)L.zdim * L.ydim * (size_t)L.xdim * 4);              // output
+  L.bytes_total = b;
+  return L;
+}
+
+// ------------------------------------------------------------------- engine
+Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Layout::of(p)), device_(device) {
+  SIMR_REQUIRE(otf != nullptr, "OTF data is required");
+  SIMR_HIP(hipSetDevice(device_));
+  SIMR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  fit_fp32_ = P.fit_fp64 == 0;
+  arena_.reserve(L.bytes_total);
+  raw_ = arena_.take((size_t)L.nimg * L.nxy * 4);
+  bg_ = (float*)arena_.take((size_t)L.nxy * 4);
+  slope_ = (float*)arena_.take((size_t)L.nxy * 4);
+  sums_ = (double*)arena_.take((size_t)L.nimg * 8);
+  ratio_ = (float*)arena_.take((size_t)L.nimg * 4);
+  sdp_ = (double*)arena_.take((size_t)L.nz * 8);
+  apofact_ = (float*)arena_.take(256);
+  bands_ = (float2*)arena_.take((size_t)L.ndirs * L.nbands * L.H * 8);
+  otf_ = (float2*)arena_.take((size_t)L.nDirsOTF * L.norders * L.otfsize * 8);
+  ovx_ = (float2*)arena_.take((size_t)2 * (2 * L.zd_ov_max + 1) * L.nxy * 8);
+  P_ = (double2*)arena_.take((size_t)L.nxy * 16);
+  SX_ = (double*)arena_.take((size_t)L.nxy * 8);
+  SY_ = (double*)arena_.take((size_t)L.nxy * 8);
+  cc_ = (float2*)arena_.take((size_t)L.nxy * 8);
+  pv_ = (float*)arena_.take(4096);
+  pi_ = (int*)arena_.take(4096);
+  peakvals_ = (float*)arena_.take(4096);
+  peakidx_ = (int*)arena_.take(4096);
+  kcand_ = (float2*)arena_.take((size_t)max_cand_ * 8);
+  part64_ = (double2*)arena_.take((size_t)max_cand_ * modamp_partials_fp64(L.nx, L.ny) * 16);
+  part32_ = (float4*)arena_.take((size_t)max_cand_ * modamp_partials_fp32(L.nx, L.ny) * 16);
+  sumparts_ = (double*)arena_.take((size_t)2 * 128 * 8);
+  stageZ_ = (float2*)arena_.take((size_t)L.nasm * L.zdim * L.nxy * 8);
+  stageY_ = (float2*)arena_.take((size_t)L.nasm * L.zdim * (size_t)L.ydim * L.nx * 8);
+  out_ = (float*)arena_.take((size_t)L.zdim * L.ydim * (size_t)L.xdim * 4);
+
+  hstage_bytes_ = std::max<size_t>((size_t)max_cand_ * modamp_partials_fp32(L.nx, L.ny) * 16,
+                                   (size_t)max_cand_ * modamp_partials_fp64(L.nx, L.ny) * 16) +
+                  8192;
+  SIMR_HIP(hipHostMalloc(&hstage_, hstage_bytes_, hipHostMallocDefault));
+
+  // OTF tables
+  const size_t notf = (size_t)L.nDirsOTF * L.norders * L.otfsize;
+  SIMR_HIP(hipMemcpyAsync(otf_, otf, notf * 8, hipMemcpyHostToDevice, stream_));
+  // separation matrix, equally spaced phases (makematrix, cudaSirecon.cpp:591-617)
+  sep_.assign((size_t)L.nbands * L.nphases, 0.f);
+  const float phi = (float)(2.0 * kPI / L.nphases);
+  for (int j = 0; j < L.nphases; ++j) {
+    sep_[0 * L.nphases + j] = 1.0f;
+    for (int order = 1; order < L.norders; ++order) {
+      const float arg = (float)(j * order) * phi;
+      sep_[(2 * order - 1) * L.nphases + j] = (float)std::cos((double)arg);
+      sep_[2 * order * L.nphases + j] = (float)std::sin((double)arg);
+    }
+  }
+  // inscale (cudaSirecon.cpp:1231-1232)
+  {
+    const int ip = L.nx * L.ny * L.nz;
+    float f = (float)ip * P.zoomfact;
+    f = f * P.zoomfact;
+    f = f * (float)P.z_zoom;
+    f = f * (float)L.ndirs;
+    inscale_ = (float)(1.0 / (double)f);
+  }
+  // edge-blend factors of apodize_x/y_kernel (gpuFunctionsImpl.cu:61,77)
+  SIMR_REQUIRE(P.napodize <= 64, "napodize must be <= 64");
+  apof_.assign(64, 0.f);
+  for (int l = 0; l < P.napodize; ++l) {
+    const double a = ((double)l + 0.5) / (double)(float)P.napodize * kPI * 0.5;
+    apof_[l] = (float)(1.0 - (double)sinf((float)a));
+  }
+  SIMR_HIP(hipMemcpyAsync(apofact_, apof_.data(), 64 * 4, hipMemcpyHostToDevice, stream_));
+  set_flatfield(nullptr, nullptr);
+  std::vector<double> zero(L.nz, 0.0);
+  SIMR_HIP(hipMemcpyAsync(sdp_, zero.data(), L.nz * 8, hipMemcpyHostToDevice, stream_));
+  SIMR_HIP(hipStreamSynchronize(stream_));
+
+  // k0 guesses (setup_common, cudaSirecon.cpp:1233-1252)
+  k0_.assign(L.ndirs, make_float2(0, 0));
+  k0_time0_ = k0_;
+  k0guess_ = k0_;
+  const float delta_angle = (float)(kPI / L.ndirs);
+  float k0magguess = (float)(1.0 / (double)P.linespacing);
+  if (L.nz > 1) k0magguess = k0magguess / (float)(L.nphases / 2 + 1 - 1);
+  for (int i = 0; i < L.ndirs; ++i) {
+    const float ang = P.nk0angles < L.ndirs ? P.k0startangle + (float)i * delta_angle : P.k0angles[i];
+    k0guess_[i] = make_float2((float)((double)k0magguess * std::cos((double)ang)),
+                              (float)((double)k0magguess * std::sin((double)ang)));
+  }
+  amp_.assign(L.ndirs, std::vector<float2>(L.norders, make_float2(0, 0)));
+  for (int i = 0; i < L.ndirs; ++i) amp_[i][0] = make_float2(1, 0);
+  corr_.assign(L.ndirs, std::vector<float>(SIMR_MAX_ORDERS, 0.f));
+  stats_.assign(L.ndirs, simr_fit{});
+}
+
+Engine::~Engine() {
+  for (auto& e : events_) {
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (hstage_) (void)hipHostFree(hstage_);
+  arena_.release();
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::sync() { SIMR_HIP(hipStreamSynchronize(stream_)); }
+
+void Engine::stage_begin(const char* name) {
+  if (!timing_) return;
+  hipEvent_t a, b;
+  SIMR_HIP(hipEventCreate(&a));
+  SIMR_HIP(hipEventCreate(&b));
+  SIMR_HIP(hipEventRecord(a, stream_));
+  events_.push_back({name, {a, b}});
+}
+void Engine::stage_end() {
+  if (!timing_ || events_.empty()) return;
+  SIMR_HIP(hipEventRecord(events_.back().second.second, stream_));
+}
+
+int Engine::timings(const char** names, float* ms, int cap) {
+  sync();
+  for (auto& e : events_) {
+    float t = 0;
+    SIMR_HIP(hipEventElapsedTime(&t, e.second.first, e.second.second));
+    last_timings_.push_back({e.first, t});
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  events_.clear();
+  static thread_local std::vector<std::string> keep;
+  keep.clear();
+  int n = 0;
+  for (auto& t : last_timings_) {
+    if (n < cap) {
+      keep.push_back(t.first);
+      ms[n] = t.second;
+    }
+    ++n;
+  }
+  for (int i = 0; i < (int)keep.size() && i < cap; ++i) names[i] = keep[i].c_str();
+  last_timings_.clear();
+  return std::min(n, cap);
+}
+
+void Engine::set_raw(const void* src, bool u16, bool on_device) {
+  SIMR_REQUIRE(src != nullptr, "raw data pointer is null");
+  const size_t bytes = (size_t)L.nimg * L.nxy * (u16 ? 2 : 4);
+  SIMR_HIP(hipMemcpyAsync(raw_, src, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+  raw_u16_ = u16;
+  raw_set_ = true;
+  if (!on_device) sync();
+}
+
+void Engine::set_flatfield(const float* bg, const float* slope) {
+  std::vector<float> b((size_t)L.nxy), s((size_t)L.nxy);
+  for (long long e = 0; e < L.nxy; ++e) {
+    b[e] = bg ? bg[e] : P.constbkgd;
+    s[e] = slope ? slope[e] : 1.0f;
+  }
+  SIMR_HIP(hipMemcpyAsync(bg_, b.data(), L.nxy * 4, hipMemcpyHostToDevice, stream_));
+  SIMR_HIP(hipMemcpyAsync(slope_, s.data(), L.nxy * 4, hipMemcpyHostToDevice, stream_));
+  sync();
+}
+
+// ------------------------------------------------------------ forward path
+void Engine::load_and_rescale(int it) {
+  if (!raw_set_) throw StateError("raw data has not been set");
+  stage_begin("section_sums");
+  launch_section_sums(raw_, raw_u16_, bg_, slope_, inscale_, L.nx, L.ny, L.nimg, sums_, stream_);
+  launch_ratios(sums_, sdp_, ratio_, L.ndirs, L.nz, L.nphases, P.bFastSIM, it, P.equalizez, P.equalizet,
+                P.do_rescale, stream_);
+  stage_end();
+  FwdParams fp{};
+  fp.nx = L.nx;
+  fp.ny = L.ny;
+  fp.nz = L.nz;
+  fp.nxh = L.nxh;
+  fp.ndirs = L.ndirs;
+  fp.nphases = L.nphases;
+  fp.nbands = L.nbands;
+  fp.fastSI = P.bFastSIM;
+  fp.napod = P.napodize > 0 ? P.napodize : 0;
+  fp.inscale = inscale_;
+  for (size_t i = 0; i < sep_.size(); ++i) fp.sep[i] = sep_[i];
+  stage_begin("rows_fwd");
+  launch_rows_fwd(fp, raw_, raw_u16_, bg_, slope_, ratio_, apofact_, bands_, stream_);
+  stage_end();
+  stage_begin("cols_fwd_y");
+  fft_cols(L.ny, bands_, L.nxh, L.nxh, (long long)L.ndirs * L.nbands * L.nz, (long long)L.nxh * L.ny, false,
+           stream_);
+  stage_end();
+  if (L.nz > 1) {
+    stage_begin("cols_fwd_z");
+    fft_cols(L.nz, bands_, (long long)L.nxh * L.ny, (long long)L.nxh * L.ny, (long long)L.ndirs * L.nbands, L.H,
+             false, stream_);
+    stage_end();
+  }
+  preprocessed_ = true;
+  fitted_ = processed_ = false;
+}
+
+// ------------------------------------------------------------ geometry
+Engine::Geo Engine::geometry() const {
+  Geo g;
+  g.dkx = 1.0f / ((float)L.nx * P.dxy);
+  g.dky = 1.0f / ((float)L.ny * P.dxy);
+  g.dkz = P.dz > 0 ? 1.0f / ((float)L.nz * P.dz) : P.dkzotf;
+  g.kzscale = g.dkz / P.dkzotf;
+  g.rdistcutoff = (float)((double)P.na * 2.0 / ((double)P.wave * 0.001));
+  if ((double)g.rdistcutoff > 1. / (2. * (double)P.dxy)) g.rdistcutoff = (float)(1. / (2. * (double)P.dxy));
+  g.lambdaem = (float)((double)((float)P.wave / P.nimm) / 1.0e3);
+  g.lambdaexc = (float)(0.88 * (double)g.lambdaem);
+  g.alpha = asinf(P.na / P.nimm);
+  return g;
+}
+
+int Engine::zd_overlaps() const { return L.zd_ov_max; }
+
+OtfRA Engine::otf_table(int dir, int order) const {
+  const int d = P.bOneOTFperAngle ? dir : 0;
+  OtfRA o;
+  o.t = otf_ + ((size_t)d * L.norders + order) * L.otfsize;
+  o.nzotf = P.nzotf;
+  o.nxotf = P.nxotf;
+  o.dkrotf = P.dkrotf;
+  return o;
+}
+
+// ------------------------------------------------------------ overlaps / fit
+void Engine::make_overlaps(int dir, int o1, int o2, float k0x, float k0y) {
+  const Geo g = geometry();
+  OvParams p{};
+  p.nx = L.nx;
+  p.ny = L.ny;
+  p.nz = L.nz;
+  p.nxh = L.nxh;
+  p.zd = zd_overlaps();
+  p.order1 = o1;
+  p.order2 = o2;
+  p.kx = k0x * (float)(o2 - o1);
+  p.ky = k0y * (float)(o2 - o1);
+  p.rdistcutoff = g.rdistcutoff;
+  p.otfcutoff = P.otfcutoff;
+  p.order0_2_factor = L.nz > 1 ? 5.0f : 1.0f;
+  p.dkx = g.dkx;
+  p.dky = g.dky;
+  p.kzscale = g.kzscale;
+  p.bNoKz0 = P.bNoKz0;
+  p.otf1 = otf_table(dir, o1);
+  p.otf2 = otf_table(dir, o2);
+  const float2* db = bands_ + (size_t)dir * L.nbands * L.H;
+  p.b1re = o1 == 0 ? db : db + (size_t)(2 * o1 - 1) * L.H;
+  p.b1im = o1 == 0 ? nullptr : db + (size_t)(2 * o1) * L.H;
+  p.b2re = db + (size_t)(2 * o2 - 1) * L.H;
+  p.b2im = db + (size_t)(2 * o2) * L.H;
+  stage_begin("overlaps");
+  launch_overlaps_xy(p, ovx_, stream_);
+  launch_overlaps_z_reduce(p, ovx_, P_, SX_, SY_, fit_fp32_, stream_);
+  if (!fit_fp32_) launch_sum2(SX_, SY_, L.nxy, sumparts_, stream_);
+  stage_end();
+  ov_totals_valid_ = false;
+  cache_.clear();
+  cache_o1_ = o1;
+  cache_o2_ = o2;
+}
+
+std::vector<Engine::Eval> Engine::eval_batch(const std::vector<float2>& kvecs) {
+  std::vector<Eval> res;
+  for (size_t c0 = 0; c0 < kvecs.size(); c0 += max_cand_) {
+    const int nc = (int)std::min<size_t>(max_cand_, kvecs.size() - c0);
+    SIMR_HIP(hipMemcpyAsync(kcand_, kvecs.data() + c0, nc * 8, hipMemcpyHostToDevice, stream_));
+    stage_begin("modamp");
+    if (fit_fp32_) {
+      const int nb = modamp_partials_fp32(L.nx, L.ny);
+      launch_modamp_fp32(P_, SX_, SY_, kcand_, nc, L.nx, L.ny, P.dxy, part32_, stream_);
+      stage_end();
+      SIMR_HIP(hipMemcpyAsync(hstage_, part32_, (size_t)nc * nb * 16, hipMemcpyDeviceToHost, stream_));
+      sync();
+      const float4* h = (const float4*)hstage_;
+      for (int c = 0; c < nc; ++c) {
+        float xr = 0, xi = 0, sx = 0, sy = 0;  // cpuReduce order (gpuFunctionsImpl_hh.cu:162-180)
+        for (int b = 0; b < nb; ++b) {
+          const float4 v = h[(size_t)c * nb + b];
+          xr += v.x;
+          xi += v.y;
+          sx += v.z;
+          sy += v.w;
+        }
+        res.push_back({xr, xi, sx, sy});
+      }
+    } else {
+      const int nb = modamp_partials_fp64(L.nx, L.ny);
+      launch_modamp_fp64(P_, kcand_, nc, L.nx, L.ny, P.dxy, part64_, stream_);
+      stage_end();
+      char* h = (char*)hstage_;
+      SIMR_HIP(hipMemcpyAsync(h, part64_, (size_t)nc * nb * 16, hipMemcpyDeviceToHost, stream_));
+      const size_t soff = (size_t)max_cand_ * nb * 16;
+      if (!ov_totals_valid_)
+        SIMR_HIP(hipMemcpyAsync(h + soff, sumparts_, 2 * 128 * 8, hipMemcpyDeviceToHost, stream_));
+      sync();
+      if (!ov_totals_valid_) {
+        const double* sp = (const double*)(h + soff);
+        double sx = 0, sy = 0;
+        for (int b = 0; b < 128; ++b) sx += sp[b];
+        for (int b = 0; b < 128; ++b) sy += sp[128 + b];
+        ov_sx_ = sx;
+        ov_sy_ = sy;
+        ov_totals_valid_ = true;
+      }
+      const double2* hp = (const double2*)h;
+      for (int c = 0; c < nc; ++c) {
+        double xr = 0, xi = 0;
+        for (int b = 0; b < nb; ++b) {
+          xr += hp[(size_t)c * nb + b].x;
+          xi += hp[(size_t)c * nb + b].y;
+        }
+        res.push_back({xr, xi, ov_sx_, ov_sy_});
+      }
+    }
+  }
+  return res;
+}
+
+// findrealspacemodamp's result arithmetic (gpuFunctionsImpl.cu:1128-1147)
+void Engine::modamp_of(const Eval& e, float2* modamp, double* amp2, float* corr) const {
+  if (fit_fp32_) {
+    const float xr = (float)e.xr, xi = (float)e.xi, sx = (float)e.sx, sy = (float)e.sy;
+    const float mx = xr / sx, my = xi / sx;
+    *modamp = make_float2(mx, my);
+    const float a2 = mx * mx + my * my;
+    *amp2 = (double)a2;
+    const float cc = (xr * xr + xi * xi) / (sx * sy);
+    if (corr) *corr = sqrtf(cc);
+  } else {
+    const double mx = e.xr / e.sx, my = e.xi / e.sx;
+    *modamp = make_float2((float)mx, (float)my);
+    *amp2 = mx * mx + my * my;
+    if (corr) *corr = (float)std::sqrt((e.xr * e.xr + e.xi * e.xi) / (e.sx * e.sy));
+  }
+}
+
+static std::pair<uint32_t, uint32_t> key_of(float a, float b) {
+  uint32_t x, y;
+  std::memcpy(&x, &a, 4);
+  std::memcpy(&y, &b, 4);
+  return {x, y};
+}
+
+static float2 kvec_of(float angle, float mag, int o1, int o2) {
+  // getmodamp (gpuFunctionsImpl.cu:1058-1059) then findrealspacemodamp (1093-1094)
+  const float kx = mag * cosf(angle), ky = mag * sinf(angle);
+  return make_float2(kx * (float)(o2 - o1), ky * (float)(o2 - o1));
+}
+
+void Engine::prefetch(int o1, int o2, const std::vector<std::pair<float, float>>& am) {
+  std::vector<float2> kv;
+  std::vector<std::pair<uint32_t, uint32_t>> keys;
+  for (auto& x : am) {
+    auto k = key_of(x.first, x.second);
+    if (cache_.count(k) || std::find(keys.begin(), keys.end(), k) != keys.end()) continue;
+    keys.push_back(k);
+    kv.push_back(kvec_of(x.first, x.second, o1, o2));
+  }
+  if (kv.empty()) return;
+  auto ev = eval_batch(kv);
+  for (size_t i = 0; i < keys.size(); ++i) cache_[keys[i]] = ev[i];
+}
+
+// getmodamp (gpuFunctionsImpl.cu:1046-1072) on the current overlap set.
+double Engine::getmodamp(int dir, float angle, float mag, int o1, int o2, float2* modamp, float* corr) {
+  auto k = key_of(angle, mag);
+  if (!cache_.count(k)) prefetch(o1, o2, {{angle, mag}});
+  double a2;
+  modamp_of(cache_[k], modamp, &a2, corr);
+  stats_[dir].n_reductions++;
+  return a2;
+}
+
+double Engine::realspacemodamp(int dir, int o1, int o2, float2 kv, bool redo, float2* modamp, float* corr) {
+  if (redo) make_overlaps(dir, o1, o2, kv.x, kv.y);
+  const float2 k = make_float2(kv.x * (float)(o2 - o1), kv.y * (float)(o2 - o1));
+  auto ev = eval_batch({k});
+  stats_[dir].n_reductions++;
+  double a2;
+  modamp_of(ev[0], modamp, &a2, corr);
+  return a2;
+}
+
+// findk0 (gpuFunctionsImpl.cu:306-372) with findpeak/fitparabola (443-495)
+static float fitparabola(float a1, float a2, float a3) {
+  const float slope = (float)(0.5 * (double)(a3 - a1));
+  const float curve = (a3 + a1) - 2 * a2;
+  if (curve == 0) return 0.0f;
+  const float peak = -slope / curve;
+  if (peak > 1.5f || peak < -1.5f) return 0.0f;
+  return peak;
+}
+
+void Engine::findk0(int dir) {
+  const int fo2 = L.nz > 1 ? 2 : 1;
+  make_overlaps(dir, 0, fo2, k0_[dir].x, k0_[dir].y);
+  stage_begin("findk0_peak");
+  launch_findpeak(P_, cc_, pv_, pi_, peakvals_, peakidx_, L.nx, L.ny, stream_);
+  stage_end();
+  char* h = (char*)hstage_;
+  SIMR_HIP(hipMemcpyAsync(h, peakvals_, 5 * 4, hipMemcpyDeviceToHost, stream_));
+  SIMR_HIP(hipMemcpyAsync(h + 64, peakidx_, 4, hipMemcpyDeviceToHost, stream_));
+  sync();
+  const float* v = (const float*)h;
+  const int idx = *(const int*)(h + 64);
+  const int xc = idx % L.nx, yc = idx / L.nx;
+  float2 pk;
+  pk.x = fitparabola(v[1], v[0], v[2]) + (float)xc;
+  pk.y = fitparabola(v[3], v[0], v[4]) + (float)yc;
+  const Geo g = geometry();
+  const float2 old = k0_[dir];
+  if (old.x / g.dkx < pk.x - (float)(L.nx / 2)) pk.x -= (float)L.nx;
+  if (old.x / g.dkx > pk.x + (float)(L.nx / 2)) pk.x += (float)L.nx;
+  if (old.y / g.dky < pk.y - (float)(L.ny / 2)) pk.y -= (float)L.ny;
+  if (old.y / g.dky > pk.y + (float)(L.ny / 2)) pk.y += (float)L.ny;
+  pk.x *= g.dkx / (float)fo2;
+  pk.y *= g.dky / (float)fo2;
+  k0_[dir] = pk;
+}
+
+template <typename U>
+static U fitxyparabola(U x1, U y1, U x2, U y2, U x3, U y3) {
+  if (x1 == x2 || x2 == x3 || x3 == x1) return 0;
+  const U xbar1 = (U)(0.5 * (double)(x1 + x2));
+  const U xbar2 = (U)(0.5 * (double)(x2 + x3));
+  const U slope1 = (y2 - y1) / (x2 - x1);
+  const U slope2 = (y3 - y2) / (x3 - x2);
+  const U curve = (slope2 - slope1) / (xbar2 - xbar1);
+  if (curve == 0) return 0;
+  return xbar2 - slope2 / curve;
+}
+
+// fitk0andmodamps (gpuFunctionsImpl.cu:874-1020).  The reference evaluates the
+// line-search steps one at a time with a synchronous reduction each; the
+// step values it will visit are deterministic (float accumulation of the
+// step), so each walk's next kLook steps in both directions are reduced in
+// one launch and the reference's step rule is replayed on the results.
+void Engine::fitk0andmodamps(int dir) {
+  constexpr int kLook = 6;
+  const int fo1 = 0, fo2 = L.nz > 1 ? 2 : 1;
+  const float2 kv = k0_[dir];
+  const float k0mag = sqrtf(kv.x * kv.x + kv.y * kv.y);
+  const float k0angle = atan2f(kv.y, kv.x);
+  auto parab = [&](float x1, double y1, float x2, double y2, float x3, double y3) -> float {
+    if (!fit_fp32_) return (float)fitxyparabola<double>(x1, y1, x2, y2, x3, y3);
+    return fitxyparabola<float>(x1, (float)y1, x2, (float)y2, x3, (float)y3);
+  };
+  const float deltaangle = 0.001f;
+  const float deltamag = (float)(0.1 / (double)((float)std::max(L.nx, L.ny) * P.dxy));
+  auto walk = [&](float start, float mag_or_angle, bool is_angle, float delta, int sign) {
+    std::vector<std::pair<float, float>> v;
+    float a = start;
+    for (int m = 0; m < kLook; ++m) {
+      v.push_back(is_angle ? std::make_pair(a, mag_or_angle) : std::make_pair(mag_or_angle, a));
+      a = sign > 0 ? a + delta : a - delta;
+    }
+    return v;
+  };
+  auto ensure = [&](float angle, float mag, bool is_angle, float delta, int sign) {
+    if (cache_.count(key_of(angle, mag))) return;
+    prefetch(fo1, fo2, is_angle ? walk(angle, mag, true, delta, sign) : walk(mag, angle, false, delta, sign));
+  };
+
+  // first evaluation recomputes the overlaps (redoarrays = recalcarrays >= 1)
+  {
+    const float2 k1 = make_float2(k0mag * cosf(k0angle), k0mag * sinf(k0angle));
+    make_overlaps(dir, fo1, fo2, k1.x, k1.y);
+  }
+  // angle walk: both directions' first kLook steps in one batch
+  {
+    auto up = walk(k0angle + deltaangle, k0mag, true, deltaangle, +1);
+    auto dn = walk(k0angle - deltaangle, k0mag, true, deltaangle, -1);
+    std::vector<std::pair<float, float>> all{{k0angle, k0mag}};
+    all.insert(all.end(), up.begin(), up.end());
+    all.insert(all.end(), dn.begin(), dn.end());
+    prefetch(fo1, fo2, all);
+  }
+  float2 modamp;
+  float x2 = k0angle;
+  double amp2 = getmodamp(dir, k0angle, k0mag, fo1, fo2, &modamp, nullptr);
+  stats_[dir].n_angle_evals++;
+  float angle = k0angle + deltaangle;
+  float x3 = angle;
+  double amp3 = getmodamp(dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
+  stats_[dir].n_angle_evals++;
+  double amp1 = 0;
+  float x1 = 0;
+  if (amp3 > amp2) {
+    while (amp3 > amp2) {
+      amp1 = amp2;
+      x1 = x2;
+      amp2 = amp3;
+      x2 = x3;
+      angle += deltaangle;
+      x3 = angle;
+      ensure(angle, k0mag, true, deltaangle, +1);
+      amp3 = getmodamp(dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
+      stats_[dir].n_angle_evals++;
+    }
+  } else {
+    angle = k0angle;
+    double a = amp3;
+    amp3 = amp2;
+    amp2 = a;
+    float ax = x3;
+    x3 = x2;
+    x2 = ax;
+    while (amp3 > amp2) {
+      amp1 = amp2;
+      x1 = x2;
+      amp2 = amp3;
+      x2 = x3;
+      angle -= deltaangle;
+      x3 = angle;
+      ensure(angle, k0mag, true, deltaangle, -1);
+      amp3 = getmodamp(dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
+      stats_[dir].n_angle_evals++;
+    }
+  }
+  angle = parab(x1, amp1, x2, amp2, x3, amp3);
+
+  // magnitude walk at the fitted angle
+  {
+    auto up = walk(k0mag + deltamag, angle, false, deltamag, +1);
+    auto dn = walk(k0mag - deltamag, angle, false, deltamag, -1);
+    std::vector<std::pair<float, float>> all{{angle, k0mag}};
+    all.insert(all.end(), up.begin(), up.end());
+    all.insert(all.end(), dn.begin(), dn.end());
+    prefetch(fo1, fo2, all);
+  }
+  x2 = k0mag;
+  amp2 = getmodamp(dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
+  stats_[dir].n_mag_evals++;
+  float mag = k0mag + deltamag;
+  x3 = mag;
+  amp3 = getmodamp(dir, angle, mag, fo1, fo2, &modamp, nullptr);
+  stats_[dir].n_mag_evals++;
+  if (amp3 > amp2) {
+    while (amp3 > amp2) {
+      amp1 = amp2;
+      x1 = x2;
+      amp2 = amp3;
+      x2 = x3;
+      mag += deltamag;
+      x3 = mag;
+      ensure(angle, mag, false, deltamag, +1);
+      amp3 = getmodamp(dir, angle, mag, fo1, fo2, &modamp, nullptr);
+      stats_[dir].n_mag_evals++;
+    }
+  } else {
+    mag = k0mag;
+    double a = amp3;
+    amp3 = amp2;
+    amp2 = a;
+    float ax = x3;
+    x3 = x2;
+    x2 = ax;
+    while (amp3 > amp2) {
+      amp1 = amp2;
+      x1 = x2;
+      amp2 = amp3;
+      x2 = x3;
+      mag -= deltamag;
+      x3 = mag;
+      ensure(angle, mag, false, deltamag, -1);
+      amp3 = getmodamp(dir, angle, mag, fo1, fo2, &modamp, nullptr);
+      stats_[dir].n_mag_evals++;
+    }
+  }
+  mag = parab(x1, amp1, x2, amp2, x3, amp3);
+
+  float cc = 0;
+  getmodamp(dir, angle, mag, fo1, fo2, &modamp, &cc);
+  corr_[dir][fo2] = cc;
+  k0_[dir] = make_float2(mag * cosf(angle), mag * sinf(angle));
+  amp_[dir][fo2] = modamp;
+  if (L.nz == 1) {
+    for (int order = 2; order < L.norders; ++order) {
+      realspacemodamp(dir, order - 1, order, make_float2(mag * cosf(angle), mag * sinf(angle)), true, &modamp, &cc);
+      amp_[dir][order] = modamp;
+      corr_[dir][order] = cc;
+    }
+  } else {
+    for (int order = 1; order < L.norders; ++order)
+      if (order != fo2) {
+        realspacemodamp(dir, 0, order, make_float2(mag * cosf(angle), mag * sinf(angle)), true, &modamp, &cc);
+        amp_[dir][order] = modamp;
+        corr_[dir][order] = cc;
+      }
+  }
+}
+
+static float2 cmulh(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+static float cmagh(float2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
+
+// k0 policy of findModulationVectorsAndPhasesForAllDirections (cudaSirecon.cpp:296-492)
+void Engine::modulation(int dir) {
+  const bool reuse = P.ntimes > 1 && cur_t_ > 0 && P.bUseTime0k0;
+  stats_[dir] = simr_fit{};
+  if (!reuse) k0_[dir] = k0guess_[dir];
+  if (P.bSearchforvector && !reuse) {
+    findk0(dir);
+    fitk0andmodamps(dir);
+    if (cur_t_ == 0) k0_time0_[dir] = k0_[dir];
+    const float dx = k0_[dir].x - k0guess_[dir].x, dy = k0_[dir].y - k0guess_[dir].y;
+    const float dist = sqrtf(dx * dx + dy * dy);
+    if (P.ntimes > 1 && cur_t_ > 0 && dist > 2 * K0_WARNING_THRESH) {
+      k0_[dir] = k0_time0_[dir];
+      for (int order = 1; order < L.norders; ++order) {
+        float2 m;
+        float cc;
+        const int o1 = L.nz > 1 ? 0 : order - 1;
+        realspacemodamp(dir, o1, order, k0_[dir], true, &m, &cc);
+        amp_[dir][order] = m;
+        corr_[dir][order] = cc;
+      }
+    }
+  } else {
+    for (int order = 1; order < L.norders; ++order) {
+      float2 m;
+      float cc;
+      realspacemodamp(dir, 0, order, k0_[dir], true, &m, &cc);
+      amp_[dir][order] = m;
+      corr_[dir][order] = cc;
+    }
+  }
+  if (L.nz == 1)
+    for (int order = 2; order < L.norders; ++order) amp_[dir][order] = cmulh(amp_[dir][order], amp_[dir][order - 1]);
+  if (P.nforceamp > 0 && P.forceamp[0] > 0.0f) {
+    for (int order = 1; order < L.norders; ++order) {
+      const float a = cmagh(amp_[dir][order]);
+      if (order - 1 < P.nforceamp && a < P.forceamp[order - 1]) {
+        const float f = P.forceamp[order - 1] / a;
+        amp_[dir][order].x *= f;
+        amp_[dir][order].y *= f;
+      }
+    }
+  }
+  if (!P.bFitallphases) {
+    const float base_phase = atan2f(amp_[dir][1].y, amp_[dir][1].x);
+    for (int order = 2; order < L.norders; ++order) {
+      const float2 amplitude = make_float2(cmagh(amp_[dir][order]), 0);
+      const float phi = (float)order * base_phase;
+      amp_[dir][order] = cmulh(amplitude, make_float2((float)std::cos((double)phi), (float)std::sin((double)phi)));
+    }
+  }
+}
+
+void Engine::find_modulation() {
+  if (!preprocessed_) throw StateError("find_modulation before load_and_rescale");
+  for (int dir = 0; dir < L.ndirs; ++dir) modulation(dir);
+  fitted_ = true;
+}
+
+// ------------------------------------------------------------ filter
+FilterParams Engine::filter_params(int dir, int order) const {
+  FilterParams f{};
+  const Geo g = geometry();
+  f.nx = L.nx;
+  f.ny = L.ny;
+  f.nz = L.nz;
+  f.nxh = L.nxh;
+  f.dir = dir;
+  f.ndirs = L.ndirs;
+  f.order = order;
+  f.norders = L.norders;
+  f.wiener = P.wiener * P.wiener;
+  f.kzscale = g.kzscale;
+  const float k0mag = sqrtf(k0_[0].x * k0_[0].x + k0_[0].y * k0_[0].y);
+  const float lambdaem = (float)((double)((float)P.wave / P.nimm) / 1000.0);
+  const float alpha = asinf(P.na / P.nimm);
+  const float lambdaexc = (float)(0.88 * (double)lambdaem);
+  float rdistcutoff = (float)((double)(P.na * 2.0f) / ((double)P.wave / 1000.0));
+  if ((double)rdistcutoff > 1. / (2. * (double)P.dxy)) rdistcutoff = (float)(1. / (2. * (double)P.dxy));
+  float zq = (1.0f - cosf(alpha)) / lambdaem;
+  zq = zq / g.dkz;
+  const int zd0 = (int)std::ceil(zq);
+  f.zd[0] = zd0;
+  f.zd[L.norders - 1] = (int)(1.3 * (double)zd0);
+  if (L.norders >= 3)
+    for (int o = 1; o < L.norders - 1; o++) f.zd[o] = (int)((1.0f + lambdaem / lambdaexc) * (float)zd0);
+  for (int o = 0; o < L.norders; o++)
+    if (f.zd[o] >= L.nz / 2) f.zd[o] = ((L.nz / 2 - 1) > 0 ? (L.nz / 2 - 1) : 0);
+  f.rdistcutoff = rdistcutoff;
+  f.apocutoff = rdistcutoff + k0mag * (float)(L.norders - 1);
+  f.zapocutoff = L.norders > 1 ? (float)f.zd[1] : (float)f.zd[0];
+  f.dkx = (float)(1. / (double)((float)L.nx * P.dxy));
+  f.dky = (float)(1. / (double)((float)L.ny * P.dxy));
+  f.min_dkr = f.dkx < f.dky ? f.dkx : f.dky;
+  f.suppRadius = (float)P.suppression_radius * f.min_dkr;
+  f.bSuppress = P.bSuppress_singularities;
+  f.bDampenOrder0 = P.bDampenOrder0;
+  f.bNoKz0 = P.bNoKz0;
+  f.bFilteroverlaps = P.bFilteroverlaps;
+  f.apodizeoutput = P.apodizeoutput;
+  for (int d = 0; d < L.ndirs; ++d) {
+    f.k0[d] = k0_[d];
+    for (int o = 0; o < L.norders; ++o) {
+      f.ampmag2_all[d * L.norders + o] = amp_[d][o].x * amp_[d][o].x + amp_[d][o].y * amp_[d][o].y;
+      f.noiseVar[d * L.norders + o] = 1.0f;  // allocSepMatrixAndNoiseVarFactors (cudaSirecon.cpp:728)
+    }
+  }
+  f.conjamp = make_float2(amp_[dir][order].x, -amp_[dir][order].y);
+  for (int o = 0; o < L.norders; ++o) f.otf[o] = otf_table(dir, o);
+  float2* db = bands_ + (size_t)dir * L.nbands * L.H;
+  f.bp = order == 0 ? db : db + (size_t)(2 * order - 1) * L.H;
+  f.bp2 = order == 0 ? nullptr : db + (size_t)(2 * order) * L.H;
+  return f;
+}
+
+void Engine::filter_dir(int dir) {
+  for (int order = 0; order < L.norders; ++order) launch_filter(filter_params(dir, order), stream_);
+}
+
+void Engine::assemble() {
+  AsmParams a{};
+  a.nx = L.nx;
+  a.ny = L.ny;
+  a.nz = L.nz;
+  a.nxh = L.nxh;
+  a.xdim = L.xdim;
+  a.ydim = L.ydim;
+  a.zdim = L.zdim;
+  a.nasm = L.nasm;
+  a.ndirs = L.ndirs;
+  a.norders = L.norders;
+  a.bands = bands_;
+  a.band_stride = L.H;
+  a.nbands = L.nbands;
+  a.stageZ = stageZ_;
+  a.stageY = stageY_;
+  a.out = out_;
+  a.fact = (float)((double)P.explodefact / 0.5);
+  a.dxyo = P.dxy / P.zoomfact;
+  for (int d = 0; d < L.ndirs; ++d) a.k0[d] = k0_[d];
+  stage_begin("asm_z");
+  launch_asm_z(a, stream_);
+  stage_end();
+  stage_begin("asm_y");
+  launch_asm_y(a, stream_);
+  stage_end();
+  stage_begin("asm_x");
+  for (int a0 = 0; a0 < L.nasm; a0 += 8) launch_asm_x(a, a0, std::min(L.nasm, a0 + 8), a0 > 0, stream_);
+  stage_end();
+}
+
+void Engine::filter_assemble() {
+  if (!fitted_) throw StateError("filter_assemble before find_modulation");
+  stage_begin("filter");
+  for (int dir = 0; dir < L.ndirs; ++dir) filter_dir(dir);
+  stage_end();
+  assemble();
+  processed_ = true;
+}
+
+void Engine::get_result(float* host) {
+  if (!processed_) throw StateError("no processed volume");
+  SIMR_HIP(hipMemcpyAsync(host, out_, result_elems() * 4, hipMemcpyDeviceToHost, stream_));
+  sync();
+}
+
+simr_fit Engine::fit(int dir) const {
+  SIMR_REQUIRE(dir >= 0 && dir < L.ndirs, "direction out of range");
+  simr_fit f = stats_[dir];
+  f.k0x = k0_[dir].x;
+  f.k0y = k0_[dir].y;
+  f.k0guess_x = k0guess_[dir].x;
+  f.k0guess_y = k0guess_[dir].y;
+  for (int o = 0; o < SIMR_MAX_ORDERS; ++o) {
+    f.amp_re[o] = o < L.norders ? amp_[dir][o].x : 0.f;
+    f.amp_im[o] = o < L.norders ? amp_[dir][o].y : 0.f;
+    f.corr[o] = corr_[dir][o];
+  }
+  return f;
+}
+
+void Engine::set_fit(int dir, const simr_fit& f) {
+  SIMR_REQUIRE(dir >= 0 && dir < L.ndirs, "direction out of range");
+  k0_[dir] = make_float2(f.k0x, f.k0y);
+  for (int o = 0; o < L.norders; ++o) amp_[dir][o] = make_float2(f.amp_re[o], f.amp_im[o]);
+  fitted_ = true;
+}
+
+void Engine::get_band(int dir, int band, float* host) {
+  SIMR_REQUIRE(dir >= 0 && dir < L.ndirs && band >= 0 && band < L.nbands, "band index out of range");
+  SIMR_HIP(hipMemcpyAsync(host, bands_ + ((size_t)dir * L.nbands + band) * L.H, L.H * 8, hipMemcpyDeviceToHost,
+                          stream_));
+  sync();
+}
+
+void Engine::set_band(int dir, int band, const float* host) {
+  SIMR_REQUIRE(dir >= 0 && dir < L.ndirs && band >= 0 && band < L.nbands, "band index out of range");
+  SIMR_HIP(hipMemcpyAsync(bands_ + ((size_t)dir * L.nbands + band) * L.H, host, L.H * 8, hipMemcpyHostToDevice,
+                          stream_));
+  sync();
+  preprocessed_ = true;
+}
+
+void Engine::debug_overlaps(int dir, int o1, int o2, float k0x, float k0y, double* Ph, double* sx, double* sy) {
+  make_overlaps(dir, o1, o2, k0x, k0y);
+  SIMR_HIP(hipMemcpyAsync(Ph, P_, L.nxy * 16, hipMemcpyDeviceToHost, stream_));
+  SIMR_HIP(hipMemcpyAsync(sx, SX_, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
+  SIMR_HIP(hipMemcpyAsync(sy, SY_, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
+  sync();
+}
+
+}  // namespace simr

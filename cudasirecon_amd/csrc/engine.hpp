@@ -1,0 +1,151 @@
+// Host orchestration of one reconstruction context (replaces the reference's
+// SIM_Reconstructor internals: setup_common, loadImageData, rescaleDriver,
+// processOneVolume and findModulationVectorsAndPhasesForAllDirections,
+// src/cudaSirecon/cudaSirecon.cpp:194-494,1139-1321,1354-1435).
+#pragma once
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/simr.h"
+#include "common.hpp"
+#include "stages.hpp"
+
+namespace simr {
+
+// One hipMalloc per context; buffers are bump-allocated from it (replaces the
+// reference's per-stage cudaMalloc/cudaFree churn in src/Buffers/GPUBuffer.cpp).
+class Arena {
+ public:
+  void reserve(size_t bytes);
+  void* take(size_t bytes);
+  void release();
+  size_t capacity() const { return cap_; }
+  size_t used() const { return off_; }
+  ~Arena() { release(); }
+
+ private:
+  char* base_ = nullptr;
+  size_t cap_ = 0, off_ = 0;
+};
+
+struct Layout {
+  // sizes derived from the parameters (identical for the footprint query and the context)
+  int nx, ny, nz, nxh, ndirs, nphases, norders, nbands, nimg, nDirsOTF;
+  long long H, nxy;
+  int xdim, ydim, zdim, nasm, zd_ov_max;
+  size_t otfsize;
+  size_t bytes_total;
+  static Layout of(const simr_params& p);
+};
+
+class Engine {
+ public:
+  Engine(const simr_params& p, const float* otf, int device);
+  ~Engine();
+
+  void set_raw(const void* src, bool u16, bool on_device);
+  void set_flatfield(const float* bg, const float* slope);
+  void load_and_rescale(int it);
+  void set_time(int it) { cur_t_ = it; }
+  void find_modulation();
+  void filter_assemble();
+  void filter_dir(int dir);
+  void get_result(float* host);
+  const float* result_dev() const { return out_; }
+  size_t result_elems() const { return (size_t)L.zdim * L.ydim * L.xdim; }
+  simr_fit fit(int dir) const;
+  void set_fit(int dir, const simr_fit& f);
+  void get_band(int dir, int band, float* host);
+  void set_band(int dir, int band, const float* host);
+  void debug_overlaps(int dir, int o1, int o2, float k0x, float k0y, double* P, double* sx, double* sy);
+  void sync();
+  hipStream_t stream() const { return stream_; }
+  const simr_params& params() const { return P; }
+  void enable_timing(bool on) { timing_ = on; }
+  int timings(const char** names, float* ms, int cap);
+
+ private:
+  struct Geo {
+    float dkx, dky, dkz, kzscale, rdistcutoff, lambdaem, lambdaexc, alpha;
+  };
+  struct Eval {
+    double xr, xi, sx, sy;
+  };
+
+  Geo geometry() const;
+  int zd_overlaps() const;
+  OtfRA otf_table(int dir, int order) const;
+  void make_overlaps(int dir, int o1, int o2, float k0x, float k0y);
+  std::vector<Eval> eval_batch(const std::vector<float2>& kvecs);
+  void modamp_of(const Eval& e, float2* modamp, double* amp2, float* corr) const;
+  double getmodamp(int dir, float angle, float mag, int o1, int o2, float2* modamp, float* corr);
+  double realspacemodamp(int dir, int o1, int o2, float2 kv, bool redo, float2* modamp, float* corr);
+  void prefetch(int o1, int o2, const std::vector<std::pair<float, float>>& am);
+  void findk0(int dir);
+  void fitk0andmodamps(int dir);
+  void modulation(int dir);
+  FilterParams filter_params(int dir, int order) const;
+  void assemble();
+  void stage_begin(const char* name);
+  void stage_end();
+
+  simr_params P;
+  Layout L;
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  Arena arena_;
+  bool fit_fp32_ = false;
+  float inscale_ = 0;
+  std::vector<float> sep_;  // [band][phase]
+  std::vector<float> apof_;
+
+  // device buffers
+  void* raw_ = nullptr;
+  bool raw_u16_ = true, raw_set_ = false;
+  float *bg_ = nullptr, *slope_ = nullptr, *ratio_ = nullptr, *apofact_ = nullptr;
+  double *sums_ = nullptr, *sdp_ = nullptr;
+  float2* bands_ = nullptr;
+  float2* otf_ = nullptr;
+  float2* ovx_ = nullptr;
+  double2* P_ = nullptr;
+  double *SX_ = nullptr, *SY_ = nullptr;
+  float2* cc_ = nullptr;
+  float* pv_ = nullptr;
+  int* pi_ = nullptr;
+  float* peakvals_ = nullptr;
+  int* peakidx_ = nullptr;
+  float2* kcand_ = nullptr;
+  double2* part64_ = nullptr;
+  float4* part32_ = nullptr;
+  double* sumparts_ = nullptr;
+  float2 *stageZ_ = nullptr, *stageY_ = nullptr;
+  float* out_ = nullptr;
+  int max_cand_ = 64;
+
+  // pinned host staging
+  void* hstage_ = nullptr;
+  size_t hstage_bytes_ = 0;
+
+  // overlap-set state
+  bool ov_totals_valid_ = false;
+  double ov_sx_ = 0, ov_sy_ = 0;
+  std::map<std::pair<uint32_t, uint32_t>, Eval> cache_;
+  int cache_o1_ = -1, cache_o2_ = -1;
+
+  // reconstruction state
+  int cur_t_ = 0;
+  bool preprocessed_ = false, fitted_ = false, processed_ = false;
+  std::vector<float2> k0_, k0_time0_, k0guess_;
+  std::vector<std::vector<float2>> amp_;
+  std::vector<std::vector<float>> corr_;
+  std::vector<simr_fit> stats_;
+
+  // timing
+  bool timing_ = false;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> events_;
+  std::vector<std::pair<std::string, float>> last_timings_;
+};
+
+}  // namespace simr

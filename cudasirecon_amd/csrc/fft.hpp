@@ -1,0 +1,279 @@
+// LDS-staged Stockham autosort FFT for gfx950 (replaces the reference's cuFFT
+// calls, src/cudaSirecon/cudaSirecon.cpp:238-257,542-579 and
+// src/cudaSirecon/gpuFunctionsImpl.cu:333-349,639-669,1781-1849).
+//
+// Formulation (self-sorting Stockham, radix R, sub-transform length Ns):
+//   for every butterfly j in [0, N/R):  k = j mod Ns
+//     v[r] = a[j + r N/R] * w_N^{r k N/(Ns R)},  r = 0..R-1
+//     v    = DFT_R(v)
+//     b[(j/Ns) Ns R + k + q Ns] = v[q]
+// run for the radix sequence of N (8s first, then 4, 2, 5, 3), Ns = product of
+// the radices already applied.  Forward uses w_N = exp(-2 pi i / N), inverse
+// its conjugate; both unnormalised (as cuFFT).
+//
+// One workgroup holds W transforms of length N in LDS; TPT threads cooperate
+// on each transform.  Everything about N (radices, butterflies per thread,
+// twiddle strides) is a compile-time constant, so all loops unroll and the
+// butterfly operands stay in VGPRs; each stage is one LDS read sweep, a
+// barrier and one LDS write sweep (in place).  Twiddles come from a per-N
+// table of exp(-2 pi i m / N), m < N, computed in long double on the host.
+#pragma once
+#include "common.hpp"
+
+namespace simr {
+
+struct Radices {
+  int n;
+  int r[16];
+};
+
+constexpr Radices make_radices(int N) {
+  Radices out{0, {}};
+  int m = N;
+  while (m % 8 == 0 && m > 8) {
+    out.r[out.n++] = 8;
+    m /= 8;
+  }
+  if (m == 8) {
+    out.r[out.n++] = 8;
+    m = 1;
+  }
+  if (m % 4 == 0) {
+    out.r[out.n++] = 4;
+    m /= 4;
+  }
+  if (m % 2 == 0) {
+    out.r[out.n++] = 2;
+    m /= 2;
+  }
+  while (m % 5 == 0) {
+    out.r[out.n++] = 5;
+    m /= 5;
+  }
+  while (m % 3 == 0) {
+    out.r[out.n++] = 3;
+    m /= 3;
+  }
+  if (m != 1) out.n = -1;  // unsupported factor
+  return out;
+}
+
+constexpr int radix_prod_before(const Radices& rd, int s) {
+  int p = 1;
+  for (int i = 0; i < s; ++i) p *= rd.r[i];
+  return p;
+}
+
+constexpr int max_radix(const Radices& rd) {
+  int m = 1;
+  for (int i = 0; i < rd.n; ++i) m = rd.r[i] > m ? rd.r[i] : m;
+  return m;
+}
+constexpr int min_radix(const Radices& rd) {
+  int m = 64;
+  for (int i = 0; i < rd.n; ++i) m = rd.r[i] < m ? rd.r[i] : m;
+  return m;
+}
+
+// Threads per transform: a power of two close to N / max_radix, at most 256.
+constexpr int fft_tpt(int N) {
+  Radices rd = make_radices(N);
+  int target = N / max_radix(rd);
+  int t = 1;
+  while (t * 2 <= target && t * 2 <= 256) t *= 2;
+  return t;
+}
+
+constexpr bool fft_supported(int N) { return N >= 1 && make_radices(N).n >= 0; }
+
+// ---------------------------------------------------------------- butterflies
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 a) {  // a * (-i) forward, a * (+i) inverse
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* v);
+
+template <>
+__device__ __forceinline__ void dft<1, false>(float2*) {}
+template <>
+__device__ __forceinline__ void dft<1, true>(float2*) {}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft2(float2* v) {
+  float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <>
+__device__ __forceinline__ void dft<2, false>(float2* v) { dft2<2, false>(v); }
+template <>
+__device__ __forceinline__ void dft<2, true>(float2* v) { dft2<2, true>(v); }
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  float2 t2 = cadd(a1, a3), t3 = mul_mi<INV>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  float2 x1 = cadd(t1, t3), x3 = csub(t1, t3);
+  a1 = x1;
+  a3 = x3;
+}
+template <>
+__device__ __forceinline__ void dft<4, false>(float2* v) { dft4<false>(v[0], v[1], v[2], v[3]); }
+template <>
+__device__ __forceinline__ void dft<4, true>(float2* v) { dft4<true>(v[0], v[1], v[2], v[3]); }
+
+template <bool INV>
+__device__ __forceinline__ void dft8(float2* v) {
+  // even/odd split: E = DFT4(v0,v2,v4,v6), O = DFT4(v1,v3,v5,v7)
+  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<INV>(e0, e1, e2, e3);
+  dft4<INV>(o0, o1, o2, o3);
+  const float h = 0.70710678118654752440f;
+  // w8^1 = h(1 -+ i), w8^2 = -+i, w8^3 = h(-1 -+ i)
+  float2 w1o1 = INV ? make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y))
+                    : make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+  float2 w2o2 = mul_mi<INV>(o2);
+  float2 w3o3 = INV ? make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y))
+                    : make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, w1o1);
+  v[5] = csub(e1, w1o1);
+  v[2] = cadd(e2, w2o2);
+  v[6] = csub(e2, w2o2);
+  v[3] = cadd(e3, w3o3);
+  v[7] = csub(e3, w3o3);
+}
+template <>
+__device__ __forceinline__ void dft<8, false>(float2* v) { dft8<false>(v); }
+template <>
+__device__ __forceinline__ void dft<8, true>(float2* v) { dft8<true>(v); }
+
+template <bool INV>
+__device__ __forceinline__ void dft3(float2* v) {
+  const float c = -0.5f;
+  const float s = 0.86602540378443864676f;  // sin(2 pi / 3)
+  float2 a0 = v[0];
+  float2 t1 = cadd(v[1], v[2]);
+  float2 t2 = csub(v[1], v[2]);
+  float2 m = make_float2(a0.x + c * t1.x, a0.y + c * t1.y);
+  // forward: X1 = m - i s t2, X2 = m + i s t2
+  float2 ist2 = make_float2(-s * t2.y, s * t2.x);  // i s t2
+  v[0] = cadd(a0, t1);
+  if (INV) {
+    v[1] = cadd(m, ist2);
+    v[2] = csub(m, ist2);
+  } else {
+    v[1] = csub(m, ist2);
+    v[2] = cadd(m, ist2);
+  }
+}
+template <>
+__device__ __forceinline__ void dft<3, false>(float2* v) { dft3<false>(v); }
+template <>
+__device__ __forceinline__ void dft<3, true>(float2* v) { dft3<true>(v); }
+
+template <bool INV>
+__device__ __forceinline__ void dft5(float2* v) {
+  const float c1 = 0.30901699437494742410f;   // cos(2 pi/5)
+  const float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
+  const float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
+  const float s2 = 0.58778525229247312917f;   // sin(4 pi/5)
+  float2 a0 = v[0];
+  float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  float2 b1 = make_float2(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+  float2 b2 = make_float2(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
+  float2 d1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+  float2 d2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+  float2 id1 = make_float2(-d1.y, d1.x), id2 = make_float2(-d2.y, d2.x);
+  v[0] = make_float2(a0.x + t1.x + t2.x, a0.y + t1.y + t2.y);
+  if (INV) {
+    v[1] = cadd(b1, id1);
+    v[4] = csub(b1, id1);
+    v[2] = cadd(b2, id2);
+    v[3] = csub(b2, id2);
+  } else {
+    v[1] = csub(b1, id1);
+    v[4] = cadd(b1, id1);
+    v[2] = csub(b2, id2);
+    v[3] = cadd(b2, id2);
+  }
+}
+template <>
+__device__ __forceinline__ void dft<5, false>(float2* v) { dft5<false>(v); }
+template <>
+__device__ __forceinline__ void dft<5, true>(float2* v) { dft5<true>(v); }
+
+// ------------------------------------------------------------- stage driver
+template <int N, bool INV, int TPT, int S>
+__device__ __forceinline__ void fft_stage(float2* buf, int t, const float2* __restrict__ tw) {
+  constexpr Radices RD = make_radices(N);
+  if constexpr (S < RD.n) {
+    constexpr int R = RD.r[S];
+    constexpr int Ns = radix_prod_before(RD, S);
+    constexpr int M = N / R;
+    constexpr int B = (M + TPT - 1) / TPT;
+    constexpr bool FULL = (M % TPT) == 0;
+    float2 v[B][R];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + b * TPT;
+      if (FULL || j < M) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * M];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + b * TPT;
+      if (FULL || j < M) {
+        const int k = j % Ns;
+        if constexpr (Ns > 1) {
+          constexpr int step = N / (Ns * R);
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            float2 w = tw[r * k * step];
+            if (INV) w.y = -w.y;
+            v[b][r] = cmul(v[b][r], w);
+          }
+        }
+        dft<R, INV>(v[b]);
+        const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) buf[idxD + q * Ns] = v[b][q];
+      }
+    }
+    __syncthreads();
+    fft_stage<N, INV, TPT, S + 1>(buf, t, tw);
+  }
+}
+
+// In-place FFT of buf[0..N) (contiguous in LDS) by the TPT threads t = 0..TPT-1
+// of one transform.  Every thread of the workgroup must call it (barriers).
+template <int N, bool INV, int TPT = fft_tpt(N)>
+__device__ __forceinline__ void fft_lds(float2* buf, int t, const float2* __restrict__ tw) {
+  static_assert(fft_supported(N), "FFT length must factor into 2, 3, 5");
+  if constexpr (N > 1) fft_stage<N, INV, TPT, 0>(buf, t, tw);
+}
+
+// LDS row pitch for a transform of length N (one extra element breaks the
+// power-of-two bank stride between transforms).
+constexpr int lds_pitch(int N) { return N + 1; }
+
+// Compile-time list of supported lengths; host code dispatches on it.
+#define SIMR_FFT_SIZES(X)                                                                    \
+  X(2) X(3) X(4) X(5) X(6) X(8) X(9) X(10) X(12) X(15) X(16) X(18) X(20) X(24) X(25) X(27)   \
+  X(30) X(32) X(36) X(40) X(45) X(48) X(50) X(54) X(60) X(64) X(72) X(75) X(80) X(81)        \
+  X(90) X(96) X(100) X(120) X(128) X(144) X(150) X(160) X(180) X(192) X(200) X(240) X(256)   \
+  X(288) X(300) X(320) X(360) X(384) X(400) X(480) X(500) X(512) X(576) X(600) X(640)        \
+  X(720) X(768) X(800) X(960) X(1000) X(1024) X(1152) X(1200) X(1280) X(1440) X(1536)        \
+  X(1600) X(1920) X(2000) X(2048) X(2400) X(2560) X(3072) X(4096)
+
+}  // namespace simr

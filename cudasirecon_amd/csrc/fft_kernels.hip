@@ -1,0 +1,180 @@
+// Batched strided FFT kernels built on the LDS Stockham core (fft.hpp), and
+// the per-device twiddle tables.
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "fft.hpp"
+#include "fft_api.hpp"
+
+namespace simr {
+
+// ------------------------------------------------------------------ twiddles
+namespace {
+std::mutex g_tw_mu;
+std::map<std::pair<int, int>, float2*> g_tw;  // (device, N) -> table, kept for the process lifetime
+}  // namespace
+
+const float2* twiddles(int N) {
+  int dev = 0;
+  SIMR_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  auto key = std::make_pair(dev, N);
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) return it->second;
+  std::vector<float2> h(N > 0 ? N : 1);
+  const long double twopi = 6.283185307179586476925286766559L;
+  for (int m = 0; m < N; ++m) {
+    long double a = -twopi * (long double)m / (long double)N;
+    h[m] = make_float2((float)cosl(a), (float)sinl(a));
+  }
+  float2* d = nullptr;
+  SIMR_HIP(hipMalloc(&d, sizeof(float2) * h.size()));
+  SIMR_HIP(hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice));
+  g_tw[key] = d;
+  return d;
+}
+
+// W transforms per workgroup so that a workgroup has at most 1024 threads and
+// at most 80 KB of LDS (two workgroups per CU within gfx950's 160 KB).
+template <int N>
+constexpr int tile_w() {
+  constexpr int tpt = fft_tpt(N);
+  int w = 1024 / tpt;
+  if (w > 128) w = 128;
+  while (w > 1 && w * lds_pitch(N) * 8 > 80 * 1024) w /= 2;
+  return w;
+}
+
+// ------------------------------------------------------------------ kernels
+// Column transforms: element i of column c of plane p at
+//   data[p * plane_stride + i * pitch + c],  c < ncols.
+// A workgroup transforms W adjacent columns, so each of its global row
+// accesses is one contiguous W-element segment.
+template <int N, bool INV, int W>
+__global__ __launch_bounds__(W* fft_tpt(N)) void k_fft_cols(float2* __restrict__ data, long long pitch,
+                                                            long long ncols, long long plane_stride,
+                                                            const float2* __restrict__ tw) {
+  constexpr int TPT = fft_tpt(N);
+  constexpr int NT = W * TPT;
+  constexpr int P = lds_pitch(N);
+  __shared__ float2 lds[W * P];
+  const int tid = threadIdx.x;
+  const long long c0 = (long long)blockIdx.x * W;
+  float2* base = data + (long long)blockIdx.y * plane_stride;
+  for (int idx = tid; idx < N * W; idx += NT) {
+    const int i = idx / W, w = idx % W;
+    const long long c = c0 + w;
+    lds[w * P + i] = (c < ncols) ? base[i * pitch + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<N, INV>(lds + (tid / TPT) * P, tid % TPT, tw);
+  for (int idx = tid; idx < N * W; idx += NT) {
+    const int i = idx / W, w = idx % W;
+    const long long c = c0 + w;
+    if (c < ncols) base[i * pitch + c] = lds[w * P + i];
+  }
+}
+
+// Row transforms: row r at data[r * pitch], N contiguous elements, r < nrows.
+template <int N, bool INV, int W>
+__global__ __launch_bounds__(W* fft_tpt(N)) void k_fft_rows(float2* __restrict__ data, long long pitch,
+                                                            long long nrows, const float2* __restrict__ tw) {
+  constexpr int TPT = fft_tpt(N);
+  constexpr int NT = W * TPT;
+  constexpr int P = lds_pitch(N);
+  __shared__ float2 lds[W * P];
+  const int tid = threadIdx.x;
+  const long long r0 = (long long)blockIdx.x * W;
+  for (int idx = tid; idx < N * W; idx += NT) {
+    const int w = idx / N, i = idx % N;
+    const long long r = r0 + w;
+    lds[w * P + i] = (r < nrows) ? data[r * pitch + i] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<N, INV>(lds + (tid / TPT) * P, tid % TPT, tw);
+  for (int idx = tid; idx < N * W; idx += NT) {
+    const int w = idx / N, i = idx % N;
+    const long long r = r0 + w;
+    if (r < nrows) data[r * pitch + i] = lds[w * P + i];
+  }
+}
+
+template <int N>
+static void launch_cols(float2* data, long long pitch, long long ncols, long long nplanes,
+                        long long plane_stride, bool inverse, hipStream_t s) {
+  constexpr int W = tile_w<N>();
+  dim3 grid(ceil_div(ncols, W), (unsigned)nplanes);
+  dim3 block(W * fft_tpt(N));
+  if (inverse)
+    hipLaunchKernelGGL((k_fft_cols<N, true, W>), grid, block, 0, s, data, pitch, ncols, plane_stride, twiddles(N));
+  else
+    hipLaunchKernelGGL((k_fft_cols<N, false, W>), grid, block, 0, s, data, pitch, ncols, plane_stride, twiddles(N));
+  SIMR_LAUNCH_CHECK();
+}
+
+template <int N>
+static void launch_rows(float2* data, long long pitch, long long nrows, bool inverse, hipStream_t s) {
+  constexpr int W = tile_w<N>();
+  dim3 grid(ceil_div(nrows, W));
+  dim3 block(W * fft_tpt(N));
+  if (inverse)
+    hipLaunchKernelGGL((k_fft_rows<N, true, W>), grid, block, 0, s, data, pitch, nrows, twiddles(N));
+  else
+    hipLaunchKernelGGL((k_fft_rows<N, false, W>), grid, block, 0, s, data, pitch, nrows, twiddles(N));
+  SIMR_LAUNCH_CHECK();
+}
+
+bool fft_size_supported(int n) {
+  switch (n) {
+#define SIMR_CASE(N) \
+  case N:            \
+    return true;
+    SIMR_FFT_SIZES(SIMR_CASE)
+#undef SIMR_CASE
+    case 1:
+      return true;
+    default:
+      return false;
+  }
+}
+
+void fft_cols(int n, float2* data, long long pitch, long long ncols, long long nplanes, long long plane_stride,
+              bool inverse, hipStream_t s) {
+  if (n == 1 || ncols <= 0 || nplanes <= 0) return;
+  if (nplanes > 65535) throw ArgError("fft_cols: too many planes");
+  switch (n) {
+#define SIMR_CASE(N)                                                          \
+  case N:                                                                     \
+    launch_cols<N>(data, pitch, ncols, nplanes, plane_stride, inverse, s);    \
+    return;
+    SIMR_FFT_SIZES(SIMR_CASE)
+#undef SIMR_CASE
+    default:
+      throw ArgError("FFT length " + std::to_string(n) + " is not supported (must be 2^a 3^b 5^c, <= 4096)");
+  }
+}
+
+void fft_rows(int n, float2* data, long long pitch, long long nrows, bool inverse, hipStream_t s) {
+  if (n == 1 || nrows <= 0) return;
+  switch (n) {
+#define SIMR_CASE(N)                                     \
+  case N:                                                \
+    launch_rows<N>(data, pitch, nrows, inverse, s);      \
+    return;
+    SIMR_FFT_SIZES(SIMR_CASE)
+#undef SIMR_CASE
+    default:
+      throw ArgError("FFT length " + std::to_string(n) + " is not supported (must be 2^a 3^b 5^c, <= 4096)");
+  }
+}
+
+void fft3d(float2* data, int nx, int ny, int nz, bool inverse, hipStream_t s) {
+  const long long nxy = (long long)nx * ny;
+  if (nx > 1) fft_rows(nx, data, nx, (long long)ny * nz, inverse, s);
+  if (ny > 1) fft_cols(ny, data, nx, nx, nz, nxy, inverse, s);
+  if (nz > 1) fft_cols(nz, data, nxy, nxy, 1, 0, inverse, s);
+}
+
+}  // namespace simr

@@ -1,0 +1,118 @@
+// Kernel parameter blocks and host launchers of the reconstruction stages
+// (stages.hip).  Everything is passed by value; no __constant__ state, so any
+// number of reconstruction contexts can run concurrently on separate streams.
+#pragma once
+#include "common.hpp"
+
+namespace simr {
+
+constexpr int kMaxDirs = 8;
+constexpr int kMaxOrders = 3;
+constexpr int kMaxBands = 5;
+
+// Rotationally averaged OTF table: [kr][kz], kz fastest (nzotf entries per kr).
+struct OtfRA {
+  const float2* t;
+  int nzotf, nxotf;
+  float dkrotf;
+};
+
+// ---- forward path -------------------------------------------------------
+struct FwdParams {
+  int nx, ny, nz, nxh;
+  int ndirs, nphases, nbands;
+  int fastSI;
+  int napod;
+  float inscale;
+  float sep[kMaxBands * kMaxBands];  // separation matrix [band][phase]
+};
+
+// sums[sec] = sum over the section of the flat-fielded pixels (double).
+void launch_section_sums(const void* raw, bool u16, const float* bg, const float* slope, float inscale, int nx,
+                         int ny, int nimg, double* sums, hipStream_t s);
+// Bleach-correction ratios per section (reference rescale semantics).
+void launch_ratios(const double* sums, double* sdp, float* ratio, int ndirs, int nz, int nphases, int fastSI,
+                   int it, int equalizez, int equalizet, int do_rescale, hipStream_t s);
+// Fused flatfield + rescale + apodize + phase separation + R2C along x.
+// bands: [dir][band][z][y][nxh] complex.
+void launch_rows_fwd(const FwdParams& p, const void* raw, bool u16, const float* bg, const float* slope,
+                     const float* ratio, const float* apofact, float2* bands, hipStream_t s);
+
+// ---- k0 search --------------------------------------------------------------
+struct OvParams {
+  int nx, ny, nz, nxh;
+  int zd;  // z support half-width in planes
+  int order1, order2;
+  float kx, ky;  // (order2-order1) * k0, 1/um
+  float rdistcutoff, otfcutoff, order0_2_factor;
+  float dkx, dky, kzscale;
+  int bNoKz0;
+  OtfRA otf1, otf2;
+  const float2 *b1re, *b1im, *b2re, *b2im;  // b1im null for order1 == 0
+};
+// ovx: [2][2zd+1][ny][nx]; builds both overlaps and inverse-transforms along x and y.
+void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s);
+// Inverse z transform of the overlap pair and the per-(x,y) z sums
+// P = sum_z conj(X) Y, SX = sum |X|^2, SY = sum |Y|^2 (double storage;
+// accumulated in float when fp32 is set, as the reference's reductionKernel threads).
+void launch_overlaps_z_reduce(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, bool fp32,
+                              hipStream_t s);
+// crosscorr = conj(P) as float, then its forward 2D FFT and |.|^2 argmax.
+// peak_out (host-visible after sync): [0] = index of the first maximum,
+// values: int peak index and the 5 intensities (center, x-1, x+1, y-1, y+1).
+void launch_findpeak(const double2* P, float2* cc, float* partial_val, int* partial_idx, float* peak_vals,
+                     int* peak_idx, int nx, int ny, hipStream_t s);
+// Modamp reductions for a batch of candidate vectors.
+//   fp64: out[c] = sum P e^{i phi_c} over the image in double; partials [ncand][nparts] double2.
+//   fp32: reference reductionKernel on 64x4 blocks; partials [ncand][nblocks] of
+//         (xsy.x, xsy.y, sx, sy) float4, summed on the host in block order.
+int modamp_partials_fp64(int nx, int ny);
+int modamp_partials_fp32(int nx, int ny);
+void launch_modamp_fp64(const double2* P, const float2* kcand, int ncand, int nx, int ny, float dxy,
+                        double2* partials, hipStream_t s);
+void launch_modamp_fp32(const double2* P, const double* SX, const double* SY, const float2* kcand, int ncand,
+                        int nx, int ny, float dxy, float4* partials, hipStream_t s);
+// Totals of SX and SY in double: partials [2][nparts].
+int sum_partials_count(long long n);
+void launch_sum2(const double* a, const double* b, long long n, double* partials, hipStream_t s);
+
+// ---- Wiener filter ------------------------------------------------------------
+struct FilterParams {
+  int nx, ny, nz, nxh;
+  int dir, ndirs, order, norders;
+  int zd[kMaxOrders];
+  float rdistcutoff, apocutoff, zapocutoff, kzscale, wiener;
+  float dkx, dky, min_dkr, suppRadius;
+  int bSuppress, bDampenOrder0, bNoKz0, bFilteroverlaps, apodizeoutput;
+  float2 k0[kMaxDirs];
+  float ampmag2_all[kMaxDirs * kMaxOrders];
+  float noiseVar[kMaxDirs * kMaxOrders];
+  float2 conjamp;  // conj(amp[dir][order])
+  OtfRA otf[kMaxOrders];
+  float2 *bp, *bp2;
+};
+void launch_filter(const FilterParams& p, hipStream_t s);
+// Fill n float2 with zero.
+void launch_zero(float2* p, long long n, hipStream_t s);
+
+// ---- assembly ----------------------------------------------------------------
+struct AsmParams {
+  int nx, ny, nz, nxh;
+  int xdim, ydim, zdim;
+  int nasm;                 // assembly bands: 0 = order-0 sum over dirs, then (dir, order>=1)
+  int ndirs, norders;
+  const float2* bands;      // [dir][band][z][y][nxh]
+  long long band_stride;    // elements between consecutive bands (H)
+  int nbands;
+  float2* stageZ;           // [asm][zdim][ny][nx]
+  float2* stageY;           // [asm][zdim][ydim][nx]
+  float* out;               // [zdim][ydim][xdim]
+  float fact, dxyo;
+  float2 k0[kMaxDirs];
+};
+void launch_asm_z(const AsmParams& p, hipStream_t s);
+void launch_asm_y(const AsmParams& p, hipStream_t s);
+// bands [a0, a1) of stageY; accumulate=false overwrites out.
+void launch_asm_x(const AsmParams& p, int a0, int a1, bool accumulate, hipStream_t s);
+
+}  // namespace simr

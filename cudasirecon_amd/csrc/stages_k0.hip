@@ -1,0 +1,468 @@
+// k0 search: overlap construction and inverse FFT, the z-summed overlap
+// products, the cross-correlation peak, and the batched modamp reductions.
+//
+// Reference (src/cudaSirecon/gpuFunctionsImpl.cu): makeoverlaps 497-675 with
+// makeOverlaps0Kernel 677-778 / makeOverlaps1Kernel 780-872, aTimesConjB
+// 374-414, computeIntensities 416-441, findrealspacemodamp 1074-1148,
+// reductionKernel 1150-1220.
+//
+// Differences in structure (same values up to FP reassociation):
+//  * overlaps are generated inside the x-pass of the inverse FFT, only for
+//    the 2*zd+1 planes the support can reach; the zero planes are never
+//    written, and the z pass reads them as zeros;
+//  * the z pass ends in the per-(x,y) sums P = sum_z conj(X) Y, SX, SY, so the
+//    3D overlap volumes are never re-read: every later line-search step is a
+//    2D reduction of P against a phase ramp;
+//  * a whole batch of line-search candidates is reduced in one launch.
+#include "device_math.hpp"
+#include "fft.hpp"
+#include "fft_api.hpp"
+#include "sizes.hpp"
+#include "stages.hpp"
+
+namespace simr {
+
+// One element of overlap0 (which == 0) or overlap1 (which == 1) at column j,
+// row i, plane z0, before the inverse transform.
+__device__ float2 ov_value(const OvParams& p, int which, int j, int i, int z0) {
+#pragma clang fp contract(off)
+  const float2 zero = make_float2(0.f, 0.f);
+  float x1f = (float)j, y1f = (float)i;
+  if (x1f > (float)(p.nx / 2)) x1f -= (float)p.nx;
+  if (y1f > (float)(p.ny / 2)) y1f -= (float)p.ny;
+  x1f *= p.dkx;
+  y1f *= p.dky;
+  const float rc = p.rdistcutoff;
+  const float rdist1 = sqrtf(x1f * x1f + y1f * y1f);
+  if (which == 0 ? !(rdist1 <= rc) : !(rdist1 < rc)) return zero;
+  const float x12 = x1f - p.kx, y12 = y1f - p.ky;
+  const float rdist12 = sqrtf(x12 * x12 + y12 * y12);
+  const float x21 = x1f + p.kx, y21 = y1f + p.ky;
+  const float rdist21 = sqrtf(x21 * x21 + y21 * y21);
+  if (!(rdist12 <= rc || rdist21 <= rc)) return zero;
+  if (z0 == 0 && p.bNoKz0) return zero;
+  int iin, jin, conj;
+  if (j <= p.nx / 2) {
+    iin = i;
+    jin = j;
+    conj = 0;
+  } else {
+    jin = p.nx - j;
+    iin = (p.ny - i) % p.ny;
+    conj = 1;
+  }
+  int zin = conj ? -z0 : z0;
+  zin = (zin + p.nz) % p.nz;
+  const long long indin = (long long)zin * p.nxh * p.ny + (long long)iin * p.nxh + jin;
+  if (which == 0) {
+    if (!(rdist12 <= rc)) return zero;
+    const float2 otf1 = otf_interp(p.otf1, x1f, y1f, z0, p.kzscale);
+    if (!(sqrtf(otf1.x * otf1.x + otf1.y * otf1.y) > p.otfcutoff)) return zero;
+    const float2 otf12 = otf_interp(p.otf2, x12, y12, z0, p.kzscale);
+    if (!(sqrtf(otf12.x * otf12.x + otf12.y * otf12.y) * p.order0_2_factor > p.otfcutoff)) return zero;
+    float2 vre = p.b1re[indin];
+    float2 vim = p.order1 > 0 ? p.b1im[indin] : zero;
+    const float root = sqrtf(otf1.x * otf1.x + otf1.y * otf1.y + otf12.x * otf12.x + otf12.y * otf12.y);
+    const float2 fact = make_float2(otf12.x / root, otf12.y / root);
+    if (conj) {
+      vre.y = -vre.y;
+      if (p.order1 > 0) vim.y = -vim.y;
+    }
+    vre = cmul_nc(vre, fact);
+    if (p.order1 > 0) vim = cmul_nc(vim, fact);
+    return make_float2(vre.x - vim.y, vre.y + vim.x);
+  } else {
+    if (!(rdist21 <= rc)) return zero;
+    const float2 otf2 = otf_interp(p.otf2, x1f, y1f, z0, p.kzscale);
+    if (!(sqrtf(otf2.x * otf2.x + otf2.y * otf2.y) * p.order0_2_factor > p.otfcutoff)) return zero;
+    const float2 otf21 = otf_interp(p.otf1, x21, y21, z0, p.kzscale);
+    if (!(sqrtf(otf21.x * otf21.x + otf21.y * otf21.y) > p.otfcutoff)) return zero;
+    float2 vre = p.b2re[indin];
+    float2 vim = p.b2im[indin];
+    const float root = sqrtf(otf2.x * otf2.x + otf2.y * otf2.y + otf21.x * otf21.x + otf21.y * otf21.y);
+    const float2 fact = make_float2(otf21.x / root, otf21.y / root);
+    if (conj) {
+      vre.y = -vre.y;
+      vim.y = -vim.y;
+    }
+    vre = cmul_nc(vre, fact);
+    vim = cmul_nc(vim, fact);
+    return make_float2(vre.x - vim.y, vre.y + vim.x);
+  }
+}
+
+// x pass: row i (blockIdx.x) of plane zb (blockIdx.y) of both overlaps.
+template <int N>
+__global__ __launch_bounds__(2 * fft_tpt(N)) void k_overlaps_x(OvParams p, float2* __restrict__ ovx,
+                                                               const float2* __restrict__ tw) {
+  constexpr int TPT = fft_tpt(N);
+  constexpr int NT = 2 * TPT;
+  constexpr int P = lds_pitch(N);
+  __shared__ float2 lds[2 * P];
+  const int i = blockIdx.x, zb = blockIdx.y;
+  const int z0 = zb - p.zd;
+  const int nzs = 2 * p.zd + 1;
+  for (int idx = threadIdx.x; idx < 2 * N; idx += NT) {
+    const int which = idx / N, j = idx - (idx / N) * N;
+    lds[which * P + j] = ov_value(p, which, j, i, z0);
+  }
+  __syncthreads();
+  fft_lds<N, true>(lds + (threadIdx.x / TPT) * P, threadIdx.x % TPT, tw);
+  for (int idx = threadIdx.x; idx < 2 * N; idx += NT) {
+    const int which = idx / N, j = idx - (idx / N) * N;
+    ovx[(((long long)which * nzs + zb) * p.ny + i) * p.nx + j] = lds[which * P + j];
+  }
+}
+
+template <int N>
+static void launch_ov_x(const OvParams& p, float2* ovx, hipStream_t s) {
+  hipLaunchKernelGGL(k_overlaps_x<N>, dim3(p.ny, 2 * p.zd + 1), dim3(2 * fft_tpt(N)), 0, s, p, ovx, twiddles(N));
+  SIMR_LAUNCH_CHECK();
+}
+
+void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s) {
+  switch (p.nx) {
+#define SIMR_CASE(N)               \
+  case N:                          \
+    launch_ov_x<N>(p, ovx, s);     \
+    break;
+    SIMR_XY_SIZES(SIMR_CASE)
+#undef SIMR_CASE
+    default:
+      throw ArgError("image width " + std::to_string(p.nx) + " is not a supported size");
+  }
+  const long long nxy = (long long)p.nx * p.ny;
+  fft_cols(p.ny, ovx, p.nx, p.nx, 2LL * (2 * p.zd + 1), nxy, true, s);
+}
+
+// z pass + per-(x,y) z sums.  Wc adjacent columns e of both overlaps per workgroup.
+template <int N>
+constexpr int zcols() {
+  constexpr int tpt = fft_tpt(N);
+  int w = 512 / tpt;
+  if (w > 64) w = 64;
+  if (w < 1) w = 1;
+  while (w > 1 && 2 * w * lds_pitch(N) * 8 > 80 * 1024) w /= 2;
+  return w;
+}
+
+template <int N, bool FP32>
+__global__ __launch_bounds__(2 * zcols<N>() * fft_tpt(N)) void k_overlaps_z(OvParams p, const float2* __restrict__ ovx,
+                                                                           double2* __restrict__ P,
+                                                                           double* __restrict__ SX,
+                                                                           double* __restrict__ SY,
+                                                                           const float2* __restrict__ tw) {
+#pragma clang fp contract(off)
+  constexpr int WC = zcols<N>();
+  constexpr int TPT = fft_tpt(N);
+  constexpr int NT = 2 * WC * TPT;
+  constexpr int LP = lds_pitch(N);
+  __shared__ float2 lds[2 * WC * LP];
+  const long long nxy = (long long)p.nx * p.ny;
+  const long long e0 = (long long)blockIdx.x * WC;
+  const int nzs = 2 * p.zd + 1;
+  for (int idx = threadIdx.x; idx < 2 * WC * N; idx += NT) {
+    const int which = idx / (WC * N);
+    const int rem = idx - which * WC * N;
+    const int z = rem / WC, w = rem - (rem / WC) * WC;
+    const long long e = e0 + w;
+    int z0 = 0;
+    bool valid = e < nxy;
+    if (z <= p.zd)
+      z0 = z;
+    else if (z >= N - p.zd)
+      z0 = z - N;
+    else
+      valid = false;
+    lds[(which * WC + w) * LP + z] =
+        valid ? ovx[((long long)which * nzs + (z0 + p.zd)) * nxy + e] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, tw);
+  for (int w = threadIdx.x; w < WC; w += NT) {
+    const long long e = e0 + w;
+    if (e >= nxy) continue;
+    const float2* X = lds + w * LP;
+    const float2* Y = lds + (WC + w) * LP;
+    if (FP32) {
+      float sxr = 0.f, sxi = 0.f, sx = 0.f, sy = 0.f;
+      for (int z = 0; z < N; ++z) {
+        const float2 a = X[z], b = Y[z];
+        sxr += a.x * b.x + a.y * b.y;
+        sxi += a.x * b.y - a.y * b.x;
+        sx += a.x * a.x + a.y * a.y;
+        sy += b.x * b.x + b.y * b.y;
+      }
+      P[e] = make_double2(sxr, sxi);
+      SX[e] = sx;
+      SY[e] = sy;
+    } else {
+      double sxr = 0.0, sxi = 0.0, sx = 0.0, sy = 0.0;
+      for (int z = 0; z < N; ++z) {
+        const double ar = X[z].x, ai = X[z].y, br = Y[z].x, bi = Y[z].y;
+        sxr += ar * br + ai * bi;
+        sxi += ar * bi - ai * br;
+        sx += ar * ar + ai * ai;
+        sy += br * br + bi * bi;
+      }
+      P[e] = make_double2(sxr, sxi);
+      SX[e] = sx;
+      SY[e] = sy;
+    }
+  }
+}
+
+template <int N>
+static void launch_ov_z(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, bool fp32,
+                        hipStream_t s) {
+  constexpr int WC = zcols<N>();
+  const long long nxy = (long long)p.nx * p.ny;
+  dim3 grid(ceil_div(nxy, WC)), block(2 * WC * fft_tpt(N));
+  if (fp32)
+    hipLaunchKernelGGL((k_overlaps_z<N, true>), grid, block, 0, s, p, ovx, P, SX, SY, twiddles(N));
+  else
+    hipLaunchKernelGGL((k_overlaps_z<N, false>), grid, block, 0, s, p, ovx, P, SX, SY, twiddles(N));
+  SIMR_LAUNCH_CHECK();
+}
+
+void launch_overlaps_z_reduce(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, bool fp32,
+                              hipStream_t s) {
+  if (2 * p.zd + 1 > p.nz) throw ArgError("overlap z support wider than the stack");
+  switch (p.nz) {
+#define SIMR_CASE(N)                                  \
+  case N:                                             \
+    launch_ov_z<N>(p, ovx, P, SX, SY, fp32, s);       \
+    return;
+    SIMR_Z_SIZES(SIMR_CASE)
+#undef SIMR_CASE
+    default:
+      throw ArgError("stack depth " + std::to_string(p.nz) + " is not a supported size");
+  }
+}
+
+// ----------------------------------------------------------------- findk0 peak
+__global__ void k_cc_prep(const double2* __restrict__ P, float2* __restrict__ cc, long long n) {
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) {
+    const double2 v = P[e];
+    cc[e] = make_float2((float)v.x, -(float)v.y);  // aTimesConjB: X conj(Y) = conj(conj(X) Y)
+  }
+}
+
+__device__ __forceinline__ void better(float& v, int& i, float v2, int i2) {
+  if (v2 > v || (v2 == v && i2 < i)) {
+    v = v2;
+    i = i2;
+  }
+}
+
+__device__ __forceinline__ float intensity(float2 a) {
+#pragma clang fp contract(off)
+  return a.x * a.x + a.y * a.y;
+}
+
+constexpr int kArgBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_argmax_partial(const float2* __restrict__ cc, long long n,
+                                                        float* __restrict__ pv, int* __restrict__ pi) {
+  __shared__ float sv[256];
+  __shared__ int si[256];
+  const long long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long long b0 = blockIdx.x * chunk;
+  const long long b1 = min(n, b0 + chunk);
+  float v = -1e30f;
+  int idx = 0x7fffffff;
+  for (long long e = b0 + threadIdx.x; e < b1; e += 256) better(v, idx, intensity(cc[e]), (int)e);
+  sv[threadIdx.x] = v;
+  si[threadIdx.x] = idx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      float a = sv[threadIdx.x];
+      int ai = si[threadIdx.x];
+      better(a, ai, sv[threadIdx.x + s], si[threadIdx.x + s]);
+      sv[threadIdx.x] = a;
+      si[threadIdx.x] = ai;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = sv[0];
+    pi[blockIdx.x] = si[0];
+  }
+}
+
+__global__ void k_argmax_final(const float2* __restrict__ cc, const float* __restrict__ pv,
+                               const int* __restrict__ pi, int nparts, int nx, int ny, float* __restrict__ vals,
+                               int* __restrict__ idx_out) {
+  if (threadIdx.x != 0) return;
+  float v = -1e30f;
+  int idx = 0x7fffffff;
+  for (int b = 0; b < nparts; ++b) better(v, idx, pv[b], pi[b]);
+  const int xc = idx % nx, yc = idx / nx;
+  auto I = [&](int y, int x) { return intensity(cc[(long long)y * nx + x]); };
+  vals[0] = I(yc, xc);
+  vals[1] = I(yc, xc == 0 ? nx - 1 : xc - 1);
+  vals[2] = I(yc, xc == nx - 1 ? 0 : xc + 1);
+  vals[3] = I(yc == 0 ? ny - 1 : yc - 1, xc);
+  vals[4] = I(yc == ny - 1 ? 0 : yc + 1, xc);
+  idx_out[0] = idx;
+}
+
+void launch_findpeak(const double2* P, float2* cc, float* partial_val, int* partial_idx, float* peak_vals,
+                     int* peak_idx, int nx, int ny, hipStream_t s) {
+  const long long n = (long long)nx * ny;
+  hipLaunchKernelGGL(k_cc_prep, dim3(ceil_div(n, 256)), dim3(256), 0, s, P, cc, n);
+  SIMR_LAUNCH_CHECK();
+  fft_rows(nx, cc, nx, ny, false, s);
+  fft_cols(ny, cc, nx, nx, 1, 0, false, s);
+  hipLaunchKernelGGL(k_argmax_partial, dim3(kArgBlocks), dim3(256), 0, s, cc, n, partial_val, partial_idx);
+  SIMR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_argmax_final, dim3(1), dim3(64), 0, s, cc, partial_val, partial_idx, kArgBlocks, nx, ny,
+                     peak_vals, peak_idx);
+  SIMR_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ modamp, FP64
+// e^{i phi} = e^{i a(x)} e^{i b(y)} with a = 2 pi dxy (x - nx/2) kx and
+// b = 2 pi dxy (y - ny/2) ky: one table of nx phasors per candidate.
+constexpr int kModRows = 4;
+
+int modamp_partials_fp64(int nx, int ny) { return (ny + kModRows - 1) / kModRows; }
+
+__global__ __launch_bounds__(256) void k_modamp_fp64(const double2* __restrict__ P, const float2* __restrict__ kc,
+                                                     int nx, int ny, float dxy, double2* __restrict__ partials) {
+  extern __shared__ double2 alpha[];  // nx phasors
+  __shared__ double2 beta[kModRows];
+  __shared__ double rx[256], ry[256];
+  const int c = blockIdx.y;
+  const double kx = (double)kc[c].x, ky = (double)kc[c].y;
+  const double twopi_dxy = 2.0 * 3.14159265358979323846 * (double)dxy;
+  for (int i = threadIdx.x; i < nx; i += 256) {
+    double sn, cs;
+    sincos(twopi_dxy * (((double)i - 0.5 * nx) * kx), &sn, &cs);
+    alpha[i] = make_double2(cs, sn);
+  }
+  const int j0 = blockIdx.x * kModRows;
+  if (threadIdx.x < kModRows) {
+    double sn, cs;
+    sincos(twopi_dxy * (((double)(j0 + threadIdx.x) - 0.5 * ny) * ky), &sn, &cs);
+    beta[threadIdx.x] = make_double2(cs, sn);
+  }
+  __syncthreads();
+  double ar = 0.0, ai = 0.0;
+  const int nrows = min(kModRows, ny - j0);
+  for (int idx = threadIdx.x; idx < nrows * nx; idx += 256) {
+    const int jj = idx / nx, i = idx - jj * nx;
+    const double2 pv = P[(long long)(j0 + jj) * nx + i];
+    const double2 a = alpha[i], b = beta[jj];
+    const double er = a.x * b.x - a.y * b.y, ei = a.x * b.y + a.y * b.x;
+    ar += pv.x * er - pv.y * ei;
+    ai += pv.x * ei + pv.y * er;
+  }
+  rx[threadIdx.x] = ar;
+  ry[threadIdx.x] = ai;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      rx[threadIdx.x] += rx[threadIdx.x + s];
+      ry[threadIdx.x] += ry[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[(long long)c * gridDim.x + blockIdx.x] = make_double2(rx[0], ry[0]);
+}
+
+void launch_modamp_fp64(const double2* P, const float2* kcand, int ncand, int nx, int ny, float dxy,
+                        double2* partials, hipStream_t s) {
+  dim3 grid(modamp_partials_fp64(nx, ny), ncand);
+  hipLaunchKernelGGL(k_modamp_fp64, grid, dim3(256), sizeof(double2) * nx, s, P, kcand, nx, ny, dxy, partials);
+  SIMR_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ modamp, FP32
+// reductionKernel on 64x4 blocks, float throughout, the thread z-sums being
+// the float P/SX/SY of the z pass; block partials are summed on the host in
+// block-index order as cpuReduce does.
+int modamp_partials_fp32(int nx, int ny) { return ((nx + 63) / 64) * ((ny + 3) / 4); }
+
+__global__ __launch_bounds__(256) void k_modamp_fp32(const double2* __restrict__ P, const double* __restrict__ SX,
+                                                     const double* __restrict__ SY, const float2* __restrict__ kc,
+                                                     int nx, int ny, float dxy, float4* __restrict__ partials) {
+#pragma clang fp contract(off)
+  __shared__ float bx[256], by[256], bs[256], bt[256];
+  const int c = blockIdx.z;
+  const float k0x = kc[c].x, k0y = kc[c].y;
+  const int i = threadIdx.x + blockIdx.x * 64;
+  const int j = threadIdx.y + blockIdx.y * 4;
+  const float a = ((float)i - 0.5f * (float)nx) * k0x;
+  const float b = ((float)j - 0.5f * (float)ny) * k0y;
+  const float angle = (float)(2.0f * 3.14159265358979323846 * (double)(a + b) * (double)dxy);
+  float sn, cs;
+  sincosf(angle, &sn, &cs);
+  float xr = 0.f, xi = 0.f, sx = 0.f, sy = 0.f;
+  if (i < nx && j < ny) {
+    const long long e = (long long)j * nx + i;
+    xr = (float)P[e].x;
+    xi = (float)P[e].y;
+    sx = (float)SX[e];
+    sy = (float)SY[e];
+  }
+  const float tr = xr * cs - xi * sn;
+  const float ti = xr * sn + xi * cs;
+  const int lt = threadIdx.y * 64 + threadIdx.x;
+  bx[lt] = tr;
+  by[lt] = ti;
+  bs[lt] = sx;
+  bt[lt] = sy;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (lt < s) {
+      bx[lt] += bx[lt + s];
+      by[lt] += by[lt + s];
+      bs[lt] += bs[lt + s];
+      bt[lt] += bt[lt + s];
+    }
+    __syncthreads();
+  }
+  if (lt == 0) {
+    const long long blk = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    partials[(long long)c * gridDim.x * gridDim.y + blk] = make_float4(bx[0], by[0], bs[0], bt[0]);
+  }
+}
+
+void launch_modamp_fp32(const double2* P, const double* SX, const double* SY, const float2* kcand, int ncand,
+                        int nx, int ny, float dxy, float4* partials, hipStream_t s) {
+  dim3 grid((nx + 63) / 64, (ny + 3) / 4, ncand);
+  hipLaunchKernelGGL(k_modamp_fp32, grid, dim3(64, 4), 0, s, P, SX, SY, kcand, nx, ny, dxy, partials);
+  SIMR_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------ sums of SX, SY
+constexpr int kSumBlocks = 128;
+int sum_partials_count(long long) { return kSumBlocks; }
+
+__global__ __launch_bounds__(256) void k_sum2(const double* __restrict__ a, const double* __restrict__ b, long long n,
+                                              double* __restrict__ partials) {
+  __shared__ double red[256];
+  const long long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long long b0 = blockIdx.x * chunk, b1 = min(n, b0 + chunk);
+  double sa = 0.0, sb = 0.0;
+  for (long long e = b0 + threadIdx.x; e < b1; e += 256) {
+    sa += a[e];
+    sb += b[e];
+  }
+  double ta = block_sum_d<256>(sa, red);
+  double tb = block_sum_d<256>(sb, red);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = ta;
+    partials[gridDim.x + blockIdx.x] = tb;
+  }
+}
+
+void launch_sum2(const double* a, const double* b, long long n, double* partials, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum2, dim3(kSumBlocks), dim3(256), 0, s, a, b, n, partials);
+  SIMR_LAUNCH_CHECK();
+}
+
+}  // namespace simr

@@ -1,0 +1,179 @@
+/*
+ * simr.h - stage-level C ABI of the MI355X 3D-SIM reconstruction engine.
+ *
+ * This is the seam between host orchestration and device code.  It replaces
+ * the reference's internal stage API (src/cudaSirecon/gpuFunctions.h:19-80 and
+ * the two cuFFT call sites in src/cudaSirecon/cudaSirecon.cpp:238-257,542-579),
+ * which took std::vector<GPUBuffer>* plus a global ReconParams and kept state in
+ * __constant__ symbols.  Here every call takes an explicit context that owns its
+ * device arena, HIP stream and FFT plans; no process-global state is used, so
+ * any number of contexts can live in one process.
+ *
+ * All functions return 0 (SIMR_OK) on success and a negative simr_status on
+ * failure; simr_last_error() returns a thread-local message.  Nothing throws
+ * across this boundary.
+ */
+#ifndef SIMR_H
+#define SIMR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIMR_MAX_DIRS 8
+#define SIMR_MAX_ORDERS 3 /* norders <= 3, i.e. nphases <= 5, as in the reference GPU path */
+#define SIMR_MAX_PHASES 5
+
+typedef enum simr_status {
+  SIMR_OK = 0,
+  SIMR_ERR_ARG = -1,       /* invalid argument / unsupported parameter combination */
+  SIMR_ERR_HIP = -2,       /* HIP runtime error */
+  SIMR_ERR_NOMEM = -3,     /* device arena exhausted / allocation failure */
+  SIMR_ERR_IO = -4,        /* file could not be read or written */
+  SIMR_ERR_FORMAT = -5,    /* malformed MRC/TIFF/config content */
+  SIMR_ERR_STATE = -6,     /* call out of order (e.g. result requested before processing) */
+  SIMR_ERR_UNSUPPORTED = -7
+} simr_status;
+
+/*
+ * Reconstruction parameters.  Field names and meanings follow the reference's
+ * ReconParams / ImageParams (src/cudaSirecon/cudaSireconImpl.h:104-195).
+ * simr_params_default() fills the values the cudasirecon CLI uses when an
+ * option is not given (src/cudaSirecon/cudaSirecon.cpp:16-72 overlaid by the
+ * option defaults at 918-1005).
+ */
+typedef struct simr_params {
+  /* image geometry */
+  int32_t nx, ny, nz;          /* nz = z sections per (direction, phase) */
+  int32_t ndirs, nphases, norders_output;
+  float dxy, dz;               /* pixel sizes in microns */
+  int32_t wave;                /* emission wavelength in nm */
+  int32_t ntimes;              /* number of time points in the series */
+  /* illumination pattern */
+  float k0startangle, linespacing, na, nimm;
+  int32_t nk0angles;
+  float k0angles[SIMR_MAX_DIRS];
+  /* algorithm */
+  float otfcutoff, zoomfact;
+  int32_t z_zoom;
+  float explodefact;
+  int32_t bFilteroverlaps, recalcarrays, napodize, bSearchforvector, bUseTime0k0;
+  int32_t apodizeoutput;
+  float apoGamma;
+  int32_t bSuppress_singularities, suppression_radius, bDampenOrder0, bFitallphases;
+  int32_t do_rescale, equalizez, equalizet, bNoKz0;
+  float wiener;
+  int32_t nforceamp;
+  float forceamp[SIMR_MAX_ORDERS];
+  /* OTF geometry (rotationally averaged: nyotf == 1, table laid out [kr][kz]) */
+  int32_t bRadAvgOTF, bOneOTFperAngle;
+  int32_t nxotf, nyotf, nzotf;
+  float dkrotf, dkzotf;
+  /* camera */
+  float constbkgd;
+  int32_t bFastSIM;
+  /* engine options (no counterpart in the reference) */
+  int32_t fit_fp64;            /* 1: k0/modamp fit reductions in FP64 (default); 0: FP32 partials */
+  int32_t reserved[15];
+} simr_params;
+
+/* Per-direction fit results, the observable outputs the reference prints. */
+typedef struct simr_fit {
+  float k0x, k0y;              /* modulation vector of order 1, in 1/micron */
+  float k0guess_x, k0guess_y;  /* initial guess, 1/micron */
+  float amp_re[SIMR_MAX_ORDERS], amp_im[SIMR_MAX_ORDERS];
+  float corr[SIMR_MAX_ORDERS]; /* correlation coefficient of each order's modamp fit */
+  int32_t n_angle_evals, n_mag_evals, n_reductions;
+} simr_fit;
+
+typedef struct simr_ctx simr_ctx;
+
+void simr_params_default(simr_params* p);
+const char* simr_last_error(void);
+void simr_set_last_error(const char* msg);
+const char* simr_version(void);
+/* sizeof(simr_params) and sizeof(simr_fit), for binding-layout checks. */
+size_t simr_params_size(void);
+size_t simr_fit_size(void);
+
+/* Number of complex OTF entries per (direction, order) for these parameters. */
+size_t simr_otf_size(const simr_params* p);
+/* Device bytes one context needs for these parameters (arena footprint). */
+size_t simr_footprint(const simr_params* p);
+
+/*
+ * Create a reconstruction context.  otf: interleaved complex float, laid out
+ * [nDirsOTF][norders][simr_otf_size()] with nDirsOTF = bOneOTFperAngle ? ndirs : 1.
+ * device: HIP device ordinal.  The context owns a stream and a device arena.
+ */
+int simr_create(const simr_params* p, const float* otf, int device, simr_ctx** out);
+int simr_destroy(simr_ctx* ctx);
+int simr_get_params(const simr_ctx* ctx, simr_params* out);
+
+/*
+ * Raw input: nImages = nz * ndirs * nphases sections of nx*ny pixels, in the
+ * reference's section order (Angle->Z->Phase, or Z->Angle->Phase with bFastSIM).
+ * The *_host variants copy from host memory; the *_dev variants take a device
+ * pointer (already resident in HBM) and copy device-to-device on the context stream.
+ */
+int simr_set_raw_f32_host(simr_ctx* ctx, const float* raw);
+int simr_set_raw_u16_host(simr_ctx* ctx, const uint16_t* raw);
+int simr_set_raw_f32_dev(simr_ctx* ctx, const float* raw_dev);
+int simr_set_raw_u16_dev(simr_ctx* ctx, const uint16_t* raw_dev);
+/* Optional flat-field maps (background, slope), nx*ny floats each; NULL resets to constbkgd/1. */
+int simr_set_flatfield(simr_ctx* ctx, const float* background, const float* slope);
+
+/* Stage calls (in this order per time point). */
+int simr_load_and_rescale(simr_ctx* ctx, int it);   /* flatfield + bleach correction + apodize + XY FFT + separation + z FFT */
+int simr_set_time_index(simr_ctx* ctx, int it);
+int simr_find_modulation(simr_ctx* ctx);             /* k0 search / fit and modamps for all directions */
+int simr_filter_assemble(simr_ctx* ctx);             /* Wiener filtering + zoomed assembly into the output volume */
+int simr_process_volume(simr_ctx* ctx);              /* find_modulation + filter_assemble */
+
+/* Results. out: (zoom*nx)*(zoom*ny)*(z_zoom*nz) floats, layout [z][y][x]. */
+size_t simr_result_elems(const simr_ctx* ctx);
+int simr_get_result_host(simr_ctx* ctx, float* out);
+const float* simr_result_dev(simr_ctx* ctx);
+int simr_get_fit(const simr_ctx* ctx, int dir, simr_fit* out);
+/* Override k0 (1/micron) and modamps of one direction (known-k0 processing, parity checks). */
+int simr_set_fit(simr_ctx* ctx, int dir, const simr_fit* in);
+
+/* Stage inspection (parity tests): copy one separated band of one direction
+ * (half-complex spectrum [z][y][nx/2+1], complex float) to the host. */
+int simr_get_band_host(simr_ctx* ctx, int dir, int band, float* out);
+int simr_set_band_host(simr_ctx* ctx, int dir, int band, const float* in);
+/* Run only the Wiener filter of one direction, in place on its bands. */
+int simr_filter_dir(simr_ctx* ctx, int dir);
+/* Build the overlap pair of (order1, order2) at k0 (1/micron) from the current
+ * bands and return its z-summed products: P = sum_z conj(X) Y (complex double
+ * [ny][nx]), SX = sum_z |X|^2, SY = sum_z |Y|^2 (double [ny][nx]). */
+int simr_debug_overlaps(simr_ctx* ctx, int dir, int order1, int order2, float k0x, float k0y, double* P,
+                        double* sx, double* sy);
+
+/*
+ * Batched complex FFT utility (the engine's own LDS Stockham kernels; replaces
+ * the cufftPlan3d/cufftExecC2C pairs at src/cudaSirecon/gpuFunctionsImpl.cu:639-669).
+ * data: interleaved complex float [nz][ny][nx], unnormalised, forward = exp(-i).
+ * _dev runs in place on device memory on `stream` (NULL = default stream);
+ * _host copies to the current device, transforms and copies back.
+ */
+int simr_fft3d_dev(float* data, int nx, int ny, int nz, int inverse, void* stream);
+int simr_fft3d_host(float* data, int nx, int ny, int nz, int inverse);
+int simr_fft_size_supported(int n);
+
+/* Synchronise the context stream.  Returns the HIP stream as void*. */
+int simr_sync(simr_ctx* ctx);
+void* simr_stream(simr_ctx* ctx);
+
+/* Per-stage device timings of the last processed volume, from HIP events on
+ * the context stream.  names/ms arrays of capacity cap; returns count. */
+int simr_stage_timings(simr_ctx* ctx, const char** names, float* ms, int cap);
+int simr_enable_timing(simr_ctx* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIMR_H */

@@ -1,0 +1,188 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same inputs.
+
+Stage by stage (same bands / same k0 and modamps in, compare out) and end to end.
+Tolerances: FFT rel-L2 2e-6; stages rel-L2 1e-5; end-to-end output rel-L2 1e-5
+against the FP32 oracle, k0 within 1e-3 px and modamp within 1e-3.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from cases import load_testdata, rel_l2, synthetic
+from cudasirecon_amd import engine
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+FFT_SHAPES = [(64,), (48,), (9,), (512,), (1024,), (2048,), (384,), (640,), (1280,), (4096,),
+              (16, 8), (64, 64), (96, 128), (10, 96, 128), (9, 64, 64), (48, 32, 16)]
+
+
+@pytest.mark.parametrize("shape", FFT_SHAPES)
+def test_fft_matches_numpy(shape):
+    log("fft", shape)
+    rng = np.random.default_rng(len(shape) * 1000 + shape[-1])
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    for inv in (False, True):
+        got = engine.fft3d(x, inverse=inv)
+        ref = np.fft.ifftn(x.astype(np.complex128)) * x.size if inv else np.fft.fftn(x.astype(np.complex128))
+        assert rel_l2(got, ref) < 2e-6, (shape, inv, rel_l2(got, ref))
+
+
+def _gpu(p, otf, fit64=True):
+    q = p.copy()
+    q.fit_fp64 = 1 if fit64 else 0
+    return engine.Reconstructor(q, otf)
+
+
+def _oracle_preprocessed(p, otf, raw, prec=32, fit64=False):
+    o = ob.Oracle(p, otf, prec, fit64)
+    o.load(raw)
+    o.rescale(0)
+    o.set_time(0)
+    o.preprocess_bands()
+    return o
+
+
+def _cases():
+    return ["testdata", "syn3d", "syn2d"]
+
+
+def _case(name):
+    if name == "testdata":
+        p, otf, raw = load_testdata()
+        return p, otf, raw
+    if name == "syn3d":
+        p, otf, raw, _ = synthetic(128, 128, 16, nphases=5)
+        return p, otf, raw
+    p, otf, raw, _ = synthetic(128, 128, 1, nphases=3)
+    return p, otf, raw
+
+
+@pytest.mark.parametrize("case", _cases())
+def test_forward_bands_match_oracle(case):
+    log("bands", case)
+    p, otf, raw = _case(case)
+    o = _oracle_preprocessed(p, otf, raw.astype(np.float32))
+    with _gpu(p, otf) as g:
+        g.set_raw(raw)
+        g.load_and_rescale(0)
+        for d in range(p.ndirs):
+            for b in range(p.nphases):
+                e = rel_l2(g.band(d, b), o.band(d, b))
+                assert e < 1e-5, (case, d, b, e)
+
+
+@pytest.mark.parametrize("case", _cases())
+def test_overlap_products_match_oracle(case):
+    log("overlaps", case)
+    p, otf, raw = _case(case)
+    o = _oracle_preprocessed(p, otf, raw.astype(np.float32))
+    fo2 = 2 if p.nz > 1 else 1
+    with _gpu(p, otf) as g:
+        for d in range(p.ndirs):
+            for b in range(p.nphases):
+                g.set_band(d, b, o.band(d, b))
+        f = o.fit(0)
+        for (o1, o2) in [(0, fo2), (0, 1)]:
+            ref = o.makeoverlaps(0, o1, o2, f.k0guess_x, f.k0guess_y)
+            got = g.debug_overlaps(0, o1, o2, f.k0guess_x, f.k0guess_y)
+            for key in ("P", "sx", "sy"):
+                e = rel_l2(got[key], ref[key])
+                assert e < 1e-5, (case, o1, o2, key, e)
+
+
+def _fitted_oracle(p, otf, raw):
+    o = _oracle_preprocessed(p, otf, raw.astype(np.float32), 32, True)
+    for d in range(p.ndirs):
+        o.modulation(d)
+    return o
+
+
+@pytest.mark.parametrize("case", _cases())
+def test_filter_matches_oracle(case):
+    log("filter", case)
+    p, otf, raw = _case(case)
+    o = _fitted_oracle(p, otf, raw)
+    with _gpu(p, otf) as g:
+        for d in range(p.ndirs):
+            for b in range(p.nphases):
+                g.set_band(d, b, o.band(d, b))
+            g.set_fit(d, o.fit(d))
+        for d in range(p.ndirs):
+            o.filter_dir(d)
+            g.filter_dir(d)
+            for b in range(p.nphases):
+                e = rel_l2(g.band(d, b), o.band(d, b))
+                assert e < 1e-5, (case, d, b, e)
+
+
+@pytest.mark.parametrize("case", _cases())
+def test_assembly_matches_oracle(case):
+    log("assembly", case)
+    p, otf, raw = _case(case)
+    o = _fitted_oracle(p, otf, raw)
+    with _gpu(p, otf) as g:
+        for d in range(p.ndirs):
+            for b in range(p.nphases):
+                g.set_band(d, b, o.band(d, b))
+            g.set_fit(d, o.fit(d))
+        o.filter_assemble()
+        g.filter_assemble()
+        e = rel_l2(g.result(), o.result())
+        assert e < 1e-5, (case, e)
+
+
+def _fit_errors(p, fa, fb):
+    dk = 1.0 / (p.nx * p.dxy)
+    norders = p.nphases // 2 + 1
+    k0e = max(math.hypot(a.k0x - b.k0x, a.k0y - b.k0y) / dk for a, b in zip(fa, fb))
+    ame = max(abs(complex(a.amp_re[o], a.amp_im[o]) - complex(b.amp_re[o], b.amp_im[o]))
+              for a, b in zip(fa, fb) for o in range(norders))
+    return k0e, ame
+
+
+@pytest.mark.parametrize("case", _cases())
+def test_end_to_end_matches_oracle(case):
+    log("end-to-end", case)
+    p, otf, raw = _case(case)
+    raw32 = raw.astype(np.float32)
+    # FP64 fit on the GPU vs the oracle carrying the same precision plan
+    ref, fref = ob.reconstruct(p, otf, raw32, 32, True)
+    with _gpu(p, otf, fit64=True) as g:
+        g.set_raw(raw)
+        g.load_and_rescale(0)
+        g.set_time(0)
+        g.process_volume()
+        vol = g.result()
+        fits = [g.fit(d) for d in range(p.ndirs)]
+    k0e, ame = _fit_errors(p, fits, fref)
+    e = rel_l2(vol, ref)
+    ref64, f64 = ob.reconstruct(p, otf, raw32, 64)
+    e64 = rel_l2(vol, ref64)
+    log("  fp64-fit: k0 err %.2e px, modamp err %.2e, rel-L2 vs FP32 oracle %.2e, vs FP64 oracle %.2e"
+        % (k0e, ame, e, e64))
+    assert k0e < 1e-3 and ame < 1e-3, (k0e, ame)
+    assert e < 1e-5, e
+    # FP32 fit (reference's float reductions) vs the reference-exact FP32 oracle
+    ref32, f32 = ob.reconstruct(p, otf, raw32, 32, False)
+    with _gpu(p, otf, fit64=False) as g:
+        g.set_raw(raw)
+        g.load_and_rescale(0)
+        g.set_time(0)
+        g.process_volume()
+        vol32 = g.result()
+        fits32 = [g.fit(d) for d in range(p.ndirs)]
+    k0e, ame = _fit_errors(p, fits32, f32)
+    e = rel_l2(vol32, ref32)
+    log("  fp32-fit: k0 err %.2e px, modamp err %.2e, rel-L2 vs FP32 oracle %.2e" % (k0e, ame, e))
+    assert k0e < 1e-3 and ame < 1e-3, (k0e, ame)
+    assert e < 1e-5, e

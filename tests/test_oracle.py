@@ -1,0 +1,251 @@
+"""Pins of the CPU oracle (oracle/) against closed forms, numpy and brute force.
+
+The reference holds no expected outputs (its test_data outputs are gitignored,
+.gitignore:5-7 of the reference), so every oracle function is pinned here by an
+independent check chosen so that a dropped term, a sign or an index slip fails.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cases import rel_l2, synthetic, load_testdata
+from cudasirecon_amd._abi import default_params, set_params
+from oracle import binding as ob
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 24, 25, 27, 40, 48, 64, 100, 384, 640, 1024])
+def test_fft_matches_numpy_1d(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for inv in (False, True):
+        ref = np.fft.ifft(x) * n if inv else np.fft.fft(x)
+        y = ob.fft3d(x.astype(np.complex128), inverse=inv)
+        assert np.abs(y - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+        yf = ob.fft3d(x.astype(np.complex64), inverse=inv)
+        assert rel_l2(yf, ref) < 2e-6
+
+
+def test_fft_brute_force_dft():
+    for n in (3, 5, 7, 9, 12, 15):
+        rng = np.random.default_rng(100 + n)
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        k = np.arange(n)
+        W = np.exp(-2j * np.pi * np.outer(k, k) / n)
+        assert np.allclose(ob.fft3d(x), W @ x, atol=1e-12)
+
+
+def test_fft_3d_and_parseval():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((6, 10, 12)) + 1j * rng.standard_normal((6, 10, 12))
+    y = ob.fft3d(x)
+    assert np.allclose(y, np.fft.fftn(x), atol=1e-11)
+    assert math.isclose(np.sum(np.abs(y) ** 2), x.size * np.sum(np.abs(x) ** 2), rel_tol=1e-12)
+
+
+# ------------------------------------------------------------------ scalar helpers
+def test_fitparabola_exact_on_quadratic():
+    # y = -(x - 0.3)^2 + 5 sampled at -1, 0, 1 -> peak at +0.3
+    f = lambda x: -(x - 0.3) ** 2 + 5  # noqa: E731
+    assert abs(ob.fitparabola(f(-1), f(0), f(1)) - 0.3) < 1e-6
+    # flat -> 0; peak outside +-1.5 -> 0 (gpuFunctionsImpl.cu:485-493)
+    assert ob.fitparabola(1.0, 1.0, 1.0) == 0.0
+    g = lambda x: -(x - 3.0) ** 2  # noqa: E731
+    assert ob.fitparabola(g(-1), g(0), g(1)) == 0.0
+
+
+def test_fitxyparabola_exact():
+    f = lambda x: -2.5 * (x - 1.234) ** 2 + 7  # noqa: E731
+    xs = [0.9, 1.3, 1.45]
+    assert abs(ob.fitxyparabola(xs[0], f(xs[0]), xs[1], f(xs[1]), xs[2], f(xs[2])) - 1.234) < 1e-9
+    assert ob.fitxyparabola(1, 2, 1, 3, 2, 4) == 0.0  # two equal x -> 0
+
+
+def test_findpeak_first_max_and_wrap():
+    a = np.zeros((8, 10), np.float32)
+    a[0, 0] = 5.0
+    a[0, 9] = 3.0  # left neighbour through the wrap
+    a[0, 1] = 4.0
+    a[7, 0] = 1.0
+    a[1, 0] = 2.0
+    px, py = ob.findpeak(a)
+    # x: fitparabola(3, 5, 4) = -0.5*(4-3)/(4+3-10) ; y: fitparabola(1, 5, 2)
+    assert abs(px - (0.5 * 1 / 3)) < 1e-6
+    assert abs(py - (0.5 * 1 / 7)) < 1e-6
+    b = np.ones((4, 4), np.float32)
+    b[2, 1] = b[3, 3] = 9.0  # tie: the first in row-major order wins
+    px, py = ob.findpeak(b)
+    assert round(px) == 1 and round(py) == 2
+
+
+# ------------------------------------------------------------------ separation matrix
+@pytest.mark.parametrize("nphases", [3, 5])
+def test_makematrix_inverts_forward_model(nphases):
+    p, tab, raw, _ = synthetic(32, 32, 4, nphases=nphases)
+    o = ob.Oracle(p, tab, 64)
+    M = o.sepmatrix()
+    norders = nphases // 2 + 1
+    phi = 2 * np.pi * np.arange(nphases) / nphases
+    # forward model img_j = a0 + sum_o c_o cos(o phi_j) + s_o sin(o phi_j)
+    F = [np.ones(nphases)]
+    for order in range(1, norders):
+        F += [np.cos(order * phi), np.sin(order * phi)]
+    F = np.array(F).T
+    expect = np.diag([nphases] + [nphases / 2.0] * (2 * norders - 2))
+    assert np.allclose(M @ F, expect, atol=1e-12)
+
+
+# ------------------------------------------------------------------ preprocessing
+def test_rescale_equalises_plane_sums():
+    p, tab, raw, _ = synthetic(32, 32, 4)
+    o = ob.Oracle(p, tab, 64)
+    o.load(raw)
+    o.rescale(0)
+    nx, ny, nz = p.nx, p.ny, p.nz
+    planes = []
+    for d in range(p.ndirs):
+        for ph in range(p.nphases):
+            b = o.band(d, ph, double=True).view(np.float64).reshape(nz, ny, (nx // 2 + 1) * 2)[:, :, :nx]
+            planes.append(b.sum(axis=(1, 2)))
+    ref = planes[0]
+    for s in planes:
+        assert np.allclose(s, ref, rtol=1e-12)
+
+
+def test_inscale_formula():
+    p, tab, raw, _ = synthetic(32, 32, 4)
+    o = ob.Oracle(p, tab, 64)
+    assert math.isclose(o.inscale(), 1.0 / (32 * 32 * 4 * 2 * 2 * 1 * 3), rel_tol=1e-15)
+
+
+# ------------------------------------------------------------------ Wiener filter special case
+def _linear_ra_otf(nxotf, nzotf, slope):
+    """OTF linear in kr and constant in kz: bilinear interpolation reproduces it exactly."""
+    kr = np.arange(nxotf, dtype=np.float64)
+    v = np.clip(1.0 - slope * kr, 0, None)
+    return np.repeat(v[:, None], nzotf, axis=1).astype(np.complex64)[None]
+
+
+def test_filter_reduces_to_textbook_wiener():
+    # ndirs = nphases = 1, no suppression, no apodisation: scale = conj(H)/(|H|^2 + w^2)
+    nx = ny = 32
+    nz = 4
+    p = default_params()
+    nzotf, nxotf, dkr = 8, 65, 0.2
+    set_params(p, nx=nx, ny=ny, nz=nz, ndirs=1, nphases=1, dxy=0.1, dz=0.2, wave=500, na=1.2, nimm=1.33,
+               bRadAvgOTF=1, nzotf=nzotf, nxotf=nxotf, nyotf=1, dkrotf=dkr, dkzotf=1.0 / (0.2 * nz),
+               bSuppress_singularities=0, apodizeoutput=0, wiener=0.05, napodize=0, k0angles=[0.0])
+    otf = _linear_ra_otf(nxotf, nzotf, 0.01)
+    o = ob.Oracle(p, otf, 64)
+    rng = np.random.default_rng(0)
+    band = (rng.standard_normal((nz, ny, nx // 2 + 1)) + 1j * rng.standard_normal((nz, ny, nx // 2 + 1)))
+    o.set_band(0, 0, band.astype(np.complex64))
+    o.filter_dir(0)
+    got = o.band(0, 0, double=True)
+    # expected, element by element (filterbands_kernel1 with a single band)
+    dkx = 1.0 / (nx * 0.1)
+    rcut = min(2 * 1.2 / 0.5, 1 / (2 * 0.1))
+    zd = o.zdist()[0]
+    exp = np.zeros_like(band)
+    for iz in range(nz):
+        z0 = iz if iz <= zd else iz - nz
+        if not (iz <= zd or iz >= nz - zd):
+            continue
+        for iy in range(ny):
+            y1 = iy if iy < ny // 2 else iy - ny
+            for ix in range(nx // 2 + 1):
+                kr = math.hypot(ix * dkx, y1 * dkx)
+                if kr > rcut:
+                    continue
+                H = max(1.0 - 0.01 * kr / dkr, 0.0)
+                exp[iz, iy, ix] = band.astype(np.complex64)[iz, iy, ix] * H / (H * H + 0.05 ** 2)
+    assert rel_l2(got, exp) < 1e-6
+
+
+# ------------------------------------------------------------------ assembly special case
+def _moved_grid(full, nx, ny, nz, Z):
+    """move_kernel's scatter (gpuFunctionsImpl.cu:1853-1913) of a full spectrum into the zoomed grid."""
+    big = np.zeros((nz, Z * ny, Z * nx), np.complex128)
+    for zc in range(nz):
+        z = zc - (nz // 2 - 1)
+        for yc in range(ny):
+            y = yc - (ny // 2 - 1)
+            for xc in range(nx):
+                x = xc - (nx // 2 - 1)
+                big[z % nz, y % (Z * ny), x % (Z * nx)] = full[z % nz, y % ny, x % nx]
+    return big
+
+
+def test_assembly_order0_is_fourier_interpolation():
+    """ndirs = nphases = 1 with a unit OTF, w = 0 and no apodisation: the filter only applies
+    its support masks, and the output is the zero-padded (Fourier) interpolation of the
+    masked spectrum, Re IFFT of the zoomed grid."""
+    nx = ny = 16
+    nz = 6
+    dxy, dz = 0.1, 0.2
+    p = default_params()
+    set_params(p, nx=nx, ny=ny, nz=nz, ndirs=1, nphases=1, dxy=dxy, dz=dz, wave=500, na=1.2, nimm=1.33,
+               bRadAvgOTF=1, nzotf=8, nxotf=65, nyotf=1, dkrotf=0.2, dkzotf=1.0 / (dz * nz), napodize=0,
+               k0angles=[0.0], wiener=0.0, apodizeoutput=0, bSuppress_singularities=0)
+    otf = _linear_ra_otf(65, 8, 0.0)
+    o = ob.Oracle(p, otf, 64)
+    rng = np.random.default_rng(1)
+    real = rng.standard_normal((nz, ny, nx))
+    full = np.fft.fftn(real)
+    o.set_band(0, 0, full[:, :, : nx // 2 + 1].astype(np.complex64))
+    o.set_fit(0, o.fit(0))
+    o.filter_assemble()
+    got = o.result(double=True)
+    zd = o.zdist()[0]
+    dk = 1.0 / (nx * dxy)
+    rcut = min(2 * 1.2 / 0.5, 1 / (2 * dxy))
+    ky = np.fft.fftfreq(ny, 1.0 / ny)
+    kx = np.fft.fftfreq(nx, 1.0 / nx)
+    ky = np.where(ky >= ny // 2, ky - ny, ky)  # y1 in [-ny/2, ny/2-1]
+    kx = np.abs(kx)
+    mask_r = np.hypot(kx[None, :] * dk, ky[:, None] * dk) <= rcut
+    iz = np.arange(nz)
+    mask_z = (iz <= zd) | (iz >= nz - zd)
+    masked = full.astype(np.complex64).astype(np.complex128) * mask_r[None] * mask_z[:, None, None]
+    big = _moved_grid(masked, nx, ny, nz, 2)
+    expect = np.real(np.fft.ifftn(big)) * big.size
+    assert got.shape == expect.shape
+    assert rel_l2(got, expect) < 1e-6
+
+
+# ------------------------------------------------------------------ end to end
+def test_testdata_k0_near_pattern_geometry():
+    """raw.dv: k0 lands next to the configured angles and line spacing (~12.6 px at 64^2)."""
+    p, otf, raw = load_testdata()
+    vol, fits = ob.reconstruct(p, otf, raw, 64)
+    dk = 1.0 / (p.nx * p.dxy)
+    for f in fits:
+        mag = math.hypot(f.k0x, f.k0y) / dk
+        assert abs(mag - (1 / 0.2035 / 2) / dk) < 0.1
+        assert math.hypot(f.k0x - f.k0guess_x, f.k0y - f.k0guess_y) / dk < 0.05
+        assert 0.3 < abs(complex(f.amp_re[1], f.amp_im[1])) < 1.5
+        assert f.corr[2] > 0.9
+    assert np.isfinite(vol).all() and vol.shape == (9, 128, 128)
+
+
+@pytest.mark.parametrize("nz,nphases", [(8, 5), (1, 3)])
+def test_synthetic_fit_recovers_ground_truth(nz, nphases):
+    p, tab, raw, truth = synthetic(64, 64, nz, nphases=nphases, photons=None)
+    vol, fits = ob.reconstruct(p, tab, raw.astype(np.float32), 64)
+    dk = 1.0 / (p.nx * p.dxy)
+    for d, f in enumerate(fits):
+        err = math.hypot(f.k0x - truth["k0"][d][0], f.k0y - truth["k0"][d][1]) / dk
+        assert err < (0.02 if nz > 1 else 0.05), (d, err)
+        for o, m in enumerate(truth["modamps"], start=1):
+            assert abs(abs(complex(f.amp_re[o], f.amp_im[o])) - m) < 0.03 * max(1, m / 0.2)
+
+
+def test_fp32_oracle_close_to_fp64_oracle():
+    p, otf, raw = load_testdata()
+    v32, f32 = ob.reconstruct(p, otf, raw, 32)
+    v64, f64 = ob.reconstruct(p, otf, raw, 64)
+    assert rel_l2(v32, v64) < 2e-5
+    dk = 1.0 / (p.nx * p.dxy)
+    for a, b in zip(f32, f64):
+        assert math.hypot(a.k0x - b.k0x, a.k0y - b.k0y) / dk < 1e-3
