@@ -41,6 +41,41 @@ __device__ __forceinline__ float2 otf_interp(const OtfRA& o, float kx, float ky,
   return v;
 }
 
+// Radial profile of one RA OTF order at a fixed kz: prof[i] is the kz-interpolated
+// value at kr index i, computed with exactly the bracketed kz term of
+// otf_interp, so that profile_lookup(prof, ...) == otf_interp(o, ..., kz, ...).
+__device__ __forceinline__ float2 otf_profile_entry(const OtfRA& o, int i, int kz, float kzscale) {
+#pragma clang fp contract(off)
+  const int nzotf = o.nzotf;
+  float kzindex = (float)kz * kzscale;
+  if (kzindex < 0) kzindex += (float)nzotf;
+  int izindex = (int)floorf(kzindex);
+  const float az = kzindex - (float)izindex;
+  if (izindex > nzotf - 1) izindex = nzotf - 1;
+  if (izindex < 0) izindex = 0;
+  const int iz1 = (izindex == nzotf - 1) ? 0 : izindex + 1;
+  const float2 a0 = o.t[i * nzotf + izindex];
+  const float2 a1 = o.t[i * nzotf + iz1];
+  return make_float2(a0.x * (1 - az) + a1.x * az, a0.y * (1 - az) + a1.y * az);
+}
+
+__device__ __forceinline__ float2 profile_lookup(const float2* prof, int nxotf, float dkrotf, float kx, float ky) {
+#pragma clang fp contract(off)
+  const float krindex = sqrtf(kx * kx + ky * ky) / dkrotf;
+  int irindex = (int)floorf(krindex);
+  float ar = krindex - (float)irindex;
+  if (irindex > nxotf - 2) {
+    irindex = nxotf - 2;
+    ar = 1.f;
+  }
+  if (irindex < 0) {
+    irindex = 0;
+    ar = 0.f;
+  }
+  const float2 t0 = prof[irindex], t1 = prof[irindex + 1];
+  return make_float2((1 - ar) * t0.x + ar * t1.x, (1 - ar) * t0.y + ar * t1.y);
+}
+
 __device__ __forceinline__ float2 cmul_nc(float2 a, float2 b) {
 #pragma clang fp contract(off)
   float re = a.x * b.x - a.y * b.y;

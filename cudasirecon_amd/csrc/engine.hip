@@ -126,7 +126,7 @@ Layout Layout::of(const simr_params& p) {
   size_t b = 0;
   b += align256((size_t)L.nimg * L.nxy * 4);            // raw (float or uint16)
   b += 2 * align256((size_t)L.nxy * 4);                  // bg, slope
-  b += align256((size_t)L.nimg * 8) + align256((size_t)L.nimg * 4) + align256((size_t)p.nz * 8) + align256(256);
+  b += align256((size_t)L.nimg * kSumSplit * 8) + align256((size_t)L.nimg * 4) + align256((size_t)p.nz * 8) + align256(256);
   b += align256((size_t)L.ndirs * L.nbands * L.H * 8);   // bands
   b += align256((size_t)L.nDirsOTF * L.norders * L.otfsize * 8);
   b += align256((size_t)2 * (2 * L.zd_ov_max + 1) * L.nxy * 8);  // overlap x/y stage
@@ -152,7 +152,7 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   raw_ = arena_.take((size_t)L.nimg * L.nxy * 4);
   bg_ = (float*)arena_.take((size_t)L.nxy * 4);
   slope_ = (float*)arena_.take((size_t)L.nxy * 4);
-  sums_ = (double*)arena_.take((size_t)L.nimg * 8);
+  sums_ = (double*)arena_.take((size_t)L.nimg * kSumSplit * 8);
   ratio_ = (float*)arena_.take((size_t)L.nimg * 4);
   sdp_ = (double*)arena_.take((size_t)L.nz * 8);
   apofact_ = (float*)arena_.take(256);
@@ -295,6 +295,10 @@ void Engine::set_raw(const void* src, bool u16, bool on_device) {
 }
 
 void Engine::set_flatfield(const float* bg, const float* slope) {
+  // bgAndSlope (cudaSirecon.cpp:139-161): without maps the background is the
+  // constant `background` option and the slope 1; that case needs no arrays.
+  flat_uniform_ = bg == nullptr && slope == nullptr;
+  if (flat_uniform_) return;
   std::vector<float> b((size_t)L.nxy), s((size_t)L.nxy);
   for (long long e = 0; e < L.nxy; ++e) {
     b[e] = bg ? bg[e] : P.constbkgd;
@@ -309,7 +313,9 @@ void Engine::set_flatfield(const float* bg, const float* slope) {
 void Engine::load_and_rescale(int it) {
   if (!raw_set_) throw StateError("raw data has not been set");
   stage_begin("section_sums");
-  launch_section_sums(raw_, raw_u16_, bg_, slope_, inscale_, L.nx, L.ny, L.nimg, sums_, stream_);
+  const float* bgp = flat_uniform_ ? nullptr : bg_;
+  const float* slp = flat_uniform_ ? nullptr : slope_;
+  launch_section_sums(raw_, raw_u16_, bgp, slp, P.constbkgd, inscale_, L.nx, L.ny, L.nimg, sums_, stream_);
   launch_ratios(sums_, sdp_, ratio_, L.ndirs, L.nz, L.nphases, P.bFastSIM, it, P.equalizez, P.equalizet,
                 P.do_rescale, stream_);
   stage_end();
@@ -324,9 +330,11 @@ void Engine::load_and_rescale(int it) {
   fp.fastSI = P.bFastSIM;
   fp.napod = P.napodize > 0 ? P.napodize : 0;
   fp.inscale = inscale_;
-  for (size_t i = 0; i < sep_.size(); ++i) fp.sep[i] = sep_[i];
+  fp.bg0 = P.constbkgd;
+  for (int b = 0; b < L.nbands; ++b)
+    for (int ph = 0; ph < L.nphases; ++ph) fp.sep[b * kMaxBands + ph] = sep_[b * L.nphases + ph];
   stage_begin("rows_fwd");
-  launch_rows_fwd(fp, raw_, raw_u16_, bg_, slope_, ratio_, apofact_, bands_, stream_);
+  launch_rows_fwd(fp, raw_, raw_u16_, bgp, slp, ratio_, apofact_, bands_, stream_);
   stage_end();
   stage_begin("cols_fwd_y");
   fft_cols(L.ny, bands_, L.nxh, L.nxh, (long long)L.ndirs * L.nbands * L.nz, (long long)L.nxh * L.ny, false,

@@ -103,7 +103,7 @@ class Engine {
 
   // device buffers
   void* raw_ = nullptr;
-  bool raw_u16_ = true, raw_set_ = false;
+  bool raw_u16_ = true, raw_set_ = false, flat_uniform_ = true;
   float *bg_ = nullptr, *slope_ = nullptr, *ratio_ = nullptr, *apofact_ = nullptr;
   double *sums_ = nullptr, *sdp_ = nullptr;
   float2* bands_ = nullptr;

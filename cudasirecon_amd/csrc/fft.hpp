@@ -210,9 +210,97 @@ __device__ __forceinline__ void dft<5, false>(float2* v) { dft5<false>(v); }
 template <>
 __device__ __forceinline__ void dft<5, true>(float2* v) { dft5<true>(v); }
 
+// ------------------------------------------------------------- LDS layout
+// Element i of a transform lives at slot lds_swz<N>(i) of its LDS row.  The
+// XOR swizzle permutes within aligned 16-element groups; it makes the
+// stride-R writes of the early Stockham stages bank-conflict free (for
+// N >= 128 every read and write sweep of every stage is conflict free; a
+// plain layout averages 2x and reaches 8x conflicts on those stages).
+template <int N>
+__host__ __device__ constexpr int lds_swz(int i) {
+  if constexpr (N >= 48)
+    return i ^ ((i >> 3) & 15);
+  else
+    return i;
+}
+
+// LDS row pitch (float2 elements) of a transform of length N.
+constexpr int lds_pitch(int N) {
+  if (N >= 48) return ((N + 15) / 16) * 16 + (N < 128 ? 4 : 2);
+  return N + 1;
+}
+
+// ------------------------------------------------------------- twiddles
+// Every stage with Ns > 1 needs B*(R-1) twiddles per thread; they depend only
+// on the thread index, so a kernel loads them once (FftTw::load) and reuses
+// them for every transform it performs.
+constexpr int tw_stage_count(int N, int TPT, int s) {
+  Radices rd = make_radices(N);
+  int R = rd.r[s];
+  int Ns = radix_prod_before(rd, s);
+  int M = N / R;
+  int B = (M + TPT - 1) / TPT;
+  return Ns > 1 ? B * (R - 1) : 0;
+}
+constexpr int tw_offset(int N, int TPT, int s) {
+  int o = 0;
+  for (int i = 0; i < s; ++i) o += tw_stage_count(N, TPT, i);
+  return o;
+}
+constexpr int tw_total(int N, int TPT) {
+  Radices rd = make_radices(N);
+  return tw_offset(N, TPT, rd.n > 0 ? rd.n : 0);
+}
+
+template <int N, int TPT = fft_tpt(N)>
+struct FftTw {
+  static constexpr int kCount = tw_total(N, TPT);
+  float2 w[kCount > 0 ? kCount : 1];
+
+  template <int S>
+  __device__ __forceinline__ void load_stage(int t, const float2* __restrict__ tab) {
+    constexpr Radices RD = make_radices(N);
+    if constexpr (S < RD.n) {
+      constexpr int R = RD.r[S];
+      constexpr int Ns = radix_prod_before(RD, S);
+      constexpr int M = N / R;
+      constexpr int B = (M + TPT - 1) / TPT;
+      constexpr int off = tw_offset(N, TPT, S);
+      if constexpr (Ns > 1) {
+        constexpr int step = N / (Ns * R);
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          const int j = t + b * TPT;
+          const int k = j < M ? j % Ns : 0;
+#pragma unroll
+          for (int r = 1; r < R; ++r) w[off + b * (R - 1) + r - 1] = tab[r * k * step];
+        }
+      }
+      load_stage<S + 1>(t, tab);
+    }
+  }
+  // forward twiddles exp(-2 pi i m / N); inverse transforms conjugate them on use
+  __device__ __forceinline__ void load(int t, const float2* __restrict__ tab) {
+    if constexpr (N > 1) load_stage<0>(t, tab);
+  }
+};
+
+// Stage the N-entry twiddle table in LDS (one coalesced read per workgroup)
+// and load this thread's twiddles from there.  stage: __shared__ float2[N];
+// contains a barrier, so every thread of the workgroup must call it.
+template <int N, int TPT>
+__device__ __forceinline__ void load_twiddles(FftTw<N, TPT>& tw, float2* stage, int t, const float2* __restrict__ tab,
+                                              int tid, int nthreads) {
+  if constexpr (N > 1) {
+    for (int i = tid; i < N; i += nthreads) stage[i] = tab[i];
+    __syncthreads();
+    tw.load(t, stage);
+  }
+}
+
 // ------------------------------------------------------------- stage driver
 template <int N, bool INV, int TPT, int S>
-__device__ __forceinline__ void fft_stage(float2* buf, int t, const float2* __restrict__ tw) {
+__device__ __forceinline__ void fft_stage(float2* buf, int t, const FftTw<N, TPT>& tw) {
   constexpr Radices RD = make_radices(N);
   if constexpr (S < RD.n) {
     constexpr int R = RD.r[S];
@@ -220,13 +308,14 @@ __device__ __forceinline__ void fft_stage(float2* buf, int t, const float2* __re
     constexpr int M = N / R;
     constexpr int B = (M + TPT - 1) / TPT;
     constexpr bool FULL = (M % TPT) == 0;
+    constexpr int off = tw_offset(N, TPT, S);
     float2 v[B][R];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       const int j = t + b * TPT;
       if (FULL || j < M) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * M];
+        for (int r = 0; r < R; ++r) v[b][r] = buf[lds_swz<N>(j + r * M)];
       }
     }
     __syncthreads();
@@ -236,18 +325,17 @@ __device__ __forceinline__ void fft_stage(float2* buf, int t, const float2* __re
       if (FULL || j < M) {
         const int k = j % Ns;
         if constexpr (Ns > 1) {
-          constexpr int step = N / (Ns * R);
 #pragma unroll
           for (int r = 1; r < R; ++r) {
-            float2 w = tw[r * k * step];
-            if (INV) w.y = -w.y;
-            v[b][r] = cmul(v[b][r], w);
+            float2 ww = tw.w[off + b * (R - 1) + r - 1];
+            if (INV) ww.y = -ww.y;
+            v[b][r] = cmul(v[b][r], ww);
           }
         }
         dft<R, INV>(v[b]);
         const int idxD = (j / Ns) * Ns * R + k;
 #pragma unroll
-        for (int q = 0; q < R; ++q) buf[idxD + q * Ns] = v[b][q];
+        for (int q = 0; q < R; ++q) buf[lds_swz<N>(idxD + q * Ns)] = v[b][q];
       }
     }
     __syncthreads();
@@ -255,17 +343,14 @@ __device__ __forceinline__ void fft_stage(float2* buf, int t, const float2* __re
   }
 }
 
-// In-place FFT of buf[0..N) (contiguous in LDS) by the TPT threads t = 0..TPT-1
-// of one transform.  Every thread of the workgroup must call it (barriers).
+// In-place FFT of one transform held in LDS (element i at buf[lds_swz<N>(i)])
+// by its TPT threads t = 0..TPT-1.  Every thread of the workgroup must call it
+// (it contains barriers).
 template <int N, bool INV, int TPT = fft_tpt(N)>
-__device__ __forceinline__ void fft_lds(float2* buf, int t, const float2* __restrict__ tw) {
+__device__ __forceinline__ void fft_lds(float2* buf, int t, const FftTw<N, TPT>& tw) {
   static_assert(fft_supported(N), "FFT length must factor into 2, 3, 5");
   if constexpr (N > 1) fft_stage<N, INV, TPT, 0>(buf, t, tw);
 }
-
-// LDS row pitch for a transform of length N (one extra element breaks the
-// power-of-two bank stride between transforms).
-constexpr int lds_pitch(int N) { return N + 1; }
 
 // Compile-time list of supported lengths; host code dispatches on it.
 #define SIMR_FFT_SIZES(X)                                                                    \

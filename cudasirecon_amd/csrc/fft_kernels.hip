@@ -61,19 +61,22 @@ __global__ __launch_bounds__(W* fft_tpt(N)) void k_fft_cols(float2* __restrict__
   constexpr int P = lds_pitch(N);
   __shared__ float2 lds[W * P];
   const int tid = threadIdx.x;
+  FftTw<N> twr;
+  __shared__ float2 twstage[N > 1 ? N : 1];
+  load_twiddles(twr, twstage, tid % TPT, tw, threadIdx.x, blockDim.x);
   const long long c0 = (long long)blockIdx.x * W;
   float2* base = data + (long long)blockIdx.y * plane_stride;
   for (int idx = tid; idx < N * W; idx += NT) {
     const int i = idx / W, w = idx % W;
     const long long c = c0 + w;
-    lds[w * P + i] = (c < ncols) ? base[i * pitch + c] : make_float2(0.f, 0.f);
+    lds[w * P + lds_swz<N>(i)] = (c < ncols) ? base[i * pitch + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<N, INV>(lds + (tid / TPT) * P, tid % TPT, tw);
+  fft_lds<N, INV>(lds + (tid / TPT) * P, tid % TPT, twr);
   for (int idx = tid; idx < N * W; idx += NT) {
     const int i = idx / W, w = idx % W;
     const long long c = c0 + w;
-    if (c < ncols) base[i * pitch + c] = lds[w * P + i];
+    if (c < ncols) base[i * pitch + c] = lds[w * P + lds_swz<N>(i)];
   }
 }
 
@@ -86,18 +89,21 @@ __global__ __launch_bounds__(W* fft_tpt(N)) void k_fft_rows(float2* __restrict__
   constexpr int P = lds_pitch(N);
   __shared__ float2 lds[W * P];
   const int tid = threadIdx.x;
+  FftTw<N> twr;
+  __shared__ float2 twstage[N > 1 ? N : 1];
+  load_twiddles(twr, twstage, tid % TPT, tw, threadIdx.x, blockDim.x);
   const long long r0 = (long long)blockIdx.x * W;
   for (int idx = tid; idx < N * W; idx += NT) {
     const int w = idx / N, i = idx % N;
     const long long r = r0 + w;
-    lds[w * P + i] = (r < nrows) ? data[r * pitch + i] : make_float2(0.f, 0.f);
+    lds[w * P + lds_swz<N>(i)] = (r < nrows) ? data[r * pitch + i] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<N, INV>(lds + (tid / TPT) * P, tid % TPT, tw);
+  fft_lds<N, INV>(lds + (tid / TPT) * P, tid % TPT, twr);
   for (int idx = tid; idx < N * W; idx += NT) {
     const int w = idx / N, i = idx % N;
     const long long r = r0 + w;
-    if (r < nrows) data[r * pitch + i] = lds[w * P + i];
+    if (r < nrows) data[r * pitch + i] = lds[w * P + lds_swz<N>(i)];
   }
 }
 

@@ -24,14 +24,17 @@ struct FwdParams {
   int fastSI;
   int napod;
   float inscale;
-  float sep[kMaxBands * kMaxBands];  // separation matrix [band][phase]
+  float bg0;                         // uniform background when no flat-field maps are set
+  float sep[kMaxBands * kMaxBands];  // separation matrix [band][phase], row stride kMaxBands
 };
 
-// sums[sec] = sum over the section of the flat-fielded pixels (double).
-void launch_section_sums(const void* raw, bool u16, const float* bg, const float* slope, float inscale, int nx,
-                         int ny, int nimg, double* sums, hipStream_t s);
+// Flat-fielded section sums: partials[sec * kSumSplit + part] (double).
+constexpr int kSumSplit = 8;
+// bg/slope == nullptr selects the uniform flat field (bg0, slope 1).
+void launch_section_sums(const void* raw, bool u16, const float* bg, const float* slope, float bg0,
+                         float inscale, int nx, int ny, int nimg, double* partials, hipStream_t s);
 // Bleach-correction ratios per section (reference rescale semantics).
-void launch_ratios(const double* sums, double* sdp, float* ratio, int ndirs, int nz, int nphases, int fastSI,
+void launch_ratios(const double* partials, double* sdp, float* ratio, int ndirs, int nz, int nphases, int fastSI,
                    int it, int equalizez, int equalizet, int do_rescale, hipStream_t s);
 // Fused flatfield + rescale + apodize + phase separation + R2C along x.
 // bands: [dir][band][z][y][nxh] complex.

@@ -79,6 +79,9 @@ __global__ __launch_bounds__(256) void k_asm_z(AsmParams p, const float2* __rest
   constexpr int NT = W * TPT;
   constexpr int LP = lds_pitch(N);
   __shared__ float2 lds[W * LP];
+  FftTw<N> twr;
+  __shared__ float2 twstage[N > 1 ? N : 1];
+  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
   const int a = blockIdx.z, yc = blockIdx.y;
   const int xc0 = blockIdx.x * W;
   const int y = yc - (p.ny / 2 - 1);
@@ -89,14 +92,14 @@ __global__ __launch_bounds__(256) void k_asm_z(AsmParams p, const float2* __rest
     const int z = z_of_zout(zout, p.nz, N, valid);
     float2 v = make_float2(0.f, 0.f);
     if (valid && xc < p.nx) v = moved_value(p, a, xc - (p.nx / 2 - 1), y, z);
-    lds[w * LP + zout] = v;
+    lds[w * LP + lds_swz<N>(zout)] = v;
   }
   __syncthreads();
-  if (threadIdx.x < NT) fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, tw);
+  fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
   for (int idx = threadIdx.x; idx < N * W; idx += NT) {
     const int zout = idx / W, w = idx - (idx / W) * W;
     const int xc = xc0 + w;
-    if (xc < p.nx) p.stageZ[(((long long)a * N + zout) * p.ny + yc) * p.nx + xc] = lds[w * LP + zout];
+    if (xc < p.nx) p.stageZ[(((long long)a * N + zout) * p.ny + yc) * p.nx + xc] = lds[w * LP + lds_swz<N>(zout)];
   }
 }
 
@@ -117,6 +120,9 @@ __global__ __launch_bounds__(asm_y_w<N>() * fft_tpt(N)) void k_asm_y(AsmParams p
   constexpr int NT = W * TPT;
   constexpr int LP = lds_pitch(N);
   __shared__ float2 lds[W * LP];
+  FftTw<N> twr;
+  __shared__ float2 twstage[N > 1 ? N : 1];
+  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
   const int a = blockIdx.z, zout = blockIdx.y;
   const int xc0 = blockIdx.x * W;
   const float2* src = p.stageZ + ((long long)a * p.zdim + zout) * p.ny * p.nx;
@@ -133,20 +139,23 @@ __global__ __launch_bounds__(asm_y_w<N>() * fft_tpt(N)) void k_asm_y(AsmParams p
       y = 0;
       valid = false;
     }
-    lds[w * LP + Y] = valid ? src[(long long)(y + (p.ny / 2 - 1)) * p.nx + xc] : make_float2(0.f, 0.f);
+    lds[w * LP + lds_swz<N>(Y)] = valid ? src[(long long)(y + (p.ny / 2 - 1)) * p.nx + xc] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, tw);
+  fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
   float2* dst = p.stageY + ((long long)a * p.zdim + zout) * (long long)N * p.nx;
   for (int idx = threadIdx.x; idx < N * W; idx += NT) {
     const int Y = idx / W, w = idx - (idx / W) * W;
     const int xc = xc0 + w;
-    if (xc < p.nx) dst[(long long)Y * p.nx + xc] = lds[w * LP + Y];
+    if (xc < p.nx) dst[(long long)Y * p.nx + xc] = lds[w * LP + lds_swz<N>(Y)];
   }
 }
 
-// ---- x pass: row Y (blockIdx.x) of every plane, bands [a0, a1) (<= 8 per launch)
+// ---- x pass: row Y (blockIdx.x) of planes [kAsmZc*blockIdx.y, +kAsmZc), bands
+// [a0, a1) (<= 8 per launch).  The modulation phasors of the row are computed
+// once per workgroup and kept in registers across its planes.
 constexpr int kAsmSlots = 8;
+constexpr int kAsmZc = 4;
 
 template <int N>
 constexpr int asm_x_tpt() {
@@ -162,6 +171,9 @@ __global__ __launch_bounds__(kAsmSlots* asm_x_tpt<N>()) void k_asm_x(AsmParams p
   constexpr int LP = lds_pitch(N);
   constexpr int E = (N + NT - 1) / NT;
   __shared__ float2 lds[kAsmSlots * LP];
+  FftTw<N, TPT> twr;
+  __shared__ float2 twstage[N > 1 ? N : 1];
+  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
   const int Y = blockIdx.x;
   const int nb = a1 - a0;
   // modulation phasors of the sideband slots for this row (cos_sin_kernel 1942-1954)
@@ -187,7 +199,9 @@ __global__ __launch_bounds__(kAsmSlots* asm_x_tpt<N>()) void k_asm_x(AsmParams p
       }
     }
   }
-  for (int zout = 0; zout < p.zdim; ++zout) {
+  const int zbeg = blockIdx.y * kAsmZc;
+  const int zend = min(p.zdim, zbeg + kAsmZc);
+  for (int zout = zbeg; zout < zend; ++zout) {
     for (int idx = threadIdx.x; idx < kAsmSlots * N; idx += NT) {
       const int sl = idx / N, X = idx - (idx / N) * N;
       float2 v = make_float2(0.f, 0.f);
@@ -205,10 +219,10 @@ __global__ __launch_bounds__(kAsmSlots* asm_x_tpt<N>()) void k_asm_x(AsmParams p
         if (valid)
           v = p.stageY[(((long long)(a0 + sl) * p.zdim + zout) * p.ydim + Y) * p.nx + (x + (p.nx / 2 - 1))];
       }
-      lds[sl * LP + X] = v;
+      lds[sl * LP + lds_swz<N>(X)] = v;
     }
     __syncthreads();
-    fft_lds<N, true, TPT>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, tw);
+    fft_lds<N, true, TPT>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
     float* orow = p.out + ((long long)zout * p.ydim + Y) * N;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -218,7 +232,7 @@ __global__ __launch_bounds__(kAsmSlots* asm_x_tpt<N>()) void k_asm_x(AsmParams p
 #pragma unroll
         for (int sl = 0; sl < kAsmSlots; ++sl) {
           if (sl < nb) {
-            const float2 v = lds[sl * LP + X];
+            const float2 v = lds[sl * LP + lds_swz<N>(X)];
             if (a0 + sl == 0) {
               acc += v.x;  // write_outbuffer_kernel1
             } else {       // write_outbuffer_kernel2: double intermediate, as in the reference
@@ -273,7 +287,8 @@ void launch_asm_x(const AsmParams& p, int a0, int a1, bool accumulate, hipStream
   switch (p.xdim) {
 #define SIMR_CASE(N)                                                                                  \
   case N:                                                                                             \
-    hipLaunchKernelGGL(k_asm_x<N>, dim3(p.ydim), dim3(kAsmSlots * asm_x_tpt<N>()), 0, s, p, a0, a1,   \
+    hipLaunchKernelGGL(k_asm_x<N>, dim3(p.ydim, ceil_div(p.zdim, kAsmZc)), dim3(kAsmSlots * asm_x_tpt<N>()),     \
+                       0, s, p, a0, a1,                                                               \
                        (int)accumulate, twiddles(N));                                                 \
     SIMR_LAUNCH_CHECK();                                                                              \
     return;

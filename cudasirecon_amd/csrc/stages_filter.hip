@@ -33,7 +33,9 @@ __device__ __forceinline__ float dev_order0damping(float radius, float zindex, f
 
 // One visit of filterbands_kernel1 at band coordinates (x1, y1, z0) whose
 // rdist1 <= rdistcutoff; updates (re, im) of the stored element.
-__device__ void filter_visit(const FilterParams& p, int x1, int y1, int z0, bool conj, float2& re, float2& im) {
+// prof: this plane's radial OTF profiles at kz = z0, [order][nxotf].
+__device__ void filter_visit(const FilterParams& p, const float2* prof, int x1, int y1, int z0, bool conj, float2& re,
+                             float2& im) {
 #pragma clang fp contract(off)
   const int order = p.order;
   const float kx = (float)order * p.k0[p.dir].x;
@@ -42,7 +44,9 @@ __device__ void filter_visit(const FilterParams& p, int x1, int y1, int z0, bool
   const float rdist1 = sqrtf(x1f * x1f + y1f * y1f);
   const float xabs = x1f + kx, yabs = y1f + ky;
   const float rdistabs = sqrtf(xabs * xabs + yabs * yabs);
-  const float2 otf1 = otf_interp(p.otf[order], x1f, y1f, z0, p.kzscale);
+  const int nxotf = p.otf[0].nxotf;
+  const float dkr = p.otf[0].dkrotf;
+  const float2 otf1 = profile_lookup(prof + order * nxotf, nxotf, dkr, x1f, y1f);
   float weight = otf1.x * otf1.x + otf1.y * otf1.y;
   if (order != 0) weight *= p.ampmag2_all[p.dir * p.norders + order];
   float dampfact = (float)(1. / (double)p.noiseVar[p.dir * p.norders + order]);
@@ -65,7 +69,7 @@ __device__ void filter_visit(const FilterParams& p, int x1, int y1, int z0, bool
       const float x2 = xabs - kx2, y2 = yabs - ky2;
       const float rdist2 = sqrtf(x2 * x2 + y2 * y2);
       if (rdist2 < p.rdistcutoff) {
-        const float2 otf2 = otf_interp(p.otf[ao2], x2, y2, z0, p.kzscale);
+        const float2 otf2 = profile_lookup(prof + ao2 * nxotf, nxotf, dkr, x2, y2);
         float w = (otf2.x * otf2.x + otf2.y * otf2.y) / p.noiseVar[dir2 * p.norders + ao2];
         if (order2 != 0) w *= amp2mag2;
         if (p.bSuppress && order2 != 0 && rdist2 <= p.suppRadius)
@@ -120,36 +124,43 @@ __device__ void filter_visit(const FilterParams& p, int x1, int y1, int z0, bool
   }
 }
 
-// One thread per stored element (jin, iin, iz) of the band pair; planes whose
-// z lies outside +-zd are zeroed (kernel3).
+// Workgroup (blockIdx.x, iz = blockIdx.y): 256 stored elements (jin, iin) of
+// plane iz of the band pair.  The plane's radial OTF profiles at kz = +z0 and
+// kz = -z0 (the conjugate visit) are built once in LDS.  Planes whose z lies
+// outside +-zd are zeroed (kernel3).
 __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
 #pragma clang fp contract(off)
+  extern __shared__ float2 prof[];  // [2][norders][nxotf]
   const long long nplane = (long long)p.nxh * p.ny;
-  const long long total = nplane * p.nz;
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= total) return;
-  const int iz = (int)(e / nplane);
-  const long long rem = e - (long long)iz * nplane;
-  const int iin = (int)(rem / p.nxh), jin = (int)(rem - (long long)iin * p.nxh);
+  const int iz = blockIdx.y;
+  const long long ep = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long e = (long long)iz * nplane + ep;
   const int zd = p.zd[p.order];
-  int z0n;
-  bool visited = true;
+  float2* bp = p.bp;
+  float2* bp2 = p.bp2;
+  const bool two = p.order != 0;
+  int z0n = 0;
   if (iz <= zd)
     z0n = iz;
   else if (iz >= p.nz - zd)
     z0n = iz - p.nz;
   else {
-    z0n = 0;
-    visited = false;
+    if (ep < nplane) {
+      bp[e] = make_float2(0.f, 0.f);
+      if (two) bp2[e] = make_float2(0.f, 0.f);
+    }
+    return;  // whole block: uniform
   }
-  float2* bp = p.bp;
-  float2* bp2 = p.bp2;
-  const bool two = p.order != 0;
-  if (!visited) {
-    bp[e] = make_float2(0.f, 0.f);
-    if (two) bp2[e] = make_float2(0.f, 0.f);
-    return;
+  const int nxotf = p.otf[0].nxotf;
+  const int nprof = p.norders * nxotf;
+  for (int i = threadIdx.x; i < 2 * nprof; i += 256) {
+    const int side = i / nprof, rem = i - side * nprof;
+    const int o = rem / nxotf, ir = rem - o * nxotf;
+    prof[i] = otf_profile_entry(p.otf[o], ir, side ? -z0n : z0n, p.kzscale);
   }
+  __syncthreads();
+  if (ep >= nplane) return;
+  const int iin = (int)(ep / p.nxh), jin = (int)(ep - (long long)iin * p.nxh);
   const int y1n = (iin < p.ny - p.ny / 2) ? iin : iin - p.ny;
   const float x1f = (float)jin * p.dkx, y1f = (float)y1n * p.dky;
   const float rdist1 = sqrtf(x1f * x1f + y1f * y1f);
@@ -161,19 +172,21 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
   float2 re = bp[e];
   float2 im = two ? bp2[e] : make_float2(0.f, 0.f);
   // first pass (x1 in [1, nx/2]) or second pass x1 == 0: the element itself
-  filter_visit(p, jin, y1n, z0n, false, re, im);
+  filter_visit(p, prof, jin, y1n, z0n, false, re, im);
   // second pass, x1 = -jin < 0: the conjugate visit of the mirrored frequency
   if (two && jin >= 1 && jin <= p.nx / 2 - 1) {
     const int yc = -((iin <= p.ny / 2) ? iin : iin - p.ny);
-    filter_visit(p, -jin, yc, -z0n, true, re, im);
+    filter_visit(p, prof + nprof, -jin, yc, -z0n, true, re, im);
   }
   bp[e] = re;
   if (two) bp2[e] = im;
 }
 
 void launch_filter(const FilterParams& p, hipStream_t s) {
-  const long long total = (long long)p.nxh * p.ny * p.nz;
-  hipLaunchKernelGGL(k_filter, dim3(ceil_div(total, 256)), dim3(256), 0, s, p);
+  const long long nplane = (long long)p.nxh * p.ny;
+  const size_t lds = (size_t)2 * p.norders * p.otf[0].nxotf * sizeof(float2);
+  if (lds > 64 * 1024) throw ArgError("OTF table too long for the filter's LDS profile");
+  hipLaunchKernelGGL(k_filter, dim3(ceil_div(nplane, 256), p.nz), dim3(256), lds, s, p);
   SIMR_LAUNCH_CHECK();
 }
 

@@ -21,43 +21,63 @@
 
 namespace simr {
 
-template <typename T>
-__device__ __forceinline__ float flat_px(const T* raw, long long sec, int nx, int ny, int y, int x, const float* bg,
-                                         const float* slope, float bg0, float inscale) {
+// (raw - background - 0) * slope * inscale (load_and_flatfield, cudaSirecon.cpp:1726)
+__device__ __forceinline__ float flat1(float v, float b, float s, float inscale) {
 #pragma clang fp contract(off)
-  const long long e = (long long)y * nx + x;
-  float v = (float)raw[sec * (long long)nx * ny + e];
-  float b = bg ? bg[e] : bg0;
-  float s = slope ? slope[e] : 1.0f;
   return ((v - b) - 0.0f) * s * inscale;
 }
 
+// Four consecutive raw pixels starting at element offset e (e % 4 == 0).
+__device__ __forceinline__ float4 load4(const uint16_t* raw, size_t e) {
+  const ushort4 u = *reinterpret_cast<const ushort4*>(raw + e);
+  return make_float4((float)u.x, (float)u.y, (float)u.z, (float)u.w);
+}
+__device__ __forceinline__ float4 load4(const float* raw, size_t e) { return *reinterpret_cast<const float4*>(raw + e); }
+
+template <typename T>
+__device__ __forceinline__ float4 flat4(const T* raw, size_t sec_off, int pix, const float* bg, const float* slope,
+                                        float bg0, float inscale) {
+  const float4 v = load4(raw, sec_off + pix);
+  float4 b = make_float4(bg0, bg0, bg0, bg0), s = make_float4(1.f, 1.f, 1.f, 1.f);
+  if (bg) {
+    b = *reinterpret_cast<const float4*>(bg + pix);
+    s = *reinterpret_cast<const float4*>(slope + pix);
+  }
+  return make_float4(flat1(v.x, b.x, s.x, inscale), flat1(v.y, b.y, s.y, inscale), flat1(v.z, b.z, s.z, inscale),
+                     flat1(v.w, b.w, s.w, inscale));
+}
+
 // ---------------------------------------------------------------- section sums
+// Block (sec, part) sums part `part` of kSumSplit of section sec in double.
 template <typename T>
 __global__ __launch_bounds__(256) void k_section_sums(const T* __restrict__ raw, const float* __restrict__ bg,
                                                       const float* __restrict__ slope, float bg0, float inscale,
-                                                      int nx, int ny, double* __restrict__ sums) {
+                                                      int npix, double* __restrict__ partials) {
   __shared__ double red[256];
-  const long long sec = blockIdx.x;
-  const long long npix = (long long)nx * ny;
+  const int sec = blockIdx.x, part = blockIdx.y;
+  const int nq = npix >> 2;
+  const int q0 = (int)((long long)nq * part / kSumSplit), q1 = (int)((long long)nq * (part + 1) / kSumSplit);
+  const size_t off = (size_t)sec * npix;
   double acc = 0.0;
-  for (long long e = threadIdx.x; e < npix; e += 256) {
-    const int y = (int)(e / nx), x = (int)(e % nx);
-    acc += (double)flat_px(raw, sec, nx, ny, y, x, bg, slope, bg0, inscale);
+#pragma unroll 4
+  for (int q = q0 + (int)threadIdx.x; q < q1; q += 256) {
+    const float4 v = flat4(raw, off, q << 2, bg, slope, bg0, inscale);
+    acc += (double)v.x + (double)v.y + (double)v.z + (double)v.w;
   }
-  double s = block_sum_d<256>(acc, red);
-  if (threadIdx.x == 0) sums[sec] = s;
+  const double s = block_sum_d<256>(acc, red);
+  if (threadIdx.x == 0) partials[sec * kSumSplit + part] = s;
 }
 
-void launch_section_sums(const void* raw, bool u16, const float* bg, const float* slope, float inscale, int nx,
-                         int ny, int nimg, double* sums, hipStream_t s) {
-  const float bg0 = 0.f;  // used only when bg is null
+void launch_section_sums(const void* raw, bool u16, const float* bg, const float* slope, float bg0, float inscale,
+                         int nx, int ny, int nimg, double* partials, hipStream_t s) {
+  if ((nx * ny) % 4) throw ArgError("section size must be a multiple of 4 pixels");
+  dim3 grid(nimg, kSumSplit);
   if (u16)
-    hipLaunchKernelGGL(k_section_sums<uint16_t>, dim3(nimg), dim3(256), 0, s, (const uint16_t*)raw, bg, slope, bg0,
-                       inscale, nx, ny, sums);
+    hipLaunchKernelGGL(k_section_sums<uint16_t>, grid, dim3(256), 0, s, (const uint16_t*)raw, bg, slope, bg0,
+                       inscale, nx * ny, partials);
   else
-    hipLaunchKernelGGL(k_section_sums<float>, dim3(nimg), dim3(256), 0, s, (const float*)raw, bg, slope, bg0,
-                       inscale, nx, ny, sums);
+    hipLaunchKernelGGL(k_section_sums<float>, grid, dim3(256), 0, s, (const float*)raw, bg, slope, bg0, inscale,
+                       nx * ny, partials);
   SIMR_LAUNCH_CHECK();
 }
 
@@ -66,31 +86,35 @@ __device__ __forceinline__ int section_of(int dir, int z, int ph, int ndirs, int
   return fastSI ? (z * ndirs * nphases + dir * nphases + ph) : (dir * nz * nphases + z * nphases + ph);
 }
 
-// rescale's ratio logic (gpuFunctionsImpl.cu:134-156): per (dir, z) in the
-// reference's loop order; sdp persists across time points (equalizet).
-__global__ void k_ratios(const double* __restrict__ sums, double* __restrict__ sdp, float* __restrict__ ratio,
-                         int ndirs, int nz, int nphases, int fastSI, int it, int equalizez, int equalizet,
-                         int do_rescale) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (int dir = 0; dir < ndirs; ++dir)
-    for (int z = 0; z < nz; ++z) {
-      if (!do_rescale) {
-        for (int ph = 0; ph < nphases; ++ph) ratio[section_of(dir, z, ph, ndirs, nz, nphases, fastSI)] = 1.0f;
-        continue;
-      }
-      const float s0 = (float)sums[section_of(dir, z, 0, ndirs, nz, nphases, fastSI)];
-      if (dir == 0 && !(equalizet && it != 0)) sdp[z] = (double)s0;
-      const float ref = equalizez ? (float)sdp[0] : (float)sdp[z];
-      for (int ph = 0; ph < nphases; ++ph) {
-        const int sec = section_of(dir, z, ph, ndirs, nz, nphases, fastSI);
-        ratio[sec] = ref / (float)sums[sec];
-      }
-    }
+__device__ __forceinline__ float section_sum(const double* partials, int sec) {
+  double s = 0.0;
+  for (int k = 0; k < kSumSplit; ++k) s += partials[sec * kSumSplit + k];
+  return (float)s;  // the reference keeps plane sums in float
 }
 
-void launch_ratios(const double* sums, double* sdp, float* ratio, int ndirs, int nz, int nphases, int fastSI,
+// rescale's ratio logic (gpuFunctionsImpl.cu:134-156).  Each z is independent
+// except through sdp[0] under equalizez, so direction 0 sets sdp first (as the
+// reference's dir-0 loop does), then every (dir, z) forms its ratios.
+__global__ __launch_bounds__(256) void k_ratios(const double* __restrict__ partials, double* __restrict__ sdp,
+                                                float* __restrict__ ratio, int ndirs, int nz, int nphases,
+                                                int fastSI, int it, int equalizez, int equalizet, int do_rescale) {
+  if (do_rescale && !(equalizet && it != 0))
+    for (int z = threadIdx.x; z < nz; z += 256)
+      sdp[z] = (double)section_sum(partials, section_of(0, z, 0, ndirs, nz, nphases, fastSI));
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < ndirs * nz; idx += 256) {
+    const int dir = idx / nz, z = idx - dir * nz;
+    const float ref = equalizez ? (float)sdp[0] : (float)sdp[z];
+    for (int ph = 0; ph < nphases; ++ph) {
+      const int sec = section_of(dir, z, ph, ndirs, nz, nphases, fastSI);
+      ratio[sec] = do_rescale ? ref / section_sum(partials, sec) : 1.0f;
+    }
+  }
+}
+
+void launch_ratios(const double* partials, double* sdp, float* ratio, int ndirs, int nz, int nphases, int fastSI,
                    int it, int equalizez, int equalizet, int do_rescale, hipStream_t s) {
-  hipLaunchKernelGGL(k_ratios, dim3(1), dim3(64), 0, s, sums, sdp, ratio, ndirs, nz, nphases, fastSI, it,
+  hipLaunchKernelGGL(k_ratios, dim3(1), dim3(256), 0, s, partials, sdp, ratio, ndirs, nz, nphases, fastSI, it,
                      equalizez, equalizet, do_rescale);
   SIMR_LAUNCH_CHECK();
 }
@@ -100,6 +124,8 @@ void launch_ratios(const double* sums, double* sdp, float* ratio, int ndirs, int
 // and y0+1.  The two real rows of each band are packed as one complex row
 // (row0 + i row1), transformed, and split with
 //   X0[k] = (Z[k] + conj Z[N-k]) / 2,   X1[k] = (Z[k] - conj Z[N-k]) / 2i.
+// All raw loads of a workgroup are 4-pixel vector loads with compile-time
+// trip counts, so they are all in flight before the first use.
 template <int N, typename T>
 __global__ __launch_bounds__(kMaxBands* fft_tpt(N)) void k_rows_fwd(FwdParams p, const T* __restrict__ raw,
                                                                      const float* __restrict__ bg,
@@ -112,85 +138,117 @@ __global__ __launch_bounds__(kMaxBands* fft_tpt(N)) void k_rows_fwd(FwdParams p,
   constexpr int TPT = fft_tpt(N);
   constexpr int NT = kMaxBands * TPT;
   constexpr int P = lds_pitch(N);
+  constexpr int Q = N / 4;
+  constexpr int NQ = kMaxBands * 2 * Q;
+  constexpr int QPT = (NQ + NT - 1) / NT;
   __shared__ float2 cz[kMaxBands * P];
-  __shared__ float rv[kMaxBands * 2 * N];
+  __shared__ __align__(16) float rv[kMaxBands * 2 * N];
   __shared__ float dy[kMaxBands * 2];
   const int tid = threadIdx.x;
+  FftTw<N> twr;
+  __shared__ float2 twstage[N > 1 ? N : 1];
+  load_twiddles(twr, twstage, tid % TPT, tw, threadIdx.x, blockDim.x);
   const int y0 = 2 * blockIdx.x;
   const int z = blockIdx.y, dir = blockIdx.z;
   const int ny = p.ny;
   const int napod = p.napod;
+  const size_t npix = (size_t)N * ny;
+  const bool edge_block = napod > 0 && (y0 < napod || y0 + 1 >= ny - napod);
   // 1. flat-field, bleach ratio, x-apodisation (apodize_x_kernel 52-66)
-  for (int idx = tid; idx < p.nphases * 2 * N; idx += NT) {
-    const int ph = idx / (2 * N);
-    const int rem = idx - ph * 2 * N;
-    const int r = rem / N, x = rem - (rem / N) * N;
-    const int y = y0 + r;
-    float v = 0.f;
-    if (y < ny) {
-      const int sec = section_of(dir, z, ph, p.ndirs, p.nz, p.nphases, p.fastSI);
+#pragma unroll
+  for (int i = 0; i < QPT; ++i) {
+    const int q = tid + i * NT;
+    if (NQ % NT == 0 || q < NQ) {
+      const int pr = q / Q, x = (q - (q / Q) * Q) * 4;
+      const int ph = pr >> 1, r = pr & 1;
+      const int phc = ph < p.nphases ? ph : p.nphases - 1;
+      const int y = y0 + r;
+      const int yc = y < ny ? y : ny - 1;
+      const int sec = section_of(dir, z, phc, p.ndirs, p.nz, p.nphases, p.fastSI);
       const float ra = ratio[sec];
-      v = flat_px(raw, sec, N, ny, y, x, bg, slope, 0.f, p.inscale) * ra;
-      if (napod > 0 && (y < napod || y >= ny - napod)) {
-        const float top = flat_px(raw, sec, N, ny, 0, x, bg, slope, 0.f, p.inscale) * ra;
-        const float bot = flat_px(raw, sec, N, ny, ny - 1, x, bg, slope, 0.f, p.inscale) * ra;
-        const float diff = (bot - top) * 0.5f;
+      float4 v = flat4(raw, sec * npix, yc * N + x, bg, slope, p.bg0, p.inscale);
+      v = make_float4(v.x * ra, v.y * ra, v.z * ra, v.w * ra);
+      if (edge_block && (y < napod || y >= ny - napod)) {
+        float4 top = flat4(raw, sec * npix, x, bg, slope, p.bg0, p.inscale);
+        float4 bot = flat4(raw, sec * npix, (ny - 1) * N + x, bg, slope, p.bg0, p.inscale);
+        const float4 d = make_float4((bot.x * ra - top.x * ra) * 0.5f, (bot.y * ra - top.y * ra) * 0.5f,
+                                     (bot.z * ra - top.z * ra) * 0.5f, (bot.w * ra - top.w * ra) * 0.5f);
         for (int l = 0; l < napod; ++l) {
           const float f = apofact[l];
-          if (y == l) v += diff * f;
-          if (y == ny - 1 - l) v -= diff * f;
+          if (y == l) {
+            v.x += d.x * f;
+            v.y += d.y * f;
+            v.z += d.z * f;
+            v.w += d.w * f;
+          }
+          if (y == ny - 1 - l) {
+            v.x -= d.x * f;
+            v.y -= d.y * f;
+            v.z -= d.z * f;
+            v.w -= d.w * f;
+          }
         }
       }
+      if (ph >= p.nphases || y >= ny) v = make_float4(0.f, 0.f, 0.f, 0.f);
+      *reinterpret_cast<float4*>(rv + pr * N + x) = v;
     }
-    rv[(ph * 2 + r) * N + x] = v;
   }
   __syncthreads();
   // 2. y-apodisation of each row from its own end values (apodize_y_kernel 68-82)
   if (napod > 0) {
-    if (tid < p.nphases * 2) dy[tid] = (rv[tid * N + N - 1] - rv[tid * N]) * 0.5f;
+    if (tid < kMaxBands * 2) dy[tid] = (rv[tid * N + N - 1] - rv[tid * N]) * 0.5f;
     __syncthreads();
-    for (int idx = tid; idx < p.nphases * 2 * N; idx += NT) {
-      const int row = idx / N, x = idx - (idx / N) * N;
-      if (x < napod || x >= N - napod) {
-        float v = rv[row * N + x];
-        const float d = dy[row];
-        for (int k = 0; k < napod; ++k) {
-          const float f = apofact[k];
-          if (x == k) v += d * f;
-          if (x == N - 1 - k) v -= d * f;
-        }
-        rv[row * N + x] = v;
+    const int ne = 2 * napod;
+    for (int idx = tid; idx < kMaxBands * 2 * ne; idx += NT) {
+      const int row = idx / ne, kk = idx - (idx / ne) * ne;
+      const int x = kk < napod ? kk : N - ne + kk;
+      float v = rv[row * N + x];
+      const float d = dy[row];
+      for (int k = 0; k < napod; ++k) {
+        const float f = apofact[k];
+        if (x == k) v += d * f;
+        if (x == N - 1 - k) v -= d * f;
       }
+      rv[row * N + x] = v;  // each element is read and written by this thread only
     }
     __syncthreads();
   }
   // 3. phase separation (separate_kernel 267-291) into packed complex rows
-  for (int idx = tid; idx < kMaxBands * N; idx += NT) {
-    const int b = idx / N, x = idx - (idx / N) * N;
-    float a0 = 0.f, a1 = 0.f;
-    if (b < p.nbands) {
-      for (int ph = 0; ph < p.nphases; ++ph) {
-        const float m = p.sep[b * p.nphases + ph];
+  constexpr int MPT = (kMaxBands * N + NT - 1) / NT;
+#pragma unroll
+  for (int i = 0; i < MPT; ++i) {
+    const int idx = tid + i * NT;
+    if ((kMaxBands * N) % NT == 0 || idx < kMaxBands * N) {
+      const int b = idx / N, x = idx - (idx / N) * N;
+      float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+      for (int ph = 0; ph < kMaxBands; ++ph) {
+        const float m = p.sep[b * kMaxBands + ph];  // zero outside nbands x nphases
         a0 += m * rv[(ph * 2 + 0) * N + x];
         a1 += m * rv[(ph * 2 + 1) * N + x];
       }
+      cz[b * P + lds_swz<N>(x)] = make_float2(a0, a1);
     }
-    cz[b * P + x] = make_float2(a0, a1);
   }
   __syncthreads();
-  fft_lds<N, false>(cz + (tid / TPT) * P, tid % TPT, tw);
+  fft_lds<N, false>(cz + (tid / TPT) * P, tid % TPT, twr);
   // 4. split the packed transforms into the two rows' half spectra
-  const int nxh = N / 2 + 1;
+  constexpr int nxh = N / 2 + 1;
+  constexpr int UPT = (kMaxBands * nxh + NT - 1) / NT;
   const bool has1 = (y0 + 1) < ny;
-  for (int idx = tid; idx < p.nbands * nxh; idx += NT) {
+#pragma unroll
+  for (int i = 0; i < UPT; ++i) {
+    const int idx = tid + i * NT;
     const int b = idx / nxh, k = idx - (idx / nxh) * nxh;
-    const float2 Z = cz[b * P + k];
-    const float2 Zm = cz[b * P + ((N - k) % N)];
-    const float2 X0 = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
-    const float2 X1 = make_float2(0.5f * (Z.y + Zm.y), 0.5f * (Zm.x - Z.x));
-    const long long base = ((long long)(dir * p.nbands + b) * p.nz + z) * ny;
-    bands[(base + y0) * nxh + k] = X0;
-    if (has1) bands[(base + y0 + 1) * nxh + k] = X1;
+    if (b < p.nbands) {
+      const float2 Zv = cz[b * P + lds_swz<N>(k)];
+      const float2 Zm = cz[b * P + lds_swz<N>((N - k) % N)];
+      const float2 X0 = make_float2(0.5f * (Zv.x + Zm.x), 0.5f * (Zv.y - Zm.y));
+      const float2 X1 = make_float2(0.5f * (Zv.y + Zm.y), 0.5f * (Zm.x - Zv.x));
+      const size_t base = ((size_t)(dir * p.nbands + b) * p.nz + z) * ny;
+      bands[(base + y0) * nxh + k] = X0;
+      if (has1) bands[(base + y0 + 1) * nxh + k] = X1;
+    }
   }
 }
 

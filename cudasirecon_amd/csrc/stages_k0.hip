@@ -23,8 +23,10 @@
 namespace simr {
 
 // One element of overlap0 (which == 0) or overlap1 (which == 1) at column j,
-// row i, plane z0, before the inverse transform.
-__device__ float2 ov_value(const OvParams& p, int which, int j, int i, int z0) {
+// row i, plane z0, before the inverse transform.  prof1/prof2: radial OTF
+// profiles of order1/order2 at kz = z0.
+__device__ float2 ov_value(const OvParams& p, const float2* prof1, const float2* prof2, int which, int j, int i,
+                           int z0) {
 #pragma clang fp contract(off)
   const float2 zero = make_float2(0.f, 0.f);
   float x1f = (float)j, y1f = (float)i;
@@ -56,9 +58,9 @@ __device__ float2 ov_value(const OvParams& p, int which, int j, int i, int z0) {
   const long long indin = (long long)zin * p.nxh * p.ny + (long long)iin * p.nxh + jin;
   if (which == 0) {
     if (!(rdist12 <= rc)) return zero;
-    const float2 otf1 = otf_interp(p.otf1, x1f, y1f, z0, p.kzscale);
+    const float2 otf1 = profile_lookup(prof1, p.otf1.nxotf, p.otf1.dkrotf, x1f, y1f);
     if (!(sqrtf(otf1.x * otf1.x + otf1.y * otf1.y) > p.otfcutoff)) return zero;
-    const float2 otf12 = otf_interp(p.otf2, x12, y12, z0, p.kzscale);
+    const float2 otf12 = profile_lookup(prof2, p.otf2.nxotf, p.otf2.dkrotf, x12, y12);
     if (!(sqrtf(otf12.x * otf12.x + otf12.y * otf12.y) * p.order0_2_factor > p.otfcutoff)) return zero;
     float2 vre = p.b1re[indin];
     float2 vim = p.order1 > 0 ? p.b1im[indin] : zero;
@@ -73,9 +75,9 @@ __device__ float2 ov_value(const OvParams& p, int which, int j, int i, int z0) {
     return make_float2(vre.x - vim.y, vre.y + vim.x);
   } else {
     if (!(rdist21 <= rc)) return zero;
-    const float2 otf2 = otf_interp(p.otf2, x1f, y1f, z0, p.kzscale);
+    const float2 otf2 = profile_lookup(prof2, p.otf2.nxotf, p.otf2.dkrotf, x1f, y1f);
     if (!(sqrtf(otf2.x * otf2.x + otf2.y * otf2.y) * p.order0_2_factor > p.otfcutoff)) return zero;
-    const float2 otf21 = otf_interp(p.otf1, x21, y21, z0, p.kzscale);
+    const float2 otf21 = profile_lookup(prof1, p.otf1.nxotf, p.otf1.dkrotf, x21, y21);
     if (!(sqrtf(otf21.x * otf21.x + otf21.y * otf21.y) > p.otfcutoff)) return zero;
     float2 vre = p.b2re[indin];
     float2 vim = p.b2im[indin];
@@ -91,32 +93,57 @@ __device__ float2 ov_value(const OvParams& p, int which, int j, int i, int z0) {
   }
 }
 
-// x pass: row i (blockIdx.x) of plane zb (blockIdx.y) of both overlaps.
+// x pass: rows [kOvRows*blockIdx.x, +kOvRows) of plane zb (blockIdx.y) of both
+// overlaps; the plane's two radial OTF profiles are built once in LDS.
+constexpr int kOvRows = 4;
+
 template <int N>
-__global__ __launch_bounds__(2 * fft_tpt(N)) void k_overlaps_x(OvParams p, float2* __restrict__ ovx,
-                                                               const float2* __restrict__ tw) {
+__global__ __launch_bounds__(2 * kOvRows * fft_tpt(N)) void k_overlaps_x(OvParams p, float2* __restrict__ ovx,
+                                                                         const float2* __restrict__ tw) {
   constexpr int TPT = fft_tpt(N);
-  constexpr int NT = 2 * TPT;
+  constexpr int NT = 2 * kOvRows * TPT;
   constexpr int P = lds_pitch(N);
-  __shared__ float2 lds[2 * P];
-  const int i = blockIdx.x, zb = blockIdx.y;
+  __shared__ float2 lds[2 * kOvRows * P];
+  extern __shared__ float2 prof[];  // [2][nxotf]
+  FftTw<N> twr;
+  __shared__ float2 twstage[N > 1 ? N : 1];
+  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
+  const int zb = blockIdx.y;
   const int z0 = zb - p.zd;
   const int nzs = 2 * p.zd + 1;
-  for (int idx = threadIdx.x; idx < 2 * N; idx += NT) {
-    const int which = idx / N, j = idx - (idx / N) * N;
-    lds[which * P + j] = ov_value(p, which, j, i, z0);
+  const int nxotf = p.otf1.nxotf;
+  for (int k = threadIdx.x; k < 2 * nxotf; k += NT)
+    prof[k] = otf_profile_entry(k < nxotf ? p.otf1 : p.otf2, k < nxotf ? k : k - nxotf, z0, p.kzscale);
+  __syncthreads();
+  const int i0 = blockIdx.x * kOvRows;
+  constexpr int EPT = (2 * kOvRows * N + NT - 1) / NT;
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    if (idx < 2 * kOvRows * N) {
+      const int t = idx / N, j = idx - (idx / N) * N;  // t = which * kOvRows + row
+      const int which = t / kOvRows, i = i0 + (t - which * kOvRows);
+      lds[t * P + lds_swz<N>(j)] = i < p.ny ? ov_value(p, prof, prof + nxotf, which, j, i, z0) : make_float2(0.f, 0.f);
+    }
   }
   __syncthreads();
-  fft_lds<N, true>(lds + (threadIdx.x / TPT) * P, threadIdx.x % TPT, tw);
-  for (int idx = threadIdx.x; idx < 2 * N; idx += NT) {
-    const int which = idx / N, j = idx - (idx / N) * N;
-    ovx[(((long long)which * nzs + zb) * p.ny + i) * p.nx + j] = lds[which * P + j];
+  fft_lds<N, true>(lds + (threadIdx.x / TPT) * P, threadIdx.x % TPT, twr);
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    if (idx < 2 * kOvRows * N) {
+      const int t = idx / N, j = idx - (idx / N) * N;
+      const int which = t / kOvRows, i = i0 + (t - which * kOvRows);
+      if (i < p.ny) ovx[(((long long)which * nzs + zb) * p.ny + i) * p.nx + j] = lds[t * P + lds_swz<N>(j)];
+    }
   }
 }
 
 template <int N>
 static void launch_ov_x(const OvParams& p, float2* ovx, hipStream_t s) {
-  hipLaunchKernelGGL(k_overlaps_x<N>, dim3(p.ny, 2 * p.zd + 1), dim3(2 * fft_tpt(N)), 0, s, p, ovx, twiddles(N));
+  const size_t lds = (size_t)2 * p.otf1.nxotf * sizeof(float2);
+  hipLaunchKernelGGL(k_overlaps_x<N>, dim3(ceil_div(p.ny, kOvRows), 2 * p.zd + 1), dim3(2 * kOvRows * fft_tpt(N)),
+                     lds, s, p, ovx, twiddles(N));
   SIMR_LAUNCH_CHECK();
 }
 
@@ -158,6 +185,9 @@ __global__ __launch_bounds__(2 * zcols<N>() * fft_tpt(N)) void k_overlaps_z(OvPa
   constexpr int NT = 2 * WC * TPT;
   constexpr int LP = lds_pitch(N);
   __shared__ float2 lds[2 * WC * LP];
+  FftTw<N> twr;
+  __shared__ float2 twstage[N > 1 ? N : 1];
+  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
   const long long nxy = (long long)p.nx * p.ny;
   const long long e0 = (long long)blockIdx.x * WC;
   const int nzs = 2 * p.zd + 1;
@@ -174,11 +204,11 @@ __global__ __launch_bounds__(2 * zcols<N>() * fft_tpt(N)) void k_overlaps_z(OvPa
       z0 = z - N;
     else
       valid = false;
-    lds[(which * WC + w) * LP + z] =
+    lds[(which * WC + w) * LP + lds_swz<N>(z)] =
         valid ? ovx[((long long)which * nzs + (z0 + p.zd)) * nxy + e] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, tw);
+  fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
   for (int w = threadIdx.x; w < WC; w += NT) {
     const long long e = e0 + w;
     if (e >= nxy) continue;
@@ -187,7 +217,7 @@ __global__ __launch_bounds__(2 * zcols<N>() * fft_tpt(N)) void k_overlaps_z(OvPa
     if (FP32) {
       float sxr = 0.f, sxi = 0.f, sx = 0.f, sy = 0.f;
       for (int z = 0; z < N; ++z) {
-        const float2 a = X[z], b = Y[z];
+        const float2 a = X[lds_swz<N>(z)], b = Y[lds_swz<N>(z)];
         sxr += a.x * b.x + a.y * b.y;
         sxi += a.x * b.y - a.y * b.x;
         sx += a.x * a.x + a.y * a.y;
@@ -199,7 +229,8 @@ __global__ __launch_bounds__(2 * zcols<N>() * fft_tpt(N)) void k_overlaps_z(OvPa
     } else {
       double sxr = 0.0, sxi = 0.0, sx = 0.0, sy = 0.0;
       for (int z = 0; z < N; ++z) {
-        const double ar = X[z].x, ai = X[z].y, br = Y[z].x, bi = Y[z].y;
+        const float2 xa = X[lds_swz<N>(z)], yb = Y[lds_swz<N>(z)];
+        const double ar = xa.x, ai = xa.y, br = yb.x, bi = yb.y;
         sxr += ar * br + ai * bi;
         sxi += ar * bi - ai * br;
         sx += ar * ar + ai * ai;
@@ -292,13 +323,29 @@ __global__ __launch_bounds__(256) void k_argmax_partial(const float2* __restrict
   }
 }
 
-__global__ void k_argmax_final(const float2* __restrict__ cc, const float* __restrict__ pv,
-                               const int* __restrict__ pi, int nparts, int nx, int ny, float* __restrict__ vals,
-                               int* __restrict__ idx_out) {
-  if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_argmax_final(const float2* __restrict__ cc, const float* __restrict__ pv,
+                                                      const int* __restrict__ pi, int nparts, int nx, int ny,
+                                                      float* __restrict__ vals, int* __restrict__ idx_out) {
+  __shared__ float sv[256];
+  __shared__ int si[256];
   float v = -1e30f;
   int idx = 0x7fffffff;
-  for (int b = 0; b < nparts; ++b) better(v, idx, pv[b], pi[b]);
+  for (int b = threadIdx.x; b < nparts; b += 256) better(v, idx, pv[b], pi[b]);
+  sv[threadIdx.x] = v;
+  si[threadIdx.x] = idx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      float a = sv[threadIdx.x];
+      int ai = si[threadIdx.x];
+      better(a, ai, sv[threadIdx.x + s], si[threadIdx.x + s]);
+      sv[threadIdx.x] = a;
+      si[threadIdx.x] = ai;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  idx = si[0];
   const int xc = idx % nx, yc = idx / nx;
   auto I = [&](int y, int x) { return intensity(cc[(long long)y * nx + x]); };
   vals[0] = I(yc, xc);
@@ -318,7 +365,7 @@ void launch_findpeak(const double2* P, float2* cc, float* partial_val, int* part
   fft_cols(ny, cc, nx, nx, 1, 0, false, s);
   hipLaunchKernelGGL(k_argmax_partial, dim3(kArgBlocks), dim3(256), 0, s, cc, n, partial_val, partial_idx);
   SIMR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_argmax_final, dim3(1), dim3(64), 0, s, cc, partial_val, partial_idx, kArgBlocks, nx, ny,
+  hipLaunchKernelGGL(k_argmax_final, dim3(1), dim3(256), 0, s, cc, partial_val, partial_idx, kArgBlocks, nx, ny,
                      peak_vals, peak_idx);
   SIMR_LAUNCH_CHECK();
 }
