@@ -27,6 +27,12 @@ import numpy as np  # noqa: E402
 
 METRIC = "output voxels/s, 3D-SIM 512²×48×15 imgs; p50/p99 latency on mixed stream"
 HBM_PEAK_GBS = 8000.0
+# stage -> kernel name (templated on the transform length) as rocprofv3 reports it
+KERNEL_OF_STAGE = {"asm_x": "simr::k_asm_x<%d>", "asm_y": "simr::k_asm_y<%d>", "asm_z": "simr::k_asm_z<%d>",
+                   "rows_fwd": "simr::k_rows_fwd<%d,", "cols_fwd_y": "simr::k_fft_cols<%d,",
+                   "cols_fwd_z": "simr::k_fft_cols<%d,"}
+XDIM_OF_STAGE = {"asm_x": "xdim", "asm_y": "ydim", "asm_z": "zdim", "rows_fwd": "nx", "cols_fwd_y": "ny",
+                 "cols_fwd_z": "nz"}
 
 
 def stage_bytes(p, L):
@@ -56,26 +62,42 @@ def stage_bytes(p, L):
     }
 
 
-def cpu_baseline(args):
-    """The FP32 oracle on a bounded sample of the same workload (256x256 lateral crop)."""
+def cpu_baseline(p, tab, raw):
+    """The FP32 oracle on a bounded sample of the same workload: the same 512x512x48
+    stack, reconstructed once or more until about 10 s of CPU work is done."""
     threads = min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    from cudasirecon_amd import synth
     from oracle import binding as ob
 
-    p, tab, raw, _ = synth.make_dataset(256, 256, 48, seed=20263202)
     raw32 = raw.astype(np.float32)
-    t0 = time.perf_counter()
-    vol, _ = ob.reconstruct(p, tab, raw32, 32, False)
-    dt = time.perf_counter() - t0
+    runs, total, vox = 0, 0.0, 0
+    while runs < 3 and (runs == 0 or total < 10.0):
+        t0 = time.perf_counter()
+        vol, _ = ob.reconstruct(p, tab, raw32, 32, False)
+        total += time.perf_counter() - t0
+        vox += vol.size
+        runs += 1
     return {
-        "value": round(vol.size / dt, 1),
+        "value": round(vox / total, 1),
         "unit": "output voxels/s",
         "cores": threads,
         "kind": "port",
-        "sample": "FP32 CPU oracle, one 256x256x48 stack (3 dirs x 5 phases, same optics and "
-                  "parameters as the GPU workload) -> 512x512x48 output; %.1f s" % dt,
+        "sample": "FP32 CPU oracle (OpenMP, %d threads), the same 512x512x48 stack (3 dirs x 5 phases) "
+                  "-> 1024x1024x48, %d run(s) in %.1f s" % (threads, runs, total),
     }
+
+
+def profiled_traffic(kernel_prefix):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a plain copy); None when absent."""
+    path = os.path.join(ROOT, "profiles", "c2_plain_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_dispatch" and parts[1].startswith(kernel_prefix):
+            return int(parts[2])
+    return None
 
 
 def main():
@@ -85,6 +107,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--fit-fp32", action="store_true", help="FP32 k0/modamp reductions (reference precision)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a plain 1 GiB device-to-device copy kernel three times, the known-size "
+                         "reference that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is described in DESIGN.md "
@@ -93,12 +118,23 @@ def main():
 
     from cudasirecon_amd import engine, synth
 
+    if args.copy_probe:
+        import torch
+
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     print("bench: generating the synthetic 512x512x48 stack", file=sys.stderr, flush=True)
     p, tab, raw, truth = synth.make_dataset(512, 512, 48, seed=20262202)
     p.fit_fp64 = 0 if args.fit_fp32 else 1
     r = engine.Reconstructor(p, tab)
     r.set_raw(raw)  # resident in HBM from here on
-    L = dict(xdim=int(round(p.zoomfact * p.nx)), ydim=int(round(p.zoomfact * p.ny)), zdim=p.z_zoom * p.nz)
+    L = dict(xdim=int(round(p.zoomfact * p.nx)), ydim=int(round(p.zoomfact * p.ny)), zdim=p.z_zoom * p.nz,
+             nx=p.nx, ny=p.ny, nz=p.nz)
     # overlap support half-width (makeoverlaps), needed for the byte count
     import math
     lam = (p.wave / p.nimm) / 1e3
@@ -171,7 +207,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(KERNEL_OF_STAGE[dom] % L[XDIM_OF_STAGE.get(dom, "xdim")]),
             "launch_ms": round(avg_ms, 4),
             "algorithmic_bytes": int(sb[dom]),
         },
@@ -180,7 +216,7 @@ def main():
     }
     if not args.no_cpu:
         print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
-        res["cpu_baseline"] = cpu_baseline(args)
+        res["cpu_baseline"] = cpu_baseline(p, tab, raw)
     else:
         res["cpu_baseline"] = None
     print(json.dumps(res), flush=True)

@@ -33,6 +33,15 @@ int guard(F&& f) {
   } catch (const simr::StateError& e) {
     g_err = e.what();
     return SIMR_ERR_STATE;
+  } catch (const simr::IoError& e) {
+    g_err = e.what();
+    return SIMR_ERR_IO;
+  } catch (const simr::FormatError& e) {
+    g_err = e.what();
+    return SIMR_ERR_FORMAT;
+  } catch (const simr::UnsupportedError& e) {
+    g_err = e.what();
+    return SIMR_ERR_UNSUPPORTED;
   } catch (const std::bad_alloc&) {
     g_err = "host allocation failed";
     return SIMR_ERR_NOMEM;
@@ -129,6 +138,13 @@ int simr_destroy(simr_ctx* ctx) {
 
 int simr_get_params(const simr_ctx* ctx, simr_params* out) {
   return guard([&] { *out = E(const_cast<simr_ctx*>(ctx)).params(); });
+}
+
+int simr_update_params(simr_ctx* ctx, const simr_params* p) {
+  return guard([&] {
+    if (!p) throw simr::ArgError("null parameter block");
+    E(ctx).update_params(*p);
+  });
 }
 
 int simr_set_raw_f32_host(simr_ctx* ctx, const float* raw) { return guard([&] { E(ctx).set_raw(raw, false, false); }); }

@@ -7,20 +7,9 @@
 #include <stdexcept>
 #include <string>
 
-namespace simr {
+#include "errors.hpp"
 
-struct HipError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-struct ArgError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-struct StateError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-struct NoMemError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
+namespace simr {
 
 #define SIMR_HIP(call)                                                                   \
   do {                                                                                   \

@@ -183,6 +183,21 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   // OTF tables
   const size_t notf = (size_t)L.nDirsOTF * L.norders * L.otfsize;
   SIMR_HIP(hipMemcpyAsync(otf_, otf, notf * 8, hipMemcpyHostToDevice, stream_));
+  set_flatfield(nullptr, nullptr);
+  std::vector<double> zero(L.nz, 0.0);
+  SIMR_HIP(hipMemcpyAsync(sdp_, zero.data(), L.nz * 8, hipMemcpyHostToDevice, stream_));
+  k0_.assign(L.ndirs, make_float2(0, 0));
+  k0_time0_ = k0_;
+  derive();
+  amp_.assign(L.ndirs, std::vector<float2>(L.norders, make_float2(0, 0)));
+  for (int i = 0; i < L.ndirs; ++i) amp_[i][0] = make_float2(1, 0);
+  corr_.assign(L.ndirs, std::vector<float>(SIMR_MAX_ORDERS, 0.f));
+  stats_.assign(L.ndirs, simr_fit{});
+}
+
+// Host-side state derived from the parameters: separation matrix, inscale,
+// edge-blend factors and k0 guesses.
+void Engine::derive() {
   // separation matrix, equally spaced phases (makematrix, cudaSirecon.cpp:591-617)
   sep_.assign((size_t)L.nbands * L.nphases, 0.f);
   const float phi = (float)(2.0 * kPI / L.nphases);
@@ -211,15 +226,10 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
     apof_[l] = (float)(1.0 - (double)sinf((float)a));
   }
   SIMR_HIP(hipMemcpyAsync(apofact_, apof_.data(), 64 * 4, hipMemcpyHostToDevice, stream_));
-  set_flatfield(nullptr, nullptr);
-  std::vector<double> zero(L.nz, 0.0);
-  SIMR_HIP(hipMemcpyAsync(sdp_, zero.data(), L.nz * 8, hipMemcpyHostToDevice, stream_));
   SIMR_HIP(hipStreamSynchronize(stream_));
 
   // k0 guesses (setup_common, cudaSirecon.cpp:1233-1252)
-  k0_.assign(L.ndirs, make_float2(0, 0));
-  k0_time0_ = k0_;
-  k0guess_ = k0_;
+  k0guess_.assign(L.ndirs, make_float2(0, 0));
   const float delta_angle = (float)(kPI / L.ndirs);
   float k0magguess = (float)(1.0 / (double)P.linespacing);
   if (L.nz > 1) k0magguess = k0magguess / (float)(L.nphases / 2 + 1 - 1);
@@ -228,10 +238,22 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
     k0guess_[i] = make_float2((float)((double)k0magguess * std::cos((double)ang)),
                               (float)((double)k0magguess * std::sin((double)ang)));
   }
-  amp_.assign(L.ndirs, std::vector<float2>(L.norders, make_float2(0, 0)));
-  for (int i = 0; i < L.ndirs; ++i) amp_[i][0] = make_float2(1, 0);
-  corr_.assign(L.ndirs, std::vector<float>(SIMR_MAX_ORDERS, 0.f));
-  stats_.assign(L.ndirs, simr_fit{});
+  SIMR_HIP(hipStreamSynchronize(stream_));
+}
+
+// Replace the parameters of a live context.  Everything that sizes device
+// buffers or the OTF tables must stay the same; fit state (the time-0 k0 of
+// a series) is kept.
+void Engine::update_params(const simr_params& p) {
+  const Layout N = Layout::of(p);
+  SIMR_REQUIRE(N.nx == L.nx && N.ny == L.ny && N.nz == L.nz && N.ndirs == L.ndirs && N.nphases == L.nphases &&
+                   N.norders == L.norders && N.xdim == L.xdim && N.ydim == L.ydim && N.zdim == L.zdim &&
+                   N.nDirsOTF == L.nDirsOTF && N.otfsize == L.otfsize && N.zd_ov_max <= L.zd_ov_max,
+               "parameter change alters the context's buffer layout; create a new context");
+  sync();
+  P = p;
+  fit_fp32_ = P.fit_fp64 == 0;
+  derive();
 }
 
 Engine::~Engine() {

@@ -45,6 +45,7 @@ class Engine {
   Engine(const simr_params& p, const float* otf, int device);
   ~Engine();
 
+  void update_params(const simr_params& p);
   void set_raw(const void* src, bool u16, bool on_device);
   void set_flatfield(const float* bg, const float* slope);
   void load_and_rescale(int it);
@@ -74,6 +75,7 @@ class Engine {
     double xr, xi, sx, sy;
   };
 
+  void derive();
   Geo geometry() const;
   int zd_overlaps() const;
   OtfRA otf_table(int dir, int order) const;

@@ -112,6 +112,11 @@ size_t simr_footprint(const simr_params* p);
 int simr_create(const simr_params* p, const float* otf, int device, simr_ctx** out);
 int simr_destroy(simr_ctx* ctx);
 int simr_get_params(const simr_ctx* ctx, simr_params* out);
+/* Replace the parameters of a live context (wiener, k0 guesses, flags, ntimes,
+ * ...).  Sizes that lay out device memory (image and zoomed dimensions, ndirs,
+ * nphases, the OTF table geometry) must not change: SIMR_ERR_ARG otherwise.
+ * Fit state of a time series (the time-0 k0) is kept. */
+int simr_update_params(simr_ctx* ctx, const simr_params* p);
 
 /*
  * Raw input: nImages = nz * ndirs * nphases sections of nx*ny pixels, in the

@@ -43,3 +43,41 @@ def load_testdata():
 def synthetic(nx, ny, nz, nphases=5, seed=11, photons=5000.0, **kw):
     p, tab, raw, truth = synth.make_dataset(nx, ny, nz, nphases=nphases, seed=seed, photons=photons, **kw)
     return p, tab, raw, truth
+
+
+def tiff_frames(path):
+    """All pages of a TIFF as one array (Pillow's libtiff-backed reader)."""
+    from PIL import Image
+
+    im = Image.open(path)
+    out = []
+    for i in range(getattr(im, "n_frames", 1)):
+        im.seek(i)
+        out.append(np.array(im))
+    return np.stack(out)
+
+
+@functools.lru_cache(maxsize=None)
+def load_testdata_tiff():
+    """raw.tif / otf.tif with test_data/config-tiff, as the TIFF mode derives them.
+
+    dxy/dz/wave come from xyres/zres/wavelength (wavelength keeps its default
+    530); the RA OTF page is [kr][kz] complex with nzotf = width/2, nxotf =
+    height, dkzotf = 1/(zresPSF*nzotf), dkrotf = 1/(dxy*(nxotf-1)*2)
+    (src/cudaSirecon/cudaSirecon.cpp:1449-1470, 1540-1571).
+    """
+    raw = tiff_frames(os.path.join(GOLDEN, "raw.tif")).astype(np.float32)
+    otf_pages = tiff_frames(os.path.join(GOLDEN, "otf.tif")).astype(np.float32)
+    otf = np.ascontiguousarray(otf_pages).view(np.complex64)  # [order][kr][kz]
+    nzotf, nxotf = otf.shape[2], otf.shape[1]
+    ndirs, nphases = 3, 5
+    nz = raw.shape[0] // (ndirs * nphases)
+    dxy, dzpsf = np.float32(0.08), np.float32(0.125)
+    p = default_params()
+    set_params(p, nx=raw.shape[2], ny=raw.shape[1], nz=nz, dxy=0.08, dz=0.125, wave=530, ntimes=1,
+               nimm=1.515, constbkgd=0.0, wiener=0.001, k0angles=[0.8043, 1.8555, -0.2388], linespacing=0.2035,
+               ndirs=ndirs, nphases=nphases, na=1.42, bRadAvgOTF=1, bDampenOrder0=1, bFastSIM=1,
+               nzotf=nzotf, nxotf=nxotf, nyotf=1,
+               dkzotf=float(np.float32(1) / (dzpsf * np.float32(nzotf))),
+               dkrotf=float(np.float32(1) / (dxy * np.float32(nxotf - 1) * np.float32(2))))
+    return p, np.ascontiguousarray(otf[:3]), raw

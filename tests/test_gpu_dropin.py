@@ -1,0 +1,185 @@
+"""GPU parity of the drop-in boundary: the cudasirecon executable and the SR_* ABI.
+
+Runs the reference's own test_data invocations (test_data/run.sh: MRC mode
+with config, TIFF-folder mode with config-tiff) through the executable, and
+the in-memory SR_new_from_shape path through the ABI, and compares every
+result with the FP32 CPU oracle on the same inputs (output rel-L2 <= 1e-5).
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from cases import GOLDEN, load_testdata, load_testdata_tiff, rel_l2, tiff_frames
+from cudasirecon_amd import mrc, sr
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "cudasirecon_amd", "bin", "cudasirecon")
+TOL = 1e-5
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_exe(args, timeout=120):
+    log("run", " ".join(os.path.basename(a) for a in args))
+    r = subprocess.run([EXE] + args, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr
+    assert "!!Error occurred" not in r.stderr, r.stderr
+    return r
+
+
+def oracle_series(p, otf, raws, fit64=True):
+    o = ob.Oracle(p, otf, 32, fit64)
+    out = []
+    for t, raw in enumerate(raws):
+        o.load(raw)
+        o.rescale(t)
+        o.set_time(t)
+        o.process_volume()
+        out.append(o.result())
+    return out
+
+
+@pytest.fixture(scope="module")
+def oracle_mrc():
+    p, otf, raw = load_testdata()
+    return oracle_series(p, otf, [raw])[0]
+
+
+@pytest.fixture(scope="module")
+def oracle_tiff():
+    p, otf, raw = load_testdata_tiff()
+    return oracle_series(p, otf, [raw])[0]
+
+
+def test_cli_mrc_matches_oracle(tmp_path, oracle_mrc):
+    out = str(tmp_path / "proc.dv")
+    run_exe([os.path.join(GOLDEN, "raw.dv"), out, os.path.join(GOLDEN, "otf.dv"), "-c", os.path.join(GOLDEN, "config")])
+    h, vol = mrc.read(out)
+    hin = mrc.read_header(open(os.path.join(GOLDEN, "raw.dv"), "rb").read(1024))
+    # setOutputHeader (src/cudaSirecon/mrc.h:47-60)
+    assert (h["nx"], h["ny"], h["nz"], h["mode"], h["next"]) == (128, 128, 9, 2, 0)
+    assert h["xlen"] == pytest.approx(hin["xlen"] / 2) and h["ylen"] == pytest.approx(hin["ylen"] / 2)
+    assert h["zlen"] == pytest.approx(hin["zlen"]) and h["iwav1"] == hin["iwav1"]
+    err = rel_l2(vol, oracle_mrc)
+    log("cli mrc rel-L2 %.3e" % err)
+    assert err < TOL
+    # amin/amax of time point 0 with the reference's else-if scan (cudaSirecon.cpp:1671-1689)
+    mx, mn = -np.float32(np.finfo(np.float32).max), np.float32(np.finfo(np.float32).max)
+    flat = vol.reshape(-1)
+    for v in flat:
+        if v > mx:
+            mx = v
+        elif v < mn:
+            mn = v
+    assert h["amax"] == mx and h["amin"] == mn
+    assert h["amean"] == hin["amean"]
+
+
+def test_cli_tiff_folder_matches_oracle(tmp_path, oracle_tiff):
+    d = tmp_path / "data"
+    d.mkdir()
+    shutil.copy(os.path.join(GOLDEN, "raw.tif"), d / "raw.tif")
+    run_exe([str(d), "raw", os.path.join(GOLDEN, "otf.tif"), "-c", os.path.join(GOLDEN, "config-tiff")])
+    out = d / "GPUsirecon" / "raw_proc.tif"
+    assert out.exists()
+    vol = tiff_frames(str(out))
+    assert vol.shape == (9, 128, 128) and vol.dtype == np.float32
+    err = rel_l2(vol, oracle_tiff)
+    log("cli tiff rel-L2 %.3e" % err)
+    assert err < TOL
+
+
+def _shape_config(tmp_path, extra=""):
+    cfg = tmp_path / "config-mem"
+    cfg.write_text(open(os.path.join(GOLDEN, "config-tiff")).read() + "\notf-file=%s\n%s"
+                   % (os.path.join(GOLDEN, "otf.tif"), extra))
+    return str(cfg)
+
+
+def test_sr_in_memory_matches_oracle_and_cli(tmp_path, oracle_tiff):
+    p, otf, raw = load_testdata_tiff()
+    with sr.SIMReconstructor(shape=(64, 64, raw.shape[0]), config=_shape_config(tmp_path)) as r:
+        im = r.image_params
+        assert (im.nx, im.ny, im.nz, im.ntimes) == (64, 64, 9, 1)
+        assert r.recon_params.bTIFF == 1 and r.recon_params.nxotf == 129 and r.recon_params.nzotf == 65
+        vol = r.reconstruct(raw)
+    err = rel_l2(vol, oracle_tiff)
+    log("SR in-memory rel-L2 %.3e" % err)
+    assert err < TOL
+    # the CLI's TIFF mode reads the same float pages: identical result
+    d = tmp_path / "data"
+    d.mkdir()
+    shutil.copy(os.path.join(GOLDEN, "raw.tif"), d / "raw.tif")
+    run_exe([str(d), "raw", os.path.join(GOLDEN, "otf.tif"), "-c", os.path.join(GOLDEN, "config-tiff")])
+    np.testing.assert_array_equal(tiff_frames(str(d / "GPUsirecon" / "raw_proc.tif")), vol)
+
+
+def test_sr_parameter_change_takes_effect(tmp_path):
+    p, otf, raw = load_testdata_tiff()
+    with sr.SIMReconstructor(shape=(64, 64, raw.shape[0]), config=_shape_config(tmp_path)) as r:
+        first = r.reconstruct(raw)
+        r.recon_params.wiener = 0.01
+        r.recon_params.bDampenOrder0 = 0
+        second = r.reconstruct(raw)
+    q = p.copy()
+    q.wiener = 0.01
+    q.bDampenOrder0 = 0
+    ref = oracle_series(q, otf, [raw])[0]
+    err = rel_l2(second, ref)
+    log("SR after parameter change rel-L2 %.3e" % err)
+    assert err < TOL and rel_l2(first, second) > 1e-3
+
+
+def test_two_handles_in_one_process(tmp_path):
+    _, _, raw = load_testdata_tiff()
+    cfg = _shape_config(tmp_path)
+    a = sr.SIMReconstructor(shape=(64, 64, raw.shape[0]), config=cfg)
+    b = sr.SIMReconstructor(shape=(64, 64, raw.shape[0]), config=cfg)
+    a.set_raw(raw)
+    b.set_raw(raw * np.float32(0.5))
+    for h in (a, b):
+        h.load_and_rescale_image(0, 0)
+        h.set_cur_time_idx(0)
+        h.process_one_volume()
+    va, vb = a.get_result(), b.get_result()
+    a.close()
+    b.close()
+    np.testing.assert_array_equal(va, sr.SIMReconstructor(shape=(64, 64, raw.shape[0]), config=cfg).reconstruct(raw))
+    # input scale cancels in the Wiener-filtered, modamp-normalised result only up to
+    # the bleach/rescale path, so just check it is a distinct, finite volume
+    assert np.isfinite(vb).all() and rel_l2(vb, va) > 0
+
+
+def test_cli_mrc_time_series_matches_oracle(tmp_path):
+    """Two time points in one DV file: t>0 reuses the time-0 k0 and refits modamps."""
+    p, otf, raw = load_testdata()
+    h, data = mrc.read(os.path.join(GOLDEN, "raw.dv"))
+    rng = np.random.default_rng(9)
+    t1 = np.clip(data.astype(np.float32) * 0.9 + rng.normal(0, 2, data.shape), 0, 65535).astype(data.dtype)
+    hh = dict(h)
+    hh["nz"] = 2 * h["nz"]
+    hh["num_times"] = 2
+    hh["interleaved"] = 0
+    src = str(tmp_path / "series.dv")
+    mrc.write(src, np.concatenate([data, t1]), hh)
+    out = str(tmp_path / "series_proc.dv")
+    run_exe([src, out, os.path.join(GOLDEN, "otf.dv"), "-c", os.path.join(GOLDEN, "config")])
+    ho, vol = mrc.read(out)
+    assert ho["nz"] == 18 and vol.shape == (18, 128, 128)
+    q = p.copy()
+    q.ntimes = 2
+    refs = oracle_series(q, otf, [raw, t1.astype(np.float32)])
+    for t in range(2):
+        err = rel_l2(vol[9 * t: 9 * (t + 1)], refs[t])
+        log("time %d rel-L2 %.3e" % (t, err))
+        assert err < TOL
