@@ -70,6 +70,53 @@ class SimrFit(C.Structure):
         return [complex(self.amp_re[i], self.amp_im[i]) for i in range(norders)]
 
 
+class SimrOtfParams(C.Structure):
+    """simr_otf_params: makeotf options (src/otf/radialft.cpp:86-114, 1381-1553)."""
+
+    _fields_ = [
+        ("nphases", C.c_int32), ("k0angle", C.c_float), ("linespacing", C.c_float), ("bead_diameter", C.c_float),
+        ("na", C.c_float), ("nimm", C.c_float), ("wavelength", C.c_int32), ("dr", C.c_float), ("dz", C.c_float),
+        ("background", C.c_float), ("napodize", C.c_int32), ("do_compen", C.c_int32), ("five_bands", C.c_int32),
+        ("conjugate", C.c_int32), ("dorescale", C.c_int32), ("twolens", C.c_int32), ("coherent_bsim", C.c_int32),
+        ("forced_pi_shift", C.c_int32), ("ifixz", C.c_int32), ("ifixr", C.c_int32),
+        ("interpkr", C.c_int32 * 2), ("leavekz", C.c_int32 * 3), ("reserved", C.c_int32 * 8),
+    ]
+
+    def copy(self):
+        p = SimrOtfParams()
+        C.memmove(C.byref(p), C.byref(self), C.sizeof(SimrOtfParams))
+        return p
+
+
+class SimrOtfInfo(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("norders", C.c_int32),
+        ("nbands_out", C.c_int32), ("nkr", C.c_int32), ("dkr", C.c_float), ("dkz", C.c_float),
+        ("xc", C.c_float), ("yc", C.c_float), ("zc", C.c_float), ("phi", C.c_float * SIMR_MAX_ORDERS),
+        ("scalefactor", C.c_float), ("reserved", C.c_int32 * 8),
+    ]
+
+
+def otf_default_params():
+    """makeotf defaults (src/otf/radialft.cpp:86-114): restated for the Python side."""
+    p = SimrOtfParams()
+    p.nphases, p.k0angle, p.linespacing, p.bead_diameter = 5, 1.57, 0.2, 0.12
+    p.na, p.nimm, p.wavelength, p.dr, p.dz = 1.4, 1.515, 525, 0.106, 0.1
+    p.background, p.napodize, p.do_compen, p.ifixz, p.ifixr = -1.0, 10, 1, 1, 0
+    return p
+
+
+def set_otf_params(p, **kw):
+    for k, v in kw.items():
+        if k in ("interpkr", "leavekz"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = int(x)
+        else:
+            setattr(p, k, v)
+    return p
+
+
 def default_params():
     """Python restatement of simr_params_default() (the cudasirecon CLI defaults).
 

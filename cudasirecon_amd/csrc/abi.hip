@@ -9,6 +9,13 @@
 #include "engine.hpp"
 #include "fft_api.hpp"
 
+namespace simr {
+void otf_params_default(simr_otf_params* p);
+size_t otf_elems(const simr_otf_params& p, int nx, int ny, int nsections);
+void make_otf(const simr_otf_params& p, const float* psf, int nx, int ny, int nsec, const float* section_bg,
+              float* otf_out, float* band0_out, simr_otf_info* info);
+}  // namespace simr
+
 struct simr_ctx {
   simr::Engine* eng;
 };
@@ -138,6 +145,22 @@ int simr_destroy(simr_ctx* ctx) {
 
 int simr_get_params(const simr_ctx* ctx, simr_params* out) {
   return guard([&] { *out = E(const_cast<simr_ctx*>(ctx)).params(); });
+}
+
+void simr_otf_params_default(simr_otf_params* p) {
+  if (p) simr::otf_params_default(p);
+}
+
+size_t simr_otf_elems(const simr_otf_params* p, int nx, int ny, int nsections) {
+  return p ? simr::otf_elems(*p, nx, ny, nsections) : 0;
+}
+
+int simr_makeotf(const simr_otf_params* p, const float* psf, int nx, int ny, int nsections, const float* section_bg,
+                 float* otf, float* band0, simr_otf_info* info) {
+  return guard([&] {
+    if (!p) throw simr::ArgError("null makeotf parameter block");
+    simr::make_otf(*p, psf, nx, ny, nsections, section_bg, otf, band0, info);
+  });
 }
 
 int simr_update_params(simr_ctx* ctx, const simr_params* p) {

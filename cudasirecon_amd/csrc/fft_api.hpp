@@ -17,4 +17,13 @@ void fft_rows(int n, float2* data, long long pitch, long long nrows, bool invers
 // In-place 3D complex transform of a [nz][ny][nx] array.
 void fft3d(float2* data, int nx, int ny, int nz, bool inverse, hipStream_t s);
 
+// Direct DFT along one axis for any length (lengths the Stockham kernels do
+// not cover).  Line element n of (outer o, inner i) is at
+// o*outer_stride + n*n_stride + i, i < ninner, o < nouter; out-of-place.
+void dft_lines(const float2* in, float2* out, int n, long long n_stride, long long ninner, long long outer_stride,
+               long long nouter, bool inverse, hipStream_t s);
+// 3D transform of any size: Stockham passes where the length is supported,
+// direct DFT passes otherwise (scratch: nx*ny*nz elements).
+void fft3d_any(float2* data, float2* scratch, int nx, int ny, int nz, bool inverse, hipStream_t s);
+
 }  // namespace simr

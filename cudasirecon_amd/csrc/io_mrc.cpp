@@ -243,6 +243,12 @@ void Writer::write_f32(const float* sec) {
   if (std::fwrite(sec, sizeof(float), n, f_) != n) throw IoError("cannot write MRC section to " + path_);
 }
 
+void Writer::write_c64(const float* sec) {
+  if (!f_) throw StateError("MRC writer already closed: " + path_);
+  const size_t n = (size_t)h_.nx * h_.ny * 2;
+  if (std::fwrite(sec, sizeof(float), n, f_) != n) throw IoError("cannot write MRC section to " + path_);
+}
+
 void Writer::close() {
   if (!f_) return;
   unsigned char buf[1024];

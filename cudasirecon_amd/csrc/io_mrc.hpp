@@ -79,6 +79,7 @@ class Writer {
   ~Writer();
   Header& header() { return h_; }
   void write_f32(const float* sec);                   // appends one section
+  void write_c64(const float* sec);                   // appends one complex section (2*nx*ny floats)
   void close();                                       // rewrites the header and closes
 
  private:

@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from ._abi import SimrFit, SimrParams, norders_of, output_shape
+from ._abi import SimrFit, SimrOtfInfo, SimrOtfParams, SimrParams, norders_of, output_shape
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcudasirecon.so")
@@ -71,6 +71,10 @@ def lib():
         "simr_fft3d_dev": (ip, [vp, ip, ip, ip, ip, vp]),
         "simr_fft3d_host": (ip, [fp, ip, ip, ip, ip]),
         "simr_fft_size_supported": (ip, [ip]),
+        "simr_update_params": (ip, [vp, P]),
+        "simr_otf_params_default": (None, [C.POINTER(SimrOtfParams)]),
+        "simr_otf_elems": (C.c_size_t, [C.POINTER(SimrOtfParams), ip, ip, ip]),
+        "simr_makeotf": (ip, [C.POINTER(SimrOtfParams), fp, ip, ip, ip, fp, fp, fp, C.POINTER(SimrOtfInfo)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -98,6 +102,30 @@ def default_params():
     p = SimrParams()
     lib().simr_params_default(C.byref(p))
     return p
+
+
+def otf_default_params():
+    p = SimrOtfParams()
+    lib().simr_otf_params_default(C.byref(p))
+    return p
+
+
+def makeotf(params, psf, section_bg=None, want_band0=False):
+    """PSF sections [nz*nphases][ny][nx] (Z->Phase order) -> (otf [nbands][nkr][nz] complex64, info, band0)."""
+    psf = np.ascontiguousarray(psf, dtype=np.float32)
+    nsec, ny, nx = psf.shape
+    n = lib().simr_otf_elems(C.byref(params), nx, ny, nsec)
+    if n == 0:
+        raise SimrError(-1, "invalid makeotf geometry")
+    out = np.empty(2 * n, np.float32)
+    bg = None if section_bg is None else np.ascontiguousarray(section_bg, dtype=np.float32)
+    nz = nsec // params.nphases
+    band0 = np.empty((nz, ny, nx), np.float32) if want_band0 else None
+    info = SimrOtfInfo()
+    _check(lib().simr_makeotf(C.byref(params), _fp(psf), nx, ny, nsec, None if bg is None else _fp(bg), _fp(out),
+                              None if band0 is None else _fp(band0), C.byref(info)))
+    otf = out.view(np.complex64).reshape(info.nbands_out, info.nkr, info.nz)
+    return otf, info, band0
 
 
 def footprint(params):

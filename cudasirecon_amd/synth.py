@@ -198,3 +198,36 @@ def make_dataset(nx, ny, nz, ndirs=3, nphases=5, seed=20261202, dxy=0.08, dz=0.1
         set_params(p, **{k: v})
     truth = dict(k0=k0, modamps=ma, sample=sample)
     return p, tab, raw, truth
+
+
+def make_psf_stack(nx, ny, nz, nphases=5, seed=0, sigma_xy=1.5, sigma_z=2.0, center=None, kz_cycles=0.08,
+                   modamps=(0.8, 0.4), phase0=0.3, background=100.0, peak=10000.0, noise=True):
+    """Bead PSF stack for makeotf: float32 sections [nz*nphases][ny][nx] in Z->Phase order.
+
+    A point bead at `center` (pixels, sub-pixel offsets allowed) under a
+    3-beam pattern: the detection PSF is an anisotropic Gaussian; phase j of
+    order o multiplies it by cos(o*(phase0 + 2*pi*j/nphases)), and order 1
+    also carries the axial modulation cos(2*pi*kz_cycles*(z - zc)).
+    """
+    rng = np.random.default_rng(seed)
+    if center is None:
+        center = (nx / 2 + 0.3, ny / 2 - 0.4, nz / 2 + 0.2)
+    xc, yc, zc = center
+    x = np.arange(nx)[None, None, :]
+    y = np.arange(ny)[None, :, None]
+    z = np.arange(nz)[:, None, None]
+    h = np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / (2 * sigma_xy ** 2) - (z - zc) ** 2 / (2 * sigma_z ** 2))
+    axial = np.cos(2 * np.pi * kz_cycles * (z - zc))
+    norders = nphases // 2 + 1
+    out = np.empty((nz, nphases, ny, nx), np.float64)
+    for j in range(nphases):
+        ph = phase0 + 2 * np.pi * j / nphases
+        pat = np.ones((nz, 1, 1))
+        if norders > 1:
+            pat = pat + modamps[0] * axial * np.cos(ph)
+        if norders > 2:
+            pat = pat + modamps[1] * np.cos(2 * ph)
+        out[:, j] = background + peak * h * pat
+    if noise:
+        out = rng.poisson(np.clip(out, 0, None)).astype(np.float64)
+    return out.reshape(nz * nphases, ny, nx).astype(np.float32)

@@ -169,6 +169,60 @@ int simr_fft3d_dev(float* data, int nx, int ny, int nz, int inverse, void* strea
 int simr_fft3d_host(float* data, int nx, int ny, int nz, int inverse);
 int simr_fft_size_supported(int n);
 
+/*
+ * makeotf: PSF -> rotationally averaged OTF (replaces the reference's
+ * makeotf program, src/otf/radialft.cpp:82-434, whose FFT is FFTW on the
+ * CPU).  Field meanings and defaults follow its command line
+ * (radialft.cpp:86-114, 1381-1553).
+ */
+typedef struct simr_otf_params {
+  int32_t nphases;         /* 5; nphases == 1 gives a widefield OTF */
+  float k0angle;           /* 1.57 rad (-angle) */
+  float linespacing;       /* 0.2 um (-ls) */
+  float bead_diameter;     /* 0.12 um (-beaddiam) */
+  float na, nimm;          /* 1.4, 1.515 */
+  int32_t wavelength;      /* emission, nm (525; MRC input: header iwav1) */
+  float dr, dz;            /* pixel sizes, um (0.106, 0.1; MRC input: header ylen, zlen) */
+  float background;        /* < 0: estimate each section from its 20-pixel border (TIFF input) */
+  int32_t napodize;        /* 10 edge-blend rows; -1: cosine apodization */
+  int32_t do_compen;       /* 1: bead-size compensation (-nocompen sets 0) */
+  int32_t five_bands;      /* 0: combine each order's re/im bands into one (-5bands sets 1) */
+  int32_t conjugate;       /* -conj */
+  int32_t dorescale;       /* -rescale: normalise every order by its own origin value */
+  int32_t twolens;         /* -2lenses */
+  int32_t coherent_bsim;   /* -bessel: skip the out-of-band cleanup */
+  int32_t forced_pi_shift; /* -PIshift */
+  int32_t ifixz, ifixr;    /* 1, 0 (-ifixkz, -ifixkr) */
+  int32_t interpkr[2];     /* -fixorigin kx1 kx2 (0 0: off) */
+  int32_t leavekz[3];      /* -leavekz a b c (0 0 0: no cleanup) */
+  int32_t reserved[8];
+} simr_otf_params;
+
+typedef struct simr_otf_info {
+  int32_t nx, ny, nz;      /* PSF geometry (nz = sections / nphases) */
+  int32_t norders;         /* (nphases + 1) / 2 */
+  int32_t nbands_out;      /* norders, or 2*norders-1 with five_bands */
+  int32_t nkr;             /* min(nx, ny)/2 + 1 radial samples */
+  float dkr, dkz;          /* OTF pixel sizes, 1/um (1/(nx*dr), 1/(nz*dz)) */
+  float xc, yc, zc;        /* fitted bead centre, pixels */
+  float phi[SIMR_MAX_ORDERS]; /* side-band phase removed by the re/im combination */
+  float scalefactor;       /* 1 / Re(order-0 OTF at the origin) */
+  int32_t reserved[8];
+} simr_otf_info;
+
+void simr_otf_params_default(simr_otf_params* p);
+/* Output size in complex values: nbands_out * nkr * nz. */
+size_t simr_otf_elems(const simr_otf_params* p, int nx, int ny, int nsections);
+/*
+ * psf: nsections = nz * nphases float sections of nx*ny, in Z->Phase order.
+ * section_bg: per-section backgrounds (NULL: p->background, or the border
+ * estimate when it is negative).  otf: complex float [nbands_out][nkr][nz]
+ * (the [kr][kz] layout cudasirecon reads).  band0: optional, receives the
+ * separated order-0 band in real space, float [nz][ny][nx] (-gen_order0).
+ */
+int simr_makeotf(const simr_otf_params* p, const float* psf, int nx, int ny, int nsections, const float* section_bg,
+                 float* otf, float* band0, simr_otf_info* info);
+
 /* Synchronise the context stream.  Returns the HIP stream as void*. */
 int simr_sync(simr_ctx* ctx);
 void* simr_stream(simr_ctx* ctx);

@@ -8,7 +8,7 @@ import subprocess
 
 import numpy as np
 
-from cudasirecon_amd._abi import SimrFit, SimrParams, norders_of, output_shape
+from cudasirecon_amd._abi import SimrFit, SimrOtfInfo, SimrOtfParams, SimrParams, norders_of, output_shape
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
@@ -53,6 +53,9 @@ def lib():
         L.orc_result_elems.restype = C.c_longlong
         L.orc_result_elems.argtypes = [vp]
         L.orc_last_error.restype = C.c_char_p
+        L.orc_makeotf.restype = C.c_int
+        L.orc_makeotf.argtypes = [C.POINTER(SimrOtfParams), fp, ip, ip, ip, fp, fp, fp, C.POINTER(SimrOtfInfo)]
+        L.orc_makeotf_error.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -211,6 +214,25 @@ class Oracle:
             res["ov0"] = ov0.view(np.complex64).reshape(vs)
             res["ov1"] = ov1.view(np.complex64).reshape(vs)
         return res
+
+
+def makeotf(params, psf, section_bg=None, want_band0=False):
+    """Oracle makeotf (src/otf/radialft.cpp restated); same contract as engine.makeotf."""
+    psf = np.ascontiguousarray(psf, dtype=np.float32)
+    nsec, ny, nx = psf.shape
+    nz = nsec // params.nphases
+    norders = (params.nphases + 1) // 2
+    nbo = 2 * norders - 1 if params.five_bands else norders
+    nkr = min(nx, ny) // 2 + 1
+    out = np.empty(2 * nbo * nkr * nz, np.float32)
+    bg = None if section_bg is None else np.ascontiguousarray(section_bg, dtype=np.float32)
+    band0 = np.empty((nz, ny, nx), np.float32) if want_band0 else None
+    info = SimrOtfInfo()
+    rc = lib().orc_makeotf(C.byref(params), _fp(psf), nx, ny, nsec, None if bg is None else _fp(bg), _fp(out),
+                           None if band0 is None else _fp(band0), C.byref(info))
+    if rc != 0:
+        raise RuntimeError(lib().orc_makeotf_error().decode())
+    return out.view(np.complex64).reshape(nbo, nkr, nz), info, band0
 
 
 def reconstruct(params, otf, raw, precision=32, fit64=False):
