@@ -880,6 +880,23 @@ FilterParams Engine::filter_params(int dir, int order) const {
   }
   f.conjamp = make_float2(amp_[dir][order].x, -amp_[dir][order].y);
   for (int o = 0; o < L.norders; ++o) f.otf[o] = otf_table(dir, o);
+  f.unit_nv = 1;
+  for (int d = 0; d < L.ndirs; ++d) {
+    for (int o2 = -(L.norders - 1); o2 < L.norders; ++o2)
+      f.kshift[d * (2 * kMaxOrders - 1) + o2 + (kMaxOrders - 1)] =
+          make_float2((float)o2 * k0_[d].x, (float)o2 * k0_[d].y);
+    for (int o = 0; o < L.norders; ++o) {
+      const float nv = f.noiseVar[d * L.norders + o];
+      f.inv_nv[d * L.norders + o] = (float)(1. / (double)nv);
+      if (nv != 1.0f) f.unit_nv = 0;
+    }
+  }
+  f.cut2 = (float)((double)rdistcutoff * (double)rdistcutoff);
+  f.inv_dkr2 = (float)(1.0 / ((double)f.otf[0].dkrotf * (double)f.otf[0].dkrotf));
+  f.inv_rcut2 = (float)(1.0 / ((double)rdistcutoff * (double)rdistcutoff));
+  f.supp2 = (float)((double)f.suppRadius * (double)f.suppRadius);
+  f.inv_nxh = 1.0f / (float)L.nxh;
+  SIMR_REQUIRE((long long)L.nxh * L.ny < (1LL << 24), "filter: plane too large");
   float2* db = bands_ + (size_t)dir * L.nbands * L.H;
   f.bp = order == 0 ? db : db + (size_t)(2 * order - 1) * L.H;
   f.bp2 = order == 0 ? nullptr : db + (size_t)(2 * order) * L.H;
@@ -918,7 +935,7 @@ void Engine::assemble() {
   launch_asm_y(a, stream_);
   stage_end();
   stage_begin("asm_x");
-  for (int a0 = 0; a0 < L.nasm; a0 += 8) launch_asm_x(a, a0, std::min(L.nasm, a0 + 8), a0 > 0, stream_);
+  launch_asm_x(a, stream_);
   stage_end();
 }
 

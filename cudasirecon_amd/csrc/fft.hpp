@@ -343,6 +343,59 @@ __device__ __forceinline__ void fft_stage(float2* buf, int t, const FftTw<N, TPT
   }
 }
 
+// Same stage with the twiddles read from an N-entry table (LDS or global)
+// at use instead of per-thread registers: for kernels whose register budget
+// is needed elsewhere.
+template <int N, bool INV, int TPT, int S>
+__device__ __forceinline__ void fft_stage_t(float2* buf, int t, const float2* __restrict__ tab) {
+  constexpr Radices RD = make_radices(N);
+  if constexpr (S < RD.n) {
+    constexpr int R = RD.r[S];
+    constexpr int Ns = radix_prod_before(RD, S);
+    constexpr int M = N / R;
+    constexpr int B = (M + TPT - 1) / TPT;
+    constexpr bool FULL = (M % TPT) == 0;
+    float2 v[B][R];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + b * TPT;
+      if (FULL || j < M) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = buf[lds_swz<N>(j + r * M)];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = t + b * TPT;
+      if (FULL || j < M) {
+        const int k = j % Ns;
+        if constexpr (Ns > 1) {
+          constexpr int step = N / (Ns * R);
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            float2 ww = tab[r * k * step];
+            if (INV) ww.y = -ww.y;
+            v[b][r] = cmul(v[b][r], ww);
+          }
+        }
+        dft<R, INV>(v[b]);
+        const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) buf[lds_swz<N>(idxD + q * Ns)] = v[b][q];
+      }
+    }
+    __syncthreads();
+    fft_stage_t<N, INV, TPT, S + 1>(buf, t, tab);
+  }
+}
+
+template <int N, bool INV, int TPT>
+__device__ __forceinline__ void fft_lds_t(float2* buf, int t, const float2* __restrict__ tab) {
+  static_assert(fft_supported(N), "FFT length must factor into 2, 3, 5");
+  if constexpr (N > 1) fft_stage_t<N, INV, TPT, 0>(buf, t, tab);
+}
+
 // In-place FFT of one transform held in LDS (element i at buf[lds_swz<N>(i)])
 // by its TPT threads t = 0..TPT-1.  Every thread of the workgroup must call it
 // (it contains barriers).

@@ -93,6 +93,15 @@ struct FilterParams {
   float2 conjamp;  // conj(amp[dir][order])
   OtfRA otf[kMaxOrders];
   float2 *bp, *bp2;
+  // host-derived constants (same float/double expressions as the per-thread ones)
+  float2 kshift[kMaxDirs * (2 * kMaxOrders - 1)];  // (float)order2 * k0[dir2], order2 = -(norders-1)..norders-1
+  float inv_nv[kMaxDirs * kMaxOrders];              // (float)(1.0 / (double)noiseVar)
+  int unit_nv;                                      // every noiseVar == 1: the divisions are exact no-ops
+  float cut2;       // rdistcutoff^2
+  float inv_dkr2;   // 1 / dkrotf^2
+  float inv_rcut2;  // 1 / rdistcutoff^2
+  float supp2;      // suppRadius^2
+  float inv_nxh;    // 1 / nxh
 };
 void launch_filter(const FilterParams& p, hipStream_t s);
 // Fill n float2 with zero.
@@ -115,7 +124,7 @@ struct AsmParams {
 };
 void launch_asm_z(const AsmParams& p, hipStream_t s);
 void launch_asm_y(const AsmParams& p, hipStream_t s);
-// bands [a0, a1) of stageY; accumulate=false overwrites out.
-void launch_asm_x(const AsmParams& p, int a0, int a1, bool accumulate, hipStream_t s);
+// all assembly bands of stageY -> out (overwritten)
+void launch_asm_x(const AsmParams& p, hipStream_t s);
 
 }  // namespace simr

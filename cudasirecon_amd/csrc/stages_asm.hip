@@ -151,100 +151,128 @@ __global__ __launch_bounds__(asm_y_w<N>() * fft_tpt(N)) void k_asm_y(AsmParams p
   }
 }
 
-// ---- x pass: row Y (blockIdx.x) of planes [kAsmZc*blockIdx.y, +kAsmZc), bands
-// [a0, a1) (<= 8 per launch).  The modulation phasors of the row are computed
-// once per workgroup and kept in registers across its planes.
-constexpr int kAsmSlots = 8;
-constexpr int kAsmZc = 4;
+// ---- x pass: row Y (blockIdx.x) of every output plane.
+// A 256-thread workgroup transforms G bands of the row at a time (TPT
+// threads each) and walks all bands, so every output element is accumulated
+// in registers and stored once.  The modulation phasors of the row (E per
+// thread for bands 1..kAsmLutBands) are computed once per workgroup and kept
+// in registers for all planes; the FFT twiddles are read from an LDS table.
+// Several workgroups share a CU, so the row loads of one overlap the
+// transforms of another.
+constexpr int kAsmXThreads = 512;
+constexpr int kAsmLutBands = 6;  // 1 + ndirs*(norders-1) <= 7 keeps every phasor in registers
 
 template <int N>
 constexpr int asm_x_tpt() {
-  return fft_tpt(N) < 128 ? fft_tpt(N) : 128;
+  return fft_tpt(N) >= 128 ? 128 : fft_tpt(N);
+}
+
+// modulation phasor of assembly band a (>= 1) at output column X of row Y
+// (cos_sin_kernel, gpuFunctionsImpl.cu:1942-1954: float angle, then sin/cos)
+template <int N>
+__device__ __forceinline__ float2 asm_phasor(const AsmParams& p, int a, int X, int Y) {
+#pragma clang fp contract(off)
+  const int no = p.norders - 1;
+  const int dir = (a - 1) / no, order = 1 + (a - 1) % no;
+  const float k0x = p.k0[dir].x * (float)order, k0y = p.k0[dir].y * (float)order;
+  const float s1 = (float)(X - N / 2) * k0x;
+  const float s2 = (float)(Y - p.ydim / 2) * k0y;
+  const float sum = s1 + s2;
+  const float angle = (float)((double)p.fact * 3.14159265358979323846 * (double)sum * (double)p.dxyo);
+  float sn, cs;
+  sincosf(angle, &sn, &cs);
+  return make_float2(cs, sn);
 }
 
 template <int N>
-__global__ __launch_bounds__(kAsmSlots* asm_x_tpt<N>()) void k_asm_x(AsmParams p, int a0, int a1, int accumulate,
-                                                                     const float2* __restrict__ tw) {
+__global__ __launch_bounds__(kAsmXThreads, 4) void k_asm_x(AsmParams p, const float2* __restrict__ tw) {
 #pragma clang fp contract(off)
   constexpr int TPT = asm_x_tpt<N>();
-  constexpr int NT = kAsmSlots * TPT;
+  constexpr int G = kAsmXThreads / TPT;
   constexpr int LP = lds_pitch(N);
-  constexpr int E = (N + NT - 1) / NT;
-  __shared__ float2 lds[kAsmSlots * LP];
-  FftTw<N, TPT> twr;
-  __shared__ float2 twstage[N > 1 ? N : 1];
-  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
+  constexpr int E = (N + kAsmXThreads - 1) / kAsmXThreads;
+  constexpr bool LUT = E <= 4;
+  __shared__ float2 lds[G * LP];
+  __shared__ float2 twt[N];
+  for (int i = threadIdx.x; i < N; i += kAsmXThreads) twt[i] = tw[i];
   const int Y = blockIdx.x;
-  const int nb = a1 - a0;
-  // modulation phasors of the sideband slots for this row (cos_sin_kernel 1942-1954)
-  float2 lut[E][kAsmSlots];
+  const int g = threadIdx.x / TPT, t = threadIdx.x % TPT;
+  const int half = p.nx / 2;
+  float2 lut[LUT ? E : 1][LUT ? kAsmLutBands : 1];
+  if constexpr (LUT) {
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int X = threadIdx.x + e * NT;
+    for (int e = 0; e < E; ++e)
 #pragma unroll
-    for (int sl = 0; sl < kAsmSlots; ++sl) {
-      lut[e][sl] = make_float2(1.f, 0.f);
-      const int a = a0 + sl;
-      if (X < N && sl < nb && a > 0) {
-        const int no = p.norders - 1;
-        const int dir = (a - 1) / no, order = 1 + (a - 1) % no;
-        const float k0x = p.k0[dir].x * (float)order, k0y = p.k0[dir].y * (float)order;
-        const float s1 = (float)(X - N / 2) * k0x;
-        const float s2 = (float)(Y - p.ydim / 2) * k0y;
-        const float sum = s1 + s2;
-        const float angle = (float)((double)p.fact * 3.14159265358979323846 * (double)sum * (double)p.dxyo);
-        float sn, cs;
-        sincosf(angle, &sn, &cs);
-        lut[e][sl] = make_float2(cs, sn);
+      for (int b = 0; b < kAsmLutBands; ++b) {
+        const int X = threadIdx.x + e * kAsmXThreads;
+        lut[e][b] = (X < N && b + 1 < p.nasm) ? asm_phasor<N>(p, b + 1, X, Y) : make_float2(1.f, 0.f);
       }
-    }
   }
-  const int zbeg = blockIdx.y * kAsmZc;
-  const int zend = min(p.zdim, zbeg + kAsmZc);
-  for (int zout = zbeg; zout < zend; ++zout) {
-    for (int idx = threadIdx.x; idx < kAsmSlots * N; idx += NT) {
-      const int sl = idx / N, X = idx - (idx / N) * N;
-      float2 v = make_float2(0.f, 0.f);
-      if (sl < nb) {
-        int x;
-        bool valid = true;
-        if (X <= p.nx - p.nx / 2)
-          x = X;
-        else if (X >= N - (p.nx / 2 - 1))
-          x = X - N;
-        else {
-          x = 0;
-          valid = false;
+  __syncthreads();
+  for (int zout = 0; zout < p.zdim; ++zout) {
+    float acc[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[e] = 0.f;
+#pragma nounroll
+    for (int a0 = 0; a0 < p.nasm; a0 += G) {
+      // G bands of row Y of plane zout, zero-padded to N, into LDS
+      for (int idx = threadIdx.x; idx < G * N; idx += kAsmXThreads) {
+        const int sl = idx / N, X = idx - sl * N;
+        const int a = a0 + sl;
+        float2 v = make_float2(0.f, 0.f);
+        if (a < p.nasm) {
+          int x;
+          bool valid = true;
+          if (X <= p.nx - half)
+            x = X;
+          else if (X >= N - (half - 1))
+            x = X - N;
+          else {
+            x = 0;
+            valid = false;
+          }
+          if (valid) v = p.stageY[(((long long)a * p.zdim + zout) * p.ydim + Y) * p.nx + (x + (half - 1))];
         }
-        if (valid)
-          v = p.stageY[(((long long)(a0 + sl) * p.zdim + zout) * p.ydim + Y) * p.nx + (x + (p.nx / 2 - 1))];
+        lds[sl * LP + lds_swz<N>(X)] = v;
       }
-      lds[sl * LP + lds_swz<N>(X)] = v;
-    }
-    __syncthreads();
-    fft_lds<N, true, TPT>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
-    float* orow = p.out + ((long long)zout * p.ydim + Y) * N;
+      __syncthreads();
+      fft_lds_t<N, true, TPT>(lds + g * LP, t, twt);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int X = threadIdx.x + e * NT;
-      if (X < N) {
-        float acc = accumulate ? orow[X] : 0.f;
+      for (int e = 0; e < E; ++e) {
+        const int X = threadIdx.x + e * kAsmXThreads;
+        if (X < N) {
 #pragma unroll
-        for (int sl = 0; sl < kAsmSlots; ++sl) {
-          if (sl < nb) {
-            const float2 v = lds[sl * LP + lds_swz<N>(X)];
-            if (a0 + sl == 0) {
-              acc += v.x;  // write_outbuffer_kernel1
-            } else {       // write_outbuffer_kernel2: double intermediate, as in the reference
-              const double d = (double)v.x * 2.0 * (double)lut[e][sl].x - (double)v.y * 2.0 * (double)lut[e][sl].y;
-              acc = (float)((double)acc + d);
+          for (int sl = 0; sl < G; ++sl) {
+            const int a = a0 + sl;
+            if (a < p.nasm) {
+              const float2 v = lds[sl * LP + lds_swz<N>(X)];
+              if (a == 0) {
+                acc[e] += v.x;  // write_outbuffer_kernel1
+              } else {          // write_outbuffer_kernel2: double intermediate, as in the reference
+                float2 ph;
+                if (LUT && a <= kAsmLutBands) {
+                  ph = lut[LUT ? e : 0][0];
+#pragma unroll
+                  for (int b = 1; b < kAsmLutBands; ++b)
+                    if (a - 1 == b) ph = lut[LUT ? e : 0][LUT ? b : 0];
+                } else {
+                  ph = asm_phasor<N>(p, a, X, Y);
+                }
+                const double d = (double)v.x * 2.0 * (double)ph.x - (double)v.y * 2.0 * (double)ph.y;
+                acc[e] = (float)((double)acc[e] + d);
+              }
             }
           }
         }
-        orow[X] = acc;
       }
+      __syncthreads();
     }
-    __syncthreads();
+    float* orow = p.out + ((long long)zout * p.ydim + Y) * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int X = threadIdx.x + e * kAsmXThreads;
+      if (X < N) orow[X] = acc[e];
+    }
   }
 }
 
@@ -282,15 +310,13 @@ void launch_asm_y(const AsmParams& p, hipStream_t s) {
   }
 }
 
-void launch_asm_x(const AsmParams& p, int a0, int a1, bool accumulate, hipStream_t s) {
-  if (a1 - a0 > kAsmSlots) throw ArgError("launch_asm_x: at most 8 bands per launch");
+void launch_asm_x(const AsmParams& p, hipStream_t s) {
   switch (p.xdim) {
-#define SIMR_CASE(N)                                                                                  \
-  case N:                                                                                             \
-    hipLaunchKernelGGL(k_asm_x<N>, dim3(p.ydim, ceil_div(p.zdim, kAsmZc)), dim3(kAsmSlots * asm_x_tpt<N>()),     \
-                       0, s, p, a0, a1,                                                               \
-                       (int)accumulate, twiddles(N));                                                 \
-    SIMR_LAUNCH_CHECK();                                                                              \
+#define SIMR_CASE(N)                                                                                       \
+  case N:                                                                                                  \
+    hipLaunchKernelGGL(k_asm_x<N>, dim3(p.ydim), dim3(kAsmXThreads), 0, s, p,                          \
+                       twiddles(N));                                                                       \
+    SIMR_LAUNCH_CHECK();                                                                                   \
     return;
     SIMR_ZOOM_SIZES(SIMR_CASE)
 #undef SIMR_CASE

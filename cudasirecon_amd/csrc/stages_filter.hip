@@ -31,9 +31,37 @@ __device__ __forceinline__ float dev_order0damping(float radius, float zindex, f
   return rf * rf + zf * zf * zf;
 }
 
+// OTF profile value at radius index krindex, bilinear as profile_lookup.
+__device__ __forceinline__ float2 prof_idx(const float2* prof, int nxotf, float krindex) {
+#pragma clang fp contract(off)
+  int irindex = (int)floorf(krindex);
+  float ar = krindex - (float)irindex;
+  if (irindex > nxotf - 2) {
+    irindex = nxotf - 2;
+    ar = 1.f;
+  }
+  if (irindex < 0) {
+    irindex = 0;
+    ar = 0.f;
+  }
+  const float2 t0 = prof[irindex], t1 = prof[irindex + 1];
+  return make_float2((1 - ar) * t0.x + ar * t1.x, (1 - ar) * t0.y + ar * t1.y);
+}
+
+// OTF profile value at radial distance rdist (= sqrtf(kx*kx + ky*ky), the
+// operand dev_otfinterpolate recomputes), as dev_otfinterpolate: rdist/dkr.
+__device__ __forceinline__ float2 prof_at(const float2* prof, int nxotf, float dkr, float rdist) {
+#pragma clang fp contract(off)
+  return prof_idx(prof, nxotf, rdist / dkr);
+}
+
 // One visit of filterbands_kernel1 at band coordinates (x1, y1, z0) whose
 // rdist1 <= rdistcutoff; updates (re, im) of the stored element.
 // prof: this plane's radial OTF profiles at kz = z0, [order][nxotf].
+// The arithmetic is the reference's; what is hoisted out of the (dir2,
+// order2) loop (the z terms, 1/noiseVar, order2*k0) is computed by the same
+// expressions once, and terms whose squared distance already exceeds the
+// cut-off are skipped before the square root.
 __device__ void filter_visit(const FilterParams& p, const float2* prof, int x1, int y1, int z0, bool conj, float2& re,
                              float2& im) {
 #pragma clang fp contract(off)
@@ -43,56 +71,70 @@ __device__ void filter_visit(const FilterParams& p, const float2* prof, int x1, 
   const float x1f = (float)x1 * p.dkx, y1f = (float)y1 * p.dky;
   const float rdist1 = sqrtf(x1f * x1f + y1f * y1f);
   const float xabs = x1f + kx, yabs = y1f + ky;
-  const float rdistabs = sqrtf(xabs * xabs + yabs * yabs);
   const int nxotf = p.otf[0].nxotf;
   const float dkr = p.otf[0].dkrotf;
-  const float2 otf1 = profile_lookup(prof + order * nxotf, nxotf, dkr, x1f, y1f);
+  // dev_order0damping's z term: fabsf(zindex / zlimit) cubed
+  float zf3 = 0.f;
+  if (p.bDampenOrder0) {
+    const float zf = fabsf((float)z0 / (float)p.zd[0]);
+    zf3 = zf * zf * zf;
+  }
+  const float2 otf1 = prof_at(prof + order * nxotf, nxotf, dkr, rdist1);
   float weight = otf1.x * otf1.x + otf1.y * otf1.y;
   if (order != 0) weight *= p.ampmag2_all[p.dir * p.norders + order];
-  float dampfact = (float)(1. / (double)p.noiseVar[p.dir * p.norders + order]);
+  float dampfact = p.inv_nv[p.dir * p.norders + order];
   if (p.bSuppress && order != 0 && rdist1 <= p.suppRadius)
     dampfact *= dev_suppress(rdist1 / p.min_dkr);
   else if (!p.bDampenOrder0 && p.bSuppress && order == 0 && rdist1 <= p.suppRadius)
     dampfact *= dev_suppress(rdist1 / p.min_dkr);
-  else if (p.bDampenOrder0 && order == 0)
-    dampfact *= dev_order0damping(rdist1, (float)z0, p.rdistcutoff, p.zd[0]);
+  else if (p.bDampenOrder0 && order == 0) {
+    const float rf = rdist1 / p.rdistcutoff;
+    dampfact *= rf * rf + zf3;
+  }
   if (order == 0 && z0 == 0 && p.bNoKz0) dampfact = 0.f;
   weight *= dampfact;
   float sumweight = weight;
+  const int nshift = 2 * kMaxOrders - 1;
   for (int dir2 = 0; dir2 < p.ndirs; dir2++) {
     for (int order2 = -(p.norders - 1); order2 < p.norders; order2++) {
       if (dir2 == p.dir && order2 == order) continue;
       if (!p.bFilteroverlaps && !(order2 == 0 && order == 0)) continue;
+      const float2 ks = p.kshift[dir2 * nshift + order2 + (kMaxOrders - 1)];
+      const float x2 = xabs - ks.x, y2 = yabs - ks.y;
+      const float d2 = x2 * x2 + y2 * y2;
+      // rdist2 < rdistcutoff, tested on the squared distance; the OTF radius
+      // index sqrt(d2)/dkr and the damping term (rdist2/rdistcutoff)^2 are
+      // formed from d2 as well (a few ulps from the reference's sqrtf and
+      // divisions, far inside the output tolerance)
+      if (!(d2 < p.cut2)) continue;
       const int ao2 = order2 < 0 ? -order2 : order2;
-      const float amp2mag2 = p.ampmag2_all[dir2 * p.norders + ao2];
-      const float kx2 = (float)order2 * p.k0[dir2].x, ky2 = (float)order2 * p.k0[dir2].y;
-      const float x2 = xabs - kx2, y2 = yabs - ky2;
-      const float rdist2 = sqrtf(x2 * x2 + y2 * y2);
-      if (rdist2 < p.rdistcutoff) {
-        const float2 otf2 = profile_lookup(prof + ao2 * nxotf, nxotf, dkr, x2, y2);
-        float w = (otf2.x * otf2.x + otf2.y * otf2.y) / p.noiseVar[dir2 * p.norders + ao2];
-        if (order2 != 0) w *= amp2mag2;
-        if (p.bSuppress && order2 != 0 && rdist2 <= p.suppRadius)
-          w *= dev_suppress(rdist2 / p.min_dkr);
-        else if (!p.bDampenOrder0 && p.bSuppress && order2 == 0 && rdist2 <= p.suppRadius)
-          w *= dev_suppress(rdist2 / p.min_dkr);
-        else if (p.bDampenOrder0 && order2 == 0)
-          w *= dev_order0damping(rdist2, (float)z0, p.rdistcutoff, p.zd[0]);
-        if (p.bNoKz0 && order2 == 0 && z0 == 0) w = 0.0f;
-        sumweight += w;
-      }
+      const float2 otf2 = prof_idx(prof + ao2 * nxotf, nxotf, __builtin_amdgcn_sqrtf(d2 * p.inv_dkr2));
+      float w = otf2.x * otf2.x + otf2.y * otf2.y;
+      if (!p.unit_nv) w = w / p.noiseVar[dir2 * p.norders + ao2];
+      if (order2 != 0) w *= p.ampmag2_all[dir2 * p.norders + ao2];
+      if (p.bSuppress && order2 != 0 && d2 <= p.supp2)
+        w *= dev_suppress(sqrtf(d2) / p.min_dkr);
+      else if (!p.bDampenOrder0 && p.bSuppress && order2 == 0 && d2 <= p.supp2)
+        w *= dev_suppress(sqrtf(d2) / p.min_dkr);
+      else if (p.bDampenOrder0 && order2 == 0)
+        w *= d2 * p.inv_rcut2 + zf3;
+      if (p.bNoKz0 && order2 == 0 && z0 == 0) w = 0.0f;
+      sumweight += w;
     }
   }
   sumweight += p.wiener;
   float2 scale = make_float2(dampfact * otf1.x / sumweight, dampfact * (-otf1.y) / sumweight);
   if (p.apodizeoutput) {
     float rho;
+    const float rdistabs = sqrtf(xabs * xabs + yabs * yabs);
     const float zdistabs = (float)abs(z0);
-    if (p.zapocutoff > 0)
-      rho = sqrtf((rdistabs / p.apocutoff) * (rdistabs / p.apocutoff) +
-                  (zdistabs / p.zapocutoff) * (zdistabs / p.zapocutoff));
-    else
-      rho = sqrtf((rdistabs / p.apocutoff) * (rdistabs / p.apocutoff));
+    const float ra = rdistabs / p.apocutoff;
+    if (p.zapocutoff > 0) {
+      const float za = zdistabs / p.zapocutoff;
+      rho = sqrtf(ra * ra + za * za);
+    } else {
+      rho = sqrtf(ra * ra);
+    }
     if (rho > 1.f) rho = 1.0f;
     float apofact = 0.f;
     if (p.apodizeoutput == 1)
@@ -124,17 +166,18 @@ __device__ void filter_visit(const FilterParams& p, const float2* prof, int x1, 
   }
 }
 
-// Workgroup (blockIdx.x, iz = blockIdx.y): 256 stored elements (jin, iin) of
-// plane iz of the band pair.  The plane's radial OTF profiles at kz = +z0 and
-// kz = -z0 (the conjugate visit) are built once in LDS.  Planes whose z lies
-// outside +-zd are zeroed (kernel3).
+// Workgroup (blockIdx.x, iz = blockIdx.y): kFiltPer*256 stored elements
+// (jin, iin) of plane iz of the band pair.  The plane's radial OTF profiles
+// at kz = +z0 and kz = -z0 (the conjugate visit) are built once in LDS.
+// Planes whose z lies outside +-zd are zeroed (kernel3).
+constexpr int kFiltPer = 4;
+
 __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
 #pragma clang fp contract(off)
   extern __shared__ float2 prof[];  // [2][norders][nxotf]
   const long long nplane = (long long)p.nxh * p.ny;
   const int iz = blockIdx.y;
-  const long long ep = (long long)blockIdx.x * 256 + threadIdx.x;
-  const long long e = (long long)iz * nplane + ep;
+  const long long ep0 = (long long)blockIdx.x * (256 * kFiltPer) + threadIdx.x;
   const int zd = p.zd[p.order];
   float2* bp = p.bp;
   float2* bp2 = p.bp2;
@@ -145,9 +188,14 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
   else if (iz >= p.nz - zd)
     z0n = iz - p.nz;
   else {
-    if (ep < nplane) {
-      bp[e] = make_float2(0.f, 0.f);
-      if (two) bp2[e] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < kFiltPer; ++k) {
+      const long long ep = ep0 + k * 256;
+      if (ep < nplane) {
+        const long long e = (long long)iz * nplane + ep;
+        bp[e] = make_float2(0.f, 0.f);
+        if (two) bp2[e] = make_float2(0.f, 0.f);
+      }
     }
     return;  // whole block: uniform
   }
@@ -159,34 +207,47 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
     prof[i] = otf_profile_entry(p.otf[o], ir, side ? -z0n : z0n, p.kzscale);
   }
   __syncthreads();
-  if (ep >= nplane) return;
-  const int iin = (int)(ep / p.nxh), jin = (int)(ep - (long long)iin * p.nxh);
-  const int y1n = (iin < p.ny - p.ny / 2) ? iin : iin - p.ny;
-  const float x1f = (float)jin * p.dkx, y1f = (float)y1n * p.dky;
-  const float rdist1 = sqrtf(x1f * x1f + y1f * y1f);
-  if (!(rdist1 <= p.rdistcutoff)) {
-    bp[e] = make_float2(0.f, 0.f);
-    if (two) bp2[e] = make_float2(0.f, 0.f);
-    return;
+  for (int k = 0; k < kFiltPer; ++k) {
+    const long long ep = ep0 + k * 256;
+    if (ep >= nplane) break;
+    const long long e = (long long)iz * nplane + ep;
+    // (iin, jin) = divmod(ep, nxh) without a 64-bit division (ep < 2^24)
+    int iin = (int)((float)ep * p.inv_nxh);
+    int jin = (int)ep - iin * p.nxh;
+    if (jin < 0) {
+      iin--;
+      jin += p.nxh;
+    } else if (jin >= p.nxh) {
+      iin++;
+      jin -= p.nxh;
+    }
+    const int y1n = (iin < p.ny - p.ny / 2) ? iin : iin - p.ny;
+    const float x1f = (float)jin * p.dkx, y1f = (float)y1n * p.dky;
+    const float rdist1 = sqrtf(x1f * x1f + y1f * y1f);
+    if (!(rdist1 <= p.rdistcutoff)) {
+      bp[e] = make_float2(0.f, 0.f);
+      if (two) bp2[e] = make_float2(0.f, 0.f);
+      continue;
+    }
+    float2 re = bp[e];
+    float2 im = two ? bp2[e] : make_float2(0.f, 0.f);
+    // first pass (x1 in [1, nx/2]) or second pass x1 == 0: the element itself
+    filter_visit(p, prof, jin, y1n, z0n, false, re, im);
+    // second pass, x1 = -jin < 0: the conjugate visit of the mirrored frequency
+    if (two && jin >= 1 && jin <= p.nx / 2 - 1) {
+      const int yc = -((iin <= p.ny / 2) ? iin : iin - p.ny);
+      filter_visit(p, prof + nprof, -jin, yc, -z0n, true, re, im);
+    }
+    bp[e] = re;
+    if (two) bp2[e] = im;
   }
-  float2 re = bp[e];
-  float2 im = two ? bp2[e] : make_float2(0.f, 0.f);
-  // first pass (x1 in [1, nx/2]) or second pass x1 == 0: the element itself
-  filter_visit(p, prof, jin, y1n, z0n, false, re, im);
-  // second pass, x1 = -jin < 0: the conjugate visit of the mirrored frequency
-  if (two && jin >= 1 && jin <= p.nx / 2 - 1) {
-    const int yc = -((iin <= p.ny / 2) ? iin : iin - p.ny);
-    filter_visit(p, prof + nprof, -jin, yc, -z0n, true, re, im);
-  }
-  bp[e] = re;
-  if (two) bp2[e] = im;
 }
 
 void launch_filter(const FilterParams& p, hipStream_t s) {
   const long long nplane = (long long)p.nxh * p.ny;
   const size_t lds = (size_t)2 * p.norders * p.otf[0].nxotf * sizeof(float2);
   if (lds > 64 * 1024) throw ArgError("OTF table too long for the filter's LDS profile");
-  hipLaunchKernelGGL(k_filter, dim3(ceil_div(nplane, 256), p.nz), dim3(256), lds, s, p);
+  hipLaunchKernelGGL(k_filter, dim3(ceil_div(nplane, 256 * kFiltPer), p.nz), dim3(256), lds, s, p);
   SIMR_LAUNCH_CHECK();
 }
 

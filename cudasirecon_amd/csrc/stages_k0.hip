@@ -50,11 +50,11 @@ __device__ float2 ov_value(const OvParams& p, const float2* prof1, const float2*
     conj = 0;
   } else {
     jin = p.nx - j;
-    iin = (p.ny - i) % p.ny;
+    iin = i == 0 ? 0 : p.ny - i;
     conj = 1;
   }
   int zin = conj ? -z0 : z0;
-  zin = (zin + p.nz) % p.nz;
+  if (zin < 0) zin += p.nz;
   const long long indin = (long long)zin * p.nxh * p.ny + (long long)iin * p.nxh + jin;
   if (which == 0) {
     if (!(rdist12 <= rc)) return zero;
