@@ -17,6 +17,7 @@
 // The order-0 bands of all directions share one transform (linearity).
 #include "device_math.hpp"
 #include "fft.hpp"
+#include "fft2.hpp"
 #include "fft_api.hpp"
 #include "sizes.hpp"
 #include "stages.hpp"
@@ -152,19 +153,29 @@ __global__ __launch_bounds__(asm_y_w<N>() * fft_tpt(N)) void k_asm_y(AsmParams p
 }
 
 // ---- x pass: row Y (blockIdx.x) of every output plane.
-// A 256-thread workgroup transforms G bands of the row at a time (TPT
-// threads each) and walks all bands, so every output element is accumulated
-// in registers and stored once.  The modulation phasors of the row (E per
-// thread for bands 1..kAsmLutBands) are computed once per workgroup and kept
-// in registers for all planes; the FFT twiddles are read from an LDS table.
-// Several workgroups share a CU, so the row loads of one overlap the
-// transforms of another.
-constexpr int kAsmXThreads = 512;
+// A workgroup transforms G bands of the row at a time (TPT threads each) and
+// walks all bands, so every output element is accumulated in registers and
+// stored once.  The modulation phasors of the row (E per thread for bands
+// 1..kAsmLutBands) are computed once per workgroup and kept in registers for
+// all planes; the FFT twiddles are read from an LDS table.  Lengths with a
+// two-pass plan (N = N1 * N2, fft2.hpp) use that core with G = 8 bands per
+// round (one round for 3 directions x 3 orders); the others use the
+// radix-stage core with 512 threads.
+constexpr int kAsmXBands = 8;
 constexpr int kAsmLutBands = 6;  // 1 + ndirs*(norders-1) <= 7 keeps every phasor in registers
 
 template <int N>
+constexpr bool asm_x_fft2() {
+  return fft2_ok(N) && fft2_tpt(N) >= 8 && 64 % fft2_tpt(N) == 0 && (kAsmXBands * fft2_tpt(N)) % 64 == 0;
+}
+template <int N>
 constexpr int asm_x_tpt() {
+  if (asm_x_fft2<N>()) return fft2_tpt(N);
   return fft_tpt(N) >= 128 ? 128 : fft_tpt(N);
+}
+template <int N>
+constexpr int asm_x_threads() {
+  return asm_x_fft2<N>() ? kAsmXBands * fft2_tpt(N) : 512;
 }
 
 // modulation phasor of assembly band a (>= 1) at output column X of row Y
@@ -184,30 +195,147 @@ __device__ __forceinline__ float2 asm_phasor(const AsmParams& p, int a, int X, i
   return make_float2(cs, sn);
 }
 
+// LDS row of one band: natural order for the two-pass core, swizzled for the
+// radix-stage core
 template <int N>
-__global__ __launch_bounds__(kAsmXThreads, 4) void k_asm_x(AsmParams p, const float2* __restrict__ tw) {
+constexpr int asm_x_pitch() {
+  return asm_x_fft2<N>() ? fft2_span(N) : lds_pitch(N);
+}
+template <int N>
+__device__ __forceinline__ int asm_x_pos(int X) {
+  if constexpr (asm_x_fft2<N>())
+    return X;
+  else
+    return lds_swz<N>(X);
+}
+
+// Adds one transformed band row (band a) in LDS to the row accumulators
+// (write_outbuffer_kernel1/2, gpuFunctionsImpl.cu:1956-1991: band 0 adds its
+// real part, side bands add Re(2 v phasor) with a double intermediate).
+// SRC >= 0: the phasor is lut[.][SRC]; SRC == -1: band 0; SRC == -2: the
+// phasor is computed.  Every table index is a compile-time constant, so the
+// table stays in registers.
+template <int N, int NT, int E, int LB, int SRC>
+__device__ __forceinline__ void asm_x_add_band(const AsmParams& p, const float2* row, int a, int Y,
+                                               float (&acc)[E], const float2 (&lut)[E][LB]) {
 #pragma clang fp contract(off)
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int X = threadIdx.x + e * NT;
+    if (X < N) {
+      const float2 v = row[asm_x_pos<N>(X)];
+      if constexpr (SRC == -1) {
+        acc[e] += v.x;
+      } else {
+        float2 ph;
+        if constexpr (SRC >= 0)
+          ph = lut[e][SRC];
+        else
+          ph = asm_phasor<N>(p, a, X, Y);
+        const double d = (double)v.x * 2.0 * (double)ph.x - (double)v.y * 2.0 * (double)ph.y;
+        acc[e] = (float)((double)acc[e] + d);
+      }
+    }
+  }
+}
+
+// Bands A .. AEND-1 of round R (slot A - R*G), each with its table entry.
+template <int N, int NT, int G, int E, int LB, int R, int A, int AEND>
+__device__ __forceinline__ void asm_x_table_bands(const AsmParams& p, const float2* lds, int Y, float (&acc)[E],
+                                                  const float2 (&lut)[E][LB]) {
+  if constexpr (A < AEND) {
+    constexpr int LP = asm_x_pitch<N>();
+    if (A < p.nasm) {
+      asm_x_add_band<N, NT, E, LB, A == 0 ? -1 : A - 1>(p, lds + (A - R * G) * LP, A, Y, acc, lut);
+      asm_x_table_bands<N, NT, G, E, LB, R, A + 1, AEND>(p, lds, Y, acc, lut);
+    }
+  }
+}
+
+// Adds the G transformed bands of round R (bands R*G + sl in LDS slot sl),
+// in band order.  Bands 0..LB come with compile-time table entries; the
+// rest compute their phasors.
+template <int N, int NT, int G, int E, int LB, int R>
+__device__ __forceinline__ void asm_x_round(const AsmParams& p, const float2* lds, int a0, int Y, float (&acc)[E],
+                                            const float2 (&lut)[E][LB]) {
+  constexpr int LP = asm_x_pitch<N>();
+  constexpr int A0 = R * G;
+  constexpr int AT = (R < 0) ? 0 : ((A0 + G < LB + 1) ? A0 + G : LB + 1);  // first band without a table entry
+  int sl = 0;
+  if constexpr (R >= 0 && AT > A0) {
+    asm_x_table_bands<N, NT, G, E, LB, R, A0, AT>(p, lds, Y, acc, lut);
+    sl = AT - A0;
+  }
+  for (; sl < G && a0 + sl < p.nasm; ++sl) asm_x_add_band<N, NT, E, LB, -2>(p, lds + sl * LP, a0 + sl, Y, acc, lut);
+}
+
+// Rows Y of bands a0 .. a0+R-1 of plane zout, each centred in a zero-padded
+// length-N row (ifftshift layout of the zoomed grid), into LDS rows dst,
+// dst + LP, ... by the 64 lanes of one wavefront.  Source column s goes to
+// X = s - (half - 1) for s >= half - 1 and to X = s + N - (half - 1) below;
+// the columns between are zero, and so are bands >= nasm.  The loads run
+// over contiguous source columns in batches of R x B with clamped (always
+// valid) addresses and no branches, so a batch is in flight at once.
+template <int N, int R>
+__device__ __forceinline__ void asm_x_load_rows(const AsmParams& p, float2* dst, int a0, int zout, int Y, int lane) {
+  constexpr int LP = asm_x_pitch<N>();
+  constexpr int B = N >= 256 ? N / 128 : 2;  // one batch for a 2x zoom
+  const int half = p.nx / 2;
+  const float2* src[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int a = a0 + r < p.nasm ? a0 + r : p.nasm - 1;
+    src[r] = p.stageY + (((long long)a * p.zdim + zout) * p.ydim + Y) * p.nx;
+  }
+  for (int s0 = lane; s0 < p.nx; s0 += 64 * B) {
+    float2 v[R][B];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const int sc = s0 + 64 * k;
+        v[r][k] = src[r][sc < p.nx ? sc : p.nx - 1];
+      }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const int sc = s0 + 64 * k;
+        const int X = sc >= half - 1 ? sc - (half - 1) : sc + N - (half - 1);
+        if (sc < p.nx) dst[r * LP + asm_x_pos<N>(X)] = a0 + r < p.nasm ? v[r][k] : make_float2(0.f, 0.f);
+      }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    for (int X = p.nx - half + 1 + lane; X < N - (half - 1); X += 64) dst[r * LP + asm_x_pos<N>(X)] = make_float2(0.f, 0.f);
+}
+
+template <int N>
+__global__ __launch_bounds__(asm_x_threads<N>(), 2) void k_asm_x(AsmParams p,
+                                                                                   const float2* __restrict__ tw) {
+#pragma clang fp contract(off)
+  constexpr int NT = asm_x_threads<N>();
   constexpr int TPT = asm_x_tpt<N>();
-  constexpr int G = kAsmXThreads / TPT;
-  constexpr int LP = lds_pitch(N);
-  constexpr int E = (N + kAsmXThreads - 1) / kAsmXThreads;
+  constexpr int G = NT / TPT;
+  constexpr int LP = asm_x_pitch<N>();
+  constexpr int E = (N + NT - 1) / NT;
   constexpr bool LUT = E <= 4;
+  constexpr int LB = LUT ? kAsmLutBands : 1;
+  static_assert(!asm_x_fft2<N>() || (64 % TPT == 0 && NT % 64 == 0), "two-pass transforms must not straddle waves");
   __shared__ float2 lds[G * LP];
   __shared__ float2 twt[N];
-  for (int i = threadIdx.x; i < N; i += kAsmXThreads) twt[i] = tw[i];
+  for (int i = threadIdx.x; i < N; i += NT) twt[i] = tw[i];
   const int Y = blockIdx.x;
   const int g = threadIdx.x / TPT, t = threadIdx.x % TPT;
   const int half = p.nx / 2;
-  float2 lut[LUT ? E : 1][LUT ? kAsmLutBands : 1];
-  if constexpr (LUT) {
+  float2 lut[E][LB];
 #pragma unroll
-    for (int e = 0; e < E; ++e)
+  for (int e = 0; e < E; ++e)
 #pragma unroll
-      for (int b = 0; b < kAsmLutBands; ++b) {
-        const int X = threadIdx.x + e * kAsmXThreads;
-        lut[e][b] = (X < N && b + 1 < p.nasm) ? asm_phasor<N>(p, b + 1, X, Y) : make_float2(1.f, 0.f);
-      }
-  }
+    for (int b = 0; b < LB; ++b) {
+      const int X = threadIdx.x + e * NT;
+      lut[e][b] = (LUT && X < N && b + 1 < p.nasm) ? asm_phasor<N>(p, b + 1, X, Y) : make_float2(1.f, 0.f);
+    }
   __syncthreads();
   for (int zout = 0; zout < p.zdim; ++zout) {
     float acc[E];
@@ -215,62 +343,51 @@ __global__ __launch_bounds__(kAsmXThreads, 4) void k_asm_x(AsmParams p, const fl
     for (int e = 0; e < E; ++e) acc[e] = 0.f;
 #pragma nounroll
     for (int a0 = 0; a0 < p.nasm; a0 += G) {
-      // G bands of row Y of plane zout, zero-padded to N, into LDS
-      for (int idx = threadIdx.x; idx < G * N; idx += kAsmXThreads) {
-        const int sl = idx / N, X = idx - sl * N;
-        const int a = a0 + sl;
-        float2 v = make_float2(0.f, 0.f);
-        if (a < p.nasm) {
-          int x;
-          bool valid = true;
-          if (X <= p.nx - half)
-            x = X;
-          else if (X >= N - (half - 1))
-            x = X - N;
-          else {
-            x = 0;
-            valid = false;
-          }
-          if (valid) v = p.stageY[(((long long)a * p.zdim + zout) * p.ydim + Y) * p.nx + (x + (half - 1))];
-        }
-        lds[sl * LP + lds_swz<N>(X)] = v;
-      }
-      __syncthreads();
-      fft_lds_t<N, true, TPT>(lds + g * LP, t, twt);
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int X = threadIdx.x + e * kAsmXThreads;
-        if (X < N) {
-#pragma unroll
-          for (int sl = 0; sl < G; ++sl) {
-            const int a = a0 + sl;
-            if (a < p.nasm) {
-              const float2 v = lds[sl * LP + lds_swz<N>(X)];
-              if (a == 0) {
-                acc[e] += v.x;  // write_outbuffer_kernel1
-              } else {          // write_outbuffer_kernel2: double intermediate, as in the reference
-                float2 ph;
-                if (LUT && a <= kAsmLutBands) {
-                  ph = lut[LUT ? e : 0][0];
-#pragma unroll
-                  for (int b = 1; b < kAsmLutBands; ++b)
-                    if (a - 1 == b) ph = lut[LUT ? e : 0][LUT ? b : 0];
-                } else {
-                  ph = asm_phasor<N>(p, a, X, Y);
-                }
-                const double d = (double)v.x * 2.0 * (double)ph.x - (double)v.y * 2.0 * (double)ph.y;
-                acc[e] = (float)((double)acc[e] + d);
-              }
+      if constexpr (asm_x_fft2<N>()) {
+        // each wavefront loads and transforms its own TPW bands, so only the
+        // accumulation below needs the whole workgroup
+        constexpr int TPW = 64 / TPT;
+        const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+        asm_x_load_rows<N, TPW>(p, lds + wave * TPW * LP, a0 + wave * TPW, zout, Y, lane);
+        wave_lds_sync();
+        fft2_lds<N, true, true>(lds + g * LP, t, twt);
+        __syncthreads();
+      } else {
+        // G bands of row Y of plane zout, zero-padded to N, into LDS
+        for (int idx = threadIdx.x; idx < G * N; idx += NT) {
+          const int sl = idx / N, X = idx - sl * N;
+          const int a = a0 + sl;
+          float2 v = make_float2(0.f, 0.f);
+          if (a < p.nasm) {
+            int x;
+            bool valid = true;
+            if (X <= p.nx - half)
+              x = X;
+            else if (X >= N - (half - 1))
+              x = X - N;
+            else {
+              x = 0;
+              valid = false;
             }
+            if (valid) v = p.stageY[(((long long)a * p.zdim + zout) * p.ydim + Y) * p.nx + (x + (half - 1))];
           }
+          lds[sl * LP + asm_x_pos<N>(X)] = v;
         }
+        __syncthreads();
+        fft_lds_t<N, true, TPT>(lds + g * LP, t, twt);
       }
+      if (a0 == 0)
+        asm_x_round<N, NT, G, E, LB, 0>(p, lds, a0, Y, acc, lut);
+      else if (a0 == G)
+        asm_x_round<N, NT, G, E, LB, 1>(p, lds, a0, Y, acc, lut);
+      else
+        asm_x_round<N, NT, G, E, LB, -1>(p, lds, a0, Y, acc, lut);
       __syncthreads();
     }
     float* orow = p.out + ((long long)zout * p.ydim + Y) * N;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int X = threadIdx.x + e * kAsmXThreads;
+      const int X = threadIdx.x + e * NT;
       if (X < N) orow[X] = acc[e];
     }
   }
@@ -314,7 +431,7 @@ void launch_asm_x(const AsmParams& p, hipStream_t s) {
   switch (p.xdim) {
 #define SIMR_CASE(N)                                                                                       \
   case N:                                                                                                  \
-    hipLaunchKernelGGL(k_asm_x<N>, dim3(p.ydim), dim3(kAsmXThreads), 0, s, p,                          \
+    hipLaunchKernelGGL(k_asm_x<N>, dim3(p.ydim), dim3(asm_x_threads<N>()), 0, s, p,                    \
                        twiddles(N));                                                                       \
     SIMR_LAUNCH_CHECK();                                                                                   \
     return;
