@@ -292,6 +292,7 @@ template <int N, int TPT>
 __device__ __forceinline__ void load_twiddles(FftTw<N, TPT>& tw, float2* stage, int t, const float2* __restrict__ tab,
                                               int tid, int nthreads) {
   if constexpr (N > 1) {
+#pragma unroll 4
     for (int i = tid; i < N; i += nthreads) stage[i] = tab[i];
     __syncthreads();
     tw.load(t, stage);

@@ -66,10 +66,22 @@ __global__ __launch_bounds__(W* fft_tpt(N)) void k_fft_cols(float2* __restrict__
   load_twiddles(twr, twstage, tid % TPT, tw, threadIdx.x, blockDim.x);
   const long long c0 = (long long)blockIdx.x * W;
   float2* base = data + (long long)blockIdx.y * plane_stride;
-  for (int idx = tid; idx < N * W; idx += NT) {
+  // all loads of a thread are issued before the first LDS write (clamped
+  // addresses, no branches), so they are in flight together
+  constexpr int EPT = (N * W + NT - 1) / NT;
+  float2 v[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int i = (idx < N * W ? idx : N * W - 1) / W, w = idx % W;
+    const long long c = c0 + w < ncols ? c0 + w : ncols - 1;
+    v[k] = base[i * pitch + c];
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
     const int i = idx / W, w = idx % W;
-    const long long c = c0 + w;
-    lds[w * P + lds_swz<N>(i)] = (c < ncols) ? base[i * pitch + c] : make_float2(0.f, 0.f);
+    if (idx < N * W) lds[w * P + lds_swz<N>(i)] = (c0 + w < ncols) ? v[k] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<N, INV>(lds + (tid / TPT) * P, tid % TPT, twr);
@@ -93,10 +105,20 @@ __global__ __launch_bounds__(W* fft_tpt(N)) void k_fft_rows(float2* __restrict__
   __shared__ float2 twstage[N > 1 ? N : 1];
   load_twiddles(twr, twstage, tid % TPT, tw, threadIdx.x, blockDim.x);
   const long long r0 = (long long)blockIdx.x * W;
-  for (int idx = tid; idx < N * W; idx += NT) {
+  constexpr int EPT = (N * W + NT - 1) / NT;
+  float2 v[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT < N * W ? tid + k * NT : N * W - 1;
     const int w = idx / N, i = idx % N;
-    const long long r = r0 + w;
-    lds[w * P + lds_swz<N>(i)] = (r < nrows) ? data[r * pitch + i] : make_float2(0.f, 0.f);
+    const long long r = r0 + w < nrows ? r0 + w : nrows - 1;
+    v[k] = data[r * pitch + i];
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int w = idx / N, i = idx % N;
+    if (idx < N * W) lds[w * P + lds_swz<N>(i)] = (r0 + w < nrows) ? v[k] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<N, INV>(lds + (tid / TPT) * P, tid % TPT, twr);

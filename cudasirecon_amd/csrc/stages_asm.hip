@@ -37,41 +37,6 @@ __device__ __forceinline__ int z_of_zout(int zout, int nz, int zdim, bool& valid
   return 0;
 }
 
-// Band value placed by move_kernel at lateral (x, y) and axial z (centred coords).
-__device__ __forceinline__ float2 moved_value(const AsmParams& p, int a, int x, int y, int z) {
-  bool conj = false;
-  if (x < 0) {
-    x = -x;
-    y = -y;
-    z = -z;
-    conj = true;
-  }
-  if (y < 0) y += p.ny;
-  if (z < 0) z += p.nz;
-  const long long indin = ((long long)z * p.ny + y) * p.nxh + x;
-  const long long dirstride = (long long)p.nbands * p.band_stride;
-  if (a == 0) {
-    float2 v = make_float2(0.f, 0.f);
-    for (int d = 0; d < p.ndirs; ++d) {
-      const float2 b = p.bands[d * dirstride + indin];
-      v.x += b.x;
-      v.y += b.y;
-    }
-    if (conj) v.y = -v.y;
-    return v;
-  }
-  const int no = p.norders - 1;
-  const int dir = (a - 1) / no, order = 1 + (a - 1) % no;
-  const float2* base = p.bands + dir * dirstride;
-  float2 re = base[(2 * order - 1) * p.band_stride + indin];
-  float2 im = base[(2 * order) * p.band_stride + indin];
-  if (conj) {
-    re.y = -re.y;
-    im.y = -im.y;
-  }
-  return make_float2(re.x - im.y, re.y + im.x);
-}
-
 // ---- z pass: W adjacent xc columns of row yc (blockIdx.y) of band a (blockIdx.z)
 template <int N>
 __global__ __launch_bounds__(256) void k_asm_z(AsmParams p, const float2* __restrict__ tw) {
@@ -86,14 +51,72 @@ __global__ __launch_bounds__(256) void k_asm_z(AsmParams p, const float2* __rest
   const int a = blockIdx.z, yc = blockIdx.y;
   const int xc0 = blockIdx.x * W;
   const int y = yc - (p.ny / 2 - 1);
-  for (int idx = threadIdx.x; idx < N * W; idx += NT) {
+  // Band value that move_kernel places at lateral (x, y) and axial z (centred
+  // coordinates; x < 0 reads the Hermitian mirror, conjugated).  Every
+  // address is computed first (invalid elements point at element 0 and are
+  // zeroed afterwards), so all loads of a thread are in flight together.
+  constexpr int B = (N * W + NT - 1) / NT;
+  long long ind[B];
+  unsigned okmask = 0, cjmask = 0;
+  static_assert(B <= 32, "per-thread element masks hold 32 bits");
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const int idx = threadIdx.x + k * NT;
     const int zout = idx / W, w = idx - (idx / W) * W;
     const int xc = xc0 + w;
     bool valid;
-    const int z = z_of_zout(zout, p.nz, N, valid);
-    float2 v = make_float2(0.f, 0.f);
-    if (valid && xc < p.nx) v = moved_value(p, a, xc - (p.nx / 2 - 1), y, z);
-    lds[w * LP + lds_swz<N>(zout)] = v;
+    int z = z_of_zout(zout, p.nz, N, valid);
+    valid = valid && xc < p.nx && idx < N * W;
+    int x = xc - (p.nx / 2 - 1), yy = y;
+    const bool cj = x < 0;
+    if (cj) {
+      x = -x;
+      yy = -yy;
+      z = -z;
+    }
+    if (yy < 0) yy += p.ny;
+    if (z < 0) z += p.nz;
+    ind[k] = valid ? ((long long)z * p.ny + yy) * p.nxh + x : 0;
+    okmask |= (valid ? 1u : 0u) << k;
+    cjmask |= (cj ? 1u : 0u) << k;
+  }
+  float2 v[B];
+  const long long dirstride = (long long)p.nbands * p.band_stride;
+  if (a == 0) {  // band 0: sum over directions (move_kernel adds the order-0 bands)
+#pragma unroll
+    for (int k = 0; k < B; ++k) v[k] = make_float2(0.f, 0.f);
+    for (int d = 0; d < p.ndirs; ++d) {
+      const float2* bd = p.bands + d * dirstride;
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const float2 b = bd[ind[k]];
+        v[k].x += b.x;
+        v[k].y += b.y;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < B; ++k)
+      if ((cjmask >> k) & 1u) v[k].y = -v[k].y;
+  } else {  // side band: re + i im of the order's two real-space bands
+    const int no = p.norders - 1;
+    const int dir = (a - 1) / no, order = 1 + (a - 1) % no;
+    const float2* bre = p.bands + dir * dirstride + (2 * order - 1) * p.band_stride;
+    const float2* bim = p.bands + dir * dirstride + (2 * order) * p.band_stride;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      float2 re = bre[ind[k]], im = bim[ind[k]];
+      if ((cjmask >> k) & 1u) {
+        re.y = -re.y;
+        im.y = -im.y;
+      }
+      v[k] = make_float2(re.x - im.y, re.y + im.x);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const int idx = threadIdx.x + k * NT;
+    const int zout = idx / W, w = idx - (idx / W) * W;
+    if (idx < N * W) lds[w * LP + lds_swz<N>(zout)] = ((okmask >> k) & 1u) ? v[k] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
@@ -127,20 +150,33 @@ __global__ __launch_bounds__(asm_y_w<N>() * fft_tpt(N)) void k_asm_y(AsmParams p
   const int a = blockIdx.z, zout = blockIdx.y;
   const int xc0 = blockIdx.x * W;
   const float2* src = p.stageZ + ((long long)a * p.zdim + zout) * p.ny * p.nx;
-  for (int idx = threadIdx.x; idx < N * W; idx += NT) {
-    const int Y = idx / W, w = idx - (idx / W) * W;
-    const int xc = xc0 + w;
-    int y;
-    bool valid = xc < p.nx;
-    if (Y <= p.ny - p.ny / 2)
-      y = Y;
-    else if (Y >= N - (p.ny / 2 - 1))
-      y = Y - N;
-    else {
-      y = 0;
-      valid = false;
+  // Source row sr goes to Y = sr - hy (sr >= hy) or Y = sr + N - hy (sr < hy);
+  // the rows between are zero.  Loads run over source rows in batches of B
+  // per thread with clamped addresses and no branches, so a batch is in
+  // flight at once (one batch for a 2x zoom).
+  const int hy = p.ny / 2 - 1;
+  constexpr int B = (N / 2 * W + NT - 1) / NT;
+  const int cnt = p.ny * W;
+  for (int i0 = threadIdx.x; i0 < cnt; i0 += NT * B) {
+    float2 v[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int idx = i0 + k * NT < cnt ? i0 + k * NT : cnt - 1;
+      const int sr = idx / W, w = idx - (idx / W) * W;
+      const int xc = xc0 + w < p.nx ? xc0 + w : p.nx - 1;
+      v[k] = src[(long long)sr * p.nx + xc];
     }
-    lds[w * LP + lds_swz<N>(Y)] = valid ? src[(long long)(y + (p.ny / 2 - 1)) * p.nx + xc] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int idx = i0 + k * NT;
+      const int sr = idx / W, w = idx - (idx / W) * W;
+      const int Y = sr >= hy ? sr - hy : sr + N - hy;
+      if (idx < cnt) lds[w * LP + lds_swz<N>(Y)] = xc0 + w < p.nx ? v[k] : make_float2(0.f, 0.f);
+    }
+  }
+  for (int idx = threadIdx.x; idx < (N - p.ny) * W; idx += NT) {
+    const int Y = p.ny - p.ny / 2 + 1 + idx / W, w = idx - (idx / W) * W;
+    lds[w * LP + lds_swz<N>(Y)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);

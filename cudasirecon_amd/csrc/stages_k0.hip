@@ -22,13 +22,21 @@
 
 namespace simr {
 
-// One element of overlap0 (which == 0) or overlap1 (which == 1) at column j,
-// row i, plane z0, before the inverse transform.  prof1/prof2: radial OTF
-// profiles of order1/order2 at kz = z0.
-__device__ float2 ov_value(const OvParams& p, const float2* prof1, const float2* prof2, int which, int j, int i,
-                           int z0) {
+// Where one element of overlap0 (which == 0) or overlap1 (which == 1) at
+// column j, row i, plane z0 comes from, before the inverse transform: its
+// band index, the conjugation of the Hermitian mirror, and the OTF factor
+// (makeOverlaps0Kernel / makeOverlaps1Kernel).  ok = false: the element is
+// zero.  prof1/prof2: radial OTF profiles of order1/order2 at kz = z0.
+struct OvSrc {
+  int ind;
+  bool ok, conj;
+  float2 fact;
+};
+
+__device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof1, const float2* prof2, int which,
+                                           int j, int i, int z0) {
 #pragma clang fp contract(off)
-  const float2 zero = make_float2(0.f, 0.f);
+  OvSrc o{0, false, false, make_float2(0.f, 0.f)};
   float x1f = (float)j, y1f = (float)i;
   if (x1f > (float)(p.nx / 2)) x1f -= (float)p.nx;
   if (y1f > (float)(p.ny / 2)) y1f -= (float)p.ny;
@@ -36,61 +44,60 @@ __device__ float2 ov_value(const OvParams& p, const float2* prof1, const float2*
   y1f *= p.dky;
   const float rc = p.rdistcutoff;
   const float rdist1 = sqrtf(x1f * x1f + y1f * y1f);
-  if (which == 0 ? !(rdist1 <= rc) : !(rdist1 < rc)) return zero;
+  if (which == 0 ? !(rdist1 <= rc) : !(rdist1 < rc)) return o;
   const float x12 = x1f - p.kx, y12 = y1f - p.ky;
   const float rdist12 = sqrtf(x12 * x12 + y12 * y12);
   const float x21 = x1f + p.kx, y21 = y1f + p.ky;
   const float rdist21 = sqrtf(x21 * x21 + y21 * y21);
-  if (!(rdist12 <= rc || rdist21 <= rc)) return zero;
-  if (z0 == 0 && p.bNoKz0) return zero;
-  int iin, jin, conj;
+  if (!(rdist12 <= rc || rdist21 <= rc)) return o;
+  if (z0 == 0 && p.bNoKz0) return o;
+  int iin, jin;
   if (j <= p.nx / 2) {
     iin = i;
     jin = j;
-    conj = 0;
   } else {
     jin = p.nx - j;
     iin = i == 0 ? 0 : p.ny - i;
-    conj = 1;
+    o.conj = true;
   }
-  int zin = conj ? -z0 : z0;
+  int zin = o.conj ? -z0 : z0;
   if (zin < 0) zin += p.nz;
-  const long long indin = (long long)zin * p.nxh * p.ny + (long long)iin * p.nxh + jin;
+  float2 oa, ob;  // own-order OTF at k, other-order OTF at the shifted k
+  float sa, sb;   // their cutoff scales
   if (which == 0) {
-    if (!(rdist12 <= rc)) return zero;
-    const float2 otf1 = profile_lookup(prof1, p.otf1.nxotf, p.otf1.dkrotf, x1f, y1f);
-    if (!(sqrtf(otf1.x * otf1.x + otf1.y * otf1.y) > p.otfcutoff)) return zero;
-    const float2 otf12 = profile_lookup(prof2, p.otf2.nxotf, p.otf2.dkrotf, x12, y12);
-    if (!(sqrtf(otf12.x * otf12.x + otf12.y * otf12.y) * p.order0_2_factor > p.otfcutoff)) return zero;
-    float2 vre = p.b1re[indin];
-    float2 vim = p.order1 > 0 ? p.b1im[indin] : zero;
-    const float root = sqrtf(otf1.x * otf1.x + otf1.y * otf1.y + otf12.x * otf12.x + otf12.y * otf12.y);
-    const float2 fact = make_float2(otf12.x / root, otf12.y / root);
-    if (conj) {
-      vre.y = -vre.y;
-      if (p.order1 > 0) vim.y = -vim.y;
-    }
-    vre = cmul_nc(vre, fact);
-    if (p.order1 > 0) vim = cmul_nc(vim, fact);
-    return make_float2(vre.x - vim.y, vre.y + vim.x);
+    if (!(rdist12 <= rc)) return o;
+    oa = profile_lookup(prof1, p.otf1.nxotf, p.otf1.dkrotf, x1f, y1f);
+    ob = profile_lookup(prof2, p.otf2.nxotf, p.otf2.dkrotf, x12, y12);
+    sa = 1.f;
+    sb = p.order0_2_factor;
   } else {
-    if (!(rdist21 <= rc)) return zero;
-    const float2 otf2 = profile_lookup(prof2, p.otf2.nxotf, p.otf2.dkrotf, x1f, y1f);
-    if (!(sqrtf(otf2.x * otf2.x + otf2.y * otf2.y) * p.order0_2_factor > p.otfcutoff)) return zero;
-    const float2 otf21 = profile_lookup(prof1, p.otf1.nxotf, p.otf1.dkrotf, x21, y21);
-    if (!(sqrtf(otf21.x * otf21.x + otf21.y * otf21.y) > p.otfcutoff)) return zero;
-    float2 vre = p.b2re[indin];
-    float2 vim = p.b2im[indin];
-    const float root = sqrtf(otf2.x * otf2.x + otf2.y * otf2.y + otf21.x * otf21.x + otf21.y * otf21.y);
-    const float2 fact = make_float2(otf21.x / root, otf21.y / root);
-    if (conj) {
-      vre.y = -vre.y;
-      vim.y = -vim.y;
-    }
-    vre = cmul_nc(vre, fact);
-    vim = cmul_nc(vim, fact);
-    return make_float2(vre.x - vim.y, vre.y + vim.x);
+    if (!(rdist21 <= rc)) return o;
+    oa = profile_lookup(prof2, p.otf2.nxotf, p.otf2.dkrotf, x1f, y1f);
+    ob = profile_lookup(prof1, p.otf1.nxotf, p.otf1.dkrotf, x21, y21);
+    sa = p.order0_2_factor;
+    sb = 1.f;
   }
+  if (!(sqrtf(oa.x * oa.x + oa.y * oa.y) * sa > p.otfcutoff)) return o;
+  if (!(sqrtf(ob.x * ob.x + ob.y * ob.y) * sb > p.otfcutoff)) return o;
+  const float root = sqrtf(oa.x * oa.x + oa.y * oa.y + ob.x * ob.x + ob.y * ob.y);
+  o.fact = make_float2(ob.x / root, ob.y / root);
+  o.ind = (zin * p.ny + iin) * p.nxh + jin;
+  o.ok = true;
+  return o;
+}
+
+// The overlap element from its source and the two band values read there.
+__device__ __forceinline__ float2 ov_combine(const OvSrc& o, float2 vre, float2 vim, bool has_im) {
+#pragma clang fp contract(off)
+  if (!o.ok) return make_float2(0.f, 0.f);
+  if (!has_im) vim = make_float2(0.f, 0.f);
+  if (o.conj) {
+    vre.y = -vre.y;
+    vim.y = -vim.y;
+  }
+  vre = cmul_nc(vre, o.fact);
+  vim = cmul_nc(vim, o.fact);
+  return make_float2(vre.x - vim.y, vre.y + vim.x);
 }
 
 // x pass: rows [kOvRows*blockIdx.x, +kOvRows) of plane zb (blockIdx.y) of both
@@ -117,13 +124,34 @@ __global__ __launch_bounds__(2 * kOvRows * fft_tpt(N)) void k_overlaps_x(OvParam
   __syncthreads();
   const int i0 = blockIdx.x * kOvRows;
   constexpr int EPT = (2 * kOvRows * N + NT - 1) / NT;
+  // sources of all elements first, then every band load (clamped to a valid
+  // index for zero elements) at once, then the products
+  OvSrc src[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    const int t = idx / N, j = idx - (idx / N) * N;  // t = which * kOvRows + row
+    const int which = t / kOvRows, i = i0 + (t - which * kOvRows);
+    src[q] = (idx < 2 * kOvRows * N && i < p.ny) ? ov_source(p, prof, prof + nxotf, which, j, i, z0)
+                                                 : OvSrc{0, false, false, make_float2(0.f, 0.f)};
+  }
+  float2 vre[EPT], vim[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    const int which = (idx / N) / kOvRows;
+    const float2* bre = which == 0 ? p.b1re : p.b2re;
+    const float2* bim = which == 0 ? (p.order1 > 0 ? p.b1im : p.b1re) : p.b2im;
+    vre[q] = bre[src[q].ind];
+    vim[q] = bim[src[q].ind];
+  }
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
     const int idx = threadIdx.x + q * NT;
     if (idx < 2 * kOvRows * N) {
-      const int t = idx / N, j = idx - (idx / N) * N;  // t = which * kOvRows + row
-      const int which = t / kOvRows, i = i0 + (t - which * kOvRows);
-      lds[t * P + lds_swz<N>(j)] = i < p.ny ? ov_value(p, prof, prof + nxotf, which, j, i, z0) : make_float2(0.f, 0.f);
+      const int t = idx / N, j = idx - (idx / N) * N;
+      const bool has_im = t >= kOvRows || p.order1 > 0;
+      lds[t * P + lds_swz<N>(j)] = ov_combine(src[q], vre[q], vim[q], has_im);
     }
   }
   __syncthreads();
@@ -191,21 +219,31 @@ __global__ __launch_bounds__(2 * zcols<N>() * fft_tpt(N)) void k_overlaps_z(OvPa
   const long long nxy = (long long)p.nx * p.ny;
   const long long e0 = (long long)blockIdx.x * WC;
   const int nzs = 2 * p.zd + 1;
-  for (int idx = threadIdx.x; idx < 2 * WC * N; idx += NT) {
+  // Slot (which, s, w): s < nzs is stored plane s (z = s - zd, wrapped);
+  // the other slots are the zero planes zd+1 .. N-zd-1.  All loads of a
+  // thread use clamped addresses and no branches, so they are in flight at
+  // once.
+  constexpr int EPT = (2 * WC * N + NT - 1) / NT;
+  float2 v[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = threadIdx.x + k * NT < 2 * WC * N ? threadIdx.x + k * NT : 2 * WC * N - 1;
     const int which = idx / (WC * N);
     const int rem = idx - which * WC * N;
-    const int z = rem / WC, w = rem - (rem / WC) * WC;
-    const long long e = e0 + w;
-    int z0 = 0;
-    bool valid = e < nxy;
-    if (z <= p.zd)
-      z0 = z;
-    else if (z >= N - p.zd)
-      z0 = z - N;
-    else
-      valid = false;
-    lds[(which * WC + w) * LP + lds_swz<N>(z)] =
-        valid ? ovx[((long long)which * nzs + (z0 + p.zd)) * nxy + e] : make_float2(0.f, 0.f);
+    const int sp = rem / WC, w = rem - (rem / WC) * WC;
+    const long long e = e0 + w < nxy ? e0 + w : nxy - 1;
+    const int sc = sp < nzs ? sp : 0;
+    v[k] = ovx[((long long)which * nzs + sc) * nxy + e];
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = threadIdx.x + k * NT;
+    const int which = idx / (WC * N);
+    const int rem = idx - which * WC * N;
+    const int sp = rem / WC, w = rem - (rem / WC) * WC;
+    const int z = sp < nzs ? (sp - p.zd >= 0 ? sp - p.zd : sp - p.zd + N) : sp - nzs + p.zd + 1;
+    if (idx < 2 * WC * N)
+      lds[(which * WC + w) * LP + lds_swz<N>(z)] = (sp < nzs && e0 + w < nxy) ? v[k] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
