@@ -100,6 +100,243 @@ def profiled_traffic(kernel_prefix):
     return None
 
 
+def bench_series(T=64, distinct=4, slots=4):
+    """Time series of 64 stacks of 256x256x32 (3 dirs x 5 phases, uint16) -> 512x512x32 f32 each.
+
+    `distinct` simulated frames (each with its own k0 draw, +-0.05 px) are
+    cycled over the T time points, and frame t is scaled by the 2 %/frame
+    bleach decay 0.98^t; every stack is staged in HBM before timing and every
+    result is copied into its own device buffer.  Plain: one context, one
+    stack after another.  Batched: the serving loop with `slots` stacks in
+    flight as one series (t = 0 first, then all others on any context with the
+    time-0 k0).  The two outputs are compared element for element.
+    """
+    import torch
+
+    from cudasirecon_amd import engine, serving, synth
+
+    print("bench: series: generating %d distinct 256x256x32 frames" % distinct, file=sys.stderr, flush=True)
+    bases = []
+    for i in range(distinct):
+        p, tab, raw, _ = synth.make_dataset(256, 256, 32, seed=20263202 + i, k0_jitter_px=0.05, gpu=True)
+        bases.append(torch.from_numpy(raw.astype(np.int32)).to("cuda"))
+    p.ntimes = T
+    d_raw = [torch.clamp(torch.round(bases[t % distinct].float() * (0.98 ** t)), 0, 65535).to(torch.int32)
+             .to(torch.int16) for t in range(T)]  # int16 storage, uint16 bits
+    del bases
+    out_elems = int(np.prod(engine.output_shape(p)))
+    d_plain = torch.empty((T, out_elems), dtype=torch.float32, device="cuda")
+    d_batch = torch.empty((T, out_elems), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    vox = T * out_elems
+
+    r = engine.Reconstructor(p, tab)
+
+    def plain(n):
+        for t in range(n):
+            r.set_raw_device(d_raw[t].data_ptr(), u16=True)
+            r.load_and_rescale(t)
+            r.set_time(t)
+            r.process_volume()
+            r.result_to_device(d_plain[t].data_ptr())
+        r.sync()
+
+    plain(min(T, 4))  # warm-up
+    t0 = time.perf_counter()
+    plain(T)
+    t_plain = time.perf_counter() - t0
+    r.close()
+
+    budget = (slots + 1) * engine.footprint(p)
+    with serving.Server(hbm_budget=budget, max_inflight=slots) as srv:
+        sid = srv.add_setup(p, tab)
+        shape = (p.nx, p.ny, p.nz)
+
+        def batched(series, n):
+            tks = [srv.submit(sid, (d_raw[t].data_ptr(), True), out=d_batch[t].data_ptr(), shape=shape,
+                              series=series, t=t) for t in range(n)]
+            return [srv.wait(tk) for tk in tks]
+
+        batched(0, min(T, 2 * slots))  # warm-up: creates and caches the contexts
+        t0 = time.perf_counter()
+        res = batched(1, T)
+        t_batch = time.perf_counter() - t0
+        st = srv.stats()
+    diff = float((d_batch - d_plain).abs().max().item())
+    return {
+        "workload": "time series: %d x 256x256x32, 3 dirs x 5 phases uint16 -> 512x512x32 f32, %d distinct "
+                    "simulated frames (k0 draws +-0.05 px) cycled, 2%%/frame bleach, stacks resident in HBM" % (T, distinct),
+        "time_points": T,
+        "plain_voxels_per_s": round(vox / t_plain, 1),
+        "plain_ms_per_stack": round(t_plain / T * 1e3, 3),
+        "batched_voxels_per_s": round(vox / t_batch, 1),
+        "batched_ms_per_stack": round(t_batch / T * 1e3, 3),
+        "speedup_vs_plain": round(t_plain / t_batch, 3),
+        "in_flight": slots,
+        "contexts_created": int(st["contexts_created"]),
+        "max_abs_diff_vs_plain": diff,
+        "mean_latency_ms": round(float(np.mean([x.latency_ms for x in res])), 3),
+    }
+
+
+def bench_serving(n_requests=200, rate=20.0, slots=4, seed=20264202):
+    """Serving stream: Poisson arrivals at `rate` req/s mixing 2D 512^2 (3 phases), 3D 256^2x24,
+    512^2x64 and 1024^2x16 (5 phases), uint16 stacks from host memory, results back to host.
+
+    Four optical setups stay registered (one 2D, three 3D that differ in
+    emission wavelength and NA); the 3D requests take their setup in turn.
+    One simulated stack per request kind is reused for every request of that
+    kind.  Latency = completion - arrival (submission at the arrival time).
+    Also reported: the same request list submitted all at once (closed loop)
+    against one plain context per kind processing it sequentially.
+    """
+    import threading
+
+    from cudasirecon_amd import engine, serving, synth
+
+    print("bench: serving: generating the request kinds", file=sys.stderr, flush=True)
+    kinds = []  # (setup index, params, raw)
+    p2, t2, raw2, _ = synth.make_dataset(512, 512, 1, nphases=3, seed=seed, gpu=True)
+    setups = [(p2, t2)]
+    optics = [dict(wave=515, na=1.42), dict(wave=605, na=1.42), dict(wave=515, na=1.27)]
+    for o in optics:
+        p3, t3, _, _ = synth.make_dataset(64, 64, 24, seed=seed + 1, **o)
+        setups.append((p3, t3))
+    kinds.append((0, (512, 512, 1), raw2))
+    for j, (nx, ny, nz) in enumerate([(256, 256, 24), (512, 512, 64), (1024, 1024, 16)]):
+        _, _, raw3, _ = synth.make_dataset(nx, ny, nz, seed=seed + 10 + j, gpu=True)
+        kinds.append((None, (nx, ny, nz), raw3))
+    rng = np.random.default_rng(seed)
+    plan = []  # (arrival s, kind, setup)
+    t = 0.0
+    n3 = 0
+    for i in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        k = int(rng.integers(0, 4))
+        s = 0 if k == 0 else 1 + (n3 % 3)
+        n3 += k != 0
+        plan.append((t, k, s))
+    shapes = {}
+    for k, (_, shape, _) in enumerate(kinds):
+        for s in ([0] if k == 0 else [1, 2, 3]):
+            q = setups[s][0].copy()
+            q.nx, q.ny, q.nz = shape
+            shapes[(k, s)] = engine.output_shape(q)
+    need = []
+    for (k, s), _ in shapes.items():
+        q = setups[s][0].copy()
+        q.nx, q.ny, q.nz = kinds[k][1]
+        need.append(engine.footprint(q))
+    # one context of every (kind, setup) plus slots-1 more of the largest fit at once (the cache keeps the
+    # OTFs and arenas resident); admission still bounds the stacks in flight to `slots`
+    budget = int(min((sum(need) + (slots - 1) * max(need)) * 1.05, 200e9))
+    vox_of = {ks: int(np.prod(v)) for ks, v in shapes.items()}
+
+    def run_stream(srv, ids, paced):
+        free = {ks: [np.empty(v, np.float32) for _ in range(slots + 2)] for ks, v in shapes.items()}
+        cond = threading.Condition()
+        lat, vox = [], [0]
+        tickets = []
+        done = threading.Event()
+
+        def collector():
+            i = 0
+            while i < len(plan):
+                with cond:
+                    while i >= len(tickets):
+                        cond.wait()
+                    tk, ks, buf, arr_us = tickets[i]
+                res = srv.wait(tk)
+                lat.append((res.t_end_us - arr_us) * 1e-6)
+                vox[0] += vox_of[ks]
+                with cond:
+                    free[ks].append(buf)
+                    cond.notify_all()
+                i += 1
+            done.set()
+
+        th = threading.Thread(target=collector)
+        th.start()
+        t0_us = serving_now_us()
+        for t_arr, k, s in plan:
+            arr_us = t0_us + (t_arr * 1e6 if paced else 0.0)  # arrival time; latency counts from here
+            if paced:
+                dt = (arr_us - serving_now_us()) * 1e-6
+                if dt > 0:
+                    time.sleep(dt)
+            ks = (k, s)
+            with cond:
+                while not free[ks]:
+                    cond.wait()
+                buf = free[ks].pop()
+            tk = srv.submit(ids[s], kinds[k][2], out=buf, shape=kinds[k][1])
+            with cond:
+                tickets.append((tk, ks, buf, arr_us))
+                cond.notify_all()
+        done.wait()
+        th.join()
+        wall = (serving_now_us() - t0_us) * 1e-6
+        return lat, vox[0], wall
+
+    with serving.Server(hbm_budget=budget, max_inflight=slots) as srv:
+        ids = [srv.add_setup(p, tab) for p, tab in setups]
+        # warm-up: one request of every (kind, setup), creates the contexts
+        for (k, s) in shapes:
+            out = np.empty(shapes[(k, s)], np.float32)
+            srv.wait(srv.submit(ids[s], kinds[k][2], out=out, shape=kinds[k][1]))
+        lat, vox, wall = run_stream(srv, ids, paced=True)
+        lat_c, vox_c, wall_c = run_stream(srv, ids, paced=False)
+        st = srv.stats()
+    # sequential plain path over the same list: one context per (kind, setup), one request after another
+    ctxs = {}
+    for (k, s) in shapes:
+        q = setups[s][0].copy()
+        q.nx, q.ny, q.nz = kinds[k][1]
+        ctxs[(k, s)] = engine.Reconstructor(q, setups[s][1])
+    outs = {ks: np.empty(v, np.float32) for ks, v in shapes.items()}
+
+    def plain_one(k, s):
+        r = ctxs[(k, s)]
+        r.set_raw(kinds[k][2])
+        r.load_and_rescale(0)
+        r.set_time(0)
+        r.process_volume()
+        engine._check(engine.lib().simr_get_result_host(r.h, engine._fp(outs[(k, s)])))
+
+    for (k, s) in shapes:
+        plain_one(k, s)
+    t0 = time.perf_counter()
+    for _, k, s in plan:
+        plain_one(k, s)
+    wall_p = time.perf_counter() - t0
+    for r in ctxs.values():
+        r.close()
+    ms = [x * 1e3 for x in lat]
+    return {
+        "workload": "serving: %d requests, Poisson arrivals at %g req/s, uniform mix of 2D 512x512 (3 phases), "
+                    "3D 256x256x24, 512x512x64, 1024x1024x16 (5 phases), 4 optical setups, uint16 stacks from "
+                    "host memory, results to host" % (n_requests, rate),
+        "requests": n_requests,
+        "in_flight": slots,
+        "p50_latency_ms": round(serving.percentile(ms, 50), 3),
+        "p99_latency_ms": round(serving.percentile(ms, 99), 3),
+        "mean_latency_ms": round(float(np.mean(ms)), 3),
+        "paced_voxels_per_s": round(vox / wall, 1),
+        "closed_loop_voxels_per_s": round(vox_c / wall_c, 1),
+        "closed_loop_p50_latency_ms": round(serving.percentile([x * 1e3 for x in lat_c], 50), 3),
+        "sequential_plain_voxels_per_s": round(vox_c / wall_p, 1),
+        "closed_loop_speedup_vs_plain": round(wall_p / wall_c, 3),
+        "contexts_created": int(st["contexts_created"]),
+        "contexts_evicted": int(st["contexts_evicted"]),
+        "hbm_budget_bytes": budget,
+    }
+
+
+def serving_now_us():
+    """The steady clock the serving loop stamps its requests with (CLOCK_MONOTONIC)."""
+    return time.monotonic_ns() * 1e-3
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +347,12 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device-to-device copy kernel three times, the known-size "
                          "reference that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
+    ap.add_argument("--no-series", action="store_true", help="skip the time-series batching measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream measurement")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests in the serving stream (2000 gives the full 100 s stream at 20 req/s)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--slots", type=int, default=4, help="stacks in flight in the batched/serving forms")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is described in DESIGN.md "
@@ -129,7 +372,7 @@ def main():
         del src, dst
 
     print("bench: generating the synthetic 512x512x48 stack", file=sys.stderr, flush=True)
-    p, tab, raw, truth = synth.make_dataset(512, 512, 48, seed=20262202)
+    p, tab, raw, truth = synth.make_dataset(512, 512, 48, seed=20262202, gpu=True)
     p.fit_fp64 = 0 if args.fit_fp32 else 1
     r = engine.Reconstructor(p, tab)
     r.set_raw(raw)  # resident in HBM from here on
@@ -225,6 +468,14 @@ def main():
                            "ms_per_stack": round(pcie_dt * 1e3, 3),
                            "note": "host uint16 stack copied in and the float32 volume copied out each stack"},
     }
+    if not args.no_series:
+        res["batched"] = bench_series(slots=args.slots)
+        print("bench: series %s" % json.dumps(res["batched"]), file=sys.stderr, flush=True)
+    if not args.no_serve:
+        sv = bench_serving(n_requests=args.serve_requests, rate=args.serve_rate, slots=args.slots)
+        res["serving"] = sv
+        res["p50_latency_ms"], res["p99_latency_ms"] = sv["p50_latency_ms"], sv["p99_latency_ms"]
+        print("bench: serving %s" % json.dumps(sv), file=sys.stderr, flush=True)
     if not args.no_cpu:
         print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline(p, tab, raw)

@@ -97,6 +97,33 @@ class SimrOtfInfo(C.Structure):
     ]
 
 
+class SimrRequest(C.Structure):
+    """simr_request: one stack for the serving loop."""
+
+    _fields_ = [
+        ("setup", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("raw", C.c_void_p), ("raw_u16", C.c_int32), ("raw_on_device", C.c_int32),
+        ("out", C.c_void_p), ("out_on_device", C.c_int32), ("series", C.c_int32), ("t", C.c_int32),
+        ("reserved", C.c_int32 * 4),
+    ]
+
+
+class SimrRequestInfo(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("context_reused", C.c_int32),
+        ("t_submit_us", C.c_double), ("t_start_us", C.c_double), ("t_end_us", C.c_double),
+        ("out_elems", C.c_size_t), ("fit", SimrFit * SIMR_MAX_DIRS), ("error", C.c_char * 256),
+    ]
+
+
+class SimrServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+        ("contexts_created", C.c_int64), ("contexts_evicted", C.c_int64),
+        ("bytes_in_use", C.c_size_t), ("bytes_peak", C.c_size_t),
+    ]
+
+
 def otf_default_params():
     """makeotf defaults (src/otf/radialft.cpp:86-114): restated for the Python side."""
     p = SimrOtfParams()

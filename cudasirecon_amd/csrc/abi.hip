@@ -114,6 +114,12 @@ size_t simr_otf_size(const simr_params* p) {
   return (size_t)p->nzotf * p->nxotf * p->nyotf;
 }
 
+size_t simr_otf_total(const simr_params* p) {
+  if (!p) return 0;
+  const int norders = p->norders_output != 0 ? p->norders_output : p->nphases / 2 + 1;
+  return (size_t)(p->bOneOTFperAngle ? p->ndirs : 1) * (size_t)(norders > 0 ? norders : 0) * simr_otf_size(p);
+}
+
 size_t simr_footprint(const simr_params* p) {
   size_t n = 0;
   int rc = guard([&] { n = simr::Layout::of(*p).bytes_total; });
@@ -191,11 +197,29 @@ int simr_process_volume(simr_ctx* ctx) {
 
 size_t simr_result_elems(const simr_ctx* ctx) { return ctx && ctx->eng ? ctx->eng->result_elems() : 0; }
 int simr_get_result_host(simr_ctx* ctx, float* out) { return guard([&] { E(ctx).get_result(out); }); }
+int simr_get_result_dev(simr_ctx* ctx, float* out_dev) {
+  return guard([&] {
+    if (!out_dev) throw simr::ArgError("null output pointer");
+    simr::Engine& e = E(ctx);
+    SIMR_HIP(hipMemcpyAsync(out_dev, e.result_dev(), e.result_elems() * sizeof(float), hipMemcpyDeviceToDevice,
+                            e.stream()));
+    e.sync();
+  });
+}
 const float* simr_result_dev(simr_ctx* ctx) { return ctx && ctx->eng ? ctx->eng->result_dev() : nullptr; }
 int simr_get_fit(const simr_ctx* ctx, int dir, simr_fit* out) {
   return guard([&] { *out = E(const_cast<simr_ctx*>(ctx)).fit(dir); });
 }
 int simr_set_fit(simr_ctx* ctx, int dir, const simr_fit* in) { return guard([&] { E(ctx).set_fit(dir, *in); }); }
+size_t simr_time0_state_size(const simr_ctx* ctx) {
+  return ctx && ctx->eng ? ctx->eng->time0_state_size() : 0;
+}
+int simr_get_time0_state(simr_ctx* ctx, void* host, size_t bytes) {
+  return guard([&] { E(ctx).get_time0_state(host, bytes); });
+}
+int simr_set_time0_state(simr_ctx* ctx, const void* host, size_t bytes) {
+  return guard([&] { E(ctx).set_time0_state(host, bytes); });
+}
 int simr_get_band_host(simr_ctx* ctx, int dir, int band, float* out) {
   return guard([&] { E(ctx).get_band(dir, band, out); });
 }

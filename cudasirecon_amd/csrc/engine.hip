@@ -976,6 +976,27 @@ void Engine::set_fit(int dir, const simr_fit& f) {
   fitted_ = true;
 }
 
+// Time-0 state: [ndirs] float2 k0 fitted at t = 0, then [nz] double
+// bleach-correction reference sums of t = 0.
+size_t Engine::time0_state_size() const { return (size_t)L.ndirs * sizeof(float2) + (size_t)L.nz * sizeof(double); }
+
+void Engine::get_time0_state(void* host, size_t bytes) {
+  SIMR_REQUIRE(host && bytes == time0_state_size(), "time-0 state buffer has the wrong size");
+  std::memcpy(host, k0_time0_.data(), (size_t)L.ndirs * sizeof(float2));
+  SIMR_HIP(hipMemcpyAsync((char*)host + (size_t)L.ndirs * sizeof(float2), sdp_, (size_t)L.nz * sizeof(double),
+                          hipMemcpyDeviceToHost, stream_));
+  sync();
+}
+
+void Engine::set_time0_state(const void* host, size_t bytes) {
+  SIMR_REQUIRE(host && bytes == time0_state_size(), "time-0 state buffer has the wrong size");
+  std::memcpy(k0_time0_.data(), host, (size_t)L.ndirs * sizeof(float2));
+  k0_ = k0_time0_;  // what a sequential context carries into t > 0
+  SIMR_HIP(hipMemcpyAsync(sdp_, (const char*)host + (size_t)L.ndirs * sizeof(float2), (size_t)L.nz * sizeof(double),
+                          hipMemcpyHostToDevice, stream_));
+  sync();
+}
+
 void Engine::get_band(int dir, int band, float* host) {
   SIMR_REQUIRE(dir >= 0 && dir < L.ndirs && band >= 0 && band < L.nbands, "band index out of range");
   SIMR_HIP(hipMemcpyAsync(host, bands_ + ((size_t)dir * L.nbands + band) * L.H, L.H * 8, hipMemcpyDeviceToHost,

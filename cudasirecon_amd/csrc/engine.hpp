@@ -58,6 +58,9 @@ class Engine {
   size_t result_elems() const { return (size_t)L.zdim * L.ydim * L.xdim; }
   simr_fit fit(int dir) const;
   void set_fit(int dir, const simr_fit& f);
+  size_t time0_state_size() const;
+  void get_time0_state(void* host, size_t bytes);
+  void set_time0_state(const void* host, size_t bytes);
   void get_band(int dir, int band, float* host);
   void set_band(int dir, int band, const float* host);
   void debug_overlaps(int dir, int o1, int o2, float k0x, float k0y, double* P, double* sx, double* sy);

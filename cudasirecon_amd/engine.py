@@ -9,7 +9,8 @@ import os
 
 import numpy as np
 
-from ._abi import SimrFit, SimrOtfInfo, SimrOtfParams, SimrParams, norders_of, output_shape
+from ._abi import (SimrFit, SimrOtfInfo, SimrOtfParams, SimrParams, SimrRequest, SimrRequestInfo, SimrServerStats,
+                   norders_of, output_shape)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcudasirecon.so")
@@ -41,6 +42,7 @@ def lib():
         "simr_params_size": (C.c_size_t, []),
         "simr_fit_size": (C.c_size_t, []),
         "simr_otf_size": (C.c_size_t, [P]),
+        "simr_otf_total": (C.c_size_t, [P]),
         "simr_footprint": (C.c_size_t, [P]),
         "simr_create": (ip, [P, fp, ip, C.POINTER(vp)]),
         "simr_destroy": (ip, [vp]),
@@ -75,6 +77,16 @@ def lib():
         "simr_otf_params_default": (None, [C.POINTER(SimrOtfParams)]),
         "simr_otf_elems": (C.c_size_t, [C.POINTER(SimrOtfParams), ip, ip, ip]),
         "simr_makeotf": (ip, [C.POINTER(SimrOtfParams), fp, ip, ip, ip, fp, fp, fp, C.POINTER(SimrOtfInfo)]),
+        "simr_get_result_dev": (ip, [vp, vp]),
+        "simr_time0_state_size": (C.c_size_t, [vp]),
+        "simr_get_time0_state": (ip, [vp, vp, C.c_size_t]),
+        "simr_set_time0_state": (ip, [vp, vp, C.c_size_t]),
+        "simr_server_create": (ip, [ip, C.c_size_t, ip, C.POINTER(vp)]),
+        "simr_server_add_setup": (ip, [vp, P, fp, C.POINTER(ip)]),
+        "simr_server_submit": (ip, [vp, C.POINTER(SimrRequest), C.POINTER(C.c_int64)]),
+        "simr_server_wait": (ip, [vp, C.c_int64, C.POINTER(SimrRequestInfo)]),
+        "simr_server_stats_get": (ip, [vp, C.POINTER(SimrServerStats)]),
+        "simr_server_destroy": (ip, [vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -227,6 +239,10 @@ class Reconstructor:
 
     def result_device_ptr(self):
         return lib().simr_result_dev(self.h)
+
+    def result_to_device(self, ptr):
+        """Copy the result into caller-owned device memory (simr_get_result_dev)."""
+        _check(lib().simr_get_result_dev(self.h, C.c_void_p(ptr)))
 
     def fit(self, d):
         f = SimrFit()

@@ -164,11 +164,74 @@ def simulate(sample, ndirs, nphases, k0_list, dxy, dz, otf_fn, modamps, phase0=N
     return out
 
 
+def simulate_gpu(sample, ndirs, nphases, k0_list, dxy, dz, otf_fn, modamps, phase0=None, seed=0,
+                 photons=None, background=90.0, u16=True, fastSI=False):
+    """simulate() with the transforms, products and Poisson draws on the GPU (torch).
+
+    Same forward model; the noise comes from torch's generator, so a stack
+    differs from simulate()'s for the same seed.  For generating large
+    benchmark stacks quickly; the tests use simulate().
+    """
+    import torch
+
+    dev = "cuda"
+    nz, ny, nx = sample.shape
+    norders = nphases // 2 + 1
+    ky = np.fft.fftfreq(ny, d=dxy)
+    kx = np.fft.fftfreq(nx, d=dxy)
+    kz = np.fft.fftfreq(nz, d=dz) if nz > 1 else np.zeros(1)
+    KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+    KR = np.sqrt(KX ** 2 + KY ** 2)
+    otfs = [torch.from_numpy(np.ascontiguousarray(otf_fn(o, KR, KZ), dtype=np.float32)).to(dev)
+            for o in range(norders)]
+    del KZ, KY, KX, KR
+    S = torch.from_numpy(np.ascontiguousarray(sample, dtype=np.float32)).to(dev).to(torch.complex64)
+    y = torch.arange(ny, device=dev, dtype=torch.float64)[None, :, None] * dxy
+    x = torch.arange(nx, device=dev, dtype=torch.float64)[None, None, :] * dxy
+    if phase0 is None:
+        phase0 = [0.0] * ndirs
+    imgs = torch.zeros((ndirs, nz, nphases, ny, nx), dtype=torch.float32, device=dev)
+    for d in range(ndirs):
+        comps = []
+        for o in range(norders):
+            if o == 0:
+                field = S
+            else:
+                ph = torch.exp(2j * math.pi * o * (k0_list[d][0] * x + k0_list[d][1] * y)).to(torch.complex64)
+                field = S * ph
+            comps.append(torch.fft.ifftn(otfs[o] * torch.fft.fftn(field)))
+        for p in range(nphases):
+            phi = 2 * math.pi * p / nphases + phase0[d]
+            im = comps[0].real.clone()
+            for o in range(1, norders):
+                im += 2 * torch.real(modamps[o - 1] * comps[o] * complex(math.cos(o * phi), math.sin(o * phi)))
+            imgs[d, :, p] = im
+        del comps
+    imgs.clamp_(min=0)
+    if photons is not None:
+        scale = photons / max(float(imgs.max()), 1e-12)
+        g = torch.Generator(device=dev)
+        g.manual_seed(int(seed))
+        imgs = torch.poisson(imgs * scale + background, generator=g)
+    else:
+        imgs += background
+    out = imgs.transpose(0, 1) if fastSI else imgs
+    out = out.reshape(-1, ny, nx)
+    if u16:
+        out = torch.clamp(torch.round(out), 0, 65535).to(torch.int32).cpu().numpy().astype(np.uint16)
+    else:
+        out = out.cpu().numpy()
+    torch.cuda.synchronize()
+    return out
+
+
 def make_dataset(nx, ny, nz, ndirs=3, nphases=5, seed=20261202, dxy=0.08, dz=0.125, wave=515, na=1.42,
                  nimm=1.515, ls=0.2035, angle0=1.648, modamps=(0.4, 0.2), photons=5000.0, background=90.0,
                  nbeads=None, nfil=20, nzotf=64, wiener=0.001, u16=True, angles=None, k0_jitter_px=0.0,
-                 **param_overrides):
-    """A complete synthetic case: (params, otf_table, raw, truth dict)."""
+                 gpu=False, **param_overrides):
+    """A complete synthetic case: (params, otf_table, raw, truth dict).
+
+    gpu=True simulates with simulate_gpu (torch on the GPU; different noise draws)."""
     norders = nphases // 2 + 1
     two_d = nz == 1
     if angles is None:
@@ -186,8 +249,8 @@ def make_dataset(nx, ny, nz, ndirs=3, nphases=5, seed=20261202, dxy=0.08, dz=0.1
     tab, og = make_otf_table(norders, na, nimm, wave, dxy, nzotf=nzotf, two_d=two_d)
     fn = lambda o, kr, kz: otf_model(o, kr, kz, na, nimm, wave)  # noqa: E731
     ma = list(modamps)[: norders - 1]
-    raw = simulate(sample, ndirs, nphases, k0, dxy, dz, fn, ma, seed=seed + 1, photons=photons,
-                   background=background, u16=u16)
+    raw = (simulate_gpu if gpu else simulate)(sample, ndirs, nphases, k0, dxy, dz, fn, ma, seed=seed + 1,
+                                              photons=photons, background=background, u16=u16)
     p = default_params()
     set_params(p, nx=nx, ny=ny, nz=nz, ndirs=ndirs, nphases=nphases, dxy=dxy, dz=dz if nz > 1 else 0.0,
                wave=wave, na=na, nimm=nimm, linespacing=ls, k0angles=list(angles), wiener=wiener,

@@ -101,6 +101,8 @@ size_t simr_fit_size(void);
 
 /* Number of complex OTF entries per (direction, order) for these parameters. */
 size_t simr_otf_size(const simr_params* p);
+/* Complex entries of the whole OTF argument of simr_create: (ndirs if bOneOTFperAngle else 1) x norders x simr_otf_size. */
+size_t simr_otf_total(const simr_params* p);
 /* Device bytes one context needs for these parameters (arena footprint). */
 size_t simr_footprint(const simr_params* p);
 
@@ -141,10 +143,24 @@ int simr_process_volume(simr_ctx* ctx);              /* find_modulation + filter
 /* Results. out: (zoom*nx)*(zoom*ny)*(z_zoom*nz) floats, layout [z][y][x]. */
 size_t simr_result_elems(const simr_ctx* ctx);
 int simr_get_result_host(simr_ctx* ctx, float* out);
+/* Copies the result into caller-owned device memory (on the context stream, then synchronises). */
+int simr_get_result_dev(simr_ctx* ctx, float* out_dev);
 const float* simr_result_dev(simr_ctx* ctx);
 int simr_get_fit(const simr_ctx* ctx, int dir, simr_fit* out);
 /* Override k0 (1/micron) and modamps of one direction (known-k0 processing, parity checks). */
 int simr_set_fit(simr_ctx* ctx, int dir, const simr_fit* in);
+
+/*
+ * Time-0 state of a time series: the k0 of every direction fitted at t = 0
+ * (which t > 0 reuses when bUseTime0k0, cudaSirecon.cpp:296-331) and the
+ * bleach-correction reference sums of t = 0 (which t > 0 reuses when
+ * equalizet, gpuFunctionsImpl.cu:140-148).  Copying it from the context that
+ * processed t = 0 into other contexts of the same shape lets later time
+ * points run on any of them with the results of one sequential context.
+ */
+size_t simr_time0_state_size(const simr_ctx* ctx);
+int simr_get_time0_state(simr_ctx* ctx, void* host, size_t bytes);
+int simr_set_time0_state(simr_ctx* ctx, const void* host, size_t bytes);
 
 /* Stage inspection (parity tests): copy one separated band of one direction
  * (half-complex spectrum [z][y][nx/2+1], complex float) to the host. */
@@ -231,6 +247,63 @@ void* simr_stream(simr_ctx* ctx);
  * the context stream.  names/ms arrays of capacity cap; returns count. */
 int simr_stage_timings(simr_ctx* ctx, const char** names, float* ms, int cap);
 int simr_enable_timing(simr_ctx* ctx, int on);
+
+/*
+ * Serving: a request loop over many stacks of mixed sizes on one GPU (no
+ * counterpart in the reference, which reconstructs one stack per process).
+ *
+ * Optical setups (parameters + OTF) are registered once.  Each request names
+ * a setup and a stack shape; up to max_inflight requests run at once, each
+ * on its own context (own stream and arena), so the work of several stacks
+ * overlaps on the device.  Contexts are cached per (setup, shape) and reused,
+ * which keeps the OTFs resident; a request is admitted when a cached idle
+ * context of its key exists or a new one fits in hbm_budget bytes (idle
+ * contexts of other keys are evicted, least recently used first).  Requests
+ * are taken in arrival order, skipping over ones that cannot be admitted yet.
+ * Time series: a request with series >= 0 and t > 0 waits for t = 0 of its
+ * series, then runs on any context with that series' time-0 state, so its
+ * result is the one a single sequential context gives.
+ */
+typedef struct simr_server simr_server;
+
+typedef struct simr_request {
+  int32_t setup;           /* id from simr_server_add_setup */
+  int32_t nx, ny, nz;      /* stack shape; nImages = nz * ndirs * nphases of the setup */
+  const void* raw;         /* the stack in the setup's section order */
+  int32_t raw_u16;         /* 1: uint16 pixels, 0: float32 */
+  int32_t raw_on_device;   /* 1: raw is a device pointer */
+  float* out;              /* receives the zoomed volume [z][y][x] (simr_result_elems floats) */
+  int32_t out_on_device;   /* 1: out is a device pointer */
+  int32_t series;          /* -1, or the id of the time series this stack belongs to */
+  int32_t t;               /* time index within the series */
+  int32_t reserved[4];
+} simr_request;
+
+typedef struct simr_request_info {
+  int32_t status;          /* SIMR_OK or the error code of the failed stage */
+  int32_t context_reused;  /* 1: ran on a cached context */
+  double t_submit_us, t_start_us, t_end_us; /* steady-clock times of submission, admission, completion */
+  size_t out_elems;
+  simr_fit fit[SIMR_MAX_DIRS];
+  char error[256];
+} simr_request_info;
+
+typedef struct simr_server_stats {
+  int64_t submitted, completed, failed;
+  int64_t contexts_created, contexts_evicted;
+  size_t bytes_in_use, bytes_peak;
+} simr_server_stats;
+
+int simr_server_create(int device, size_t hbm_budget, int max_inflight, simr_server** out);
+/* p: the setup's parameters (nx, ny, nz are taken from each request); otf as for simr_create. */
+int simr_server_add_setup(simr_server* srv, const simr_params* p, const float* otf, int* setup_id);
+/* Queue a request; returns at once with its ticket.  raw and out must stay valid until it completes. */
+int simr_server_submit(simr_server* srv, const simr_request* rq, int64_t* ticket);
+/* Block until the request completes; info may be NULL.  Returns the request's status. */
+int simr_server_wait(simr_server* srv, int64_t ticket, simr_request_info* info);
+int simr_server_stats_get(simr_server* srv, simr_server_stats* st);
+/* Waits for every queued request, then frees every context. */
+int simr_server_destroy(simr_server* srv);
 
 #ifdef __cplusplus
 }
