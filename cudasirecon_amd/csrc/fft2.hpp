@@ -211,15 +211,115 @@ __device__ __forceinline__ void fft2_sync() {
     __syncthreads();
 }
 
+// Per-thread twiddles of the two-pass core.  Thread t multiplies column
+// n2 = t + N1 c by W_N^{n2 k1}, k1 < N1, whatever row it transforms, so the
+// factors are built once per kernel: W^{n2 k1} = W^{n2 8a} W^{n2 b} with
+// k1 = 8a + b, from 8 + N1/8 table values per column (one product, about one
+// ulp more than the table entry, instead of N1 - 1 dependent loads per row).
+template <int N>
+struct Fft2Tw {
+  static constexpr int N1 = fft2_plan(N).n1, MC = fft2_plan(N).n2 / (N1 > 0 ? N1 : 1);
+  static constexpr int NB = N1 < 8 ? N1 : 8, NA = (N1 + 7) / 8;
+  float2 lo[MC][NB];  // W^{n2 b}
+  float2 hi[MC][NA];  // W^{n2 8 a}
+  // tab: exp(-2 pi i m / N), m < N (global or LDS)
+  __device__ __forceinline__ void load(int t, const float2* __restrict__ tab) {
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+      const int n2 = t + N1 * c;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) lo[c][b] = tab[n2 * b];  // n2 b < N2 N1 = N
+#pragma unroll
+      for (int a = 0; a < NA; ++a) hi[c][a] = tab[n2 * 8 * a];  // 8a < N1
+    }
+  }
+  template <int C, int K1, bool INV>
+  __device__ __forceinline__ float2 get() const {
+#pragma clang fp contract(off)
+    constexpr int A = K1 / 8, B = K1 % 8;
+    float2 w;
+    if constexpr (A == 0)
+      w = lo[C][B];
+    else if constexpr (B == 0)
+      w = hi[C][A];
+    else
+      w = cmul(hi[C][A], lo[C][B]);
+    if (INV) w.y = -w.y;
+    return w;
+  }
+};
+
+// v[K1] *= W^{n2 K1} for K1 = 1 .. N1-1 of column slot C
+template <int N, bool INV, int C, int K1 = 1>
+__device__ __forceinline__ void fft2_twiddle(float2* v, const Fft2Tw<N>& tw) {
+  if constexpr (K1 < Fft2Tw<N>::N1) {
+    v[K1] = cmul(v[K1], tw.template get<C, K1, INV>());
+    fft2_twiddle<N, INV, C, K1 + 1>(v, tw);
+  }
+}
+
+template <int N, bool INV, int C = 0>
+__device__ __forceinline__ void fft2_twiddle_all(float2* v, const Fft2Tw<N>& tw) {
+  if constexpr (C < Fft2Tw<N>::MC) {
+    fft2_twiddle<N, INV, C>(v + C * Fft2Tw<N>::N1, tw);
+    fft2_twiddle_all<N, INV, C + 1>(v, tw);
+  }
+}
+
 // In-place transform of one length-N row held in LDS in natural order
 // (element i at buf[i], fft2_span(N) elements available) by its TPT = N1
-// threads t.  tab: exp(-2 pi i m / N), m < N (LDS or global).
+// threads t, with the twiddles of thread t (Fft2Tw::load(t, ...)).
 // WAVE = false: contains workgroup barriers, every thread of the workgroup
 // must call it.  WAVE = true: the caller guarantees that the TPT threads of
 // each transform lie in one wavefront and that no other wavefront touches
 // buf during the call; the exchanges then synchronise the wave only.
+// The same with the twiddles read from the table at use (no twiddle
+// registers held across the call; for kernels short of registers).  The
+// scheduling fence after each product stops the compiler from hoisting all
+// N1 loads ahead of the multiplies, which would double the register
+// footprint.
 template <int N, bool INV, bool WAVE = false>
 __device__ __forceinline__ void fft2_lds(float2* buf, int t, const float2* __restrict__ tab) {
+#pragma clang fp contract(off)
+  constexpr F2Plan P = fft2_plan(N);
+  constexpr int N1 = P.n1, N2 = P.n2, MC = N2 / N1, RP = N2 + 1;
+  static_assert(N1 > 0, "no two-pass plan for this length");
+  float2 v[N2];
+  const float2* col = buf + t;
+#pragma unroll
+  for (int c = 0; c < MC; ++c)
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) v[c * N1 + n1] = col[N2 * n1 + N1 * c];
+  fft2_sync<WAVE>();
+  float2* tcol = buf + t;
+#pragma unroll
+  for (int c = 0; c < MC; ++c) {
+    const int n2 = t + N1 * c;
+    reg_dft<N1, INV>(v + c * N1);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+      float2 w = tab[n2 * k1];  // n2 k1 < N2 N1 = N
+      if (INV) w.y = -w.y;
+      v[c * N1 + k1] = cmul(v[c * N1 + k1], w);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) tcol[RP * k1 + N1 * c] = v[c * N1 + k1];
+  }
+  fft2_sync<WAVE>();
+  const float2* row = buf + RP * t;
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) v[n2] = row[n2];
+  fft2_sync<WAVE>();
+  reg_dft<N2, INV>(v);
+  float2* out = buf + t;
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) out[N1 * k2] = v[k2];
+  fft2_sync<WAVE>();
+}
+
+template <int N, bool INV, bool WAVE = false>
+__device__ __forceinline__ void fft2_lds(float2* buf, int t, const Fft2Tw<N>& tw) {
 #pragma clang fp contract(off)
   constexpr F2Plan P = fft2_plan(N);
   constexpr int N1 = P.n1, N2 = P.n2, MC = N2 / N1, RP = N2 + 1;
@@ -232,24 +332,14 @@ __device__ __forceinline__ void fft2_lds(float2* buf, int t, const float2* __res
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) v[c * N1 + n1] = col[N2 * n1 + N1 * c];
   fft2_sync<WAVE>();
+#pragma unroll
+  for (int c = 0; c < MC; ++c) reg_dft<N1, INV>(v + c * N1);
+  fft2_twiddle_all<N, INV>(v, tw);
   float2* tcol = buf + t;
 #pragma unroll
-  for (int c = 0; c < MC; ++c) {
-    const int n2 = t + N1 * c;
-    reg_dft<N1, INV>(v + c * N1);
-    // twiddles W_N^{n2 k1} (n2 k1 < N1 N2 = N); the scheduling fence after
-    // each product stops the compiler from hoisting all N1 loads ahead of the
-    // multiplies (which would double the register footprint)
-#pragma unroll
-    for (int k1 = 1; k1 < N1; ++k1) {
-      float2 w = tab[n2 * k1];
-      if (INV) w.y = -w.y;
-      v[c * N1 + k1] = cmul(v[c * N1 + k1], w);
-      __builtin_amdgcn_sched_barrier(0);
-    }
+  for (int c = 0; c < MC; ++c)
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) tcol[RP * k1 + N1 * c] = v[c * N1 + k1];
-  }
   fft2_sync<WAVE>();
   // pass 2: row k1 = t, N2-point DFT
   const float2* row = buf + RP * t;

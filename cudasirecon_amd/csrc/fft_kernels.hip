@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "fft.hpp"
+#include "fft2.hpp"
 #include "fft_api.hpp"
 
 namespace simr {
@@ -129,9 +130,127 @@ __global__ __launch_bounds__(W* fft_tpt(N)) void k_fft_rows(float2* __restrict__
   }
 }
 
+// ---------------------------------------------------------- two-pass kernels
+// Lengths with a two-pass plan (fft2.hpp) whose transforms do not straddle a
+// wavefront use the register core: each transform's row is kept in natural
+// order in LDS at a pitch of span + pad (pitch = 1 mod 32, so the lanes of a
+// tile row, W columns apart, fall in different banks), the twiddles of each
+// thread are built once in registers, and the two exchanges inside a
+// transform synchronise its wavefront only.
+template <int N>
+constexpr bool use_fft2() {
+  return fft2_ok(N) && 64 % fft2_tpt(N) == 0 && fft2_tpt(N) >= 2;
+}
+template <int N>
+constexpr int fft2_pitch() {
+  return fft2_span(N) + ((33 - fft2_span(N) % 32) % 32);
+}
+// W transforms per workgroup: at most 512 threads and 80 KB of LDS, whole waves.
+template <int N>
+constexpr int tile_w2() {
+  constexpr int tpt = fft2_tpt(N) > 0 ? fft2_tpt(N) : 1;
+  int w = 512 / tpt;
+  if (w > 64) w = 64;
+  while (w > 1 && w * fft2_pitch<N>() * 8 > 80 * 1024) w /= 2;
+  return w;
+}
+
+template <int N, bool INV, int W>
+__global__ __launch_bounds__(W* fft2_tpt(N)) void k_fft2_cols(float2* __restrict__ data, long long pitch,
+                                                             long long ncols, long long plane_stride,
+                                                             const float2* __restrict__ tw) {
+  constexpr int TPT = fft2_tpt(N);
+  constexpr int NT = W * TPT;
+  constexpr int SP = fft2_pitch<N>();
+  static_assert(NT % 64 == 0, "whole wavefronts");
+  __shared__ float2 lds[W * SP];
+  const int tid = threadIdx.x;
+  Fft2Tw<N> tw2;
+  tw2.load(tid % TPT, tw);
+  const long long c0 = (long long)blockIdx.x * W;
+  float2* base = data + (long long)blockIdx.y * plane_stride;
+  constexpr int EPT = N * W / NT;
+  float2 v[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int i = idx / W, w = idx % W;
+    const long long c = c0 + w < ncols ? c0 + w : ncols - 1;
+    v[k] = base[i * pitch + c];
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int i = idx / W, w = idx % W;
+    lds[w * SP + i] = (c0 + w < ncols) ? v[k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft2_lds<N, INV, true>(lds + (tid / TPT) * SP, tid % TPT, tw2);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int i = idx / W, w = idx % W;
+    if (c0 + w < ncols) base[i * pitch + c0 + w] = lds[w * SP + i];
+  }
+}
+
+template <int N, bool INV, int W>
+__global__ __launch_bounds__(W* fft2_tpt(N)) void k_fft2_rows(float2* __restrict__ data, long long pitch,
+                                                             long long nrows, const float2* __restrict__ tw) {
+  constexpr int TPT = fft2_tpt(N);
+  constexpr int NT = W * TPT;
+  constexpr int SP = fft2_pitch<N>();
+  static_assert(NT % 64 == 0, "whole wavefronts");
+  __shared__ float2 lds[W * SP];
+  const int tid = threadIdx.x;
+  Fft2Tw<N> tw2;
+  tw2.load(tid % TPT, tw);
+  const long long r0 = (long long)blockIdx.x * W;
+  constexpr int EPT = N * W / NT;
+  float2 v[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int w = idx / N, i = idx % N;
+    const long long r = r0 + w < nrows ? r0 + w : nrows - 1;
+    v[k] = data[r * pitch + i];
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int w = idx / N, i = idx % N;
+    lds[w * SP + i] = (r0 + w < nrows) ? v[k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft2_lds<N, INV, true>(lds + (tid / TPT) * SP, tid % TPT, tw2);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int w = idx / N, i = idx % N;
+    if (r0 + w < nrows) data[(r0 + w) * pitch + i] = lds[w * SP + i];
+  }
+}
+
 template <int N>
 static void launch_cols(float2* data, long long pitch, long long ncols, long long nplanes,
                         long long plane_stride, bool inverse, hipStream_t s) {
+  if constexpr (use_fft2<N>()) {
+    constexpr int W = tile_w2<N>();
+    if constexpr (W * fft2_tpt(N) % 64 == 0) {
+      dim3 grid(ceil_div(ncols, W), (unsigned)nplanes);
+      dim3 block(W * fft2_tpt(N));
+      if (inverse)
+        hipLaunchKernelGGL((k_fft2_cols<N, true, W>), grid, block, 0, s, data, pitch, ncols, plane_stride,
+                           twiddles(N));
+      else
+        hipLaunchKernelGGL((k_fft2_cols<N, false, W>), grid, block, 0, s, data, pitch, ncols, plane_stride,
+                           twiddles(N));
+      SIMR_LAUNCH_CHECK();
+      return;
+    }
+  }
   constexpr int W = tile_w<N>();
   dim3 grid(ceil_div(ncols, W), (unsigned)nplanes);
   dim3 block(W * fft_tpt(N));
@@ -144,6 +263,19 @@ static void launch_cols(float2* data, long long pitch, long long ncols, long lon
 
 template <int N>
 static void launch_rows(float2* data, long long pitch, long long nrows, bool inverse, hipStream_t s) {
+  if constexpr (use_fft2<N>()) {
+    constexpr int W = tile_w2<N>();
+    if constexpr (W * fft2_tpt(N) % 64 == 0) {
+      dim3 grid(ceil_div(nrows, W));
+      dim3 block(W * fft2_tpt(N));
+      if (inverse)
+        hipLaunchKernelGGL((k_fft2_rows<N, true, W>), grid, block, 0, s, data, pitch, nrows, twiddles(N));
+      else
+        hipLaunchKernelGGL((k_fft2_rows<N, false, W>), grid, block, 0, s, data, pitch, nrows, twiddles(N));
+      SIMR_LAUNCH_CHECK();
+      return;
+    }
+  }
   constexpr int W = tile_w<N>();
   dim3 grid(ceil_div(nrows, W));
   dim3 block(W * fft_tpt(N));

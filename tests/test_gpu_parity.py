@@ -22,8 +22,12 @@ def log(*a):
     sys.stdout.flush()
 
 
+# every two-pass plan (4, 8, 12, 16, 20, 24, 32, 40, 48, 60, 64, 80, 96, 128, 192, 256, 512, 1024) and the
+# radix-stage lengths, as rows, columns (y) and z lines
 FFT_SHAPES = [(64,), (48,), (9,), (512,), (1024,), (2048,), (384,), (640,), (1280,), (4096,),
-              (16, 8), (64, 64), (96, 128), (10, 96, 128), (9, 64, 64), (48, 32, 16)]
+              (4,), (8,), (12,), (20,), (24,), (60,), (80,), (128,), (192,), (256,),
+              (16, 8), (64, 64), (96, 128), (40, 24), (1024, 16), (512, 12), (10, 96, 128), (9, 64, 64),
+              (48, 32, 16), (12, 20, 8), (60, 4, 32)]
 
 
 @pytest.mark.parametrize("shape", FFT_SHAPES)
