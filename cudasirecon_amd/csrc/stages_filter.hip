@@ -62,8 +62,12 @@ __device__ __forceinline__ float2 prof_at(const float2* prof, int nxotf, float d
 // order2) loop (the z terms, 1/noiseVar, order2*k0) is computed by the same
 // expressions once, and terms whose squared distance already exceeds the
 // cut-off are skipped before the square root.
-__device__ void filter_visit(const FilterParams& p, const float2* prof, int x1, int y1, int z0, bool conj, float2& re,
-                             float2& im) {
+// ND, NO > 0: the direction and order counts as compile-time constants (the
+// (dir2, order2) loop is unrolled, so its per-term constants are read once
+// and the OTF lookups of different terms overlap); 0: taken from p.
+template <int ND, int NO>
+__device__ __forceinline__ void filter_visit(const FilterParams& p, const float2* prof, int x1, int y1, int z0,
+                                             bool conj, float2& re, float2& im) {
 #pragma clang fp contract(off)
   const int order = p.order;
   const float kx = (float)order * p.k0[p.dir].x;
@@ -95,8 +99,11 @@ __device__ void filter_visit(const FilterParams& p, const float2* prof, int x1, 
   weight *= dampfact;
   float sumweight = weight;
   const int nshift = 2 * kMaxOrders - 1;
-  for (int dir2 = 0; dir2 < p.ndirs; dir2++) {
-    for (int order2 = -(p.norders - 1); order2 < p.norders; order2++) {
+  const int ndirs = ND > 0 ? ND : p.ndirs, norders = NO > 0 ? NO : p.norders;
+#pragma unroll
+  for (int dir2 = 0; dir2 < ndirs; dir2++) {
+#pragma unroll
+    for (int order2 = -(norders - 1); order2 < norders; order2++) {
       if (dir2 == p.dir && order2 == order) continue;
       if (!p.bFilteroverlaps && !(order2 == 0 && order == 0)) continue;
       const float2 ks = p.kshift[dir2 * nshift + order2 + (kMaxOrders - 1)];
@@ -172,6 +179,7 @@ __device__ void filter_visit(const FilterParams& p, const float2* prof, int x1, 
 // Planes whose z lies outside +-zd are zeroed (kernel3).
 constexpr int kFiltPer = 4;
 
+template <int ND, int NO>
 __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
 #pragma clang fp contract(off)
   extern __shared__ float2 prof[];  // [2][norders][nxotf]
@@ -232,11 +240,11 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
     float2 re = bp[e];
     float2 im = two ? bp2[e] : make_float2(0.f, 0.f);
     // first pass (x1 in [1, nx/2]) or second pass x1 == 0: the element itself
-    filter_visit(p, prof, jin, y1n, z0n, false, re, im);
+    filter_visit<ND, NO>(p, prof, jin, y1n, z0n, false, re, im);
     // second pass, x1 = -jin < 0: the conjugate visit of the mirrored frequency
     if (two && jin >= 1 && jin <= p.nx / 2 - 1) {
       const int yc = -((iin <= p.ny / 2) ? iin : iin - p.ny);
-      filter_visit(p, prof + nprof, -jin, yc, -z0n, true, re, im);
+      filter_visit<ND, NO>(p, prof + nprof, -jin, yc, -z0n, true, re, im);
     }
     bp[e] = re;
     if (two) bp2[e] = im;
@@ -247,7 +255,13 @@ void launch_filter(const FilterParams& p, hipStream_t s) {
   const long long nplane = (long long)p.nxh * p.ny;
   const size_t lds = (size_t)2 * p.norders * p.otf[0].nxotf * sizeof(float2);
   if (lds > 64 * 1024) throw ArgError("OTF table too long for the filter's LDS profile");
-  hipLaunchKernelGGL(k_filter, dim3(ceil_div(nplane, 256 * kFiltPer), p.nz), dim3(256), lds, s, p);
+  const dim3 grid(ceil_div(nplane, 256 * kFiltPer), p.nz);
+  if (p.ndirs == 3 && p.norders == 3)  // the 3-beam 3D case
+    hipLaunchKernelGGL((k_filter<3, 3>), grid, dim3(256), lds, s, p);
+  else if (p.ndirs == 3 && p.norders == 2)  // 2D, 3 phases
+    hipLaunchKernelGGL((k_filter<3, 2>), grid, dim3(256), lds, s, p);
+  else
+    hipLaunchKernelGGL((k_filter<0, 0>), grid, dim3(256), lds, s, p);
   SIMR_LAUNCH_CHECK();
 }
 
