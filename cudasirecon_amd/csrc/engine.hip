@@ -426,6 +426,11 @@ void Engine::make_overlaps(int dir, int o1, int o2, float k0x, float k0y) {
   p.b1im = o1 == 0 ? nullptr : db + (size_t)(2 * o1) * L.H;
   p.b2re = db + (size_t)(2 * o2 - 1) * L.H;
   p.b2im = db + (size_t)(2 * o2) * L.H;
+  p.rc2 = p.rdistcutoff * p.rdistcutoff;
+  p.cut2 = p.otfcutoff * p.otfcutoff;
+  p.f2 = p.order0_2_factor * p.order0_2_factor;
+  p.inv_dkr2_1 = (float)(1.0 / ((double)p.otf1.dkrotf * (double)p.otf1.dkrotf));
+  p.inv_dkr2_2 = (float)(1.0 / ((double)p.otf2.dkrotf * (double)p.otf2.dkrotf));
   stage_begin("overlaps");
   launch_overlaps_xy(p, ovx_, stream_);
   launch_overlaps_z_reduce(p, ovx_, P_, SX_, SY_, fit_fp32_, stream_);

@@ -52,6 +52,8 @@ struct OvParams {
   int bNoKz0;
   OtfRA otf1, otf2;
   const float2 *b1re, *b1im, *b2re, *b2im;  // b1im null for order1 == 0
+  // host-derived: rdistcutoff^2, otfcutoff^2, order0_2_factor^2, 1/dkrotf^2 of otf1 and otf2
+  float rc2, cut2, f2, inv_dkr2_1, inv_dkr2_2;
 };
 // ovx: [2][2zd+1][ny][nx]; builds both overlaps and inverse-transforms along x and y.
 void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s);

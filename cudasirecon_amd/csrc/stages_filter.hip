@@ -62,6 +62,38 @@ __device__ __forceinline__ float2 prof_at(const float2* prof, int nxotf, float d
 // order2) loop (the z terms, 1/noiseVar, order2*k0) is computed by the same
 // expressions once, and terms whose squared distance already exceeds the
 // cut-off are skipped before the square root.
+// One (dir2, order2) term of the Wiener denominator at absolute frequency
+// (xabs, yabs), plane z0; 0 for the band's own term and for terms outside the
+// support.
+__device__ __forceinline__ float filter_term(const FilterParams& p, const float2* prof, int nxotf, int order, int dir2,
+                                             int order2, float xabs, float yabs, int z0, float zf3) {
+#pragma clang fp contract(off)
+  constexpr int nshift = 2 * kMaxOrders - 1;
+  if (dir2 == p.dir && order2 == order) return 0.f;
+  if (!p.bFilteroverlaps && !(order2 == 0 && order == 0)) return 0.f;
+  const float2 ks = p.kshift[dir2 * nshift + order2 + (kMaxOrders - 1)];
+  const float x2 = xabs - ks.x, y2 = yabs - ks.y;
+  const float d2 = x2 * x2 + y2 * y2;
+  // rdist2 < rdistcutoff, tested on the squared distance; the OTF radius
+  // index sqrt(d2)/dkr and the damping term (rdist2/rdistcutoff)^2 are
+  // formed from d2 as well (a few ulps from the reference's sqrtf and
+  // divisions, far inside the output tolerance)
+  if (!(d2 < p.cut2)) return 0.f;
+  const int ao2 = order2 < 0 ? -order2 : order2;
+  const float2 otf2 = prof_idx(prof + ao2 * nxotf, nxotf, __builtin_amdgcn_sqrtf(d2 * p.inv_dkr2));
+  float w = otf2.x * otf2.x + otf2.y * otf2.y;
+  if (!p.unit_nv) w = w / p.noiseVar[dir2 * p.norders + ao2];
+  if (order2 != 0) w *= p.ampmag2_all[dir2 * p.norders + ao2];
+  if (p.bSuppress && order2 != 0 && d2 <= p.supp2)
+    w *= dev_suppress(sqrtf(d2) / p.min_dkr);
+  else if (!p.bDampenOrder0 && p.bSuppress && order2 == 0 && d2 <= p.supp2)
+    w *= dev_suppress(sqrtf(d2) / p.min_dkr);
+  else if (p.bDampenOrder0 && order2 == 0)
+    w *= d2 * p.inv_rcut2 + zf3;
+  if (p.bNoKz0 && order2 == 0 && z0 == 0) w = 0.0f;
+  return w;
+}
+
 // ND, NO > 0: the direction and order counts as compile-time constants (the
 // (dir2, order2) loop is unrolled, so its per-term constants are read once
 // and the OTF lookups of different terms overlap); 0: taken from p.
@@ -98,36 +130,16 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
   if (order == 0 && z0 == 0 && p.bNoKz0) dampfact = 0.f;
   weight *= dampfact;
   float sumweight = weight;
-  const int nshift = 2 * kMaxOrders - 1;
-  const int ndirs = ND > 0 ? ND : p.ndirs, norders = NO > 0 ? NO : p.norders;
+  if constexpr (ND > 0) {
 #pragma unroll
-  for (int dir2 = 0; dir2 < ndirs; dir2++) {
+    for (int dir2 = 0; dir2 < ND; dir2++)
 #pragma unroll
-    for (int order2 = -(norders - 1); order2 < norders; order2++) {
-      if (dir2 == p.dir && order2 == order) continue;
-      if (!p.bFilteroverlaps && !(order2 == 0 && order == 0)) continue;
-      const float2 ks = p.kshift[dir2 * nshift + order2 + (kMaxOrders - 1)];
-      const float x2 = xabs - ks.x, y2 = yabs - ks.y;
-      const float d2 = x2 * x2 + y2 * y2;
-      // rdist2 < rdistcutoff, tested on the squared distance; the OTF radius
-      // index sqrt(d2)/dkr and the damping term (rdist2/rdistcutoff)^2 are
-      // formed from d2 as well (a few ulps from the reference's sqrtf and
-      // divisions, far inside the output tolerance)
-      if (!(d2 < p.cut2)) continue;
-      const int ao2 = order2 < 0 ? -order2 : order2;
-      const float2 otf2 = prof_idx(prof + ao2 * nxotf, nxotf, __builtin_amdgcn_sqrtf(d2 * p.inv_dkr2));
-      float w = otf2.x * otf2.x + otf2.y * otf2.y;
-      if (!p.unit_nv) w = w / p.noiseVar[dir2 * p.norders + ao2];
-      if (order2 != 0) w *= p.ampmag2_all[dir2 * p.norders + ao2];
-      if (p.bSuppress && order2 != 0 && d2 <= p.supp2)
-        w *= dev_suppress(sqrtf(d2) / p.min_dkr);
-      else if (!p.bDampenOrder0 && p.bSuppress && order2 == 0 && d2 <= p.supp2)
-        w *= dev_suppress(sqrtf(d2) / p.min_dkr);
-      else if (p.bDampenOrder0 && order2 == 0)
-        w *= d2 * p.inv_rcut2 + zf3;
-      if (p.bNoKz0 && order2 == 0 && z0 == 0) w = 0.0f;
-      sumweight += w;
-    }
+      for (int order2 = -(NO - 1); order2 < NO; order2++)
+        sumweight += filter_term(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
+  } else {
+    for (int dir2 = 0; dir2 < p.ndirs; dir2++)
+      for (int order2 = -(p.norders - 1); order2 < p.norders; order2++)
+        sumweight += filter_term(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
   }
   sumweight += p.wiener;
   float2 scale = make_float2(dampfact * otf1.x / sumweight, dampfact * (-otf1.y) / sumweight);

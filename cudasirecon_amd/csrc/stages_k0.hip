@@ -27,11 +27,35 @@ namespace simr {
 // band index, the conjugation of the Hermitian mirror, and the OTF factor
 // (makeOverlaps0Kernel / makeOverlaps1Kernel).  ok = false: the element is
 // zero.  prof1/prof2: radial OTF profiles of order1/order2 at kz = z0.
+//
+// The reference's tests sqrtf(d) <= rdistcutoff and sqrtf(|otf|^2) * f >
+// otfcutoff are made on the squares, the OTF radius index is the hardware
+// square root of d / dkr^2 and the normalisation 1/sqrt(|a|^2 + |b|^2) the
+// hardware reciprocal square root: within a few ulps of the reference's
+// correctly rounded sqrtf and divisions (an element exactly on a cut-off
+// circle may land on the other side), far inside the overlap tolerance, at
+// a third of the instructions.
 struct OvSrc {
   int ind;
   bool ok, conj;
   float2 fact;
 };
+
+__device__ __forceinline__ float2 prof_lerp(const float2* prof, int nxotf, float krindex) {
+#pragma clang fp contract(off)
+  int irindex = (int)floorf(krindex);
+  float ar = krindex - (float)irindex;
+  if (irindex > nxotf - 2) {
+    irindex = nxotf - 2;
+    ar = 1.f;
+  }
+  if (irindex < 0) {
+    irindex = 0;
+    ar = 0.f;
+  }
+  const float2 t0 = prof[irindex], t1 = prof[irindex + 1];
+  return make_float2((1 - ar) * t0.x + ar * t1.x, (1 - ar) * t0.y + ar * t1.y);
+}
 
 __device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof1, const float2* prof2, int which,
                                            int j, int i, int z0) {
@@ -42,14 +66,13 @@ __device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof
   if (y1f > (float)(p.ny / 2)) y1f -= (float)p.ny;
   x1f *= p.dkx;
   y1f *= p.dky;
-  const float rc = p.rdistcutoff;
-  const float rdist1 = sqrtf(x1f * x1f + y1f * y1f);
-  if (which == 0 ? !(rdist1 <= rc) : !(rdist1 < rc)) return o;
+  const float d1 = x1f * x1f + y1f * y1f;
+  if (which == 0 ? !(d1 <= p.rc2) : !(d1 < p.rc2)) return o;
   const float x12 = x1f - p.kx, y12 = y1f - p.ky;
-  const float rdist12 = sqrtf(x12 * x12 + y12 * y12);
+  const float d12 = x12 * x12 + y12 * y12;
   const float x21 = x1f + p.kx, y21 = y1f + p.ky;
-  const float rdist21 = sqrtf(x21 * x21 + y21 * y21);
-  if (!(rdist12 <= rc || rdist21 <= rc)) return o;
+  const float d21 = x21 * x21 + y21 * y21;
+  if (!(d12 <= p.rc2 || d21 <= p.rc2)) return o;
   if (z0 == 0 && p.bNoKz0) return o;
   int iin, jin;
   if (j <= p.nx / 2) {
@@ -63,24 +86,25 @@ __device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof
   int zin = o.conj ? -z0 : z0;
   if (zin < 0) zin += p.nz;
   float2 oa, ob;  // own-order OTF at k, other-order OTF at the shifted k
-  float sa, sb;   // their cutoff scales
+  float sa2, sb2; // their cutoff scales, squared
   if (which == 0) {
-    if (!(rdist12 <= rc)) return o;
-    oa = profile_lookup(prof1, p.otf1.nxotf, p.otf1.dkrotf, x1f, y1f);
-    ob = profile_lookup(prof2, p.otf2.nxotf, p.otf2.dkrotf, x12, y12);
-    sa = 1.f;
-    sb = p.order0_2_factor;
+    if (!(d12 <= p.rc2)) return o;
+    oa = prof_lerp(prof1, p.otf1.nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2_1));
+    ob = prof_lerp(prof2, p.otf2.nxotf, __builtin_amdgcn_sqrtf(d12 * p.inv_dkr2_2));
+    sa2 = 1.f;
+    sb2 = p.f2;
   } else {
-    if (!(rdist21 <= rc)) return o;
-    oa = profile_lookup(prof2, p.otf2.nxotf, p.otf2.dkrotf, x1f, y1f);
-    ob = profile_lookup(prof1, p.otf1.nxotf, p.otf1.dkrotf, x21, y21);
-    sa = p.order0_2_factor;
-    sb = 1.f;
+    if (!(d21 <= p.rc2)) return o;
+    oa = prof_lerp(prof2, p.otf2.nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2_2));
+    ob = prof_lerp(prof1, p.otf1.nxotf, __builtin_amdgcn_sqrtf(d21 * p.inv_dkr2_1));
+    sa2 = p.f2;
+    sb2 = 1.f;
   }
-  if (!(sqrtf(oa.x * oa.x + oa.y * oa.y) * sa > p.otfcutoff)) return o;
-  if (!(sqrtf(ob.x * ob.x + ob.y * ob.y) * sb > p.otfcutoff)) return o;
-  const float root = sqrtf(oa.x * oa.x + oa.y * oa.y + ob.x * ob.x + ob.y * ob.y);
-  o.fact = make_float2(ob.x / root, ob.y / root);
+  const float ma = oa.x * oa.x + oa.y * oa.y, mb = ob.x * ob.x + ob.y * ob.y;
+  if (!(ma * sa2 > p.cut2)) return o;
+  if (!(mb * sb2 > p.cut2)) return o;
+  const float rroot = __builtin_amdgcn_rsqf(ma + mb);
+  o.fact = make_float2(ob.x * rroot, ob.y * rroot);
   o.ind = (zin * p.ny + iin) * p.nxh + jin;
   o.ok = true;
   return o;
