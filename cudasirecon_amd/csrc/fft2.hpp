@@ -193,6 +193,18 @@ constexpr bool fft2_ok(int N) { return fft2_plan(N).n1 > 0; }
 constexpr int fft2_tpt(int N) { return fft2_plan(N).n1; }
 // LDS elements one transform needs (the padded intermediate is the larger)
 constexpr int fft2_span(int N) { return fft2_plan(N).n1 * (fft2_plan(N).n2 + 1); }
+// Whether a length has a plan whose transforms stay inside one wavefront.
+template <int N>
+constexpr bool use_fft2() {
+  return fft2_ok(N) && 64 % fft2_tpt(N) == 0 && fft2_tpt(N) >= 2;
+}
+// LDS pitch of one transform in a tile of transforms: span rounded up to
+// 1 mod 32, so the lanes of a tile row (one element of each of consecutive
+// transforms) fall in different banks.
+template <int N>
+constexpr int fft2_pitch() {
+  return fft2_span(N) + ((33 - fft2_span(N) % 32) % 32);
+}
 
 // Orders the LDS accesses of one wavefront: its LDS instructions execute in
 // issue order, so a fence that the compiler may not move memory accesses

@@ -131,20 +131,11 @@ __global__ __launch_bounds__(W* fft_tpt(N)) void k_fft_rows(float2* __restrict__
 }
 
 // ---------------------------------------------------------- two-pass kernels
-// Lengths with a two-pass plan (fft2.hpp) whose transforms do not straddle a
-// wavefront use the register core: each transform's row is kept in natural
-// order in LDS at a pitch of span + pad (pitch = 1 mod 32, so the lanes of a
-// tile row, W columns apart, fall in different banks), the twiddles of each
-// thread are built once in registers, and the two exchanges inside a
-// transform synchronise its wavefront only.
-template <int N>
-constexpr bool use_fft2() {
-  return fft2_ok(N) && 64 % fft2_tpt(N) == 0 && fft2_tpt(N) >= 2;
-}
-template <int N>
-constexpr int fft2_pitch() {
-  return fft2_span(N) + ((33 - fft2_span(N) % 32) % 32);
-}
+// Lengths with a two-pass plan whose transforms do not straddle a wavefront
+// (use_fft2) use the register core: each transform's row is kept in natural
+// order in LDS at fft2_pitch, the twiddles of each thread are built once in
+// registers, and the two exchanges inside a transform synchronise its
+// wavefront only.
 // W transforms per workgroup: at most 512 threads and 80 KB of LDS, whole waves.
 template <int N>
 constexpr int tile_w2() {

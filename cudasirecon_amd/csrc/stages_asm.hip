@@ -128,8 +128,36 @@ __global__ __launch_bounds__(256) void k_asm_z(AsmParams p, const float2* __rest
 }
 
 // ---- y pass: W adjacent xc columns of plane zout (blockIdx.y) of band a (blockIdx.z)
+// Lengths with a wave-local two-pass plan use that core (natural-order LDS
+// columns, register twiddles) with 16 columns per workgroup, so every global
+// row segment is a full 128-byte line; the others the radix-stage core.
+template <int N>
+constexpr bool asm_y_fft2() {
+  return use_fft2<N>() && fft2_tpt(N) >= 4;
+}
+template <int N>
+constexpr int asm_y_tpt() {
+  return asm_y_fft2<N>() ? fft2_tpt(N) : fft_tpt(N);
+}
+template <int N>
+constexpr int asm_y_pitch() {
+  return asm_y_fft2<N>() ? fft2_pitch<N>() : lds_pitch(N);
+}
+template <int N>
+__device__ __forceinline__ int asm_y_pos(int Y) {
+  if constexpr (asm_y_fft2<N>())
+    return Y;
+  else
+    return lds_swz<N>(Y);
+}
 template <int N>
 constexpr int asm_y_w() {
+  if (asm_y_fft2<N>()) {
+    int w = 16;
+    while (w > 1 && w * asm_y_pitch<N>() * 8 > 150 * 1024) w /= 2;
+    while (w * asm_y_tpt<N>() < 64) w *= 2;
+    return w;
+  }
   constexpr int tpt = fft_tpt(N);
   int w = 1024 / tpt;
   if (w > 16) w = 16;
@@ -138,15 +166,21 @@ constexpr int asm_y_w() {
 }
 
 template <int N>
-__global__ __launch_bounds__(asm_y_w<N>() * fft_tpt(N)) void k_asm_y(AsmParams p, const float2* __restrict__ tw) {
-  constexpr int TPT = fft_tpt(N);
+__global__ __launch_bounds__(asm_y_w<N>() * asm_y_tpt<N>()) void k_asm_y(AsmParams p, const float2* __restrict__ tw) {
+  constexpr bool F2 = asm_y_fft2<N>();
+  constexpr int TPT = asm_y_tpt<N>();
   constexpr int W = asm_y_w<N>();
   constexpr int NT = W * TPT;
-  constexpr int LP = lds_pitch(N);
+  constexpr int LP = asm_y_pitch<N>();
+  static_assert(!F2 || NT % 64 == 0, "two-pass tiles are whole wavefronts");
   __shared__ float2 lds[W * LP];
-  FftTw<N> twr;
-  __shared__ float2 twstage[N > 1 ? N : 1];
-  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
+  FftTw<F2 ? 1 : N> twr;
+  __shared__ float2 twstage[(N > 1 && !F2) ? N : 1];
+  Fft2Tw<F2 ? N : 64> tw2;
+  if constexpr (F2)
+    tw2.load(threadIdx.x % TPT, tw);
+  else
+    load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
   const int a = blockIdx.z, zout = blockIdx.y;
   const int xc0 = blockIdx.x * W;
   const float2* src = p.stageZ + ((long long)a * p.zdim + zout) * p.ny * p.nx;
@@ -171,20 +205,25 @@ __global__ __launch_bounds__(asm_y_w<N>() * fft_tpt(N)) void k_asm_y(AsmParams p
       const int idx = i0 + k * NT;
       const int sr = idx / W, w = idx - (idx / W) * W;
       const int Y = sr >= hy ? sr - hy : sr + N - hy;
-      if (idx < cnt) lds[w * LP + lds_swz<N>(Y)] = xc0 + w < p.nx ? v[k] : make_float2(0.f, 0.f);
+      if (idx < cnt) lds[w * LP + asm_y_pos<N>(Y)] = xc0 + w < p.nx ? v[k] : make_float2(0.f, 0.f);
     }
   }
   for (int idx = threadIdx.x; idx < (N - p.ny) * W; idx += NT) {
     const int Y = p.ny - p.ny / 2 + 1 + idx / W, w = idx - (idx / W) * W;
-    lds[w * LP + lds_swz<N>(Y)] = make_float2(0.f, 0.f);
+    lds[w * LP + asm_y_pos<N>(Y)] = make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
+  if constexpr (F2) {
+    fft2_lds<N, true, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, tw2);
+    __syncthreads();
+  } else {
+    fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
+  }
   float2* dst = p.stageY + ((long long)a * p.zdim + zout) * (long long)N * p.nx;
   for (int idx = threadIdx.x; idx < N * W; idx += NT) {
     const int Y = idx / W, w = idx - (idx / W) * W;
     const int xc = xc0 + w;
-    if (xc < p.nx) dst[(long long)Y * p.nx + xc] = lds[w * LP + lds_swz<N>(Y)];
+    if (xc < p.nx) dst[(long long)Y * p.nx + xc] = lds[w * LP + asm_y_pos<N>(Y)];
   }
 }
 
@@ -451,7 +490,7 @@ void launch_asm_y(const AsmParams& p, hipStream_t s) {
 #define SIMR_CASE(N)                                                                                \
   case N: {                                                                                         \
     constexpr int W = asm_y_w<N>();                                                                 \
-    hipLaunchKernelGGL(k_asm_y<N>, dim3(ceil_div(p.nx, W), p.zdim, p.nasm), dim3(W * fft_tpt(N)),   \
+    hipLaunchKernelGGL(k_asm_y<N>, dim3(ceil_div(p.nx, W), p.zdim, p.nasm), dim3(W * asm_y_tpt<N>()), \
                        0, s, p, twiddles(N));                                                       \
     SIMR_LAUNCH_CHECK();                                                                            \
     return;                                                                                         \
