@@ -372,7 +372,7 @@ def main():
         del src, dst
 
     print("bench: generating the synthetic 512x512x48 stack", file=sys.stderr, flush=True)
-    p, tab, raw, truth = synth.make_dataset(512, 512, 48, seed=20262202, gpu=True)
+    p, tab, raw, truth = synth.make_dataset(512, 512, 48, seed=20262202)
     p.fit_fp64 = 0 if args.fit_fp32 else 1
     r = engine.Reconstructor(p, tab)
     r.set_raw(raw)  # resident in HBM from here on
