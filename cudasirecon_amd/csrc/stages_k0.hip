@@ -14,8 +14,11 @@
 //    3D overlap volumes are never re-read: every later line-search step is a
 //    2D reduction of P against a phase ramp;
 //  * a whole batch of line-search candidates is reduced in one launch.
+#include <type_traits>
+
 #include "device_math.hpp"
 #include "fft.hpp"
+#include "fft2.hpp"
 #include "fft_api.hpp"
 #include "sizes.hpp"
 #include "stages.hpp"
@@ -305,9 +308,113 @@ __global__ __launch_bounds__(2 * zcols<N>() * fft_tpt(N)) void k_overlaps_z(OvPa
   }
 }
 
+// z pass on the two-pass core: the TPT threads of a column transform its X
+// and then its Y overlap (one LDS slot each) and keep both results in
+// registers (elements z = t + N1 k2), form their share of the z sums there
+// and combine the TPT shares with cross-lane adds; thread 0 of the column
+// writes P, SX, SY.  Sums run in double (FP64 fit) or float.
+template <int N>
+constexpr bool ovz_fft2() {
+  return use_fft2<N>() && fft2_tpt(N) >= 2 && fft2_tpt(N) <= 16;
+}
+// columns per workgroup: at most 64 and 80 KB of LDS (two slots per column)
+template <int N>
+constexpr int ovz_cols() {
+  int w = 64;
+  while (w > 1 && 2 * w * fft2_pitch<N>() * 8 > 80 * 1024) w /= 2;
+  return w;
+}
+
+template <typename T>
+__device__ __forceinline__ T xor_add(T v, int mask) {
+  return v + __shfl_xor(v, mask, 64);
+}
+
+template <int N, bool FP32>
+__global__ __launch_bounds__(ovz_cols<N>() * fft2_tpt(N)) void k_overlaps_z2(OvParams p, const float2* __restrict__ ovx,
+                                                                       double2* __restrict__ P,
+                                                                       double* __restrict__ SX,
+                                                                       double* __restrict__ SY,
+                                                                       const float2* __restrict__ tw) {
+#pragma clang fp contract(off)
+  constexpr int WC = ovz_cols<N>();
+  constexpr int TPT = fft2_tpt(N);
+  constexpr int NT = WC * TPT;
+  constexpr int N2 = fft2_plan(N).n2;
+  constexpr int SP = fft2_pitch<N>();
+  static_assert(NT % 64 == 0 && 64 % TPT == 0, "columns must not straddle wavefronts");
+  __shared__ float2 lds[2 * WC * SP];  // slot 2w: X of column w, slot 2w+1: Y
+  const int tid = threadIdx.x, w = tid / TPT, t = tid % TPT;
+  Fft2Tw<N> tw2;
+  tw2.load(t, tw);
+  const long long nxy = (long long)p.nx * p.ny;
+  const long long e0 = (long long)blockIdx.x * WC;
+  const int nzs = 2 * p.zd + 1;
+  // Slot (which, s, w): s < nzs is stored plane s (z = s - zd, wrapped); the
+  // other slots are the zero planes zd+1 .. N-zd-1.  Clamped loads, no branches.
+  constexpr int EPT = 2 * WC * N / NT;
+  float2 v[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int which = idx / (WC * N);
+    const int rem = idx - which * WC * N;
+    const int sp = rem / WC, wc = rem - (rem / WC) * WC;
+    const long long e = e0 + wc < nxy ? e0 + wc : nxy - 1;
+    const int sc = sp < nzs ? sp : 0;
+    v[k] = ovx[((long long)which * nzs + sc) * nxy + e];
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int which = idx / (WC * N);
+    const int rem = idx - which * WC * N;
+    const int sp = rem / WC, wc = rem - (rem / WC) * WC;
+    const int z = sp < nzs ? (sp - p.zd >= 0 ? sp - p.zd : sp - p.zd + N) : sp - nzs + p.zd + 1;
+    lds[(2 * wc + which) * SP + z] = (sp < nzs && e0 + wc < nxy) ? v[k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2 X[N2], Y[N2];
+  fft2_lds_regs<N, true, true>(lds + (2 * w) * SP, t, tw2, X);
+  fft2_lds_regs<N, true, true>(lds + (2 * w + 1) * SP, t, tw2, Y);
+  using A = typename std::conditional<FP32, float, double>::type;
+  A sxr = 0, sxi = 0, sx = 0, sy = 0;
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) {
+    const A ar = X[k2].x, ai = X[k2].y, br = Y[k2].x, bi = Y[k2].y;
+    sxr += ar * br + ai * bi;
+    sxi += ar * bi - ai * br;
+    sx += ar * ar + ai * ai;
+    sy += br * br + bi * bi;
+  }
+#pragma unroll
+  for (int m = 1; m < TPT; m <<= 1) {
+    sxr = xor_add(sxr, m);
+    sxi = xor_add(sxi, m);
+    sx = xor_add(sx, m);
+    sy = xor_add(sy, m);
+  }
+  const long long e = e0 + w;
+  if (t == 0 && e < nxy) {
+    P[e] = make_double2((double)sxr, (double)sxi);
+    SX[e] = (double)sx;
+    SY[e] = (double)sy;
+  }
+}
+
 template <int N>
 static void launch_ov_z(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, bool fp32,
                         hipStream_t s) {
+  if constexpr (ovz_fft2<N>() && (ovz_cols<N>() * fft2_tpt(N)) % 64 == 0) {
+    const long long nxy = (long long)p.nx * p.ny;
+    dim3 grid(ceil_div(nxy, ovz_cols<N>())), block(ovz_cols<N>() * fft2_tpt(N));
+    if (fp32)
+      hipLaunchKernelGGL((k_overlaps_z2<N, true>), grid, block, 0, s, p, ovx, P, SX, SY, twiddles(N));
+    else
+      hipLaunchKernelGGL((k_overlaps_z2<N, false>), grid, block, 0, s, p, ovx, P, SX, SY, twiddles(N));
+    SIMR_LAUNCH_CHECK();
+    return;
+  }
   constexpr int WC = zcols<N>();
   const long long nxy = (long long)p.nx * p.ny;
   dim3 grid(ceil_div(nxy, WC)), block(2 * WC * fft_tpt(N));
