@@ -1,6 +1,8 @@
 // Host orchestration of one reconstruction context.  See engine.hpp.
 #include <algorithm>
 #include <cmath>
+#include <exception>
+#include <thread>
 #include <cstring>
 
 #include "engine.hpp"
@@ -14,6 +16,7 @@ constexpr double kPI = 3.14159265358979323846;
 constexpr float K0_WARNING_THRESH = 0.05f;  // cudaSireconImpl.h:54
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr int kMaxCand = 64;  // line-search candidates reduced per launch
 
 int norders_of(const simr_params& p) { return p.norders_output != 0 ? p.norders_output : p.nphases / 2 + 1; }
 
@@ -87,6 +90,18 @@ void Arena::release() {
 }
 
 // ------------------------------------------------------------------- layout
+// Bytes of one direction's fit scratch, in the order Engine::carve_fit lays it out.
+static size_t fit_set_bytes(const Layout& L, int ncand) {
+  size_t b = 0;
+  b += align256((size_t)2 * (2 * L.zd_ov_max + 1) * L.nxy * 8);  // overlap x/y stage
+  b += align256((size_t)L.nxy * 16) + 2 * align256((size_t)L.nxy * 8) + align256((size_t)L.nxy * 8);  // P, SX, SY, cc
+  b += 4 * align256(4096) + align256((size_t)ncand * 8);  // peak search, candidates
+  b += align256((size_t)ncand * modamp_partials_fp64(L.nx, L.ny) * 16);
+  b += align256((size_t)ncand * modamp_partials_fp32(L.nx, L.ny) * 16);
+  b += align256((size_t)2 * 128 * 8);
+  return b;
+}
+
 Layout Layout::of(const simr_params& p) {
   Layout L{};
   SIMR_REQUIRE(p.nx > 0 && p.ny > 0 && p.nz > 0, "image dimensions must be positive");
@@ -122,21 +137,19 @@ Layout Layout::of(const simr_params& p) {
   SIMR_REQUIRE(in_list_zoom(L.xdim) && in_list_zoom(L.ydim), "zoomed lateral size is not a supported size");
   SIMR_REQUIRE(in_list_z(L.zdim), "zoomed stack depth is not a supported size");
   SIMR_REQUIRE(2 * L.zd_ov_max + 1 <= p.nz, "overlap z support wider than the stack");
-  const int ncand = 64;
+  const int ncand = kMaxCand;
   size_t b = 0;
   b += align256((size_t)L.nimg * L.nxy * 4);            // raw (float or uint16)
   b += 2 * align256((size_t)L.nxy * 4);                  // bg, slope
   b += align256((size_t)L.nimg * kSumSplit * 8) + align256((size_t)L.nimg * 4) + align256((size_t)p.nz * 8) + align256(256);
   b += align256((size_t)L.ndirs * L.nbands * L.H * 8);   // bands
   b += align256((size_t)L.nDirsOTF * L.norders * L.otfsize * 8);
-  b += align256((size_t)2 * (2 * L.zd_ov_max + 1) * L.nxy * 8);  // overlap x/y stage
-  b += align256((size_t)L.nxy * 16) + 2 * align256((size_t)L.nxy * 8) + align256((size_t)L.nxy * 8);
-  b += 4 * align256(4096) + align256((size_t)ncand * 8);
-  b += align256((size_t)ncand * modamp_partials_fp64(p.nx, p.ny) * 16);
-  b += align256((size_t)ncand * modamp_partials_fp32(p.nx, p.ny) * 16);
-  b += align256((size_t)2 * 128 * 8);
-  b += align256((size_t)L.nasm * L.zdim * L.nxy * 8);                       // stageZ
-  b += align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * p.nx * 8);       // stageY
+  const size_t fit = fit_set_bytes(L, ncand);  // direction 0's fit scratch
+  b += fit;
+  // assembly staging; the fit scratch of directions 1.. aliases it
+  const size_t stage = align256((size_t)L.nasm * L.zdim * L.nxy * 8) +            // stageZ
+                       align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * p.nx * 8);  // stageY
+  b += std::max(stage, (size_t)(L.ndirs - 1) * fit);
   b += align256((size_t)L.zdim * L.ydim * (size_t)L.xdim * 4);              // output
   L.bytes_total = b;
   return L;
@@ -158,27 +171,32 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   apofact_ = (float*)arena_.take(256);
   bands_ = (float2*)arena_.take((size_t)L.ndirs * L.nbands * L.H * 8);
   otf_ = (float2*)arena_.take((size_t)L.nDirsOTF * L.norders * L.otfsize * 8);
-  ovx_ = (float2*)arena_.take((size_t)2 * (2 * L.zd_ov_max + 1) * L.nxy * 8);
-  P_ = (double2*)arena_.take((size_t)L.nxy * 16);
-  SX_ = (double*)arena_.take((size_t)L.nxy * 8);
-  SY_ = (double*)arena_.take((size_t)L.nxy * 8);
-  cc_ = (float2*)arena_.take((size_t)L.nxy * 8);
-  pv_ = (float*)arena_.take(4096);
-  pi_ = (int*)arena_.take(4096);
-  peakvals_ = (float*)arena_.take(4096);
-  peakidx_ = (int*)arena_.take(4096);
-  kcand_ = (float2*)arena_.take((size_t)max_cand_ * 8);
-  part64_ = (double2*)arena_.take((size_t)max_cand_ * modamp_partials_fp64(L.nx, L.ny) * 16);
-  part32_ = (float4*)arena_.take((size_t)max_cand_ * modamp_partials_fp32(L.nx, L.ny) * 16);
-  sumparts_ = (double*)arena_.take((size_t)2 * 128 * 8);
-  stageZ_ = (float2*)arena_.take((size_t)L.nasm * L.zdim * L.nxy * 8);
-  stageY_ = (float2*)arena_.take((size_t)L.nasm * L.zdim * (size_t)L.ydim * L.nx * 8);
+  const size_t fitb = fit_set_bytes(L, kMaxCand);
+  char* fit0 = (char*)arena_.take(fitb);
+  const size_t zb = align256((size_t)L.nasm * L.zdim * L.nxy * 8);
+  const size_t yb = align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * L.nx * 8);
+  char* stage = (char*)arena_.take(std::max(zb + yb, (size_t)(L.ndirs - 1) * fitb));
+  stageZ_ = (float2*)stage;
+  stageY_ = (float2*)(stage + zb);
   out_ = (float*)arena_.take((size_t)L.zdim * L.ydim * (size_t)L.xdim * 4);
 
-  hstage_bytes_ = std::max<size_t>((size_t)max_cand_ * modamp_partials_fp32(L.nx, L.ny) * 16,
-                                   (size_t)max_cand_ * modamp_partials_fp64(L.nx, L.ny) * 16) +
+  hstage_bytes_ = std::max<size_t>((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16,
+                                   (size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16) +
                   8192;
-  SIMR_HIP(hipHostMalloc(&hstage_, hstage_bytes_, hipHostMallocDefault));
+  fw_.resize(L.ndirs);
+  for (int d = 0; d < L.ndirs; ++d) {
+    FitWork& w = fw_[d];
+    char* at = d == 0 ? fit0 : stage + (size_t)(d - 1) * fitb;
+    size_t unused = 0;
+    carve_fit(w, at, false, unused);
+    if (d == 0) {
+      w.stream = stream_;
+    } else {
+      SIMR_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+      w.own_stream = true;
+    }
+    SIMR_HIP(hipHostMalloc(&w.hstage, hstage_bytes_, hipHostMallocDefault));
+  }
 
   // OTF tables
   const size_t notf = (size_t)L.nDirsOTF * L.norders * L.otfsize;
@@ -262,24 +280,62 @@ Engine::~Engine() {
     (void)hipEventDestroy(e.second.second);
   }
   if (stream_) (void)hipStreamSynchronize(stream_);
-  if (hstage_) (void)hipHostFree(hstage_);
+  for (auto& w : fw_) {
+    if (w.own_stream) {
+      (void)hipStreamSynchronize(w.stream);
+      (void)hipStreamDestroy(w.stream);
+    }
+    if (w.hstage) (void)hipHostFree(w.hstage);
+  }
   arena_.release();
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 void Engine::sync() { SIMR_HIP(hipStreamSynchronize(stream_)); }
 
-void Engine::stage_begin(const char* name) {
-  if (!timing_) return;
+// Stage timing: a pair of events around the stage's launches on its stream
+// (the fit's directions record on their own streams, from their threads).
+int Engine::stage_begin(const char* name, hipStream_t s) {
+  if (!timing_) return -1;
   hipEvent_t a, b;
   SIMR_HIP(hipEventCreate(&a));
   SIMR_HIP(hipEventCreate(&b));
-  SIMR_HIP(hipEventRecord(a, stream_));
+  SIMR_HIP(hipEventRecord(a, s));
+  std::lock_guard<std::mutex> lk(ev_mu_);
   events_.push_back({name, {a, b}});
+  return (int)events_.size() - 1;
 }
-void Engine::stage_end() {
-  if (!timing_ || events_.empty()) return;
-  SIMR_HIP(hipEventRecord(events_.back().second.second, stream_));
+void Engine::stage_end(int id, hipStream_t s) {
+  if (id < 0) return;
+  hipEvent_t b;
+  {
+    std::lock_guard<std::mutex> lk(ev_mu_);
+    b = events_[id].second.second;
+  }
+  SIMR_HIP(hipEventRecord(b, s));
+}
+
+// Fit scratch of one direction at `at` (the layout fit_set_bytes counts).
+void Engine::carve_fit(FitWork& w, char*& at, bool count_only, size_t& bytes) const {
+  auto take = [&](size_t n) {
+    char* p = at;
+    at += align256(n);
+    bytes += align256(n);
+    return count_only ? nullptr : p;
+  };
+  w.ovx = (float2*)take((size_t)2 * (2 * L.zd_ov_max + 1) * L.nxy * 8);
+  w.P = (double2*)take((size_t)L.nxy * 16);
+  w.SX = (double*)take((size_t)L.nxy * 8);
+  w.SY = (double*)take((size_t)L.nxy * 8);
+  w.cc = (float2*)take((size_t)L.nxy * 8);
+  w.pv = (float*)take(4096);
+  w.pi = (int*)take(4096);
+  w.peakvals = (float*)take(4096);
+  w.peakidx = (int*)take(4096);
+  w.kcand = (float2*)take((size_t)kMaxCand * 8);
+  w.part64 = (double2*)take((size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16);
+  w.part32 = (float4*)take((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
+  w.sumparts = (double*)take((size_t)2 * 128 * 8);
 }
 
 int Engine::timings(const char** names, float* ms, int cap) {
@@ -334,13 +390,13 @@ void Engine::set_flatfield(const float* bg, const float* slope) {
 // ------------------------------------------------------------ forward path
 void Engine::load_and_rescale(int it) {
   if (!raw_set_) throw StateError("raw data has not been set");
-  stage_begin("section_sums");
+  cur_stage_ = stage_begin("section_sums", stream_);
   const float* bgp = flat_uniform_ ? nullptr : bg_;
   const float* slp = flat_uniform_ ? nullptr : slope_;
   launch_section_sums(raw_, raw_u16_, bgp, slp, P.constbkgd, inscale_, L.nx, L.ny, L.nimg, sums_, stream_);
   launch_ratios(sums_, sdp_, ratio_, L.ndirs, L.nz, L.nphases, P.bFastSIM, it, P.equalizez, P.equalizet,
                 P.do_rescale, stream_);
-  stage_end();
+  stage_end(cur_stage_, stream_);
   FwdParams fp{};
   fp.nx = L.nx;
   fp.ny = L.ny;
@@ -355,18 +411,18 @@ void Engine::load_and_rescale(int it) {
   fp.bg0 = P.constbkgd;
   for (int b = 0; b < L.nbands; ++b)
     for (int ph = 0; ph < L.nphases; ++ph) fp.sep[b * kMaxBands + ph] = sep_[b * L.nphases + ph];
-  stage_begin("rows_fwd");
+  cur_stage_ = stage_begin("rows_fwd", stream_);
   launch_rows_fwd(fp, raw_, raw_u16_, bgp, slp, ratio_, apofact_, bands_, stream_);
-  stage_end();
-  stage_begin("cols_fwd_y");
+  stage_end(cur_stage_, stream_);
+  cur_stage_ = stage_begin("cols_fwd_y", stream_);
   fft_cols(L.ny, bands_, L.nxh, L.nxh, (long long)L.ndirs * L.nbands * L.nz, (long long)L.nxh * L.ny, false,
            stream_);
-  stage_end();
+  stage_end(cur_stage_, stream_);
   if (L.nz > 1) {
-    stage_begin("cols_fwd_z");
+    cur_stage_ = stage_begin("cols_fwd_z", stream_);
     fft_cols(L.nz, bands_, (long long)L.nxh * L.ny, (long long)L.nxh * L.ny, (long long)L.ndirs * L.nbands, L.H,
              false, stream_);
-    stage_end();
+    stage_end(cur_stage_, stream_);
   }
   preprocessed_ = true;
   fitted_ = processed_ = false;
@@ -400,7 +456,7 @@ OtfRA Engine::otf_table(int dir, int order) const {
 }
 
 // ------------------------------------------------------------ overlaps / fit
-void Engine::make_overlaps(int dir, int o1, int o2, float k0x, float k0y) {
+void Engine::make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float k0y) {
   const Geo g = geometry();
   OvParams p{};
   p.nx = L.nx;
@@ -431,30 +487,30 @@ void Engine::make_overlaps(int dir, int o1, int o2, float k0x, float k0y) {
   p.f2 = p.order0_2_factor * p.order0_2_factor;
   p.inv_dkr2_1 = (float)(1.0 / ((double)p.otf1.dkrotf * (double)p.otf1.dkrotf));
   p.inv_dkr2_2 = (float)(1.0 / ((double)p.otf2.dkrotf * (double)p.otf2.dkrotf));
-  stage_begin("overlaps");
-  launch_overlaps_xy(p, ovx_, stream_);
-  launch_overlaps_z_reduce(p, ovx_, P_, SX_, SY_, fit_fp32_, stream_);
-  if (!fit_fp32_) launch_sum2(SX_, SY_, L.nxy, sumparts_, stream_);
-  stage_end();
-  ov_totals_valid_ = false;
-  cache_.clear();
-  cache_o1_ = o1;
-  cache_o2_ = o2;
+  const int st = stage_begin("overlaps", w.stream);
+  launch_overlaps_xy(p, w.ovx, w.stream);
+  launch_overlaps_z_reduce(p, w.ovx, w.P, w.SX, w.SY, fit_fp32_, w.stream);
+  if (!fit_fp32_) launch_sum2(w.SX, w.SY, L.nxy, w.sumparts, w.stream);
+  stage_end(st, w.stream);
+  w.ov_totals_valid = false;
+  w.cache.clear();
+  w.cache_o1 = o1;
+  w.cache_o2 = o2;
 }
 
-std::vector<Engine::Eval> Engine::eval_batch(const std::vector<float2>& kvecs) {
+std::vector<Engine::Eval> Engine::eval_batch(FitWork& w, const std::vector<float2>& kvecs) {
   std::vector<Eval> res;
-  for (size_t c0 = 0; c0 < kvecs.size(); c0 += max_cand_) {
-    const int nc = (int)std::min<size_t>(max_cand_, kvecs.size() - c0);
-    SIMR_HIP(hipMemcpyAsync(kcand_, kvecs.data() + c0, nc * 8, hipMemcpyHostToDevice, stream_));
-    stage_begin("modamp");
+  for (size_t c0 = 0; c0 < kvecs.size(); c0 += kMaxCand) {
+    const int nc = (int)std::min<size_t>(kMaxCand, kvecs.size() - c0);
+    SIMR_HIP(hipMemcpyAsync(w.kcand, kvecs.data() + c0, nc * 8, hipMemcpyHostToDevice, w.stream));
+    const int st = stage_begin("modamp", w.stream);
     if (fit_fp32_) {
       const int nb = modamp_partials_fp32(L.nx, L.ny);
-      launch_modamp_fp32(P_, SX_, SY_, kcand_, nc, L.nx, L.ny, P.dxy, part32_, stream_);
-      stage_end();
-      SIMR_HIP(hipMemcpyAsync(hstage_, part32_, (size_t)nc * nb * 16, hipMemcpyDeviceToHost, stream_));
-      sync();
-      const float4* h = (const float4*)hstage_;
+      launch_modamp_fp32(w.P, w.SX, w.SY, w.kcand, nc, L.nx, L.ny, P.dxy, w.part32, w.stream);
+      stage_end(st, w.stream);
+      SIMR_HIP(hipMemcpyAsync(w.hstage, w.part32, (size_t)nc * nb * 16, hipMemcpyDeviceToHost, w.stream));
+      SIMR_HIP(hipStreamSynchronize(w.stream));
+      const float4* h = (const float4*)w.hstage;
       for (int c = 0; c < nc; ++c) {
         float xr = 0, xi = 0, sx = 0, sy = 0;  // cpuReduce order (gpuFunctionsImpl_hh.cu:162-180)
         for (int b = 0; b < nb; ++b) {
@@ -468,22 +524,22 @@ std::vector<Engine::Eval> Engine::eval_batch(const std::vector<float2>& kvecs) {
       }
     } else {
       const int nb = modamp_partials_fp64(L.nx, L.ny);
-      launch_modamp_fp64(P_, kcand_, nc, L.nx, L.ny, P.dxy, part64_, stream_);
-      stage_end();
-      char* h = (char*)hstage_;
-      SIMR_HIP(hipMemcpyAsync(h, part64_, (size_t)nc * nb * 16, hipMemcpyDeviceToHost, stream_));
-      const size_t soff = (size_t)max_cand_ * nb * 16;
-      if (!ov_totals_valid_)
-        SIMR_HIP(hipMemcpyAsync(h + soff, sumparts_, 2 * 128 * 8, hipMemcpyDeviceToHost, stream_));
-      sync();
-      if (!ov_totals_valid_) {
+      launch_modamp_fp64(w.P, w.kcand, nc, L.nx, L.ny, P.dxy, w.part64, w.stream);
+      stage_end(st, w.stream);
+      char* h = (char*)w.hstage;
+      SIMR_HIP(hipMemcpyAsync(h, w.part64, (size_t)nc * nb * 16, hipMemcpyDeviceToHost, w.stream));
+      const size_t soff = (size_t)kMaxCand * nb * 16;
+      if (!w.ov_totals_valid)
+        SIMR_HIP(hipMemcpyAsync(h + soff, w.sumparts, 2 * 128 * 8, hipMemcpyDeviceToHost, w.stream));
+      SIMR_HIP(hipStreamSynchronize(w.stream));
+      if (!w.ov_totals_valid) {
         const double* sp = (const double*)(h + soff);
         double sx = 0, sy = 0;
         for (int b = 0; b < 128; ++b) sx += sp[b];
         for (int b = 0; b < 128; ++b) sy += sp[128 + b];
-        ov_sx_ = sx;
-        ov_sy_ = sy;
-        ov_totals_valid_ = true;
+        w.ov_sx = sx;
+        w.ov_sy = sy;
+        w.ov_totals_valid = true;
       }
       const double2* hp = (const double2*)h;
       for (int c = 0; c < nc; ++c) {
@@ -492,7 +548,7 @@ std::vector<Engine::Eval> Engine::eval_batch(const std::vector<float2>& kvecs) {
           xr += hp[(size_t)c * nb + b].x;
           xi += hp[(size_t)c * nb + b].y;
         }
-        res.push_back({xr, xi, ov_sx_, ov_sy_});
+        res.push_back({xr, xi, w.ov_sx, w.ov_sy});
       }
     }
   }
@@ -530,34 +586,36 @@ static float2 kvec_of(float angle, float mag, int o1, int o2) {
   return make_float2(kx * (float)(o2 - o1), ky * (float)(o2 - o1));
 }
 
-void Engine::prefetch(int o1, int o2, const std::vector<std::pair<float, float>>& am) {
+void Engine::prefetch(FitWork& w, int o1, int o2, const std::vector<std::pair<float, float>>& am) {
   std::vector<float2> kv;
   std::vector<std::pair<uint32_t, uint32_t>> keys;
   for (auto& x : am) {
     auto k = key_of(x.first, x.second);
-    if (cache_.count(k) || std::find(keys.begin(), keys.end(), k) != keys.end()) continue;
+    if (w.cache.count(k) || std::find(keys.begin(), keys.end(), k) != keys.end()) continue;
     keys.push_back(k);
     kv.push_back(kvec_of(x.first, x.second, o1, o2));
   }
   if (kv.empty()) return;
-  auto ev = eval_batch(kv);
-  for (size_t i = 0; i < keys.size(); ++i) cache_[keys[i]] = ev[i];
+  auto ev = eval_batch(w, kv);
+  for (size_t i = 0; i < keys.size(); ++i) w.cache[keys[i]] = ev[i];
 }
 
 // getmodamp (gpuFunctionsImpl.cu:1046-1072) on the current overlap set.
-double Engine::getmodamp(int dir, float angle, float mag, int o1, int o2, float2* modamp, float* corr) {
+double Engine::getmodamp(FitWork& w, int dir, float angle, float mag, int o1, int o2, float2* modamp,
+                         float* corr) {
   auto k = key_of(angle, mag);
-  if (!cache_.count(k)) prefetch(o1, o2, {{angle, mag}});
+  if (!w.cache.count(k)) prefetch(w, o1, o2, {{angle, mag}});
   double a2;
-  modamp_of(cache_[k], modamp, &a2, corr);
+  modamp_of(w.cache[k], modamp, &a2, corr);
   stats_[dir].n_reductions++;
   return a2;
 }
 
-double Engine::realspacemodamp(int dir, int o1, int o2, float2 kv, bool redo, float2* modamp, float* corr) {
-  if (redo) make_overlaps(dir, o1, o2, kv.x, kv.y);
+double Engine::realspacemodamp(FitWork& w, int dir, int o1, int o2, float2 kv, bool redo, float2* modamp,
+                               float* corr) {
+  if (redo) make_overlaps(w, dir, o1, o2, kv.x, kv.y);
   const float2 k = make_float2(kv.x * (float)(o2 - o1), kv.y * (float)(o2 - o1));
-  auto ev = eval_batch({k});
+  auto ev = eval_batch(w, {k});
   stats_[dir].n_reductions++;
   double a2;
   modamp_of(ev[0], modamp, &a2, corr);
@@ -574,16 +632,16 @@ static float fitparabola(float a1, float a2, float a3) {
   return peak;
 }
 
-void Engine::findk0(int dir) {
+void Engine::findk0(FitWork& w, int dir) {
   const int fo2 = L.nz > 1 ? 2 : 1;
-  make_overlaps(dir, 0, fo2, k0_[dir].x, k0_[dir].y);
-  stage_begin("findk0_peak");
-  launch_findpeak(P_, cc_, pv_, pi_, peakvals_, peakidx_, L.nx, L.ny, stream_);
-  stage_end();
-  char* h = (char*)hstage_;
-  SIMR_HIP(hipMemcpyAsync(h, peakvals_, 5 * 4, hipMemcpyDeviceToHost, stream_));
-  SIMR_HIP(hipMemcpyAsync(h + 64, peakidx_, 4, hipMemcpyDeviceToHost, stream_));
-  sync();
+  make_overlaps(w, dir, 0, fo2, k0_[dir].x, k0_[dir].y);
+  const int st = stage_begin("findk0_peak", w.stream);
+  launch_findpeak(w.P, w.cc, w.pv, w.pi, w.peakvals, w.peakidx, L.nx, L.ny, w.stream);
+  stage_end(st, w.stream);
+  char* h = (char*)w.hstage;
+  SIMR_HIP(hipMemcpyAsync(h, w.peakvals, 5 * 4, hipMemcpyDeviceToHost, w.stream));
+  SIMR_HIP(hipMemcpyAsync(h + 64, w.peakidx, 4, hipMemcpyDeviceToHost, w.stream));
+  SIMR_HIP(hipStreamSynchronize(w.stream));
   const float* v = (const float*)h;
   const int idx = *(const int*)(h + 64);
   const int xc = idx % L.nx, yc = idx / L.nx;
@@ -618,7 +676,7 @@ static U fitxyparabola(U x1, U y1, U x2, U y2, U x3, U y3) {
 // step values it will visit are deterministic (float accumulation of the
 // step), so each walk's next kLook steps in both directions are reduced in
 // one launch and the reference's step rule is replayed on the results.
-void Engine::fitk0andmodamps(int dir) {
+void Engine::fitk0andmodamps(FitWork& w, int dir) {
   constexpr int kLook = 6;
   const int fo1 = 0, fo2 = L.nz > 1 ? 2 : 1;
   const float2 kv = k0_[dir];
@@ -640,14 +698,14 @@ void Engine::fitk0andmodamps(int dir) {
     return v;
   };
   auto ensure = [&](float angle, float mag, bool is_angle, float delta, int sign) {
-    if (cache_.count(key_of(angle, mag))) return;
-    prefetch(fo1, fo2, is_angle ? walk(angle, mag, true, delta, sign) : walk(mag, angle, false, delta, sign));
+    if (w.cache.count(key_of(angle, mag))) return;
+    prefetch(w, fo1, fo2, is_angle ? walk(angle, mag, true, delta, sign) : walk(mag, angle, false, delta, sign));
   };
 
   // first evaluation recomputes the overlaps (redoarrays = recalcarrays >= 1)
   {
     const float2 k1 = make_float2(k0mag * cosf(k0angle), k0mag * sinf(k0angle));
-    make_overlaps(dir, fo1, fo2, k1.x, k1.y);
+    make_overlaps(w, dir, fo1, fo2, k1.x, k1.y);
   }
   // angle walk: both directions' first kLook steps in one batch
   {
@@ -656,15 +714,15 @@ void Engine::fitk0andmodamps(int dir) {
     std::vector<std::pair<float, float>> all{{k0angle, k0mag}};
     all.insert(all.end(), up.begin(), up.end());
     all.insert(all.end(), dn.begin(), dn.end());
-    prefetch(fo1, fo2, all);
+    prefetch(w, fo1, fo2, all);
   }
   float2 modamp;
   float x2 = k0angle;
-  double amp2 = getmodamp(dir, k0angle, k0mag, fo1, fo2, &modamp, nullptr);
+  double amp2 = getmodamp(w, dir, k0angle, k0mag, fo1, fo2, &modamp, nullptr);
   stats_[dir].n_angle_evals++;
   float angle = k0angle + deltaangle;
   float x3 = angle;
-  double amp3 = getmodamp(dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
+  double amp3 = getmodamp(w, dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
   stats_[dir].n_angle_evals++;
   double amp1 = 0;
   float x1 = 0;
@@ -677,7 +735,7 @@ void Engine::fitk0andmodamps(int dir) {
       angle += deltaangle;
       x3 = angle;
       ensure(angle, k0mag, true, deltaangle, +1);
-      amp3 = getmodamp(dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
+      amp3 = getmodamp(w, dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
       stats_[dir].n_angle_evals++;
     }
   } else {
@@ -696,7 +754,7 @@ void Engine::fitk0andmodamps(int dir) {
       angle -= deltaangle;
       x3 = angle;
       ensure(angle, k0mag, true, deltaangle, -1);
-      amp3 = getmodamp(dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
+      amp3 = getmodamp(w, dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
       stats_[dir].n_angle_evals++;
     }
   }
@@ -709,14 +767,14 @@ void Engine::fitk0andmodamps(int dir) {
     std::vector<std::pair<float, float>> all{{angle, k0mag}};
     all.insert(all.end(), up.begin(), up.end());
     all.insert(all.end(), dn.begin(), dn.end());
-    prefetch(fo1, fo2, all);
+    prefetch(w, fo1, fo2, all);
   }
   x2 = k0mag;
-  amp2 = getmodamp(dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
+  amp2 = getmodamp(w, dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
   stats_[dir].n_mag_evals++;
   float mag = k0mag + deltamag;
   x3 = mag;
-  amp3 = getmodamp(dir, angle, mag, fo1, fo2, &modamp, nullptr);
+  amp3 = getmodamp(w, dir, angle, mag, fo1, fo2, &modamp, nullptr);
   stats_[dir].n_mag_evals++;
   if (amp3 > amp2) {
     while (amp3 > amp2) {
@@ -727,7 +785,7 @@ void Engine::fitk0andmodamps(int dir) {
       mag += deltamag;
       x3 = mag;
       ensure(angle, mag, false, deltamag, +1);
-      amp3 = getmodamp(dir, angle, mag, fo1, fo2, &modamp, nullptr);
+      amp3 = getmodamp(w, dir, angle, mag, fo1, fo2, &modamp, nullptr);
       stats_[dir].n_mag_evals++;
     }
   } else {
@@ -746,27 +804,27 @@ void Engine::fitk0andmodamps(int dir) {
       mag -= deltamag;
       x3 = mag;
       ensure(angle, mag, false, deltamag, -1);
-      amp3 = getmodamp(dir, angle, mag, fo1, fo2, &modamp, nullptr);
+      amp3 = getmodamp(w, dir, angle, mag, fo1, fo2, &modamp, nullptr);
       stats_[dir].n_mag_evals++;
     }
   }
   mag = parab(x1, amp1, x2, amp2, x3, amp3);
 
   float cc = 0;
-  getmodamp(dir, angle, mag, fo1, fo2, &modamp, &cc);
+  getmodamp(w, dir, angle, mag, fo1, fo2, &modamp, &cc);
   corr_[dir][fo2] = cc;
   k0_[dir] = make_float2(mag * cosf(angle), mag * sinf(angle));
   amp_[dir][fo2] = modamp;
   if (L.nz == 1) {
     for (int order = 2; order < L.norders; ++order) {
-      realspacemodamp(dir, order - 1, order, make_float2(mag * cosf(angle), mag * sinf(angle)), true, &modamp, &cc);
+      realspacemodamp(w, dir, order - 1, order, make_float2(mag * cosf(angle), mag * sinf(angle)), true, &modamp, &cc);
       amp_[dir][order] = modamp;
       corr_[dir][order] = cc;
     }
   } else {
     for (int order = 1; order < L.norders; ++order)
       if (order != fo2) {
-        realspacemodamp(dir, 0, order, make_float2(mag * cosf(angle), mag * sinf(angle)), true, &modamp, &cc);
+        realspacemodamp(w, dir, 0, order, make_float2(mag * cosf(angle), mag * sinf(angle)), true, &modamp, &cc);
         amp_[dir][order] = modamp;
         corr_[dir][order] = cc;
       }
@@ -777,13 +835,13 @@ static float2 cmulh(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b
 static float cmagh(float2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
 
 // k0 policy of findModulationVectorsAndPhasesForAllDirections (cudaSirecon.cpp:296-492)
-void Engine::modulation(int dir) {
+void Engine::modulation(FitWork& w, int dir) {
   const bool reuse = P.ntimes > 1 && cur_t_ > 0 && P.bUseTime0k0;
   stats_[dir] = simr_fit{};
   if (!reuse) k0_[dir] = k0guess_[dir];
   if (P.bSearchforvector && !reuse) {
-    findk0(dir);
-    fitk0andmodamps(dir);
+    findk0(w, dir);
+    fitk0andmodamps(w, dir);
     if (cur_t_ == 0) k0_time0_[dir] = k0_[dir];
     const float dx = k0_[dir].x - k0guess_[dir].x, dy = k0_[dir].y - k0guess_[dir].y;
     const float dist = sqrtf(dx * dx + dy * dy);
@@ -793,7 +851,7 @@ void Engine::modulation(int dir) {
         float2 m;
         float cc;
         const int o1 = L.nz > 1 ? 0 : order - 1;
-        realspacemodamp(dir, o1, order, k0_[dir], true, &m, &cc);
+        realspacemodamp(w, dir, o1, order, k0_[dir], true, &m, &cc);
         amp_[dir][order] = m;
         corr_[dir][order] = cc;
       }
@@ -802,7 +860,7 @@ void Engine::modulation(int dir) {
     for (int order = 1; order < L.norders; ++order) {
       float2 m;
       float cc;
-      realspacemodamp(dir, 0, order, k0_[dir], true, &m, &cc);
+      realspacemodamp(w, dir, 0, order, k0_[dir], true, &m, &cc);
       amp_[dir][order] = m;
       corr_[dir][order] = cc;
     }
@@ -829,9 +887,40 @@ void Engine::modulation(int dir) {
   }
 }
 
+// The directions' fits are independent (findModulationVectorsAndPhasesForAllDirections
+// loops over them): direction 0 runs on the calling thread and the context
+// stream, every other one on a thread and stream of its own after the bands
+// are ready, so one direction's kernels fill the GPU while another waits for
+// its line-search read-back.  Each thread ends with its stream drained.
 void Engine::find_modulation() {
   if (!preprocessed_) throw StateError("find_modulation before load_and_rescale");
-  for (int dir = 0; dir < L.ndirs; ++dir) modulation(dir);
+  if (L.ndirs > 1) {
+    hipEvent_t ready;
+    SIMR_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    SIMR_HIP(hipEventRecord(ready, stream_));
+    for (int dir = 1; dir < L.ndirs; ++dir) SIMR_HIP(hipStreamWaitEvent(fw_[dir].stream, ready, 0));
+    SIMR_HIP(hipEventDestroy(ready));
+  }
+  std::vector<std::exception_ptr> err(L.ndirs);
+  std::vector<std::thread> th;
+  for (int dir = 1; dir < L.ndirs; ++dir)
+    th.emplace_back([this, dir, &err] {
+      try {
+        SIMR_HIP(hipSetDevice(device_));
+        modulation(fw_[dir], dir);
+        SIMR_HIP(hipStreamSynchronize(fw_[dir].stream));
+      } catch (...) {
+        err[dir] = std::current_exception();
+      }
+    });
+  try {
+    modulation(fw_[0], 0);
+  } catch (...) {
+    err[0] = std::current_exception();
+  }
+  for (auto& t : th) t.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
   fitted_ = true;
 }
 
@@ -933,22 +1022,22 @@ void Engine::assemble() {
   a.fact = (float)((double)P.explodefact / 0.5);
   a.dxyo = P.dxy / P.zoomfact;
   for (int d = 0; d < L.ndirs; ++d) a.k0[d] = k0_[d];
-  stage_begin("asm_z");
+  cur_stage_ = stage_begin("asm_z", stream_);
   launch_asm_z(a, stream_);
-  stage_end();
-  stage_begin("asm_y");
+  stage_end(cur_stage_, stream_);
+  cur_stage_ = stage_begin("asm_y", stream_);
   launch_asm_y(a, stream_);
-  stage_end();
-  stage_begin("asm_x");
+  stage_end(cur_stage_, stream_);
+  cur_stage_ = stage_begin("asm_x", stream_);
   launch_asm_x(a, stream_);
-  stage_end();
+  stage_end(cur_stage_, stream_);
 }
 
 void Engine::filter_assemble() {
   if (!fitted_) throw StateError("filter_assemble before find_modulation");
-  stage_begin("filter");
+  cur_stage_ = stage_begin("filter", stream_);
   for (int dir = 0; dir < L.ndirs; ++dir) filter_dir(dir);
-  stage_end();
+  stage_end(cur_stage_, stream_);
   assemble();
   processed_ = true;
 }
@@ -1018,10 +1107,11 @@ void Engine::set_band(int dir, int band, const float* host) {
 }
 
 void Engine::debug_overlaps(int dir, int o1, int o2, float k0x, float k0y, double* Ph, double* sx, double* sy) {
-  make_overlaps(dir, o1, o2, k0x, k0y);
-  SIMR_HIP(hipMemcpyAsync(Ph, P_, L.nxy * 16, hipMemcpyDeviceToHost, stream_));
-  SIMR_HIP(hipMemcpyAsync(sx, SX_, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
-  SIMR_HIP(hipMemcpyAsync(sy, SY_, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
+  FitWork& w = fw_[0];
+  make_overlaps(w, dir, o1, o2, k0x, k0y);
+  SIMR_HIP(hipMemcpyAsync(Ph, w.P, L.nxy * 16, hipMemcpyDeviceToHost, stream_));
+  SIMR_HIP(hipMemcpyAsync(sx, w.SX, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
+  SIMR_HIP(hipMemcpyAsync(sy, w.SY, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
   sync();
 }
 

@@ -4,6 +4,7 @@
 // src/cudaSirecon/cudaSirecon.cpp:194-494,1139-1321,1354-1435).
 #pragma once
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -77,24 +78,50 @@ class Engine {
   struct Eval {
     double xr, xi, sx, sy;
   };
+  // Scratch and state of one direction's k0/modamp fit.  Directions are
+  // fitted concurrently, each on its own stream and host thread; direction
+  // 0 uses the context stream, the others alias the assembly staging
+  // buffers, which are idle while the fit runs.
+  struct FitWork {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    float2* ovx = nullptr;
+    double2* P = nullptr;
+    double *SX = nullptr, *SY = nullptr;
+    float2* cc = nullptr;
+    float* pv = nullptr;
+    int* pi = nullptr;
+    float* peakvals = nullptr;
+    int* peakidx = nullptr;
+    float2* kcand = nullptr;
+    double2* part64 = nullptr;
+    float4* part32 = nullptr;
+    double* sumparts = nullptr;
+    void* hstage = nullptr;  // pinned host staging
+    bool ov_totals_valid = false;
+    double ov_sx = 0, ov_sy = 0;
+    std::map<std::pair<uint32_t, uint32_t>, Eval> cache;
+    int cache_o1 = -1, cache_o2 = -1;
+  };
 
   void derive();
   Geo geometry() const;
   int zd_overlaps() const;
   OtfRA otf_table(int dir, int order) const;
-  void make_overlaps(int dir, int o1, int o2, float k0x, float k0y);
-  std::vector<Eval> eval_batch(const std::vector<float2>& kvecs);
+  void carve_fit(FitWork& w, char*& at, bool count_only, size_t& bytes) const;
+  void make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float k0y);
+  std::vector<Eval> eval_batch(FitWork& w, const std::vector<float2>& kvecs);
   void modamp_of(const Eval& e, float2* modamp, double* amp2, float* corr) const;
-  double getmodamp(int dir, float angle, float mag, int o1, int o2, float2* modamp, float* corr);
-  double realspacemodamp(int dir, int o1, int o2, float2 kv, bool redo, float2* modamp, float* corr);
-  void prefetch(int o1, int o2, const std::vector<std::pair<float, float>>& am);
-  void findk0(int dir);
-  void fitk0andmodamps(int dir);
-  void modulation(int dir);
+  double getmodamp(FitWork& w, int dir, float angle, float mag, int o1, int o2, float2* modamp, float* corr);
+  double realspacemodamp(FitWork& w, int dir, int o1, int o2, float2 kv, bool redo, float2* modamp, float* corr);
+  void prefetch(FitWork& w, int o1, int o2, const std::vector<std::pair<float, float>>& am);
+  void findk0(FitWork& w, int dir);
+  void fitk0andmodamps(FitWork& w, int dir);
+  void modulation(FitWork& w, int dir);
   FilterParams filter_params(int dir, int order) const;
   void assemble();
-  void stage_begin(const char* name);
-  void stage_end();
+  int stage_begin(const char* name, hipStream_t s);
+  void stage_end(int id, hipStream_t s);
 
   simr_params P;
   Layout L;
@@ -113,31 +140,11 @@ class Engine {
   double *sums_ = nullptr, *sdp_ = nullptr;
   float2* bands_ = nullptr;
   float2* otf_ = nullptr;
-  float2* ovx_ = nullptr;
-  double2* P_ = nullptr;
-  double *SX_ = nullptr, *SY_ = nullptr;
-  float2* cc_ = nullptr;
-  float* pv_ = nullptr;
-  int* pi_ = nullptr;
-  float* peakvals_ = nullptr;
-  int* peakidx_ = nullptr;
-  float2* kcand_ = nullptr;
-  double2* part64_ = nullptr;
-  float4* part32_ = nullptr;
-  double* sumparts_ = nullptr;
+  std::vector<FitWork> fw_;  // one per direction
   float2 *stageZ_ = nullptr, *stageY_ = nullptr;
   float* out_ = nullptr;
-  int max_cand_ = 64;
 
-  // pinned host staging
-  void* hstage_ = nullptr;
   size_t hstage_bytes_ = 0;
-
-  // overlap-set state
-  bool ov_totals_valid_ = false;
-  double ov_sx_ = 0, ov_sy_ = 0;
-  std::map<std::pair<uint32_t, uint32_t>, Eval> cache_;
-  int cache_o1_ = -1, cache_o2_ = -1;
 
   // reconstruction state
   int cur_t_ = 0;
@@ -149,6 +156,8 @@ class Engine {
 
   // timing
   bool timing_ = false;
+  int cur_stage_ = -1;  // open stage on the context stream
+  std::mutex ev_mu_;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> events_;
   std::vector<std::pair<std::string, float>> last_timings_;
 };
