@@ -65,6 +65,9 @@ __device__ __forceinline__ float2 prof_at(const float2* prof, int nxotf, float d
 // One (dir2, order2) term of the Wiener denominator at absolute frequency
 // (xabs, yabs), plane z0; 0 for the band's own term and for terms outside the
 // support.
+// UNV: every noiseVar is 1 (known at compile time; the division would
+// otherwise be computed and discarded by a select in the unrolled loop).
+template <bool UNV>
 __device__ __forceinline__ float filter_term(const FilterParams& p, const float2* prof, int nxotf, int order, int dir2,
                                              int order2, float xabs, float yabs, int z0, float zf3) {
 #pragma clang fp contract(off)
@@ -82,7 +85,7 @@ __device__ __forceinline__ float filter_term(const FilterParams& p, const float2
   const int ao2 = order2 < 0 ? -order2 : order2;
   const float2 otf2 = prof_idx(prof + ao2 * nxotf, nxotf, __builtin_amdgcn_sqrtf(d2 * p.inv_dkr2));
   float w = otf2.x * otf2.x + otf2.y * otf2.y;
-  if (!p.unit_nv) w = w / p.noiseVar[dir2 * p.norders + ao2];
+  if (!UNV && !p.unit_nv) w = w / p.noiseVar[dir2 * p.norders + ao2];
   if (order2 != 0) w *= p.ampmag2_all[dir2 * p.norders + ao2];
   if (p.bSuppress && order2 != 0 && d2 <= p.supp2)
     w *= dev_suppress(sqrtf(d2) / p.min_dkr);
@@ -97,7 +100,7 @@ __device__ __forceinline__ float filter_term(const FilterParams& p, const float2
 // ND, NO > 0: the direction and order counts as compile-time constants (the
 // (dir2, order2) loop is unrolled, so its per-term constants are read once
 // and the OTF lookups of different terms overlap); 0: taken from p.
-template <int ND, int NO>
+template <int ND, int NO, bool UNV>
 __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2* prof, int x1, int y1, int z0,
                                              bool conj, float2& re, float2& im) {
 #pragma clang fp contract(off)
@@ -135,11 +138,11 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
     for (int dir2 = 0; dir2 < ND; dir2++)
 #pragma unroll
       for (int order2 = -(NO - 1); order2 < NO; order2++)
-        sumweight += filter_term(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
+        sumweight += filter_term<UNV>(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
   } else {
     for (int dir2 = 0; dir2 < p.ndirs; dir2++)
       for (int order2 = -(p.norders - 1); order2 < p.norders; order2++)
-        sumweight += filter_term(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
+        sumweight += filter_term<false>(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
   }
   sumweight += p.wiener;
   float2 scale = make_float2(dampfact * otf1.x / sumweight, dampfact * (-otf1.y) / sumweight);
@@ -191,7 +194,7 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
 // Planes whose z lies outside +-zd are zeroed (kernel3).
 constexpr int kFiltPer = 4;
 
-template <int ND, int NO>
+template <int ND, int NO, bool UNV>
 __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
 #pragma clang fp contract(off)
   extern __shared__ float2 prof[];  // [2][norders][nxotf]
@@ -252,11 +255,11 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
     float2 re = bp[e];
     float2 im = two ? bp2[e] : make_float2(0.f, 0.f);
     // first pass (x1 in [1, nx/2]) or second pass x1 == 0: the element itself
-    filter_visit<ND, NO>(p, prof, jin, y1n, z0n, false, re, im);
+    filter_visit<ND, NO, UNV>(p, prof, jin, y1n, z0n, false, re, im);
     // second pass, x1 = -jin < 0: the conjugate visit of the mirrored frequency
     if (two && jin >= 1 && jin <= p.nx / 2 - 1) {
       const int yc = -((iin <= p.ny / 2) ? iin : iin - p.ny);
-      filter_visit<ND, NO>(p, prof + nprof, -jin, yc, -z0n, true, re, im);
+      filter_visit<ND, NO, UNV>(p, prof + nprof, -jin, yc, -z0n, true, re, im);
     }
     bp[e] = re;
     if (two) bp2[e] = im;
@@ -268,12 +271,20 @@ void launch_filter(const FilterParams& p, hipStream_t s) {
   const size_t lds = (size_t)2 * p.norders * p.otf[0].nxotf * sizeof(float2);
   if (lds > 64 * 1024) throw ArgError("OTF table too long for the filter's LDS profile");
   const dim3 grid(ceil_div(nplane, 256 * kFiltPer), p.nz);
-  if (p.ndirs == 3 && p.norders == 3)  // the 3-beam 3D case
-    hipLaunchKernelGGL((k_filter<3, 3>), grid, dim3(256), lds, s, p);
-  else if (p.ndirs == 3 && p.norders == 2)  // 2D, 3 phases
-    hipLaunchKernelGGL((k_filter<3, 2>), grid, dim3(256), lds, s, p);
-  else
-    hipLaunchKernelGGL((k_filter<0, 0>), grid, dim3(256), lds, s, p);
+  const bool unv = p.unit_nv != 0;
+  if (p.ndirs == 3 && p.norders == 3) {  // the 3-beam 3D case
+    if (unv)
+      hipLaunchKernelGGL((k_filter<3, 3, true>), grid, dim3(256), lds, s, p);
+    else
+      hipLaunchKernelGGL((k_filter<3, 3, false>), grid, dim3(256), lds, s, p);
+  } else if (p.ndirs == 3 && p.norders == 2) {  // 2D, 3 phases
+    if (unv)
+      hipLaunchKernelGGL((k_filter<3, 2, true>), grid, dim3(256), lds, s, p);
+    else
+      hipLaunchKernelGGL((k_filter<3, 2, false>), grid, dim3(256), lds, s, p);
+  } else {
+    hipLaunchKernelGGL((k_filter<0, 0, false>), grid, dim3(256), lds, s, p);
+  }
   SIMR_LAUNCH_CHECK();
 }
 
