@@ -39,7 +39,8 @@ class SimrParams(C.Structure):
         ("constbkgd", C.c_float),
         ("bFastSIM", C.c_int32),
         ("fit_fp64", C.c_int32),
-        ("reserved", C.c_int32 * 15),
+        ("nx_raw", C.c_int32), ("deskew_factor", C.c_float), ("deskew_shift", C.c_int32),
+        ("reserved", C.c_int32 * 12),
     ]
 
     def copy(self):

@@ -125,6 +125,10 @@ Layout Layout::of(const simr_params& p) {
   L.nDirsOTF = p.bOneOTFperAngle ? p.ndirs : 1;
   L.H = (long long)L.nxh * p.ny * p.nz;
   L.nxy = (long long)p.nx * p.ny;
+  SIMR_REQUIRE(p.nx_raw >= 0, "nx_raw must not be negative");
+  L.nx_raw = p.nx_raw > 0 ? p.nx_raw : p.nx;
+  L.nxy_raw = (long long)L.nx_raw * p.ny;
+  L.resample = L.nx_raw != p.nx || p.deskew_factor != 0.f;
   L.xdim = (int)std::rint((double)p.zoomfact * p.nx);
   L.ydim = (int)std::rint((double)p.zoomfact * p.ny);
   L.zdim = p.z_zoom * p.nz;
@@ -139,8 +143,8 @@ Layout Layout::of(const simr_params& p) {
   SIMR_REQUIRE(2 * L.zd_ov_max + 1 <= p.nz, "overlap z support wider than the stack");
   const int ncand = kMaxCand;
   size_t b = 0;
-  b += align256((size_t)L.nimg * L.nxy * 4);            // raw (float or uint16)
-  b += 2 * align256((size_t)L.nxy * 4);                  // bg, slope
+  b += align256((size_t)L.nimg * L.nxy_raw * 4);        // raw (float or uint16)
+  b += 2 * align256((size_t)L.nxy_raw * 4);              // bg, slope
   b += align256((size_t)L.nimg * kSumSplit * 8) + align256((size_t)L.nimg * 4) + align256((size_t)p.nz * 8) + align256(256);
   b += align256((size_t)L.ndirs * L.nbands * L.H * 8);   // bands
   b += align256((size_t)L.nDirsOTF * L.norders * L.otfsize * 8);
@@ -149,7 +153,8 @@ Layout Layout::of(const simr_params& p) {
   // assembly staging; the fit scratch of directions 1.. aliases it
   const size_t stage = align256((size_t)L.nasm * L.zdim * L.nxy * 8) +            // stageZ
                        align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * p.nx * 8);  // stageY
-  b += std::max(stage, (size_t)(L.ndirs - 1) * fit);
+  // (and the resampled float planes, which live there while the bands are formed)
+  b += std::max(std::max(stage, (size_t)(L.ndirs - 1) * fit), L.resample ? (size_t)L.nimg * (size_t)L.nxy * 4 : (size_t)0);
   b += align256((size_t)L.zdim * L.ydim * (size_t)L.xdim * 4);              // output
   L.bytes_total = b;
   return L;
@@ -162,9 +167,9 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   SIMR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   fit_fp32_ = P.fit_fp64 == 0;
   arena_.reserve(L.bytes_total);
-  raw_ = arena_.take((size_t)L.nimg * L.nxy * 4);
-  bg_ = (float*)arena_.take((size_t)L.nxy * 4);
-  slope_ = (float*)arena_.take((size_t)L.nxy * 4);
+  raw_ = arena_.take((size_t)L.nimg * L.nxy_raw * 4);
+  bg_ = (float*)arena_.take((size_t)L.nxy_raw * 4);
+  slope_ = (float*)arena_.take((size_t)L.nxy_raw * 4);
   sums_ = (double*)arena_.take((size_t)L.nimg * kSumSplit * 8);
   ratio_ = (float*)arena_.take((size_t)L.nimg * 4);
   sdp_ = (double*)arena_.take((size_t)L.nz * 8);
@@ -175,8 +180,10 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   char* fit0 = (char*)arena_.take(fitb);
   const size_t zb = align256((size_t)L.nasm * L.zdim * L.nxy * 8);
   const size_t yb = align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * L.nx * 8);
-  char* stage = (char*)arena_.take(std::max(zb + yb, (size_t)(L.ndirs - 1) * fitb));
+  char* stage = (char*)arena_.take(std::max(std::max(zb + yb, (size_t)(L.ndirs - 1) * fitb),
+                                            L.resample ? (size_t)L.nimg * (size_t)L.nxy * 4 : (size_t)0));
   stageZ_ = (float2*)stage;
+  resampled_ = (float*)stage;
   stageY_ = (float2*)(stage + zb);
   out_ = (float*)arena_.take((size_t)L.zdim * L.ydim * (size_t)L.xdim * 4);
 
@@ -265,6 +272,7 @@ void Engine::derive() {
 void Engine::update_params(const simr_params& p) {
   const Layout N = Layout::of(p);
   SIMR_REQUIRE(N.nx == L.nx && N.ny == L.ny && N.nz == L.nz && N.ndirs == L.ndirs && N.nphases == L.nphases &&
+                   N.nx_raw == L.nx_raw && N.resample == L.resample &&
                    N.norders == L.norders && N.xdim == L.xdim && N.ydim == L.ydim && N.zdim == L.zdim &&
                    N.nDirsOTF == L.nDirsOTF && N.otfsize == L.otfsize && N.zd_ov_max <= L.zd_ov_max,
                "parameter change alters the context's buffer layout; create a new context");
@@ -365,7 +373,7 @@ int Engine::timings(const char** names, float* ms, int cap) {
 
 void Engine::set_raw(const void* src, bool u16, bool on_device) {
   SIMR_REQUIRE(src != nullptr, "raw data pointer is null");
-  const size_t bytes = (size_t)L.nimg * L.nxy * (u16 ? 2 : 4);
+  const size_t bytes = (size_t)L.nimg * L.nxy_raw * (u16 ? 2 : 4);
   SIMR_HIP(hipMemcpyAsync(raw_, src, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
   raw_u16_ = u16;
   raw_set_ = true;
@@ -377,23 +385,39 @@ void Engine::set_flatfield(const float* bg, const float* slope) {
   // constant `background` option and the slope 1; that case needs no arrays.
   flat_uniform_ = bg == nullptr && slope == nullptr;
   if (flat_uniform_) return;
-  std::vector<float> b((size_t)L.nxy), s((size_t)L.nxy);
-  for (long long e = 0; e < L.nxy; ++e) {
+  std::vector<float> b((size_t)L.nxy_raw), s((size_t)L.nxy_raw);
+  for (long long e = 0; e < L.nxy_raw; ++e) {
     b[e] = bg ? bg[e] : P.constbkgd;
     s[e] = slope ? slope[e] : 1.0f;
   }
-  SIMR_HIP(hipMemcpyAsync(bg_, b.data(), L.nxy * 4, hipMemcpyHostToDevice, stream_));
-  SIMR_HIP(hipMemcpyAsync(slope_, s.data(), L.nxy * 4, hipMemcpyHostToDevice, stream_));
+  SIMR_HIP(hipMemcpyAsync(bg_, b.data(), L.nxy_raw * 4, hipMemcpyHostToDevice, stream_));
+  SIMR_HIP(hipMemcpyAsync(slope_, s.data(), L.nxy_raw * 4, hipMemcpyHostToDevice, stream_));
   sync();
 }
 
 // ------------------------------------------------------------ forward path
 void Engine::load_and_rescale(int it) {
   if (!raw_set_) throw StateError("raw data has not been set");
-  cur_stage_ = stage_begin("section_sums", stream_);
   const float* bgp = flat_uniform_ ? nullptr : bg_;
   const float* slp = flat_uniform_ ? nullptr : slope_;
-  launch_section_sums(raw_, raw_u16_, bgp, slp, P.constbkgd, inscale_, L.nx, L.ny, L.nimg, sums_, stream_);
+  const void* src = raw_;
+  bool src_u16 = raw_u16_;
+  float bg0 = P.constbkgd, insc = inscale_;
+  if (L.resample) {
+    // flat-field and resample into float planes; the stages below then see
+    // them with the identity flat-field ((v - 0 - 0) * 1 * 1 = v exactly)
+    cur_stage_ = stage_begin("resample", stream_);
+    launch_resample(raw_, raw_u16_, bgp, slp, P.constbkgd, inscale_, L.nx_raw, L.nx, L.ny, L.nz, L.nimg, L.ndirs,
+                    L.nphases, P.bFastSIM, P.deskew_factor, P.deskew_shift, resampled_, stream_);
+    stage_end(cur_stage_, stream_);
+    src = resampled_;
+    src_u16 = false;
+    bgp = slp = nullptr;
+    bg0 = 0.f;
+    insc = 1.f;
+  }
+  cur_stage_ = stage_begin("section_sums", stream_);
+  launch_section_sums(src, src_u16, bgp, slp, bg0, insc, L.nx, L.ny, L.nimg, sums_, stream_);
   launch_ratios(sums_, sdp_, ratio_, L.ndirs, L.nz, L.nphases, P.bFastSIM, it, P.equalizez, P.equalizet,
                 P.do_rescale, stream_);
   stage_end(cur_stage_, stream_);
@@ -407,12 +431,12 @@ void Engine::load_and_rescale(int it) {
   fp.nbands = L.nbands;
   fp.fastSI = P.bFastSIM;
   fp.napod = P.napodize > 0 ? P.napodize : 0;
-  fp.inscale = inscale_;
-  fp.bg0 = P.constbkgd;
+  fp.inscale = insc;
+  fp.bg0 = bg0;
   for (int b = 0; b < L.nbands; ++b)
     for (int ph = 0; ph < L.nphases; ++ph) fp.sep[b * kMaxBands + ph] = sep_[b * L.nphases + ph];
   cur_stage_ = stage_begin("rows_fwd", stream_);
-  launch_rows_fwd(fp, raw_, raw_u16_, bgp, slp, ratio_, apofact_, bands_, stream_);
+  launch_rows_fwd(fp, src, src_u16, bgp, slp, ratio_, apofact_, bands_, stream_);
   stage_end(cur_stage_, stream_);
   cur_stage_ = stage_begin("cols_fwd_y", stream_);
   fft_cols(L.ny, bands_, L.nxh, L.nxh, (long long)L.ndirs * L.nbands * L.nz, (long long)L.nxh * L.ny, false,

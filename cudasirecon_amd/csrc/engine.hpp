@@ -35,6 +35,9 @@ struct Layout {
   // sizes derived from the parameters (identical for the footprint query and the context)
   int nx, ny, nz, nxh, ndirs, nphases, norders, nbands, nimg, nDirsOTF;
   long long H, nxy;
+  int nx_raw;             // width of the raw sections
+  long long nxy_raw;      // nx_raw * ny
+  bool resample;          // raw sections are resampled (deskew / crop) into nx-wide planes
   int xdim, ydim, zdim, nasm, zd_ov_max;
   size_t otfsize;
   size_t bytes_total;
@@ -142,6 +145,7 @@ class Engine {
   float2* otf_ = nullptr;
   std::vector<FitWork> fw_;  // one per direction
   float2 *stageZ_ = nullptr, *stageY_ = nullptr;
+  float* resampled_ = nullptr;  // [nimg][ny][nx] resampled raw planes (aliases the staging region)
   float* out_ = nullptr;
 
   size_t hstage_bytes_ = 0;

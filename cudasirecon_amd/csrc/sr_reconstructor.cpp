@@ -181,11 +181,30 @@ Reconstructor::~Reconstructor() {
 
 // The reference's optional branches that this engine does not implement are
 // refused up front instead of producing a partial result.
+// notGoodDimension / findOptimalDimension (cudaSirecon.cpp:77-97): step down
+// to the next size whose only prime factors are 2, 3, 5 and 7.
+static bool not_good_dimension(unsigned n) {
+  if (n == 2 || n == 3 || n == 5 || n == 7) return false;
+  if (n % 2 == 0) return not_good_dimension(n / 2);
+  if (n % 3 == 0) return not_good_dimension(n / 3);
+  if (n % 5 == 0) return not_good_dimension(n / 5);
+  if (n % 7 == 0) return not_good_dimension(n / 7);
+  return true;
+}
+static unsigned find_optimal_dimension(unsigned n) {
+  while (n > 1 && not_good_dimension(n)) n -= 1;
+  return n;
+}
+
 void Reconstructor::check_supported() const {
-  if (std::fabs(P_.deskewAngle) > 0.f) throw UnsupportedError("--deskew is not supported by this engine");
+  // --noRecon only acts together with --bessel in the reference (loadAndRescaleImage,
+  // cudaSirecon.cpp:1325); --bessel itself is refused below.
   if (P_.bBessel) throw UnsupportedError("--bessel is not supported by this engine");
   if (P_.bTwolens) throw UnsupportedError("--2lenses is not supported by this engine");
-  if (P_.bNoRecon) throw UnsupportedError("--noRecon is not supported by this engine");
+  if (P_.bUsecorr && (std::fabs(P_.deskewAngle) > 0.f || (P_.cropXYto > 0 && (int)P_.cropXYto < I_.nx_raw)))
+    throw UnsupportedError(
+        "--usecorr together with --deskew or --cropXY is not supported (the reference indexes the "
+        "calibration maps with the raw width but sizes them with the resampled one)");
   if (P_.bSaveSeparated || P_.bSaveAlignedRaw || P_.bSaveOverlaps)
     throw UnsupportedError("--saveprefiltered/--savealignedraw/--saveoverlaps are not supported by this engine");
   if (!P_.bRadAvgOTF)
@@ -199,9 +218,21 @@ void Reconstructor::check_supported() const {
 void Reconstructor::setup_common() {
   if (I_.nz < 1) throw ArgError("fewer raw images than ndirs*nphases");
   I_.nz0 = I_.nz;  // nzPadTo is never set
-  I_.nx = I_.nx_raw;
-  I_.dz_raw = 0;
-  if (P_.cropXYto > 0 && (int)P_.cropXYto < I_.nx) throw UnsupportedError("--cropXY is not supported by this engine");
+  // deskewing widens nx to the next smaller 2/3/5/7-smooth size that holds the
+  // sheared stack and scales dz by sin(angle) (cudaSirecon.cpp:1153-1165);
+  // cropping then narrows nx (1168-1171)
+  if (std::fabs(P_.deskewAngle) > 0.f) {
+    if (P_.deskewAngle < 0) P_.deskewAngle += 180.f;
+    const double a = (double)P_.deskewAngle * M_PI / 180.;
+    I_.nx = (int)find_optimal_dimension(
+        (unsigned)((double)I_.nx_raw + (double)I_.nz0 * std::fabs(std::cos(a)) * (double)I_.dz / (double)I_.dxy));
+    I_.dz_raw = I_.dz;
+    I_.dz = (float)((double)I_.dz * std::fabs(std::sin(a)));
+  } else {
+    I_.nx = I_.nx_raw;
+    I_.dz_raw = 0;
+  }
+  if (P_.cropXYto > 0 && (int)P_.cropXYto < I_.nx) I_.nx = (int)P_.cropXYto;
   std::printf("nx=%d, ny=%d, nz=%d, nz0 = %d, nwaves=%d\n", I_.nx, I_.ny, I_.nz, I_.nz0, I_.nwaves);
   std::printf("dxy=%f, dz=%f um\n", I_.dxy, I_.dz);
   P_.norders = P_.norders_output != 0 ? P_.norders_output : P_.nphases / 2 + 1;
@@ -329,6 +360,12 @@ simr_params Reconstructor::engine_params() const {
   simr_params p;
   simr_params_default(&p);
   p.nx = I_.nx;
+  p.nx_raw = I_.nx_raw;
+  // loadImageData (cudaSirecon.cpp:1355-1360): cos(angle) dz_raw / dxy in double, stored as float
+  p.deskew_factor = std::fabs(P_.deskewAngle) > 0.f
+                        ? (float)(std::cos((double)P_.deskewAngle * M_PI / 180.) * (double)I_.dz_raw / (double)I_.dxy)
+                        : 0.f;
+  p.deskew_shift = P_.extraShift;
   p.ny = I_.ny;
   p.nz = I_.nz0;
   p.ndirs = P_.ndirs;

@@ -38,6 +38,11 @@ void launch_ratios(const double* partials, double* sdp, float* ratio, int ndirs,
                    int it, int equalizez, int equalizet, int do_rescale, hipStream_t s);
 // Fused flatfield + rescale + apodize + phase separation + R2C along x.
 // bands: [dir][band][z][y][nxh] complex.
+// deskewOneSection (cudaSirecon.cpp:1733-1760) of the flat-fielded sections
+// (load_and_flatfield, 1715-1731): raw [nimg][ny][nx_raw] -> out [nimg][ny][nx].
+void launch_resample(const void* raw, bool u16, const float* bg, const float* slope, float bg0, float inscale,
+                     int nx_raw, int nx, int ny, int nz, int nimg, int ndirs, int nphases, int fastSI, float factor,
+                     int shift, float* out, hipStream_t s);
 void launch_rows_fwd(const FwdParams& p, const void* raw, bool u16, const float* bg, const float* slope,
                      const float* ratio, const float* apofact, float2* bands, hipStream_t s);
 

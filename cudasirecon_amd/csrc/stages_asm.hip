@@ -307,7 +307,12 @@ __device__ __forceinline__ void asm_x_add_band(const AsmParams& p, const float2*
           ph = lut[e][SRC];
         else
           ph = asm_phasor<N>(p, a, X, Y);
-        const double d = (double)v.x * 2.0 * (double)ph.x - (double)v.y * 2.0 * (double)ph.y;
+        // d = 2 v.x ph.x - 2 v.y ph.y in double as the reference forms it: 2 v is
+        // exact in float, each float x float product is exact in double, so
+        // one product plus one fused multiply-add rounds once, as the
+        // reference's subtraction does (same double, half the FP64 work)
+        const double bp = (double)(2.f * v.y) * (double)ph.y;
+        const double d = __builtin_fma((double)(2.f * v.x), (double)ph.x, -bp);
         acc[e] = (float)((double)acc[e] + d);
       }
     }

@@ -47,6 +47,60 @@ __device__ __forceinline__ float4 flat4(const T* raw, size_t sec_off, int pix, c
                      flat1(v.w, b.w, s.w, inscale));
 }
 
+// ---------------------------------------------------------------- resampling
+// One output pixel (xout, y) of section sec: the z index of the section
+// (Angle->Z->Phase, or Z->Angle->Phase with fastSI) sets the shift; xin and
+// the interpolation weight follow the reference's double/float expression
+// order.  in[indin + 1] past the last pixel of the section (the reference
+// reads beyond its buffer there) is taken as 0.
+template <typename T>
+__device__ __forceinline__ float flat_px(const T* raw, size_t off, long long i, const float* bg, const float* slope,
+                                         float bg0, float inscale) {
+  const float b = bg ? bg[i] : bg0, s = slope ? slope[i] : 1.f;
+  return flat1((float)raw[off + i], b, s, inscale);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_resample(const T* __restrict__ raw, const float* __restrict__ bg,
+                                                  const float* __restrict__ slope, float bg0, float inscale,
+                                                  int nx_raw, int nx, int ny, int nz, int ndirs, int nphases,
+                                                  int fastSI, float factor, int shift, float* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int xout = blockIdx.x * 256 + threadIdx.x;
+  const int y = blockIdx.y, sec = blockIdx.z;
+  if (xout >= nx) return;
+  const int z = fastSI ? sec / (ndirs * nphases) : (sec / nphases) % nz;
+  float xin = (float)xout;
+  if (fabsf(factor) > 0)
+    xin = (float)((double)xout - (double)nx / 2.0 + (double)shift - (double)factor * ((double)z - (double)nz / 2.0) +
+                  (double)nx_raw / 2.0);
+  float v = 0.f;
+  if (xin >= 0.f && xin < (float)nx_raw) {
+    const float fl = floorf(xin);
+    const long long i0 = (long long)y * nx_raw + (long long)fl;
+    const float offset = xin - fl;
+    const size_t off = (size_t)sec * (size_t)nx_raw * ny;
+    const float a = flat_px(raw, off, i0, bg, slope, bg0, inscale);
+    const float b = i0 + 1 < (long long)nx_raw * ny ? flat_px(raw, off, i0 + 1, bg, slope, bg0, inscale) : 0.f;
+    v = (1 - offset) * a + offset * b;
+  }
+  out[((size_t)sec * ny + y) * nx + xout] = v;
+}
+
+void launch_resample(const void* raw, bool u16, const float* bg, const float* slope, float bg0, float inscale,
+                     int nx_raw, int nx, int ny, int nz, int nimg, int ndirs, int nphases, int fastSI, float factor,
+                     int shift, float* out, hipStream_t s) {
+  if (nimg > 65535 || ny > 65535) throw ArgError("resampling: too many sections or rows");
+  const dim3 grid(ceil_div(nx, 256), ny, nimg);
+  if (u16)
+    hipLaunchKernelGGL(k_resample<uint16_t>, grid, dim3(256), 0, s, (const uint16_t*)raw, bg, slope, bg0, inscale,
+                       nx_raw, nx, ny, nz, ndirs, nphases, fastSI, factor, shift, out);
+  else
+    hipLaunchKernelGGL(k_resample<float>, grid, dim3(256), 0, s, (const float*)raw, bg, slope, bg0, inscale, nx_raw,
+                       nx, ny, nz, ndirs, nphases, fastSI, factor, shift, out);
+  SIMR_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------- section sums
 // Block (sec, part) sums part `part` of kSumSplit of section sec in double.
 template <typename T>

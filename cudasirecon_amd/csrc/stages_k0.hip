@@ -382,10 +382,19 @@ __global__ __launch_bounds__(ovz_cols<N>() * fft2_tpt(N)) void k_overlaps_z2(OvP
 #pragma unroll
   for (int k2 = 0; k2 < N2; ++k2) {
     const A ar = X[k2].x, ai = X[k2].y, br = Y[k2].x, bi = Y[k2].y;
-    sxr += ar * br + ai * bi;
-    sxi += ar * bi - ai * br;
-    sx += ar * ar + ai * ai;
-    sy += br * br + bi * bi;
+    if constexpr (FP32) {
+      sxr += ar * br + ai * bi;
+      sxi += ar * bi - ai * br;
+      sx += ar * ar + ai * ai;
+      sy += br * br + bi * bi;
+    } else {
+      // float x float products are exact in double, so a product plus a fused
+      // multiply-add gives the correctly rounded a*b + c*d of the plain form
+      sxr += __builtin_fma(ar, br, ai * bi);
+      sxi += __builtin_fma(ar, bi, -(ai * br));
+      sx += __builtin_fma(ar, ar, ai * ai);
+      sy += __builtin_fma(br, br, bi * bi);
+    }
   }
 #pragma unroll
   for (int m = 1; m < TPT; m <<= 1) {

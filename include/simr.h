@@ -77,7 +77,16 @@ typedef struct simr_params {
   int32_t bFastSIM;
   /* engine options (no counterpart in the reference) */
   int32_t fit_fp64;            /* 1: k0/modamp fit reductions in FP64 (default); 0: FP32 partials */
-  int32_t reserved[15];
+  /* lateral resampling of the raw sections before the reconstruction
+     (deskewOneSection, cudaSirecon.cpp:1733-1760): raw sections are nx_raw
+     wide; output column x of z section z samples the flat-fielded raw row at
+     x - nx/2 + deskew_shift - deskew_factor (z - nz/2) + nx_raw/2 (linear
+     interpolation), or at x when deskew_factor is 0 (a crop keeps the left
+     nx columns).  nx_raw 0 or nx with deskew_factor 0: no resampling. */
+  int32_t nx_raw;              /* width of the raw sections (0: nx) */
+  float deskew_factor;         /* cos(deskew angle) * raw dz / dxy (0: none) */
+  int32_t deskew_shift;        /* --deskewshift */
+  int32_t reserved[12];
 } simr_params;
 
 /* Per-direction fit results, the observable outputs the reference prints. */
@@ -130,7 +139,8 @@ int simr_set_raw_f32_host(simr_ctx* ctx, const float* raw);
 int simr_set_raw_u16_host(simr_ctx* ctx, const uint16_t* raw);
 int simr_set_raw_f32_dev(simr_ctx* ctx, const float* raw_dev);
 int simr_set_raw_u16_dev(simr_ctx* ctx, const uint16_t* raw_dev);
-/* Optional flat-field maps (background, slope), nx*ny floats each; NULL resets to constbkgd/1. */
+/* Optional flat-field maps (background, slope), nx_raw*ny floats each (nx*ny without
+ * resampling); NULL resets to constbkgd/1. */
 int simr_set_flatfield(simr_ctx* ctx, const float* background, const float* slope);
 
 /* Stage calls (in this order per time point). */

@@ -62,6 +62,7 @@ template <typename T>
 struct Oracle {
   simr_params P{};
   int nx = 0, ny = 0, nz = 0, nz0 = 0, ndirs = 0, nphases = 0, norders = 0;
+  int nx_raw = 0;     // raw section width (deskew / crop resample it to nx)
   int xdim = 0;       // (nx/2+1)*2 floats per padded row
   long long H = 0;    // complex elements of one half-complex band
   T dxy = 0, dz = 0;
@@ -144,8 +145,9 @@ struct Oracle {
     sum_dir0_phase0.assign(nz, 0.0);
     amp.assign(ndirs, std::vector<C2<T>>(norders, {0, 0}));
     for (int i = 0; i < ndirs; ++i) amp[i][0] = {(T)1, (T)0};
-    background.assign((size_t)nx * ny, (T)p.constbkgd);  // bgAndSlope 139-161
-    slope.assign((size_t)nx * ny, (T)1);
+    nx_raw = p.nx_raw > 0 ? p.nx_raw : nx;
+    background.assign((size_t)std::max(nx, nx_raw) * ny, (T)p.constbkgd);  // bgAndSlope 139-161
+    slope.assign((size_t)std::max(nx, nx_raw) * ny, (T)1);
     stats.assign(ndirs, FitStats());
     corr.assign(ndirs, std::vector<float>(SIMR_MAX_ORDERS, 0.f));
   }
@@ -164,9 +166,48 @@ struct Oracle {
     }
   }
 
-  // loadImageData + load_and_flatfield + deskewOneSection(factor 0)
-  // (cudaSirecon.cpp:1354-1435, 1715-1760).  raw: [nImages][ny][nx] floats.
+  // deskewOneSection (cudaSirecon.cpp:1733-1760): the flat-fielded section
+  // `in` (nx_raw wide, float as the reference's CImg) into dst (xdim floats
+  // per row).  in[indin + 1] past the end of the section, which the reference
+  // reads beyond its buffer, is taken as 0.
+  void deskew_section(const std::vector<float>& in, T* dst, int z) const {
+    const float factor = P.deskew_factor;
+    for (int xout = 0; xout < xdim; ++xout) {
+      float xin = (float)xout;
+      if (std::fabs(factor) > 0)
+        xin = (float)(xout - nx / 2. + P.deskew_shift - factor * (z - nz / 2.) + nx_raw / 2.);
+      for (int y = 0; y < ny; ++y) {
+        const size_t indout = (size_t)y * xdim + xout;
+        if (xin >= 0 && xin < nx_raw) {
+          const size_t indin = (size_t)y * nx_raw + (size_t)std::floor(xin);
+          const float offset = xin - (float)std::floor(xin);
+          const float b = indin + 1 < in.size() ? in[indin + 1] : 0.f;
+          dst[indout] = (T)((1 - offset) * in[indin] + offset * b);
+        } else {
+          dst[indout] = (T)0;
+        }
+      }
+    }
+  }
+
+  // loadImageData + load_and_flatfield + deskewOneSection
+  // (cudaSirecon.cpp:1354-1435, 1715-1760).  raw: [nImages][ny][nx_raw] floats.
   void load(const float* raw) {
+    if (nx_raw != nx || P.deskew_factor != 0.f) {
+      std::vector<float> sec((size_t)nx_raw * ny);
+      for (int dir = 0; dir < ndirs; ++dir)
+        for (int z = 0; z < nz; ++z) {
+          int zsec = P.bFastSIM ? (z * ndirs * nphases + dir * nphases) : (dir * nz * nphases + z * nphases);
+          for (int phase = 0; phase < nphases; ++phase) {
+            const float* src = raw + (size_t)zsec * nx_raw * ny;
+            for (size_t e = 0; e < sec.size(); ++e)  // load_and_flatfield in float, raw-width indices
+              sec[e] = (((float)src[e] - (float)background[e]) - 0.0f) * (float)slope[e] * (float)inscale;
+            deskew_section(sec, bands[dir][phase].data() + (size_t)z * xdim * ny, z);
+            ++zsec;
+          }
+        }
+      return;
+    }
     for (int dir = 0; dir < ndirs; ++dir) {
       for (int z = 0; z < nz; ++z) {
         int zsec = P.bFastSIM ? (z * ndirs * nphases + dir * nphases) : (dir * nz * nphases + z * nphases);

@@ -180,8 +180,17 @@ def test_executable_help_and_error_exit_status(tmp_path):
     assert "!!Error occurred: No matching input TIFF or MRC files not found" in r.stderr
 
 
+def test_deskew_to_an_unsupported_width_is_refused(tmp_path):
+    # raw.dv: nx_raw 64, 9 sections per phase, dz 0.2 / dxy 0.1 -> findOptimalDimension(64 + 9 cos(30) 2) = 75,
+    # a width the engine has no kernels for: a clear error, no partial result
+    r = subprocess.run([EXE, os.path.join(GOLDEN, "raw.dv"), str(tmp_path / "o.dv"), os.path.join(GOLDEN, "otf.dv"),
+                        "--deskew", "30", "--otfRA"], capture_output=True, text=True)
+    assert "!!Error occurred: image width 75 is not a supported size" in r.stderr, r.stderr
+    assert "nx=75" in r.stdout
+
+
 def test_unsupported_branches_are_refused(tmp_path):
-    for extra in (["--deskew", "30"], ["--bessel"], ["--2lenses"], ["--saveoverlaps", "x.dv"], []):
+    for extra in (["--bessel"], ["--2lenses"], ["--saveoverlaps", "x.dv"], []):
         r = subprocess.run([EXE, os.path.join(GOLDEN, "raw.dv"), str(tmp_path / "o.dv"),
                             os.path.join(GOLDEN, "otf.dv")] + extra + (["--otfRA"] if extra else []),
                            capture_output=True, text=True)

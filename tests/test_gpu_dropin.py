@@ -183,3 +183,47 @@ def test_cli_mrc_time_series_matches_oracle(tmp_path):
         err = rel_l2(vol[9 * t: 9 * (t + 1)], refs[t])
         log("time %d rel-L2 %.3e" % (t, err))
         assert err < TOL
+
+
+def _resampled_case(extra_cfg):
+    """(oracle params, otf, raw) for the test_data TIFF with deskew/crop options applied
+    the way setup_common and loadImageData derive them (cudaSirecon.cpp:1153-1171, 1355-1360)."""
+    import math
+    p, otf, raw = load_testdata_tiff()
+    q = p.copy()
+    q.nx_raw = p.nx
+    if "deskew" in extra_cfg:
+        ang = float(np.float32(extra_cfg["deskew"]))
+        if ang < 0:
+            ang = float(np.float32(ang + 180.0))  # setup_common folds negative angles
+        a = ang * math.pi / 180.0
+        dz_raw = float(np.float32(p.dz))
+        nx = int(64 + p.nz * abs(math.cos(a)) * dz_raw / float(np.float32(p.dxy)))
+        assert nx == 64  # a good dimension already: findOptimalDimension keeps it
+        q.nx = 64
+        q.dz = float(np.float32(dz_raw * abs(math.sin(a))))
+        q.deskew_factor = float(np.float32(math.cos(a) * dz_raw / float(np.float32(p.dxy))))
+    if "cropXY" in extra_cfg:
+        q.nx = int(extra_cfg["cropXY"])
+    return q, otf, raw
+
+
+@pytest.mark.parametrize("opts", [{"deskew": 87.0}, {"cropXY": 48}, {"deskew": 87.0, "deskewshift": 3},
+                                  {"deskew": -93.0}])
+def test_sr_deskew_and_crop_match_oracle(tmp_path, opts):
+    log("case: resampling", opts)
+    q, otf, raw = _resampled_case(opts)
+    if "deskewshift" in opts:
+        q.deskew_shift = int(opts["deskewshift"])
+    extra = "\n".join("%s=%s" % kv for kv in opts.items())
+    with sr.SIMReconstructor(shape=(64, 64, raw.shape[0]), config=_shape_config(tmp_path, extra)) as r:
+        im = r.image_params
+        assert (im.nx, im.nx_raw) == (q.nx, 64)
+        if "deskew" in opts:
+            assert im.dz == pytest.approx(q.dz, rel=1e-6) and im.dz_raw == pytest.approx(0.125)
+        vol = r.reconstruct(raw)
+    ref = oracle_series(q, otf, [raw])[0]
+    assert vol.shape == ref.shape == (9, 2 * q.ny, 2 * q.nx)
+    err = rel_l2(vol, ref)
+    log("resampled %s rel-L2 %.3e" % (opts, err))
+    assert err < TOL
