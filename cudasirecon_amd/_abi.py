@@ -40,7 +40,8 @@ class SimrParams(C.Structure):
         ("bFastSIM", C.c_int32),
         ("fit_fp64", C.c_int32),
         ("nx_raw", C.c_int32), ("deskew_factor", C.c_float), ("deskew_shift", C.c_int32),
-        ("reserved", C.c_int32 * 12),
+        ("bBessel", C.c_int32), ("BesselNA", C.c_float), ("BesselLambdaEx", C.c_float),
+        ("reserved", C.c_int32 * 9),
     ]
 
     def copy(self):
@@ -194,6 +195,9 @@ def default_params():
     p.constbkgd = 0.0
     p.bFastSIM = 0
     p.fit_fp64 = 1
+    p.bBessel = 0
+    p.BesselNA = 0.144
+    p.BesselLambdaEx = 0.488
     return p
 
 

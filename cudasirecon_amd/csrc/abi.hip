@@ -107,6 +107,8 @@ void simr_params_default(simr_params* p) {
   p->do_rescale = 1;
   p->wiener = 0.01f;
   p->fit_fp64 = 1;
+  p->BesselNA = 0.144f;       // --besselNA default (cudaSirecon.cpp:983)
+  p->BesselLambdaEx = 0.488f;  // --besselExWave default (cudaSirecon.cpp:981)
 }
 
 size_t simr_otf_size(const simr_params* p) {

@@ -21,13 +21,34 @@ constexpr int kMaxCand = 64;  // line-search candidates reduced per launch
 int norders_of(const simr_params& p) { return p.norders_output != 0 ? p.norders_output : p.nphases / 2 + 1; }
 
 // makeoverlaps' axial support (gpuFunctionsImpl.cu:508-548), float semantics.
-int overlap_zd(const simr_params& p) {
+// Without --bessel it does not depend on k0.  With --bessel it is
+// ceil((kzExMax sin(halfangle) + (1 - cos(alpha)) / lambdaem) / dkz) with
+// cos(halfangle) = |k0| order2 / (norders - 1) / kzExMax; k0mag < 0 gives the
+// bound over every k0 (halfangle = pi/2), which sizes the overlap stage.
+int overlap_zd(const simr_params& p, float k0mag = -1.f, int order2 = 1, int norders = 2) {
   const int nz = p.nz;
   const float dz = p.dz;
   const float dkz = dz > 0 ? 1.0f / ((float)nz * dz) : p.dkzotf;
   const float lambdaem = (float)((double)((float)p.wave / p.nimm) / 1.0e3);
   const float alpha = asinf(p.na / p.nimm);
-  float zd = (float)(int)std::ceil(((1.0 - (double)cosf(alpha)) / (double)lambdaem) / (double)dkz);
+  float zd;
+  if (!p.bBessel) {
+    zd = (float)(int)std::ceil(((1.0 - (double)cosf(alpha)) / (double)lambdaem) / (double)dkz);
+  } else {
+    const float kzExMax = (float)(2.0 * (double)p.BesselNA / (double)p.BesselLambdaEx);
+    float s = 1.0f;
+    if (k0mag >= 0.f) {
+      const float c = k0mag * (float)order2 / (float)(norders - 1) / kzExMax;
+      SIMR_REQUIRE(c <= 1.0f, "Bessel excitation kz extent 2*besselNA/besselExWave is smaller than the "
+                              "pattern's lateral frequency |k0|*order/(norders-1)");
+      s = sinf(acosf(c));
+    }
+    const double q = ((double)(kzExMax * s) + (1.0 - (double)cosf(alpha)) / (double)lambdaem) / (double)dkz;
+    zd = (float)std::ceil(q);
+    // the bound: a margin for the float rounding of sin(acos(c)) near 1, and
+    // no clamp, since any value up to it can come out unclamped
+    if (k0mag < 0.f) return (int)zd + 1;
+  }
   if (zd > (float)(nz / 2)) zd = (float)(((nz / 2 - 1) > 0) ? (nz / 2 - 1) : 0);
   return (int)zd;
 }
@@ -134,6 +155,8 @@ Layout Layout::of(const simr_params& p) {
   L.zdim = p.z_zoom * p.nz;
   L.nasm = 1 + p.ndirs * (L.norders - 1);
   L.zd_ov_max = overlap_zd(p);
+  // Bessel: every per-call support up to the bound, within 2 zd + 1 <= nz
+  if (p.bBessel) L.zd_ov_max = std::min(L.zd_ov_max, (p.nz - 1) / 2);
   L.otfsize = (size_t)p.nzotf * p.nxotf * p.nyotf;
   SIMR_REQUIRE(in_list_xy(p.nx), "image width " + std::to_string(p.nx) + " is not a supported size");
   SIMR_REQUIRE(fft_size_supported(p.ny) && p.ny <= 4096, "image height " + std::to_string(p.ny) + " is not a supported size");
@@ -469,6 +492,9 @@ Engine::Geo Engine::geometry() const {
 
 int Engine::zd_overlaps() const { return L.zd_ov_max; }
 
+// fitorder2 of findk0 / fitk0andmodamps (gpuFunctionsImpl.cu:316-323, 880-887)
+int Engine::fit_order2() const { return L.nz > 1 && !P.bBessel ? 2 : 1; }
+
 OtfRA Engine::otf_table(int dir, int order) const {
   const int d = P.bOneOTFperAngle ? dir : 0;
   OtfRA o;
@@ -487,14 +513,21 @@ void Engine::make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float
   p.ny = L.ny;
   p.nz = L.nz;
   p.nxh = L.nxh;
-  p.zd = zd_overlaps();
+  if (P.bBessel) {
+    // the reference's k0mag = sqrt(k0x*k0x + k0y*k0y) of the float operands
+    const float k0mag = (float)std::sqrt((double)(k0x * k0x + k0y * k0y));
+    p.zd = overlap_zd(P, k0mag, o2, L.norders);
+    SIMR_REQUIRE(p.zd <= L.zd_ov_max, "overlap z support wider than the stack");
+  } else {
+    p.zd = zd_overlaps();
+  }
   p.order1 = o1;
   p.order2 = o2;
   p.kx = k0x * (float)(o2 - o1);
   p.ky = k0y * (float)(o2 - o1);
   p.rdistcutoff = g.rdistcutoff;
-  p.otfcutoff = P.otfcutoff;
-  p.order0_2_factor = L.nz > 1 ? 5.0f : 1.0f;
+  p.otfcutoff = P.bBessel ? 0.01f : P.otfcutoff;
+  p.order0_2_factor = L.nz > 1 ? (P.bBessel ? 4.0f : 5.0f) : 1.0f;
   p.dkx = g.dkx;
   p.dky = g.dky;
   p.kzscale = g.kzscale;
@@ -657,7 +690,7 @@ static float fitparabola(float a1, float a2, float a3) {
 }
 
 void Engine::findk0(FitWork& w, int dir) {
-  const int fo2 = L.nz > 1 ? 2 : 1;
+  const int fo2 = fit_order2();
   make_overlaps(w, dir, 0, fo2, k0_[dir].x, k0_[dir].y);
   const int st = stage_begin("findk0_peak", w.stream);
   launch_findpeak(w.P, w.cc, w.pv, w.pi, w.peakvals, w.peakidx, L.nx, L.ny, w.stream);
@@ -702,7 +735,7 @@ static U fitxyparabola(U x1, U y1, U x2, U y2, U x3, U y3) {
 // one launch and the reference's step rule is replayed on the results.
 void Engine::fitk0andmodamps(FitWork& w, int dir) {
   constexpr int kLook = 6;
-  const int fo1 = 0, fo2 = L.nz > 1 ? 2 : 1;
+  const int fo1 = 0, fo2 = fit_order2();
   const float2 kv = k0_[dir];
   const float k0mag = sqrtf(kv.x * kv.x + kv.y * kv.y);
   const float k0angle = atan2f(kv.y, kv.x);
@@ -968,18 +1001,35 @@ FilterParams Engine::filter_params(int dir, int order) const {
   const float lambdaexc = (float)(0.88 * (double)lambdaem);
   float rdistcutoff = (float)((double)(P.na * 2.0f) / ((double)P.wave / 1000.0));
   if ((double)rdistcutoff > 1. / (2. * (double)P.dxy)) rdistcutoff = (float)(1. / (2. * (double)P.dxy));
-  float zq = (1.0f - cosf(alpha)) / lambdaem;
-  zq = zq / g.dkz;
-  const int zd0 = (int)std::ceil(zq);
-  f.zd[0] = zd0;
-  f.zd[L.norders - 1] = (int)(1.3 * (double)zd0);
-  if (L.norders >= 3)
-    for (int o = 1; o < L.norders - 1; o++) f.zd[o] = (int)((1.0f + lambdaem / lambdaexc) * (float)zd0);
+  if (!P.bBessel) {
+    float zq = (1.0f - cosf(alpha)) / lambdaem;
+    zq = zq / g.dkz;
+    const int zd0 = (int)std::ceil(zq);
+    f.zd[0] = zd0;
+    f.zd[L.norders - 1] = (int)(1.3 * (double)zd0);
+    if (L.norders >= 3)
+      for (int o = 1; o < L.norders - 1; o++) f.zd[o] = (int)((1.0f + lambdaem / lambdaexc) * (float)zd0);
+  } else {
+    // gpuFunctionsImpl.cu:1260-1271: order 0 spans the excitation's kz extent
+    // plus the detection OTF's; order o the part of the excitation left at
+    // lateral frequency |k0| o / (norders - 1)
+    const float kzExMax = (2.0f * P.BesselNA) / P.BesselLambdaEx;
+    f.zd[0] = (int)std::rint((kzExMax + (1.0f - cosf(alpha)) / lambdaem) / g.dkz);
+    for (int o = 1; o < L.norders; o++) {
+      const float c = k0mag * (float)o / (float)(L.norders - 1) / kzExMax;
+      SIMR_REQUIRE(c <= 1.0f, "Bessel excitation kz extent 2*besselNA/besselExWave is smaller than the "
+                              "pattern's lateral frequency |k0|*order/(norders-1)");
+      const double q = ((double)(kzExMax * sinf(acosf(c))) + (1.0 - (double)cosf(alpha)) / (double)lambdaem) /
+                       (double)g.dkz;
+      f.zd[o] = (int)std::ceil(q);
+    }
+  }
   for (int o = 0; o < L.norders; o++)
     if (f.zd[o] >= L.nz / 2) f.zd[o] = ((L.nz / 2 - 1) > 0 ? (L.nz / 2 - 1) : 0);
   f.rdistcutoff = rdistcutoff;
   f.apocutoff = rdistcutoff + k0mag * (float)(L.norders - 1);
-  f.zapocutoff = L.norders > 1 ? (float)f.zd[1] : (float)f.zd[0];
+  f.zapocutoff = (L.norders > 1 && !P.bBessel) ? (float)f.zd[1] : (float)f.zd[0];
+  f.bessel = P.bBessel ? 1 : 0;
   f.dkx = (float)(1. / (double)((float)L.nx * P.dxy));
   f.dky = (float)(1. / (double)((float)L.ny * P.dxy));
   f.min_dkr = f.dkx < f.dky ? f.dkx : f.dky;

@@ -110,6 +110,7 @@ class Engine {
   void derive();
   Geo geometry() const;
   int zd_overlaps() const;
+  int fit_order2() const;
   OtfRA otf_table(int dir, int order) const;
   void carve_fit(FitWork& w, char*& at, bool count_only, size_t& bytes) const;
   void make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float k0y);

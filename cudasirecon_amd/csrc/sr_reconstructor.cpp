@@ -197,9 +197,12 @@ static unsigned find_optimal_dimension(unsigned n) {
 }
 
 void Reconstructor::check_supported() const {
-  // --noRecon only acts together with --bessel in the reference (loadAndRescaleImage,
-  // cudaSirecon.cpp:1325); --bessel itself is refused below.
-  if (P_.bBessel) throw UnsupportedError("--bessel is not supported by this engine");
+  // --noRecon only acts together with --bessel in the reference: loadAndRescaleImage
+  // (cudaSirecon.cpp:1325) then returns before loading the stack, and the
+  // reconstruction that follows runs on whatever the device buffers held.
+  if (P_.bBessel && P_.bNoRecon)
+    throw UnsupportedError("--noRecon together with --bessel is not supported (the reference skips loading the "
+                           "data and reconstructs uninitialised buffers)");
   if (P_.bTwolens) throw UnsupportedError("--2lenses is not supported by this engine");
   if (P_.bUsecorr && (std::fabs(P_.deskewAngle) > 0.f || (P_.cropXYto > 0 && (int)P_.cropXYto < I_.nx_raw)))
     throw UnsupportedError(
@@ -412,6 +415,9 @@ simr_params Reconstructor::engine_params() const {
   p.dkzotf = P_.dkzotf;
   p.constbkgd = P_.constbkgd;
   p.bFastSIM = P_.bFastSIM;
+  p.bBessel = P_.bBessel ? 1 : 0;
+  p.BesselNA = P_.BesselNA;
+  p.BesselLambdaEx = P_.BesselLambdaEx;
   return p;
 }
 

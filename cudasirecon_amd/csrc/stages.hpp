@@ -94,6 +94,7 @@ struct FilterParams {
   float rdistcutoff, apocutoff, zapocutoff, kzscale, wiener;
   float dkx, dky, min_dkr, suppRadius;
   int bSuppress, bDampenOrder0, bNoKz0, bFilteroverlaps, apodizeoutput;
+  int bessel;  // apodization radius x/rdistcutoff, y/apocutoff, z/zapocutoff (gpuFunctionsImpl.cu:1570-1577)
   float2 k0[kMaxDirs];
   float ampmag2_all[kMaxDirs * kMaxOrders];
   float noiseVar[kMaxDirs * kMaxOrders];

@@ -151,7 +151,10 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
     const float rdistabs = sqrtf(xabs * xabs + yabs * yabs);
     const float zdistabs = (float)abs(z0);
     const float ra = rdistabs / p.apocutoff;
-    if (p.zapocutoff > 0) {
+    if (p.zapocutoff > 0 && p.bessel) {
+      const float rx = xabs / p.rdistcutoff, ry = yabs / p.apocutoff, rz = zdistabs / p.zapocutoff;
+      rho = sqrtf(rx * rx + ry * ry + rz * rz);
+    } else if (p.zapocutoff > 0) {
       const float za = zdistabs / p.zapocutoff;
       rho = sqrtf(ra * ra + za * za);
     } else {

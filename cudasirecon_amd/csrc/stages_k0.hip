@@ -9,16 +9,16 @@
 // Differences in structure (same values up to FP reassociation):
 //  * overlaps are generated inside the x-pass of the inverse FFT, only for
 //    the 2*zd+1 planes the support can reach; the zero planes are never
-//    written, and the z pass reads them as zeros;
-//  * the z pass ends in the per-(x,y) sums P = sum_z conj(X) Y, SX, SY, so the
-//    3D overlap volumes are never re-read: every later line-search step is a
-//    2D reduction of P against a phase ramp;
+//    written;
+//  * the fit reads only the per-(x,y) z sums P = sum_z conj(X) Y, SX, SY,
+//    which Parseval's identity along z forms from the stored kz planes, so
+//    there is no z transform, and the 3D overlap volumes are never re-read:
+//    every later line-search step is a 2D reduction of P against a phase ramp;
 //  * a whole batch of line-search candidates is reduced in one launch.
 #include <type_traits>
 
 #include "device_math.hpp"
 #include "fft.hpp"
-#include "fft2.hpp"
 #include "fft_api.hpp"
 #include "sizes.hpp"
 #include "stages.hpp"
@@ -217,171 +217,28 @@ void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s) {
   fft_cols(p.ny, ovx, p.nx, p.nx, 2LL * (2 * p.zd + 1), nxy, true, s);
 }
 
-// z pass + per-(x,y) z sums.  Wc adjacent columns e of both overlaps per workgroup.
-template <int N>
-constexpr int zcols() {
-  constexpr int tpt = fft_tpt(N);
-  int w = 512 / tpt;
-  if (w > 64) w = 64;
-  if (w < 1) w = 1;
-  while (w > 1 && 2 * w * lds_pitch(N) * 8 > 80 * 1024) w /= 2;
-  return w;
-}
-
-template <int N, bool FP32>
-__global__ __launch_bounds__(2 * zcols<N>() * fft_tpt(N)) void k_overlaps_z(OvParams p, const float2* __restrict__ ovx,
-                                                                           double2* __restrict__ P,
-                                                                           double* __restrict__ SX,
-                                                                           double* __restrict__ SY,
-                                                                           const float2* __restrict__ tw) {
+// z pass, reduced to what the fit reads.  The modamp and cross-correlation
+// kernels use only the per-(x,y) sums over z of conj(X) Y, |X|^2 and |Y|^2 of
+// the inverse-transformed overlaps, and the phase factor they apply depends on
+// (x, y) alone (reductionKernel, gpuFunctionsImpl.cu:1150-1190; aTimesConjB
+// 391-413).  For the unnormalised inverse DFT along z, Parseval gives
+// sum_z conj(x_z) y_z = nz sum_kz conj(X_kz) Y_kz, and only the 2zd+1 stored
+// kz planes are nonzero, so the sums are formed from those planes directly
+// and the z transform is not needed.  One thread per (x, y); loads of four
+// planes in flight at once.  Sums in double (FP64 fit) or float.
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_overlaps_zsum(const float2* __restrict__ ovx, int nzs, int nz, long long nxy,
+                                                       double2* __restrict__ P, double* __restrict__ SX,
+                                                       double* __restrict__ SY) {
 #pragma clang fp contract(off)
-  constexpr int WC = zcols<N>();
-  constexpr int TPT = fft_tpt(N);
-  constexpr int NT = 2 * WC * TPT;
-  constexpr int LP = lds_pitch(N);
-  __shared__ float2 lds[2 * WC * LP];
-  FftTw<N> twr;
-  __shared__ float2 twstage[N > 1 ? N : 1];
-  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
-  const long long nxy = (long long)p.nx * p.ny;
-  const long long e0 = (long long)blockIdx.x * WC;
-  const int nzs = 2 * p.zd + 1;
-  // Slot (which, s, w): s < nzs is stored plane s (z = s - zd, wrapped);
-  // the other slots are the zero planes zd+1 .. N-zd-1.  All loads of a
-  // thread use clamped addresses and no branches, so they are in flight at
-  // once.
-  constexpr int EPT = (2 * WC * N + NT - 1) / NT;
-  float2 v[EPT];
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int idx = threadIdx.x + k * NT < 2 * WC * N ? threadIdx.x + k * NT : 2 * WC * N - 1;
-    const int which = idx / (WC * N);
-    const int rem = idx - which * WC * N;
-    const int sp = rem / WC, w = rem - (rem / WC) * WC;
-    const long long e = e0 + w < nxy ? e0 + w : nxy - 1;
-    const int sc = sp < nzs ? sp : 0;
-    v[k] = ovx[((long long)which * nzs + sc) * nxy + e];
-  }
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int idx = threadIdx.x + k * NT;
-    const int which = idx / (WC * N);
-    const int rem = idx - which * WC * N;
-    const int sp = rem / WC, w = rem - (rem / WC) * WC;
-    const int z = sp < nzs ? (sp - p.zd >= 0 ? sp - p.zd : sp - p.zd + N) : sp - nzs + p.zd + 1;
-    if (idx < 2 * WC * N)
-      lds[(which * WC + w) * LP + lds_swz<N>(z)] = (sp < nzs && e0 + w < nxy) ? v[k] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
-  for (int w = threadIdx.x; w < WC; w += NT) {
-    const long long e = e0 + w;
-    if (e >= nxy) continue;
-    const float2* X = lds + w * LP;
-    const float2* Y = lds + (WC + w) * LP;
-    if (FP32) {
-      float sxr = 0.f, sxi = 0.f, sx = 0.f, sy = 0.f;
-      for (int z = 0; z < N; ++z) {
-        const float2 a = X[lds_swz<N>(z)], b = Y[lds_swz<N>(z)];
-        sxr += a.x * b.x + a.y * b.y;
-        sxi += a.x * b.y - a.y * b.x;
-        sx += a.x * a.x + a.y * a.y;
-        sy += b.x * b.x + b.y * b.y;
-      }
-      P[e] = make_double2(sxr, sxi);
-      SX[e] = sx;
-      SY[e] = sy;
-    } else {
-      double sxr = 0.0, sxi = 0.0, sx = 0.0, sy = 0.0;
-      for (int z = 0; z < N; ++z) {
-        const float2 xa = X[lds_swz<N>(z)], yb = Y[lds_swz<N>(z)];
-        const double ar = xa.x, ai = xa.y, br = yb.x, bi = yb.y;
-        sxr += ar * br + ai * bi;
-        sxi += ar * bi - ai * br;
-        sx += ar * ar + ai * ai;
-        sy += br * br + bi * bi;
-      }
-      P[e] = make_double2(sxr, sxi);
-      SX[e] = sx;
-      SY[e] = sy;
-    }
-  }
-}
-
-// z pass on the two-pass core: the TPT threads of a column transform its X
-// and then its Y overlap (one LDS slot each) and keep both results in
-// registers (elements z = t + N1 k2), form their share of the z sums there
-// and combine the TPT shares with cross-lane adds; thread 0 of the column
-// writes P, SX, SY.  Sums run in double (FP64 fit) or float.
-template <int N>
-constexpr bool ovz_fft2() {
-  return use_fft2<N>() && fft2_tpt(N) >= 2 && fft2_tpt(N) <= 16;
-}
-// columns per workgroup: at most 64 and 80 KB of LDS (two slots per column)
-template <int N>
-constexpr int ovz_cols() {
-  int w = 64;
-  while (w > 1 && 2 * w * fft2_pitch<N>() * 8 > 80 * 1024) w /= 2;
-  return w;
-}
-
-template <typename T>
-__device__ __forceinline__ T xor_add(T v, int mask) {
-  return v + __shfl_xor(v, mask, 64);
-}
-
-template <int N, bool FP32>
-__global__ __launch_bounds__(ovz_cols<N>() * fft2_tpt(N)) void k_overlaps_z2(OvParams p, const float2* __restrict__ ovx,
-                                                                       double2* __restrict__ P,
-                                                                       double* __restrict__ SX,
-                                                                       double* __restrict__ SY,
-                                                                       const float2* __restrict__ tw) {
-#pragma clang fp contract(off)
-  constexpr int WC = ovz_cols<N>();
-  constexpr int TPT = fft2_tpt(N);
-  constexpr int NT = WC * TPT;
-  constexpr int N2 = fft2_plan(N).n2;
-  constexpr int SP = fft2_pitch<N>();
-  static_assert(NT % 64 == 0 && 64 % TPT == 0, "columns must not straddle wavefronts");
-  __shared__ float2 lds[2 * WC * SP];  // slot 2w: X of column w, slot 2w+1: Y
-  const int tid = threadIdx.x, w = tid / TPT, t = tid % TPT;
-  Fft2Tw<N> tw2;
-  tw2.load(t, tw);
-  const long long nxy = (long long)p.nx * p.ny;
-  const long long e0 = (long long)blockIdx.x * WC;
-  const int nzs = 2 * p.zd + 1;
-  // Slot (which, s, w): s < nzs is stored plane s (z = s - zd, wrapped); the
-  // other slots are the zero planes zd+1 .. N-zd-1.  Clamped loads, no branches.
-  constexpr int EPT = 2 * WC * N / NT;
-  float2 v[EPT];
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int idx = tid + k * NT;
-    const int which = idx / (WC * N);
-    const int rem = idx - which * WC * N;
-    const int sp = rem / WC, wc = rem - (rem / WC) * WC;
-    const long long e = e0 + wc < nxy ? e0 + wc : nxy - 1;
-    const int sc = sp < nzs ? sp : 0;
-    v[k] = ovx[((long long)which * nzs + sc) * nxy + e];
-  }
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int idx = tid + k * NT;
-    const int which = idx / (WC * N);
-    const int rem = idx - which * WC * N;
-    const int sp = rem / WC, wc = rem - (rem / WC) * WC;
-    const int z = sp < nzs ? (sp - p.zd >= 0 ? sp - p.zd : sp - p.zd + N) : sp - nzs + p.zd + 1;
-    lds[(2 * wc + which) * SP + z] = (sp < nzs && e0 + wc < nxy) ? v[k] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  float2 X[N2], Y[N2];
-  fft2_lds_regs<N, true, true>(lds + (2 * w) * SP, t, tw2, X);
-  fft2_lds_regs<N, true, true>(lds + (2 * w + 1) * SP, t, tw2, Y);
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= nxy) return;
   using A = typename std::conditional<FP32, float, double>::type;
   A sxr = 0, sxi = 0, sx = 0, sy = 0;
-#pragma unroll
-  for (int k2 = 0; k2 < N2; ++k2) {
-    const A ar = X[k2].x, ai = X[k2].y, br = Y[k2].x, bi = Y[k2].y;
+  const float2* X = ovx + e;
+  const float2* Y = ovx + (long long)nzs * nxy + e;
+  auto acc = [&](float2 xa, float2 yb) {
+    const A ar = xa.x, ai = xa.y, br = yb.x, bi = yb.y;
     if constexpr (FP32) {
       sxr += ar * br + ai * bi;
       sxi += ar * bi - ai * br;
@@ -395,58 +252,36 @@ __global__ __launch_bounds__(ovz_cols<N>() * fft2_tpt(N)) void k_overlaps_z2(OvP
       sx += __builtin_fma(ar, ar, ai * ai);
       sy += __builtin_fma(br, br, bi * bi);
     }
-  }
+  };
+  int s = 0;
+  for (; s + 4 <= nzs; s += 4) {
+    float2 xa[4], yb[4];
 #pragma unroll
-  for (int m = 1; m < TPT; m <<= 1) {
-    sxr = xor_add(sxr, m);
-    sxi = xor_add(sxi, m);
-    sx = xor_add(sx, m);
-    sy = xor_add(sy, m);
+    for (int k = 0; k < 4; ++k) {
+      xa[k] = X[(long long)(s + k) * nxy];
+      yb[k] = Y[(long long)(s + k) * nxy];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc(xa[k], yb[k]);
   }
-  const long long e = e0 + w;
-  if (t == 0 && e < nxy) {
-    P[e] = make_double2((double)sxr, (double)sxi);
-    SX[e] = (double)sx;
-    SY[e] = (double)sy;
-  }
-}
-
-template <int N>
-static void launch_ov_z(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, bool fp32,
-                        hipStream_t s) {
-  if constexpr (ovz_fft2<N>() && (ovz_cols<N>() * fft2_tpt(N)) % 64 == 0) {
-    const long long nxy = (long long)p.nx * p.ny;
-    dim3 grid(ceil_div(nxy, ovz_cols<N>())), block(ovz_cols<N>() * fft2_tpt(N));
-    if (fp32)
-      hipLaunchKernelGGL((k_overlaps_z2<N, true>), grid, block, 0, s, p, ovx, P, SX, SY, twiddles(N));
-    else
-      hipLaunchKernelGGL((k_overlaps_z2<N, false>), grid, block, 0, s, p, ovx, P, SX, SY, twiddles(N));
-    SIMR_LAUNCH_CHECK();
-    return;
-  }
-  constexpr int WC = zcols<N>();
-  const long long nxy = (long long)p.nx * p.ny;
-  dim3 grid(ceil_div(nxy, WC)), block(2 * WC * fft_tpt(N));
-  if (fp32)
-    hipLaunchKernelGGL((k_overlaps_z<N, true>), grid, block, 0, s, p, ovx, P, SX, SY, twiddles(N));
-  else
-    hipLaunchKernelGGL((k_overlaps_z<N, false>), grid, block, 0, s, p, ovx, P, SX, SY, twiddles(N));
-  SIMR_LAUNCH_CHECK();
+  for (; s < nzs; ++s) acc(X[(long long)s * nxy], Y[(long long)s * nxy]);
+  const A n = (A)nz;
+  P[e] = make_double2((double)(sxr * n), (double)(sxi * n));
+  SX[e] = (double)(sx * n);
+  SY[e] = (double)(sy * n);
 }
 
 void launch_overlaps_z_reduce(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, bool fp32,
                               hipStream_t s) {
   if (2 * p.zd + 1 > p.nz) throw ArgError("overlap z support wider than the stack");
-  switch (p.nz) {
-#define SIMR_CASE(N)                                  \
-  case N:                                             \
-    launch_ov_z<N>(p, ovx, P, SX, SY, fp32, s);       \
-    return;
-    SIMR_Z_SIZES(SIMR_CASE)
-#undef SIMR_CASE
-    default:
-      throw ArgError("stack depth " + std::to_string(p.nz) + " is not a supported size");
-  }
+  const long long nxy = (long long)p.nx * p.ny;
+  const int nzs = 2 * p.zd + 1;
+  const dim3 grid(ceil_div(nxy, 256));
+  if (fp32)
+    hipLaunchKernelGGL(k_overlaps_zsum<true>, grid, dim3(256), 0, s, ovx, nzs, p.nz, nxy, P, SX, SY);
+  else
+    hipLaunchKernelGGL(k_overlaps_zsum<false>, grid, dim3(256), 0, s, ovx, nzs, p.nz, nxy, P, SX, SY);
+  SIMR_LAUNCH_CHECK();
 }
 
 // ----------------------------------------------------------------- findk0 peak
@@ -607,7 +442,7 @@ void launch_modamp_fp64(const double2* P, const float2* kcand, int ncand, int nx
 
 // ------------------------------------------------------------ modamp, FP32
 // reductionKernel on 64x4 blocks, float throughout, the thread z-sums being
-// the float P/SX/SY of the z pass; block partials are summed on the host in
+// the float P/SX/SY of k_overlaps_zsum; block partials are summed on the host in
 // block-index order as cpuReduce does.
 int modamp_partials_fp32(int nx, int ny) { return ((nx + 63) / 64) * ((ny + 3) / 4); }
 

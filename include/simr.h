@@ -86,7 +86,15 @@ typedef struct simr_params {
   int32_t nx_raw;              /* width of the raw sections (0: nx) */
   float deskew_factor;         /* cos(deskew angle) * raw dz / dxy (0: none) */
   int32_t deskew_shift;        /* --deskewshift */
-  int32_t reserved[12];
+  /* Bessel-sheet SIM (--bessel, --besselNA, --besselExWave): k0 fitted on
+     order 1 in 3D, order-0 weight 4, overlap OTF cutoff 0.01, axial cutoffs
+     from the excitation's kz extent 2 NA / lambda, anisotropic apodization
+     (gpuFunctionsImpl.cu:317-323, 505-506, 533-539, 554-555, 1260-1271, 1294,
+     1570-1577) */
+  int32_t bBessel;
+  float BesselNA;              /* excitation NA of the Bessel beam */
+  float BesselLambdaEx;        /* excitation wavelength in microns */
+  int32_t reserved[9];
 } simr_params;
 
 /* Per-direction fit results, the observable outputs the reference prints. */

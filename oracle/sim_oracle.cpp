@@ -51,6 +51,8 @@ inline float Tsin(float v) { return sinf(v); }
 inline double Tsin(double v) { return std::sin(v); }
 inline float Tasin(float v) { return asinf(v); }
 inline double Tasin(double v) { return std::asin(v); }
+inline float Tacos(float v) { return acosf(v); }
+inline double Tacos(double v) { return std::acos(v); }
 inline float Tatan2(float a, float b) { return atan2f(a, b); }
 inline double Tatan2(double a, double b) { return std::atan2(a, b); }
 
@@ -399,12 +401,26 @@ struct Oracle {
   void makeoverlaps(int dir, int order1, int order2, T k0x, T k0y) {
     const int dirOTF = P.bOneOTFperAngle ? dir : 0;
     Geo g = geometry();
-    const T order0_2_factor = nz > 1 ? (T)5 : (T)1;
-    T zdistcutoff = (T)(int)std::ceil(((1.0 - (double)Tcos(g.alpha)) / (double)g.lambdaem) / (double)g.dkz);
+    // 498-506: order-0 weight 5 in 3D, 4 for Bessel-sheet data
+    const T order0_2_factor = nz > 1 ? (P.bBessel ? (T)4 : (T)5) : (T)1;
+    T zdistcutoff;
+    if (!P.bBessel) {  // 530-532
+      zdistcutoff = (T)(int)std::ceil(((1.0 - (double)Tcos(g.alpha)) / (double)g.lambdaem) / (double)g.dkz);
+    } else {  // 533-539: kzExMax in double then float; the sum in double
+      const T kzExMax = (T)(2.0 * (double)P.BesselNA / (double)P.BesselLambdaEx);
+      const T k0mag = Tsqrt(k0x * k0x + k0y * k0y);
+      const T c = k0mag * (T)order2 / (T)(norders - 1) / kzExMax;
+      if (!(c <= (T)1)) throw std::runtime_error("oracle: Bessel kz extent below the pattern frequency");
+      const T halfangle = Tacos(c);
+      zdistcutoff = (T)std::ceil(((double)(kzExMax * Tsin(halfangle)) + (1.0 - (double)Tcos(g.alpha)) /
+                                  (double)g.lambdaem) / (double)g.dkz);
+    }
     if (zdistcutoff > (T)(nz / 2)) zdistcutoff = (T)(((nz / 2 - 1) > 0) ? (nz / 2 - 1) : 0);
+    // zdistcutoff == nz/2 would make planes +-nz/2 write the same output plane
+    if (2 * (int)zdistcutoff + 1 > nz) throw std::runtime_error("oracle: overlap z support wider than the stack");
     const T kx = k0x * (T)(order2 - order1);
     const T ky = k0y * (T)(order2 - order1);
-    const T otfcutoff = (T)P.otfcutoff;
+    const T otfcutoff = P.bBessel ? (T)0.01f : (T)P.otfcutoff;  // 553-555
     const T rc = g.rdistcutoff;
     const long long N = (long long)nx * ny * nz;
     std::vector<C2<T>> ov0(N, C2<T>{0, 0}), ov1(N, C2<T>{0, 0});
@@ -550,7 +566,8 @@ struct Oracle {
     return peak;
   }
 
-  int fitorder2() const { return nz > 1 ? 2 : 1; }
+  // findk0 316-323, fitk0andmodamps 880-887: Bessel-sheet data fit order 1 in 3D
+  int fitorder2() const { return nz > 1 && !P.bBessel ? 2 : 1; }
 
   // findk0 (gpuFunctionsImpl.cu:306-372)
   void findk0(int dir) {
@@ -903,18 +920,30 @@ struct Oracle {
     T rdistcutoff = (T)((double)((T)P.na * (T)2) / ((double)wave / 1000.0));
     if ((double)rdistcutoff > 1. / (2. * (double)dxy)) rdistcutoff = (T)(1. / (2. * (double)dxy));
     // filterbands 1254: evaluated in float (unlike makeoverlaps 531, which promotes to double)
-    T zq = ((T)1 - Tcos(alpha)) / lambdaem;
-    zq = zq / g.dkz;
-    int zd0 = (int)std::ceil(zq);
-    fc.zd[0] = zd0;
-    fc.zd[norders - 1] = (int)(1.3 * (double)zd0);
-    if (norders >= 3)
-      for (int o = 1; o < norders - 1; o++) fc.zd[o] = (int)(((T)1 + lambdaem / lambdaexc) * (T)zd0);
+    if (!P.bBessel) {
+      T zq = ((T)1 - Tcos(alpha)) / lambdaem;
+      zq = zq / g.dkz;
+      int zd0 = (int)std::ceil(zq);
+      fc.zd[0] = zd0;
+      fc.zd[norders - 1] = (int)(1.3 * (double)zd0);
+      if (norders >= 3)
+        for (int o = 1; o < norders - 1; o++) fc.zd[o] = (int)(((T)1 + lambdaem / lambdaexc) * (T)zd0);
+    } else {  // 1260-1271: kzExMax and the order-0 sum in the working type, the others in double
+      T kzExMax = (T)2 * (T)P.BesselNA / (T)P.BesselLambdaEx;
+      fc.zd[0] = (int)std::rint((kzExMax + ((T)1 - Tcos(alpha)) / lambdaem) / g.dkz);
+      for (int o = 1; o < norders; o++) {
+        T c = k0mag * (T)o / (T)(norders - 1) / kzExMax;
+        if (!(c <= (T)1)) throw std::runtime_error("oracle: Bessel kz extent below the pattern frequency");
+        T halfangle = Tacos(c);
+        fc.zd[o] = (int)std::ceil(((double)(kzExMax * Tsin(halfangle)) + (1.0 - (double)Tcos(alpha)) / (double)lambdaem) /
+                                  (double)g.dkz);
+      }
+    }
     for (int o = 0; o < norders; o++)
       if (fc.zd[o] >= nz / 2) fc.zd[o] = ((nz / 2 - 1) > 0 ? (nz / 2 - 1) : 0);
     fc.rdistcutoff = rdistcutoff;
     fc.apocutoff = rdistcutoff + k0mag * (T)(norders - 1);
-    fc.zapocutoff = norders > 1 ? (T)fc.zd[1] : (T)fc.zd[0];
+    fc.zapocutoff = (norders > 1 && !P.bBessel) ? (T)fc.zd[1] : (T)fc.zd[0];  // 1292-1296
     for (int o = 0; o < norders; o++) {
       fc.ampmag2[o] = amp[dir][o].x * amp[dir][o].x + amp[dir][o].y * amp[dir][o].y;
       fc.conjamp[o] = {amp[dir][o].x, -amp[dir][o].y};
@@ -991,7 +1020,10 @@ struct Oracle {
       if (P.apodizeoutput) {
         T rho;
         T zdistabs = (T)std::abs(z0);
-        if (fc.zapocutoff > 0)
+        if (fc.zapocutoff > 0 && P.bBessel) {  // 1570-1577: x against rdistcutoff, y against apocutoff
+          T rhox = xabs / fc.rdistcutoff, rhoy = yabs / fc.apocutoff, rhoz = zdistabs / fc.zapocutoff;
+          rho = Tsqrt(rhox * rhox + rhoy * rhoy + rhoz * rhoz);
+        } else if (fc.zapocutoff > 0)
           rho = Tsqrt((rdistabs / fc.apocutoff) * (rdistabs / fc.apocutoff) +
                       (zdistabs / fc.zapocutoff) * (zdistabs / fc.zapocutoff));
         else

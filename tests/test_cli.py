@@ -190,7 +190,7 @@ def test_deskew_to_an_unsupported_width_is_refused(tmp_path):
 
 
 def test_unsupported_branches_are_refused(tmp_path):
-    for extra in (["--bessel"], ["--2lenses"], ["--saveoverlaps", "x.dv"], []):
+    for extra in (["--bessel", "--noRecon"], ["--2lenses"], ["--saveoverlaps", "x.dv"], []):
         r = subprocess.run([EXE, os.path.join(GOLDEN, "raw.dv"), str(tmp_path / "o.dv"),
                             os.path.join(GOLDEN, "otf.dv")] + extra + (["--otfRA"] if extra else []),
                            capture_output=True, text=True)

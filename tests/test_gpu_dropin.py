@@ -227,3 +227,21 @@ def test_sr_deskew_and_crop_match_oracle(tmp_path, opts):
     err = rel_l2(vol, ref)
     log("resampled %s rel-L2 %.3e" % (opts, err))
     assert err < TOL
+
+
+def test_sr_bessel_matches_oracle(tmp_path):
+    """--bessel through the config file: the reference's Bessel-sheet branches
+    (order-1 fit, overlap cutoff 0.01, order-0 weight 4, Bessel axial cutoffs,
+    anisotropic apodization) against the oracle's."""
+    log("case: bessel besselNA=0.65 besselExWave=0.488")
+    p, otf, raw = load_testdata_tiff()
+    q = p.copy()
+    q.bBessel, q.BesselNA, q.BesselLambdaEx = 1, float(np.float32(0.65)), float(np.float32(0.488))
+    with sr.SIMReconstructor(shape=(64, 64, raw.shape[0]),
+                             config=_shape_config(tmp_path, "bessel=true\nbesselNA=0.65\nbesselExWave=0.488")) as r:
+        assert r.recon_params.bBessel == 1
+        vol = r.reconstruct(raw)
+    ref = oracle_series(q, otf, [raw])[0]
+    err = rel_l2(vol, ref)
+    log("bessel rel-L2 %.3e" % err)
+    assert err < TOL

@@ -57,7 +57,7 @@ def _oracle_preprocessed(p, otf, raw, prec=32, fit64=False):
 
 
 def _cases():
-    return ["testdata", "syn3d", "syn2d"]
+    return ["testdata", "syn3d", "syn2d", "bessel3d"]
 
 
 def _case(name):
@@ -66,6 +66,9 @@ def _case(name):
         return p, otf, raw
     if name == "syn3d":
         p, otf, raw, _ = synthetic(128, 128, 16, nphases=5)
+        return p, otf, raw
+    if name == "bessel3d":  # --bessel: order-1 fit, Bessel axial cutoffs 15/14/10, anisotropic apodization
+        p, otf, raw, _ = synthetic(64, 64, 32, nphases=5, dz=0.1, bBessel=1, BesselNA=0.65, BesselLambdaEx=0.488)
         return p, otf, raw
     p, otf, raw, _ = synthetic(128, 128, 1, nphases=3)
     return p, otf, raw

@@ -249,3 +249,163 @@ def test_fp32_oracle_close_to_fp64_oracle():
     dk = 1.0 / (p.nx * p.dxy)
     for a, b in zip(f32, f64):
         assert math.hypot(a.k0x - b.k0x, a.k0y - b.k0y) / dk < 1e-3
+
+
+# ------------------------------------------------------------------ Bessel-sheet branches
+def test_bessel_2d_is_the_plain_path_with_otfcutoff_001():
+    """In 2D every Bessel branch but the overlap OTF cutoff collapses: fitorder2 is
+    1 either way, order0_2_factor is 1, every axial cutoff clamps to 0 and the
+    apodization takes its 2D form (gpuFunctionsImpl.cu:316-323, 505, 548, 554,
+    1286-1287, 1566-1586).  So --bessel in 2D equals the plain path run with
+    otfcutoff 0.01, and differs from the plain path with the default 0.006."""
+    p, tab, raw, _ = synthetic(64, 64, 1, nphases=3, photons=None)
+    plain = p.copy()
+    plain.otfcutoff = 0.01
+    bes = p.copy()
+    bes.bBessel, bes.BesselNA, bes.BesselLambdaEx = 1, 1.3, 0.488  # kzExMax 5.33 > |k0| 4.91
+    va, fa = ob.reconstruct(plain, tab, raw.astype(np.float32), 32)
+    vb, fb = ob.reconstruct(bes, tab, raw.astype(np.float32), 32)
+    np.testing.assert_array_equal(va, vb)
+    for a, b in zip(fa, fb):
+        assert (a.k0x, a.k0y, list(a.amp_re)) == (b.k0x, b.k0y, list(b.amp_re))
+    v0, _ = ob.reconstruct(p, tab, raw.astype(np.float32), 32)
+    assert rel_l2(v0, vb) > 0
+
+
+def _bessel3d(**kw):
+    # |k0| = 1/0.2035/2 = 2.457/um against kzExMax = 2*0.65/0.488 = 2.664/um;
+    # dz 0.1 keeps the axial cutoffs (15, 14, 10 for orders 0, 1, 2) below nz/2
+    return synthetic(64, 64, 32, nphases=5, dz=0.1, bBessel=1, BesselNA=0.65, BesselLambdaEx=0.488, **kw)
+
+
+def test_bessel_3d_fit_on_order_1_recovers_ground_truth():
+    """--bessel fits k0 on the (0, 1) overlap and divides the cross-correlation
+    peak by 1, not 2 (findk0, gpuFunctionsImpl.cu:316-372): the fitted k0 must
+    land on the simulated first-order frequency and the modamps on the truth."""
+    p, tab, raw, truth = _bessel3d(photons=None)
+    vol, fits = ob.reconstruct(p, tab, raw.astype(np.float32), 64)
+    dk = 1.0 / (p.nx * p.dxy)
+    for d, f in enumerate(fits):
+        err = math.hypot(f.k0x - truth["k0"][d][0], f.k0y - truth["k0"][d][1]) / dk
+        # an order-1 fit has half the order-2 fit's lever: 0.05 px (halving k0 would miss by ~6 px)
+        assert err < 0.05, (d, err)
+        for o, m in enumerate(truth["modamps"], start=1):
+            assert abs(abs(complex(f.amp_re[o], f.amp_im[o])) - m) < 0.03 * max(1, m / 0.2), (d, o)
+    assert np.isfinite(vol).all()
+
+
+def test_bessel_refuses_a_pattern_beyond_the_excitation_kz_extent():
+    """cos(halfangle) = |k0| order / (norders-1) / kzExMax above 1 has no angle
+    (the reference takes acosf of it and truncates a NaN): a clear error."""
+    p, tab, raw, _ = _bessel3d(photons=None)
+    q = p.copy()
+    q.BesselNA = 0.3  # kzExMax 1.23 < |k0| 2.457
+    with pytest.raises(Exception, match="Bessel"):
+        ob.reconstruct(q, tab, raw.astype(np.float32), 32)
+
+
+def test_bessel_filter_axial_cutoffs_at_known_half_angles():
+    """The filter's Bessel axial cutoffs (gpuFunctionsImpl.cu:1260-1271) where the
+    half angle is exact: with besselExWave 2 and besselNA = |k0|, kzExMax = |k0|,
+    so order 2 of 3 sits at cos(halfangle) = 1 and keeps no excitation kz extent
+    (its cutoff is the plain path's detection-only zdistcutoff[0]), order 1 sits
+    at 60 degrees (sin = sqrt(3)/2), and order 0 spans kzExMax plus the detection
+    extent, rounded to nearest."""
+    p, tab, _, _ = synthetic(64, 64, 32, nphases=5, dz=0.1, photons=None)
+    k = float(np.float32(2.4))
+
+    def cutoffs(q):
+        o = ob.Oracle(q, tab, 32)
+        for d in range(q.ndirs):
+            f = o.fit(d)
+            f.k0x, f.k0y = k, 0.0
+            for i in range(3):
+                f.amp_re[i], f.amp_im[i] = 1.0, 0.0
+            o.set_fit(d, f)
+        o.filter_dir(0)
+        return o.zdist()
+
+    plain = cutoffs(p)
+    q = p.copy()
+    q.bBessel, q.BesselNA, q.BesselLambdaEx = 1, k, 2.0
+    bes = cutoffs(q)
+    lam = 515 / 1.515 / 1000.0
+    det = (1 - math.cos(math.asin(1.42 / 1.515))) / lam  # detection OTF's axial extent, 1/um
+    dkz = 1 / (32 * 0.1)
+    assert bes[2] == plain[0] == math.ceil(det / dkz)
+    assert bes[1] == math.ceil((k * math.sqrt(3) / 2 + det) / dkz)
+    assert bes[0] == round((k + det) / dkz)
+    assert len(set(bes)) == 3  # three distinct cutoffs: an order mix-up shows
+
+
+# ------------------------------------------------------------------ deskew / cropXY resampling
+def _loaded(q, tab, raw):
+    """Flat-fielded, resampled sections the oracle loads: [dir][phase] -> (nz, ny, nx)."""
+    o = ob.Oracle(q, tab, 64)
+    o.load(raw)
+    secs = [[o.band(d, ph, double=True).view(np.float64).reshape(q.nz, q.ny, -1)[:, :, :q.nx]
+             for ph in range(q.nphases)] for d in range(q.ndirs)]
+    return secs, o.inscale()
+
+
+def _flat(r, bg, inscale):
+    # load_and_flatfield in float: ((raw - bg) - 0) * slope * inscale, slope 1
+    return ((r.astype(np.float32) - np.float32(bg)) - np.float32(0)) * np.float32(1) * np.float32(inscale)
+
+
+def test_crop_keeps_the_left_columns():
+    """cropXY without deskew (deskewOneSection with factor 0, cudaSirecon.cpp:1743-1745):
+    output column x is raw column x of the flat-fielded section."""
+    p, tab, _, _ = synthetic(64, 64, 8, nphases=5, photons=None)
+    q = p.copy()
+    q.nx_raw = 96
+    rng = np.random.default_rng(3)
+    raw = rng.integers(80, 4000, size=(3 * 5 * 8, 64, 96)).astype(np.float32)
+    secs, inscale = _loaded(q, tab, raw)
+    for d in range(3):
+        for z in range(8):
+            for ph in range(5):
+                exp = _flat(raw[(d * 8 + z) * 5 + ph], p.constbkgd, inscale)[:, :64]
+                np.testing.assert_array_equal(secs[d][ph][z], exp.astype(np.float64))
+
+
+def test_deskew_by_whole_pixels_is_an_exact_shift():
+    """deskew factor 1 (cudaSirecon.cpp:1746-1757): section z is read at
+    x - nx/2 + shift - (z - nz/2) + nx_raw/2, an integer, so the interpolation
+    returns the raw value there exactly, and 0 outside the section."""
+    p, tab, _, _ = synthetic(64, 64, 8, nphases=5, photons=None)
+    q = p.copy()
+    q.nx_raw, q.deskew_factor, q.deskew_shift = 64, 1.0, 3
+    rng = np.random.default_rng(4)
+    raw = rng.integers(80, 4000, size=(3 * 5 * 8, 64, 64)).astype(np.float32)
+    secs, inscale = _loaded(q, tab, raw)
+    for d in range(3):
+        for z in range(8):
+            for ph in range(5):
+                f = _flat(raw[(d * 8 + z) * 5 + ph], p.constbkgd, inscale)
+                exp = np.zeros((64, 64), np.float32)
+                for x in range(64):
+                    xin = x - 32 + 3 - (z - 4) + 32
+                    if 0 <= xin < 64:
+                        exp[:, x] = f[:, xin]
+                np.testing.assert_array_equal(secs[d][ph][z], exp.astype(np.float64))
+
+
+def test_deskew_interpolation_reproduces_a_linear_ramp():
+    """deskew factor 0.5 puts odd sections at half pixels; linear interpolation of
+    a ramp along x returns the ramp at the fractional position (away from the
+    last raw column, whose right neighbour is the next row's first pixel)."""
+    p, tab, _, _ = synthetic(64, 64, 8, nphases=5, photons=None)
+    q = p.copy()
+    q.nx_raw, q.deskew_factor, q.deskew_shift = 64, 0.5, 0
+    xs = np.arange(64, dtype=np.float64)
+    row = 200.0 + 30.0 * xs
+    raw = np.broadcast_to(row.astype(np.float32), (3 * 5 * 8, 64, 64)).copy()
+    secs, inscale = _loaded(q, tab, raw)
+    for z in range(8):
+        xin = xs - 0.5 * (z - 4)
+        ok = (xin >= 0) & (xin < 63)
+        exp = (200.0 + 30.0 * xin - p.constbkgd) * inscale
+        got = secs[1][2][z]
+        assert np.allclose(got[:, ok], exp[ok], rtol=2e-6, atol=0), z
+        assert (got[:, xin < 0] == 0).all()
