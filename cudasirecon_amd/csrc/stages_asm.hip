@@ -127,6 +127,135 @@ __global__ __launch_bounds__(256) void k_asm_z(AsmParams p, const float2* __rest
   }
 }
 
+// ---- z pass, Hermitian-paired (zdim == nz, even nx and ny).
+// An output column with x < 0 holds conj(B(-x, -y, -z)), and with zdim == nz
+// the z map is the identity mod nz, so its inverse transform is the conjugate
+// of the stored column's: one transform of stored column (x, s) gives output
+// (x, s) and, conjugated, output (-x, -s).  A workgroup therefore reads the
+// stored columns x in [x0, x0+W) of stored rows s and -s once (the other
+// kernel reads every mirrored element a second time) and writes the four
+// output columns they feed.  Side band a: value re + i im, so re and im are
+// transformed separately: direct R + iI, mirror conj(R) + i conj(I).
+// Band 0 (sum over directions): direct B, mirror conj(B).
+template <int N, bool B0>
+constexpr int asm_zp_slots() {
+  return B0 ? 2 : 4;  // transforms per stored column: rows s, -s (x re, im)
+}
+template <int N, bool B0>
+constexpr int asm_zp_w() {
+  return 256 / (asm_zp_slots<N, B0>() * fft_tpt(N));
+}
+template <int N>
+constexpr bool asm_zp_ok() {
+  return N > 1 && 4 * fft_tpt(N) <= 256;
+}
+
+template <int N, bool B0>
+__global__ __launch_bounds__(256) void k_asm_z_pair(AsmParams p, const float2* __restrict__ tw) {
+  constexpr int TPT = fft_tpt(N);
+  constexpr int S = asm_zp_slots<N, B0>();
+  constexpr int W = asm_zp_w<N, B0>();
+  constexpr int NT = S * W * TPT;
+  static_assert(NT == 256, "one transform slot per TPT threads");
+  constexpr int LP = lds_pitch(N);
+  __shared__ float2 lds[S * W * LP];
+  FftTw<N> twr;
+  __shared__ float2 twstage[N > 1 ? N : 1];
+  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
+  const int a = B0 ? 0 : 1 + blockIdx.z;
+  const int s = blockIdx.y, sp = s == 0 ? 0 : p.ny - s;  // stored rows s and -s
+  const int x0 = blockIdx.x * W;
+  const int xmax = p.nx / 2;  // stored columns 0 .. nx/2 feed the output
+  const long long dirstride = (long long)p.nbands * p.band_stride;
+  const float2* b0;
+  const float2* b1;
+  if (B0) {
+    b0 = p.bands;
+    b1 = nullptr;
+  } else {
+    const int no = p.norders - 1;
+    const int dir = (a - 1) / no, order = 1 + (a - 1) % no;
+    b0 = p.bands + dir * dirstride + (2 * order - 1) * p.band_stride;
+    b1 = b0 + p.band_stride;
+  }
+  // element e = (j * N + z) * W + w: slot j (B0: row s, row -s; else re s,
+  // im s, re -s, im -s), plane z, column x0 + w (clamped: the columns past
+  // nx/2 are transformed but never written)
+  constexpr int E = S * N * W;
+  constexpr int B = (E + NT - 1) / NT;
+  long long off[B];
+  int slot[B];
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const int e = threadIdx.x + k * NT < E ? threadIdx.x + k * NT : E - 1;  // past E: a valid address, not stored
+    const int w = e % W, jz = e / W;
+    const int j = jz / N, z = jz - j * N;
+    const int row = (B0 ? j : j >> 1) == 0 ? s : sp;
+    const int x = x0 + w <= xmax ? x0 + w : xmax;
+    off[k] = ((long long)z * p.ny + row) * p.nxh + x;
+    slot[k] = (j * W + w) * LP + lds_swz<N>(z);
+  }
+  float2 v[B];
+  if (B0) {
+#pragma unroll
+    for (int k = 0; k < B; ++k) v[k] = make_float2(0.f, 0.f);
+    for (int d = 0; d < p.ndirs; ++d) {
+      const float2* bd = b0 + d * dirstride;
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const float2 t = bd[off[k]];
+        v[k].x += t.x;
+        v[k].y += t.y;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int e = threadIdx.x + k * NT < E ? threadIdx.x + k * NT : E - 1;
+      const int j = (e / W) / N;
+      v[k] = ((j & 1) ? b1 : b0)[off[k]];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < B; ++k)
+    if (threadIdx.x + k * NT < E) lds[slot[k]] = v[k];
+  __syncthreads();
+  fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
+  const int ycs = s + p.ny / 2 - 1;                       // output row of centred y = s
+  const int ycp = (s == 0 ? 0 : -s) + p.ny / 2 - 1;       // output row of centred y = -s
+  const int ycp2 = s == p.ny / 2 ? ycs : ycp;             // s = ny/2 is its own mirror
+  for (int idx = threadIdx.x; idx < W * N; idx += NT) {
+    const int w = idx % W, zout = idx / W;
+    const int xs = x0 + w;
+    if (xs > xmax) continue;
+    const int zp = lds_swz<N>(zout);
+    float2 ds, dp, ms, mp;
+    if (B0) {
+      const float2 bs = lds[w * LP + zp], bp = lds[(W + w) * LP + zp];
+      ds = bs;
+      dp = bp;
+      ms = make_float2(bs.x, -bs.y);
+      mp = make_float2(bp.x, -bp.y);
+    } else {
+      const float2 rs = lds[w * LP + zp], is = lds[(W + w) * LP + zp];
+      const float2 rp = lds[(2 * W + w) * LP + zp], ip = lds[(3 * W + w) * LP + zp];
+      ds = make_float2(rs.x - is.y, rs.y + is.x);
+      dp = make_float2(rp.x - ip.y, rp.y + ip.x);
+      ms = make_float2(rs.x + is.y, is.x - rs.y);
+      mp = make_float2(rp.x + ip.y, ip.x - rp.y);
+    }
+    float2* plane = p.stageZ + ((long long)a * N + zout) * p.ny * p.nx;
+    const int xd = xs + p.nx / 2 - 1;
+    plane[(long long)ycs * p.nx + xd] = ds;
+    plane[(long long)ycp2 * p.nx + xd] = dp;
+    if (xs >= 1 && xs <= xmax - 1) {
+      const int xm = p.nx / 2 - 1 - xs;
+      plane[(long long)ycp2 * p.nx + xm] = ms;  // mirror of row s lands in row -s
+      plane[(long long)ycs * p.nx + xm] = mp;   // and that of row -s in row s
+    }
+  }
+}
+
 // ---- y pass: W adjacent xc columns of plane zout (blockIdx.y) of band a (blockIdx.z)
 // Lengths with a wave-local two-pass plan use that core (natural-order LDS
 // columns, register twiddles) with 16 columns per workgroup, so every global
@@ -473,10 +602,30 @@ __global__ __launch_bounds__(asm_x_threads<N>(), 2) void k_asm_x(AsmParams p,
   }
 }
 
+template <int N>
+static bool launch_asm_z_pair(const AsmParams& p, hipStream_t s) {
+  if constexpr (asm_zp_ok<N>()) {
+    if (p.zdim != p.nz || p.nz == 1 || (p.nx & 1) || (p.ny & 1)) return false;
+    const int nstored = p.nx / 2 + 1;
+    hipLaunchKernelGGL((k_asm_z_pair<N, true>), dim3(ceil_div(nstored, asm_zp_w<N, true>()), p.ny / 2 + 1, 1),
+                       dim3(256), 0, s, p, twiddles(N));
+    SIMR_LAUNCH_CHECK();
+    if (p.nasm > 1) {
+      hipLaunchKernelGGL((k_asm_z_pair<N, false>),
+                         dim3(ceil_div(nstored, asm_zp_w<N, false>()), p.ny / 2 + 1, p.nasm - 1), dim3(256), 0, s, p,
+                         twiddles(N));
+      SIMR_LAUNCH_CHECK();
+    }
+    return true;
+  }
+  return false;
+}
+
 void launch_asm_z(const AsmParams& p, hipStream_t s) {
   switch (p.zdim) {
 #define SIMR_CASE(N)                                                                              \
   case N: {                                                                                       \
+    if (launch_asm_z_pair<N>(p, s)) return;                                                       \
     constexpr int W = (256 / fft_tpt(N)) > 0 ? 256 / fft_tpt(N) : 1;                              \
     hipLaunchKernelGGL(k_asm_z<N>, dim3(ceil_div(p.nx, W), p.ny, p.nasm), dim3(256), 0, s, p,     \
                        twiddles(N));                                                              \
