@@ -1,9 +1,11 @@
 // Host orchestration of one reconstruction context.  See engine.hpp.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <cstring>
 #include <exception>
 #include <thread>
-#include <cstring>
 
 #include "engine.hpp"
 #include "fft_api.hpp"
@@ -119,7 +121,7 @@ static size_t fit_set_bytes(const Layout& L, int ncand) {
   b += 4 * align256(4096) + align256((size_t)ncand * 8);  // peak search, candidates
   b += align256((size_t)ncand * modamp_partials_fp64(L.nx, L.ny) * 16);
   b += align256((size_t)ncand * modamp_partials_fp32(L.nx, L.ny) * 16);
-  b += align256((size_t)2 * 128 * 8);
+  b += align256((size_t)2 * kZsumMaxBlocks * 8);
   return b;
 }
 
@@ -212,7 +214,7 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
 
   hstage_bytes_ = std::max<size_t>((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16,
                                    (size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16) +
-                  8192;
+                  (size_t)2 * kZsumMaxBlocks * 8 + 8192;
   fw_.resize(L.ndirs);
   for (int d = 0; d < L.ndirs; ++d) {
     FitWork& w = fw_[d];
@@ -226,6 +228,7 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
       w.own_stream = true;
     }
     SIMR_HIP(hipHostMalloc(&w.hstage, hstage_bytes_, hipHostMallocDefault));
+    SIMR_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
   }
 
   // OTF tables
@@ -306,16 +309,18 @@ void Engine::update_params(const simr_params& p) {
 }
 
 Engine::~Engine() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& e : events_) {
     (void)hipEventDestroy(e.second.first);
     (void)hipEventDestroy(e.second.second);
   }
-  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
   for (auto& w : fw_) {
     if (w.own_stream) {
       (void)hipStreamSynchronize(w.stream);
       (void)hipStreamDestroy(w.stream);
     }
+    if (w.done) (void)hipEventDestroy(w.done);
     if (w.hstage) (void)hipHostFree(w.hstage);
   }
   arena_.release();
@@ -324,17 +329,57 @@ Engine::~Engine() {
 
 void Engine::sync() { SIMR_HIP(hipStreamSynchronize(stream_)); }
 
+// Waits for the work queued on a fit stream.  The fit's read-backs are small
+// and on its critical path, so the event is polled: the wait ends within
+// microseconds of the copy, where a blocking synchronize adds tens of them.
+// After 2 ms of polling (a busy device, e.g. other contexts' work) it falls
+// back to the blocking wait.
+void Engine::wait_fit(FitWork& w) {
+  SIMR_HIP(hipEventRecord(w.done, w.stream));
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0;; ++i) {
+    const hipError_t e = hipEventQuery(w.done);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) SIMR_HIP(e);
+    if ((i & 15) == 15 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+    for (int k = 0; k < 32; ++k) __builtin_ia32_pause();  // a few microseconds between queries
+  }
+  SIMR_HIP(hipEventSynchronize(w.done));
+}
+
+void Engine::enable_timing(bool on) {
+  timing_ = on;
+  if (!on) return;
+  // the timed region takes its events from here instead of creating them
+  std::lock_guard<std::mutex> lk(ev_mu_);
+  while (ev_pool_.size() < 2048) {
+    hipEvent_t e;
+    SIMR_HIP(hipEventCreate(&e));
+    ev_pool_.push_back(e);
+  }
+}
+
 // Stage timing: a pair of events around the stage's launches on its stream
 // (the fit's directions record on their own streams, from their threads).
 int Engine::stage_begin(const char* name, hipStream_t s) {
   if (!timing_) return -1;
   hipEvent_t a, b;
-  SIMR_HIP(hipEventCreate(&a));
-  SIMR_HIP(hipEventCreate(&b));
+  int id;
+  {
+    std::lock_guard<std::mutex> lk(ev_mu_);
+    for (hipEvent_t* e : {&a, &b}) {
+      if (ev_pool_.empty()) {
+        SIMR_HIP(hipEventCreate(e));
+      } else {
+        *e = ev_pool_.back();
+        ev_pool_.pop_back();
+      }
+    }
+    events_.push_back({name, {a, b}});
+    id = (int)events_.size() - 1;
+  }
   SIMR_HIP(hipEventRecord(a, s));
-  std::lock_guard<std::mutex> lk(ev_mu_);
-  events_.push_back({name, {a, b}});
-  return (int)events_.size() - 1;
+  return id;
 }
 void Engine::stage_end(int id, hipStream_t s) {
   if (id < 0) return;
@@ -366,7 +411,7 @@ void Engine::carve_fit(FitWork& w, char*& at, bool count_only, size_t& bytes) co
   w.kcand = (float2*)take((size_t)kMaxCand * 8);
   w.part64 = (double2*)take((size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16);
   w.part32 = (float4*)take((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
-  w.sumparts = (double*)take((size_t)2 * 128 * 8);
+  w.sumparts = (double*)take((size_t)2 * kZsumMaxBlocks * 8);
 }
 
 int Engine::timings(const char** names, float* ms, int cap) {
@@ -375,8 +420,8 @@ int Engine::timings(const char** names, float* ms, int cap) {
     float t = 0;
     SIMR_HIP(hipEventElapsedTime(&t, e.second.first, e.second.second));
     last_timings_.push_back({e.first, t});
-    (void)hipEventDestroy(e.second.first);
-    (void)hipEventDestroy(e.second.second);
+    ev_pool_.push_back(e.second.first);
+    ev_pool_.push_back(e.second.second);
   }
   events_.clear();
   static thread_local std::vector<std::string> keep;
@@ -506,7 +551,7 @@ OtfRA Engine::otf_table(int dir, int order) const {
 }
 
 // ------------------------------------------------------------ overlaps / fit
-void Engine::make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float k0y) {
+void Engine::make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float k0y, bool per_elem) {
   const Geo g = geometry();
   OvParams p{};
   p.nx = L.nx;
@@ -546,8 +591,7 @@ void Engine::make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float
   p.inv_dkr2_2 = (float)(1.0 / ((double)p.otf2.dkrotf * (double)p.otf2.dkrotf));
   const int st = stage_begin("overlaps", w.stream);
   launch_overlaps_xy(p, w.ovx, w.stream);
-  launch_overlaps_z_reduce(p, w.ovx, w.P, w.SX, w.SY, fit_fp32_, w.stream);
-  if (!fit_fp32_) launch_sum2(w.SX, w.SY, L.nxy, w.sumparts, w.stream);
+  launch_overlaps_z_reduce(p, w.ovx, w.P, w.SX, w.SY, w.sumparts, fit_fp32_, per_elem, w.stream);
   stage_end(st, w.stream);
   w.ov_totals_valid = false;
   w.cache.clear();
@@ -566,7 +610,7 @@ std::vector<Engine::Eval> Engine::eval_batch(FitWork& w, const std::vector<float
       launch_modamp_fp32(w.P, w.SX, w.SY, w.kcand, nc, L.nx, L.ny, P.dxy, w.part32, w.stream);
       stage_end(st, w.stream);
       SIMR_HIP(hipMemcpyAsync(w.hstage, w.part32, (size_t)nc * nb * 16, hipMemcpyDeviceToHost, w.stream));
-      SIMR_HIP(hipStreamSynchronize(w.stream));
+      wait_fit(w);
       const float4* h = (const float4*)w.hstage;
       for (int c = 0; c < nc; ++c) {
         float xr = 0, xi = 0, sx = 0, sy = 0;  // cpuReduce order (gpuFunctionsImpl_hh.cu:162-180)
@@ -586,14 +630,15 @@ std::vector<Engine::Eval> Engine::eval_batch(FitWork& w, const std::vector<float
       char* h = (char*)w.hstage;
       SIMR_HIP(hipMemcpyAsync(h, w.part64, (size_t)nc * nb * 16, hipMemcpyDeviceToHost, w.stream));
       const size_t soff = (size_t)kMaxCand * nb * 16;
+      const int zb = zsum_blocks(L.nxy);
       if (!w.ov_totals_valid)
-        SIMR_HIP(hipMemcpyAsync(h + soff, w.sumparts, 2 * 128 * 8, hipMemcpyDeviceToHost, w.stream));
-      SIMR_HIP(hipStreamSynchronize(w.stream));
+        SIMR_HIP(hipMemcpyAsync(h + soff, w.sumparts, (size_t)2 * zb * 8, hipMemcpyDeviceToHost, w.stream));
+      wait_fit(w);
       if (!w.ov_totals_valid) {
         const double* sp = (const double*)(h + soff);
         double sx = 0, sy = 0;
-        for (int b = 0; b < 128; ++b) sx += sp[b];
-        for (int b = 0; b < 128; ++b) sy += sp[128 + b];
+        for (int b = 0; b < zb; ++b) sx += sp[b];
+        for (int b = 0; b < zb; ++b) sy += sp[zb + b];
         w.ov_sx = sx;
         w.ov_sy = sy;
         w.ov_totals_valid = true;
@@ -698,7 +743,7 @@ void Engine::findk0(FitWork& w, int dir) {
   char* h = (char*)w.hstage;
   SIMR_HIP(hipMemcpyAsync(h, w.peakvals, 5 * 4, hipMemcpyDeviceToHost, w.stream));
   SIMR_HIP(hipMemcpyAsync(h + 64, w.peakidx, 4, hipMemcpyDeviceToHost, w.stream));
-  SIMR_HIP(hipStreamSynchronize(w.stream));
+  wait_fit(w);
   const float* v = (const float*)h;
   const int idx = *(const int*)(h + 64);
   const int xc = idx % L.nx, yc = idx / L.nx;
@@ -960,21 +1005,29 @@ void Engine::find_modulation() {
   }
   std::vector<std::exception_ptr> err(L.ndirs);
   std::vector<std::thread> th;
+  std::atomic<int> running(L.ndirs - 1);
   for (int dir = 1; dir < L.ndirs; ++dir)
-    th.emplace_back([this, dir, &err] {
+    th.emplace_back([this, dir, &err, &running] {
       try {
         SIMR_HIP(hipSetDevice(device_));
         modulation(fw_[dir], dir);
-        SIMR_HIP(hipStreamSynchronize(fw_[dir].stream));
+        wait_fit(fw_[dir]);
       } catch (...) {
         err[dir] = std::current_exception();
       }
+      running.fetch_sub(1, std::memory_order_release);
     });
   try {
     modulation(fw_[0], 0);
   } catch (...) {
     err[0] = std::current_exception();
   }
+  // the filter that follows needs every direction's fit: wait for the last
+  // one by polling (a blocking join wakes up tens of microseconds late)
+  const auto t0 = std::chrono::steady_clock::now();
+  while (running.load(std::memory_order_acquire) > 0 &&
+         std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(2))
+    for (int k = 0; k < 32; ++k) __builtin_ia32_pause();
   for (auto& t : th) t.join();
   for (auto& e : err)
     if (e) std::rethrow_exception(e);
@@ -1182,7 +1235,7 @@ void Engine::set_band(int dir, int band, const float* host) {
 
 void Engine::debug_overlaps(int dir, int o1, int o2, float k0x, float k0y, double* Ph, double* sx, double* sy) {
   FitWork& w = fw_[0];
-  make_overlaps(w, dir, o1, o2, k0x, k0y);
+  make_overlaps(w, dir, o1, o2, k0x, k0y, true);
   SIMR_HIP(hipMemcpyAsync(Ph, w.P, L.nxy * 16, hipMemcpyDeviceToHost, stream_));
   SIMR_HIP(hipMemcpyAsync(sx, w.SX, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
   SIMR_HIP(hipMemcpyAsync(sy, w.SY, L.nxy * 8, hipMemcpyDeviceToHost, stream_));

@@ -71,7 +71,7 @@ class Engine {
   void sync();
   hipStream_t stream() const { return stream_; }
   const simr_params& params() const { return P; }
-  void enable_timing(bool on) { timing_ = on; }
+  void enable_timing(bool on);
   int timings(const char** names, float* ms, int cap);
 
  private:
@@ -101,6 +101,7 @@ class Engine {
     float4* part32 = nullptr;
     double* sumparts = nullptr;
     void* hstage = nullptr;  // pinned host staging
+    hipEvent_t done = nullptr;  // read-back completion (polled by wait_fit)
     bool ov_totals_valid = false;
     double ov_sx = 0, ov_sy = 0;
     std::map<std::pair<uint32_t, uint32_t>, Eval> cache;
@@ -113,7 +114,8 @@ class Engine {
   int fit_order2() const;
   OtfRA otf_table(int dir, int order) const;
   void carve_fit(FitWork& w, char*& at, bool count_only, size_t& bytes) const;
-  void make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float k0y);
+  void wait_fit(FitWork& w);
+  void make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float k0y, bool per_elem = false);
   std::vector<Eval> eval_batch(FitWork& w, const std::vector<float2>& kvecs);
   void modamp_of(const Eval& e, float2* modamp, double* amp2, float* corr) const;
   double getmodamp(FitWork& w, int dir, float angle, float mag, int o1, int o2, float2* modamp, float* corr);
@@ -164,6 +166,7 @@ class Engine {
   int cur_stage_ = -1;  // open stage on the context stream
   std::mutex ev_mu_;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> events_;
+  std::vector<hipEvent_t> ev_pool_;  // timing events not in use (created once, reused)
   std::vector<std::pair<std::string, float>> last_timings_;
 };
 

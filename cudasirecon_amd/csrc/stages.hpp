@@ -62,11 +62,16 @@ struct OvParams {
 };
 // ovx: [2][2zd+1][ny][nx]; builds both overlaps and inverse-transforms along x and y.
 void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s);
-// Inverse z transform of the overlap pair and the per-(x,y) z sums
-// P = sum_z conj(X) Y, SX = sum |X|^2, SY = sum |Y|^2 (double storage;
-// accumulated in float when fp32 is set, as the reference's reductionKernel threads).
-void launch_overlaps_z_reduce(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, bool fp32,
-                              hipStream_t s);
+// The per-(x,y) z sums of the inverse-transformed overlap pair,
+// P = sum_z conj(X) Y, SX = sum |X|^2, SY = sum |Y|^2 (double storage).
+// fp32: accumulated in float as the reference's reductionKernel threads, P,
+// SX, SY stored per (x, y).  Otherwise in double, P per (x, y) and the totals
+// of SX and SY as zsum_blocks(nx*ny) partials each: totals[b], totals[nb + b]
+// (per_elem: SX and SY per (x, y) as well).
+constexpr int kZsumMaxBlocks = 2048;
+int zsum_blocks(long long nxy);
+void launch_overlaps_z_reduce(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, double* totals,
+                              bool fp32, bool per_elem, hipStream_t s);
 // crosscorr = conj(P) as float, then its forward 2D FFT and |.|^2 argmax.
 // peak_out (host-visible after sync): [0] = index of the first maximum,
 // values: int peak index and the 5 intensities (center, x-1, x+1, y-1, y+1).
@@ -82,9 +87,6 @@ void launch_modamp_fp64(const double2* P, const float2* kcand, int ncand, int nx
                         double2* partials, hipStream_t s);
 void launch_modamp_fp32(const double2* P, const double* SX, const double* SY, const float2* kcand, int ncand,
                         int nx, int ny, float dxy, float4* partials, hipStream_t s);
-// Totals of SX and SY in double: partials [2][nparts].
-int sum_partials_count(long long n);
-void launch_sum2(const double* a, const double* b, long long n, double* partials, hipStream_t s);
 
 // ---- Wiener filter ------------------------------------------------------------
 struct FilterParams {

@@ -224,63 +224,90 @@ void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s) {
 // 391-413).  For the unnormalised inverse DFT along z, Parseval gives
 // sum_z conj(x_z) y_z = nz sum_kz conj(X_kz) Y_kz, and only the 2zd+1 stored
 // kz planes are nonzero, so the sums are formed from those planes directly
-// and the z transform is not needed.  One thread per (x, y); loads of four
-// planes in flight at once.  Sums in double (FP64 fit) or float.
-template <bool FP32>
+// and the z transform is not needed.  Loads of eight planes of both overlaps
+// are in flight at once.
+// FP32: float sums, stored per (x, y) in P, SX, SY (k_modamp_fp32 reduces
+// them in the reference's blocks).  FP64: double sums; P per (x, y), and of
+// SX and SY only what the fit reads, their totals, as one partial per block
+// (totals[block], totals[nblocks + block]); ELEM also stores them per (x, y).
+int zsum_blocks(long long nxy) { return (int)std::min<long long>(ceil_div(nxy, 256), kZsumMaxBlocks); }
+
+template <bool FP32, bool ELEM>
 __global__ __launch_bounds__(256) void k_overlaps_zsum(const float2* __restrict__ ovx, int nzs, int nz, long long nxy,
                                                        double2* __restrict__ P, double* __restrict__ SX,
-                                                       double* __restrict__ SY) {
+                                                       double* __restrict__ SY, double* __restrict__ totals) {
 #pragma clang fp contract(off)
-  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (e >= nxy) return;
   using A = typename std::conditional<FP32, float, double>::type;
-  A sxr = 0, sxi = 0, sx = 0, sy = 0;
-  const float2* X = ovx + e;
-  const float2* Y = ovx + (long long)nzs * nxy + e;
-  auto acc = [&](float2 xa, float2 yb) {
-    const A ar = xa.x, ai = xa.y, br = yb.x, bi = yb.y;
-    if constexpr (FP32) {
-      sxr += ar * br + ai * bi;
-      sxi += ar * bi - ai * br;
-      sx += ar * ar + ai * ai;
-      sy += br * br + bi * bi;
-    } else {
-      // float x float products are exact in double, so a product plus a fused
-      // multiply-add gives the correctly rounded a*b + c*d of the plain form
-      sxr += __builtin_fma(ar, br, ai * bi);
-      sxi += __builtin_fma(ar, bi, -(ai * br));
-      sx += __builtin_fma(ar, ar, ai * ai);
-      sy += __builtin_fma(br, br, bi * bi);
-    }
-  };
-  int s = 0;
-  for (; s + 4 <= nzs; s += 4) {
-    float2 xa[4], yb[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      xa[k] = X[(long long)(s + k) * nxy];
-      yb[k] = Y[(long long)(s + k) * nxy];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) acc(xa[k], yb[k]);
-  }
-  for (; s < nzs; ++s) acc(X[(long long)s * nxy], Y[(long long)s * nxy]);
+  constexpr int KB = 8;
   const A n = (A)nz;
-  P[e] = make_double2((double)(sxr * n), (double)(sxi * n));
-  SX[e] = (double)(sx * n);
-  SY[e] = (double)(sy * n);
+  double tsx = 0.0, tsy = 0.0;
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < nxy; e += (long long)gridDim.x * 256) {
+    A sxr = 0, sxi = 0, sx = 0, sy = 0;
+    const float2* X = ovx + e;
+    const float2* Y = ovx + (long long)nzs * nxy + e;
+    for (int s = 0; s < nzs; s += KB) {
+      const int cnt = nzs - s < KB ? nzs - s : KB;  // uniform
+      float2 xa[KB], yb[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        const long long o = (long long)(k < cnt ? s + k : s) * nxy;
+        xa[k] = X[o];
+        yb[k] = Y[o];
+      }
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        if (k >= cnt) break;
+        const A ar = xa[k].x, ai = xa[k].y, br = yb[k].x, bi = yb[k].y;
+        if constexpr (FP32) {
+          sxr += ar * br + ai * bi;
+          sxi += ar * bi - ai * br;
+          sx += ar * ar + ai * ai;
+          sy += br * br + bi * bi;
+        } else {
+          // float x float products are exact in double, so a product plus a
+          // fused multiply-add gives the correctly rounded a*b + c*d of the
+          // plain form
+          sxr += __builtin_fma(ar, br, ai * bi);
+          sxi += __builtin_fma(ar, bi, -(ai * br));
+          sx += __builtin_fma(ar, ar, ai * ai);
+          sy += __builtin_fma(br, br, bi * bi);
+        }
+      }
+    }
+    P[e] = make_double2((double)(sxr * n), (double)(sxi * n));
+    if constexpr (FP32 || ELEM) {
+      SX[e] = (double)(sx * n);
+      SY[e] = (double)(sy * n);
+    }
+    if constexpr (!FP32) {
+      tsx += sx * n;
+      tsy += sy * n;
+    }
+  }
+  if constexpr (!FP32) {
+    __shared__ double red[256];
+    const double a = block_sum_d<256>(tsx, red);
+    const double b = block_sum_d<256>(tsy, red);
+    if (threadIdx.x == 0) {
+      totals[blockIdx.x] = a;
+      totals[gridDim.x + blockIdx.x] = b;
+    }
+  }
 }
 
-void launch_overlaps_z_reduce(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, bool fp32,
-                              hipStream_t s) {
+void launch_overlaps_z_reduce(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, double* totals,
+                              bool fp32, bool per_elem, hipStream_t s) {
   if (2 * p.zd + 1 > p.nz) throw ArgError("overlap z support wider than the stack");
   const long long nxy = (long long)p.nx * p.ny;
   const int nzs = 2 * p.zd + 1;
-  const dim3 grid(ceil_div(nxy, 256));
+  const dim3 grid(zsum_blocks(nxy));
   if (fp32)
-    hipLaunchKernelGGL(k_overlaps_zsum<true>, grid, dim3(256), 0, s, ovx, nzs, p.nz, nxy, P, SX, SY);
+    hipLaunchKernelGGL((k_overlaps_zsum<true, true>), grid, dim3(256), 0, s, ovx, nzs, p.nz, nxy, P, SX, SY, totals);
+  else if (per_elem)
+    hipLaunchKernelGGL((k_overlaps_zsum<false, true>), grid, dim3(256), 0, s, ovx, nzs, p.nz, nxy, P, SX, SY, totals);
   else
-    hipLaunchKernelGGL(k_overlaps_zsum<false>, grid, dim3(256), 0, s, ovx, nzs, p.nz, nxy, P, SX, SY);
+    hipLaunchKernelGGL((k_overlaps_zsum<false, false>), grid, dim3(256), 0, s, ovx, nzs, p.nz, nxy, P, SX, SY,
+                       totals);
   SIMR_LAUNCH_CHECK();
 }
 
@@ -495,33 +522,6 @@ void launch_modamp_fp32(const double2* P, const double* SX, const double* SY, co
                         int nx, int ny, float dxy, float4* partials, hipStream_t s) {
   dim3 grid((nx + 63) / 64, (ny + 3) / 4, ncand);
   hipLaunchKernelGGL(k_modamp_fp32, grid, dim3(64, 4), 0, s, P, SX, SY, kcand, nx, ny, dxy, partials);
-  SIMR_LAUNCH_CHECK();
-}
-
-// ------------------------------------------------------------ sums of SX, SY
-constexpr int kSumBlocks = 128;
-int sum_partials_count(long long) { return kSumBlocks; }
-
-__global__ __launch_bounds__(256) void k_sum2(const double* __restrict__ a, const double* __restrict__ b, long long n,
-                                              double* __restrict__ partials) {
-  __shared__ double red[256];
-  const long long chunk = (n + gridDim.x - 1) / gridDim.x;
-  const long long b0 = blockIdx.x * chunk, b1 = min(n, b0 + chunk);
-  double sa = 0.0, sb = 0.0;
-  for (long long e = b0 + threadIdx.x; e < b1; e += 256) {
-    sa += a[e];
-    sb += b[e];
-  }
-  double ta = block_sum_d<256>(sa, red);
-  double tb = block_sum_d<256>(sb, red);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = ta;
-    partials[gridDim.x + blockIdx.x] = tb;
-  }
-}
-
-void launch_sum2(const double* a, const double* b, long long n, double* partials, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum2, dim3(kSumBlocks), dim3(256), 0, s, a, b, n, partials);
   SIMR_LAUNCH_CHECK();
 }
 
