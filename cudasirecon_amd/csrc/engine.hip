@@ -1116,6 +1116,12 @@ FilterParams Engine::filter_params(int dir, int order) const {
   f.inv_dkr2 = (float)(1.0 / ((double)f.otf[0].dkrotf * (double)f.otf[0].dkrotf));
   f.inv_rcut2 = (float)(1.0 / ((double)rdistcutoff * (double)rdistcutoff));
   f.supp2 = (float)((double)f.suppRadius * (double)f.suppRadius);
+  f.inv_min_dkr2 = (float)(1.0 / ((double)f.min_dkr * (double)f.min_dkr));
+  f.inv_zd0 = 1.0f / (float)f.zd[0];  // inf for zd[0] == 0: 0 * inf is the reference's 0 / 0
+  f.inv_rcut = 1.0f / f.rdistcutoff;
+  f.inv_apo = 1.0f / f.apocutoff;
+  f.inv_apo2 = (float)(1.0 / ((double)f.apocutoff * (double)f.apocutoff));
+  f.inv_zapo = f.zapocutoff > 0 ? 1.0f / f.zapocutoff : 0.f;
   f.inv_nxh = 1.0f / (float)L.nxh;
   SIMR_REQUIRE((long long)L.nxh * L.ny < (1LL << 24), "filter: plane too large");
   float2* db = bands_ + (size_t)dir * L.nbands * L.H;

@@ -111,6 +111,7 @@ struct FilterParams {
   float inv_dkr2;   // 1 / dkrotf^2
   float inv_rcut2;  // 1 / rdistcutoff^2
   float supp2;      // suppRadius^2
+  float inv_min_dkr2, inv_zd0, inv_rcut, inv_apo, inv_apo2, inv_zapo;  // reciprocals of the divisors
   float inv_nxh;    // 1 / nxh
 };
 void launch_filter(const FilterParams& p, hipStream_t s);

@@ -48,13 +48,6 @@ __device__ __forceinline__ float2 prof_idx(const float2* prof, int nxotf, float 
   return make_float2((1 - ar) * t0.x + ar * t1.x, (1 - ar) * t0.y + ar * t1.y);
 }
 
-// OTF profile value at radial distance rdist (= sqrtf(kx*kx + ky*ky), the
-// operand dev_otfinterpolate recomputes), as dev_otfinterpolate: rdist/dkr.
-__device__ __forceinline__ float2 prof_at(const float2* prof, int nxotf, float dkr, float rdist) {
-#pragma clang fp contract(off)
-  return prof_idx(prof, nxotf, rdist / dkr);
-}
-
 // One visit of filterbands_kernel1 at band coordinates (x1, y1, z0) whose
 // rdist1 <= rdistcutoff; updates (re, im) of the stored element.
 // prof: this plane's radial OTF profiles at kz = z0, [order][nxotf].
@@ -88,9 +81,9 @@ __device__ __forceinline__ float filter_term(const FilterParams& p, const float2
   if (!UNV && !p.unit_nv) w = w / p.noiseVar[dir2 * p.norders + ao2];
   if (order2 != 0) w *= p.ampmag2_all[dir2 * p.norders + ao2];
   if (p.bSuppress && order2 != 0 && d2 <= p.supp2)
-    w *= dev_suppress(sqrtf(d2) / p.min_dkr);
+    w *= dev_suppress(__builtin_amdgcn_sqrtf(d2 * p.inv_min_dkr2));
   else if (!p.bDampenOrder0 && p.bSuppress && order2 == 0 && d2 <= p.supp2)
-    w *= dev_suppress(sqrtf(d2) / p.min_dkr);
+    w *= dev_suppress(__builtin_amdgcn_sqrtf(d2 * p.inv_min_dkr2));
   else if (p.bDampenOrder0 && order2 == 0)
     w *= d2 * p.inv_rcut2 + zf3;
   if (p.bNoKz0 && order2 == 0 && z0 == 0) w = 0.0f;
@@ -108,28 +101,28 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
   const float kx = (float)order * p.k0[p.dir].x;
   const float ky = (float)order * p.k0[p.dir].y;
   const float x1f = (float)x1 * p.dkx, y1f = (float)y1 * p.dky;
-  const float rdist1 = sqrtf(x1f * x1f + y1f * y1f);
+  // divisions by per-launch constants are products with their reciprocals and
+  // square roots are the hardware's (each within an ulp or two of the
+  // reference's correctly rounded forms, far inside the output tolerance)
+  const float d1 = x1f * x1f + y1f * y1f;
   const float xabs = x1f + kx, yabs = y1f + ky;
   const int nxotf = p.otf[0].nxotf;
-  const float dkr = p.otf[0].dkrotf;
   // dev_order0damping's z term: fabsf(zindex / zlimit) cubed
   float zf3 = 0.f;
   if (p.bDampenOrder0) {
-    const float zf = fabsf((float)z0 / (float)p.zd[0]);
+    const float zf = fabsf((float)z0 * p.inv_zd0);
     zf3 = zf * zf * zf;
   }
-  const float2 otf1 = prof_at(prof + order * nxotf, nxotf, dkr, rdist1);
+  const float2 otf1 = prof_idx(prof + order * nxotf, nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2));
   float weight = otf1.x * otf1.x + otf1.y * otf1.y;
   if (order != 0) weight *= p.ampmag2_all[p.dir * p.norders + order];
   float dampfact = p.inv_nv[p.dir * p.norders + order];
-  if (p.bSuppress && order != 0 && rdist1 <= p.suppRadius)
-    dampfact *= dev_suppress(rdist1 / p.min_dkr);
-  else if (!p.bDampenOrder0 && p.bSuppress && order == 0 && rdist1 <= p.suppRadius)
-    dampfact *= dev_suppress(rdist1 / p.min_dkr);
-  else if (p.bDampenOrder0 && order == 0) {
-    const float rf = rdist1 / p.rdistcutoff;
-    dampfact *= rf * rf + zf3;
-  }
+  if (p.bSuppress && order != 0 && d1 <= p.supp2)
+    dampfact *= dev_suppress(__builtin_amdgcn_sqrtf(d1 * p.inv_min_dkr2));
+  else if (!p.bDampenOrder0 && p.bSuppress && order == 0 && d1 <= p.supp2)
+    dampfact *= dev_suppress(__builtin_amdgcn_sqrtf(d1 * p.inv_min_dkr2));
+  else if (p.bDampenOrder0 && order == 0)
+    dampfact *= d1 * p.inv_rcut2 + zf3;
   if (order == 0 && z0 == 0 && p.bNoKz0) dampfact = 0.f;
   weight *= dampfact;
   float sumweight = weight;
@@ -145,20 +138,19 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
         sumweight += filter_term<false>(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
   }
   sumweight += p.wiener;
-  float2 scale = make_float2(dampfact * otf1.x / sumweight, dampfact * (-otf1.y) / sumweight);
+  const float rs = __builtin_amdgcn_rcpf(sumweight);
+  float2 scale = make_float2((dampfact * otf1.x) * rs, (dampfact * (-otf1.y)) * rs);
   if (p.apodizeoutput) {
     float rho;
-    const float rdistabs = sqrtf(xabs * xabs + yabs * yabs);
     const float zdistabs = (float)abs(z0);
-    const float ra = rdistabs / p.apocutoff;
     if (p.zapocutoff > 0 && p.bessel) {
-      const float rx = xabs / p.rdistcutoff, ry = yabs / p.apocutoff, rz = zdistabs / p.zapocutoff;
-      rho = sqrtf(rx * rx + ry * ry + rz * rz);
-    } else if (p.zapocutoff > 0) {
-      const float za = zdistabs / p.zapocutoff;
-      rho = sqrtf(ra * ra + za * za);
+      const float rx = xabs * p.inv_rcut, ry = yabs * p.inv_apo, rz = zdistabs * p.inv_zapo;
+      rho = __builtin_amdgcn_sqrtf(rx * rx + ry * ry + rz * rz);
     } else {
-      rho = sqrtf(ra * ra);
+      // (r / apocutoff)^2 + (z / zapocutoff)^2, the z term absent in 2D
+      const float ra2 = (xabs * xabs + yabs * yabs) * p.inv_apo2;
+      const float za = p.zapocutoff > 0 ? zdistabs * p.inv_zapo : 0.f;
+      rho = __builtin_amdgcn_sqrtf(ra2 + za * za);
     }
     if (rho > 1.f) rho = 1.0f;
     float apofact = 0.f;
