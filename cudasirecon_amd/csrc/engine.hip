@@ -1103,9 +1103,6 @@ FilterParams Engine::filter_params(int dir, int order) const {
   for (int o = 0; o < L.norders; ++o) f.otf[o] = otf_table(dir, o);
   f.unit_nv = 1;
   for (int d = 0; d < L.ndirs; ++d) {
-    for (int o2 = -(L.norders - 1); o2 < L.norders; ++o2)
-      f.kshift[d * (2 * kMaxOrders - 1) + o2 + (kMaxOrders - 1)] =
-          make_float2((float)o2 * k0_[d].x, (float)o2 * k0_[d].y);
     for (int o = 0; o < L.norders; ++o) {
       const float nv = f.noiseVar[d * L.norders + o];
       f.inv_nv[d * L.norders + o] = (float)(1. / (double)nv);

@@ -104,7 +104,6 @@ struct FilterParams {
   OtfRA otf[kMaxOrders];
   float2 *bp, *bp2;
   // host-derived constants (same float/double expressions as the per-thread ones)
-  float2 kshift[kMaxDirs * (2 * kMaxOrders - 1)];  // (float)order2 * k0[dir2], order2 = -(norders-1)..norders-1
   float inv_nv[kMaxDirs * kMaxOrders];              // (float)(1.0 / (double)noiseVar)
   int unit_nv;                                      // every noiseVar == 1: the divisions are exact no-ops
   float cut2;       // rdistcutoff^2

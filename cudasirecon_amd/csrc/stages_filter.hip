@@ -31,21 +31,19 @@ __device__ __forceinline__ float dev_order0damping(float radius, float zindex, f
   return rf * rf + zf * zf * zf;
 }
 
-// OTF profile value at radius index krindex, bilinear as profile_lookup.
-__device__ __forceinline__ float2 prof_idx(const float2* prof, int nxotf, float krindex) {
-#pragma clang fp contract(off)
-  int irindex = (int)floorf(krindex);
-  float ar = krindex - (float)irindex;
+// OTF profile value at radius index krindex, bilinear as profile_lookup.  The
+// index is a square root, so never negative: truncation is the floor (the
+// reference's clamp at 0 cannot act), and the interpolation is
+// t0 + ar (t1 - t0) in fused form (within an ulp of (1 - ar) t0 + ar t1).
+__device__ __forceinline__ float2 prof_pos(const float2* prof, int nxotf, float krindex) {
+  int irindex = (int)krindex;
+  float ar = __builtin_amdgcn_fractf(krindex);
   if (irindex > nxotf - 2) {
     irindex = nxotf - 2;
     ar = 1.f;
   }
-  if (irindex < 0) {
-    irindex = 0;
-    ar = 0.f;
-  }
   const float2 t0 = prof[irindex], t1 = prof[irindex + 1];
-  return make_float2((1 - ar) * t0.x + ar * t1.x, (1 - ar) * t0.y + ar * t1.y);
+  return make_float2(__builtin_fmaf(ar, t1.x - t0.x, t0.x), __builtin_fmaf(ar, t1.y - t0.y, t0.y));
 }
 
 // One visit of filterbands_kernel1 at band coordinates (x1, y1, z0) whose
@@ -64,20 +62,23 @@ template <bool UNV>
 __device__ __forceinline__ float filter_term(const FilterParams& p, const float2* prof, int nxotf, int order, int dir2,
                                              int order2, float xabs, float yabs, int z0, float zf3) {
 #pragma clang fp contract(off)
-  constexpr int nshift = 2 * kMaxOrders - 1;
   if (dir2 == p.dir && order2 == order) return 0.f;
   if (!p.bFilteroverlaps && !(order2 == 0 && order == 0)) return 0.f;
-  const float2 ks = p.kshift[dir2 * nshift + order2 + (kMaxOrders - 1)];
-  const float x2 = xabs - ks.x, y2 = yabs - ks.y;
-  const float d2 = x2 * x2 + y2 * y2;
+  // x2 = xabs - order2 k0x: order2 k0x is exact for |order2| <= 2, so the
+  // fused form rounds exactly as the reference's subtraction (k0 is read from
+  // the direction's two values rather than a per-term table, which keeps the
+  // unrolled loop's constants in scalar registers)
+  const float2 k0 = p.k0[dir2];
+  const float x2 = __builtin_fmaf(-(float)order2, k0.x, xabs), y2 = __builtin_fmaf(-(float)order2, k0.y, yabs);
+  const float d2 = __builtin_fmaf(x2, x2, y2 * y2);
   // rdist2 < rdistcutoff, tested on the squared distance; the OTF radius
   // index sqrt(d2)/dkr and the damping term (rdist2/rdistcutoff)^2 are
   // formed from d2 as well (a few ulps from the reference's sqrtf and
   // divisions, far inside the output tolerance)
   if (!(d2 < p.cut2)) return 0.f;
   const int ao2 = order2 < 0 ? -order2 : order2;
-  const float2 otf2 = prof_idx(prof + ao2 * nxotf, nxotf, __builtin_amdgcn_sqrtf(d2 * p.inv_dkr2));
-  float w = otf2.x * otf2.x + otf2.y * otf2.y;
+  const float2 otf2 = prof_pos(prof + ao2 * nxotf, nxotf, __builtin_amdgcn_sqrtf(d2 * p.inv_dkr2));
+  float w = __builtin_fmaf(otf2.x, otf2.x, otf2.y * otf2.y);
   if (!UNV && !p.unit_nv) w = w / p.noiseVar[dir2 * p.norders + ao2];
   if (order2 != 0) w *= p.ampmag2_all[dir2 * p.norders + ao2];
   if (p.bSuppress && order2 != 0 && d2 <= p.supp2)
@@ -113,7 +114,7 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
     const float zf = fabsf((float)z0 * p.inv_zd0);
     zf3 = zf * zf * zf;
   }
-  const float2 otf1 = prof_idx(prof + order * nxotf, nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2));
+  const float2 otf1 = prof_pos(prof + order * nxotf, nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2));
   float weight = otf1.x * otf1.x + otf1.y * otf1.y;
   if (order != 0) weight *= p.ampmag2_all[p.dir * p.norders + order];
   float dampfact = p.inv_nv[p.dir * p.norders + order];
@@ -249,13 +250,15 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
     }
     float2 re = bp[e];
     float2 im = two ? bp2[e] : make_float2(0.f, 0.f);
-    // first pass (x1 in [1, nx/2]) or second pass x1 == 0: the element itself
-    filter_visit<ND, NO, UNV>(p, prof, jin, y1n, z0n, false, re, im);
-    // second pass, x1 = -jin < 0: the conjugate visit of the mirrored frequency
-    if (two && jin >= 1 && jin <= p.nx / 2 - 1) {
-      const int yc = -((iin <= p.ny / 2) ? iin : iin - p.ny);
-      filter_visit<ND, NO, UNV>(p, prof + nprof, -jin, yc, -z0n, true, re, im);
-    }
+    // visit 0: first pass (x1 in [1, nx/2]) or second pass x1 == 0, the
+    // element itself; visit 1: second pass, x1 = -jin < 0, the conjugate
+    // visit of the mirrored frequency.  One loop body for both keeps one copy
+    // of the unrolled denominator (and its constants) in the kernel.
+    const int nvis = (two && jin >= 1 && jin <= p.nx / 2 - 1) ? 2 : 1;
+    const int yc = -((iin <= p.ny / 2) ? iin : iin - p.ny);
+    for (int v = 0; v < nvis; ++v)
+      filter_visit<ND, NO, UNV>(p, v ? prof + nprof : prof, v ? -jin : jin, v ? yc : y1n, v ? -z0n : z0n, v != 0,
+                                re, im);
     bp[e] = re;
     if (two) bp2[e] = im;
   }
