@@ -32,9 +32,10 @@ namespace simr {
 // zero.  prof1/prof2: radial OTF profiles of order1/order2 at kz = z0.
 //
 // The reference's tests sqrtf(d) <= rdistcutoff and sqrtf(|otf|^2) * f >
-// otfcutoff are made on the squares, the OTF radius index is the hardware
-// square root of d / dkr^2 and the normalisation 1/sqrt(|a|^2 + |b|^2) the
-// hardware reciprocal square root: within a few ulps of the reference's
+// otfcutoff are made on the squares (sums of squares in fused form), the OTF
+// radius index is the hardware square root of d / dkr^2 and the
+// normalisation 1/sqrt(|a|^2 + |b|^2) the hardware reciprocal square root:
+// within a few ulps of the reference's
 // correctly rounded sqrtf and divisions (an element exactly on a cut-off
 // circle may land on the other side), far inside the overlap tolerance, at
 // a third of the instructions.
@@ -44,20 +45,19 @@ struct OvSrc {
   float2 fact;
 };
 
+// Bilinear profile lookup (dev_otfinterpolate's radial interpolation) at a
+// radius index that is a square root, so never negative: truncation is the
+// floor and the clamp at 0 cannot act; t0 + ar (t1 - t0) in fused form is
+// within an ulp of (1 - ar) t0 + ar t1.
 __device__ __forceinline__ float2 prof_lerp(const float2* prof, int nxotf, float krindex) {
-#pragma clang fp contract(off)
-  int irindex = (int)floorf(krindex);
-  float ar = krindex - (float)irindex;
+  int irindex = (int)krindex;
+  float ar = __builtin_amdgcn_fractf(krindex);
   if (irindex > nxotf - 2) {
     irindex = nxotf - 2;
     ar = 1.f;
   }
-  if (irindex < 0) {
-    irindex = 0;
-    ar = 0.f;
-  }
   const float2 t0 = prof[irindex], t1 = prof[irindex + 1];
-  return make_float2((1 - ar) * t0.x + ar * t1.x, (1 - ar) * t0.y + ar * t1.y);
+  return make_float2(__builtin_fmaf(ar, t1.x - t0.x, t0.x), __builtin_fmaf(ar, t1.y - t0.y, t0.y));
 }
 
 __device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof1, const float2* prof2, int which,
@@ -69,12 +69,12 @@ __device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof
   if (y1f > (float)(p.ny / 2)) y1f -= (float)p.ny;
   x1f *= p.dkx;
   y1f *= p.dky;
-  const float d1 = x1f * x1f + y1f * y1f;
+  const float d1 = __builtin_fmaf(x1f, x1f, y1f * y1f);
   if (which == 0 ? !(d1 <= p.rc2) : !(d1 < p.rc2)) return o;
   const float x12 = x1f - p.kx, y12 = y1f - p.ky;
-  const float d12 = x12 * x12 + y12 * y12;
+  const float d12 = __builtin_fmaf(x12, x12, y12 * y12);
   const float x21 = x1f + p.kx, y21 = y1f + p.ky;
-  const float d21 = x21 * x21 + y21 * y21;
+  const float d21 = __builtin_fmaf(x21, x21, y21 * y21);
   if (!(d12 <= p.rc2 || d21 <= p.rc2)) return o;
   if (z0 == 0 && p.bNoKz0) return o;
   int iin, jin;
@@ -103,7 +103,7 @@ __device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof
     sa2 = p.f2;
     sb2 = 1.f;
   }
-  const float ma = oa.x * oa.x + oa.y * oa.y, mb = ob.x * ob.x + ob.y * ob.y;
+  const float ma = __builtin_fmaf(oa.x, oa.x, oa.y * oa.y), mb = __builtin_fmaf(ob.x, ob.x, ob.y * ob.y);
   if (!(ma * sa2 > p.cut2)) return o;
   if (!(mb * sb2 > p.cut2)) return o;
   const float rroot = __builtin_amdgcn_rsqf(ma + mb);
