@@ -267,21 +267,31 @@ __global__ __launch_bounds__(kMaxBands* fft_tpt(N)) void k_rows_fwd(FwdParams p,
     }
     __syncthreads();
   }
-  // 3. phase separation (separate_kernel 267-291) into packed complex rows
-  constexpr int MPT = (kMaxBands * N + NT - 1) / NT;
+  // 3. phase separation (separate_kernel 267-291) into packed complex rows:
+  // a thread reads the ten phase values of its column x once and forms all
+  // bands there
+  constexpr int XPT = (N + NT - 1) / NT;
 #pragma unroll
-  for (int i = 0; i < MPT; ++i) {
-    const int idx = tid + i * NT;
-    if ((kMaxBands * N) % NT == 0 || idx < kMaxBands * N) {
-      const int b = idx / N, x = idx - (idx / N) * N;
-      float a0 = 0.f, a1 = 0.f;
+  for (int i = 0; i < XPT; ++i) {
+    const int x = tid + i * NT;
+    if (N % NT == 0 || x < N) {
+      float r0[kMaxBands], r1[kMaxBands];
 #pragma unroll
       for (int ph = 0; ph < kMaxBands; ++ph) {
-        const float m = p.sep[b * kMaxBands + ph];  // zero outside nbands x nphases
-        a0 += m * rv[(ph * 2 + 0) * N + x];
-        a1 += m * rv[(ph * 2 + 1) * N + x];
+        r0[ph] = rv[(ph * 2 + 0) * N + x];
+        r1[ph] = rv[(ph * 2 + 1) * N + x];
       }
-      cz[b * P + lds_swz<N>(x)] = make_float2(a0, a1);
+#pragma unroll
+      for (int b = 0; b < kMaxBands; ++b) {
+        float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+        for (int ph = 0; ph < kMaxBands; ++ph) {
+          const float m = p.sep[b * kMaxBands + ph];  // zero outside nbands x nphases
+          a0 += m * r0[ph];
+          a1 += m * r1[ph];
+        }
+        cz[b * P + lds_swz<N>(x)] = make_float2(a0, a1);
+      }
     }
   }
   __syncthreads();
