@@ -249,6 +249,10 @@ def bench_serving(n_requests=200, rate=20.0, slots=4, seed=20264202):
                 res = srv.wait(tk)
                 lat.append((res.t_end_us - arr_us) * 1e-6)
                 vox[0] += vox_of[ks]
+                if (i + 1) % 250 == 0:
+                    print("bench: serving (%s): %d of %d requests done" % ("paced" if paced else "closed loop",
+                                                                          i + 1, len(plan)),
+                          file=sys.stderr, flush=True)
                 with cond:
                     free[ks].append(buf)
                     cond.notify_all()
@@ -306,8 +310,11 @@ def bench_serving(n_requests=200, rate=20.0, slots=4, seed=20264202):
     for (k, s) in shapes:
         plain_one(k, s)
     t0 = time.perf_counter()
-    for _, k, s in plan:
+    for i, (_, k, s) in enumerate(plan):
         plain_one(k, s)
+        if (i + 1) % 250 == 0:
+            print("bench: serving (sequential plain): %d of %d requests done" % (i + 1, len(plan)),
+                  file=sys.stderr, flush=True)
     wall_p = time.perf_counter() - t0
     for r in ctxs.values():
         r.close()

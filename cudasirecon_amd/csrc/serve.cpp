@@ -138,7 +138,8 @@ struct simr_server {
         }
       }
       const Key key{q.setup, q.nx, q.ny, q.nz};
-      for (auto& s : slots)
+      bool key_busy = false;  // a context of this key is running another request
+      for (auto& s : slots) {
         if (!s->busy && s->key == key) {
           s->busy = true;
           ++busy;
@@ -147,6 +148,8 @@ struct simr_server {
           queue.erase(it);
           return r;
         }
+        key_busy = key_busy || s->key == key;
+      }
       const simr_params p = params_of(key);
       const size_t need = simr_footprint(&p);
       if (need == 0 || need > budget) {
@@ -160,6 +163,13 @@ struct simr_server {
         complete(r);
         it = queue.erase(it);
         cv.notify_all();
+        continue;
+      }
+      // A second context of a key that has a running one is made only when it
+      // fits as it is: evicting other keys' contexts for it costs more (their
+      // re-creation) than waiting for the running one to come free.
+      if (key_busy && used + need > budget) {
+        ++it;
         continue;
       }
       // evict idle contexts, least recently used first, until the new one fits
