@@ -148,3 +148,27 @@ def test_device_resident_input_and_output():
         srv.wait(tk)
     got = d_out.cpu().numpy().reshape(ref.shape)
     assert np.array_equal(got, ref)
+
+
+def test_busy_key_waits_instead_of_evicting():
+    """A budget that holds one context per key: a request whose key's context
+    is running waits for it rather than evicting the other key's idle context
+    (which the next request of that key would then have to re-create)."""
+    log("case: 64x64x8 and 128x128x8 interleaved 5:1, budget for one context of each, 4 in flight")
+    p3, otf3, items = cases_3d_2d()[0]
+    need = [engine.footprint(shaped(p3, shape)) for shape, _ in items]
+    ref = {shape: alone(shaped(p3, shape), otf3, raw)[0] for shape, raw in items}
+    order = [0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 1]
+    with serving.Server(hbm_budget=int(sum(need) * 1.05), max_inflight=4) as srv:
+        sid = srv.add_setup(p3, otf3)
+        jobs = []
+        for k in order:
+            shape, raw = items[k]
+            out = np.empty(srv.output_shape(sid, *shape), np.float32)
+            jobs.append((shape, out, srv.submit(sid, raw, out=out, shape=shape)))
+        for shape, out, tk in jobs:
+            srv.wait(tk)
+            assert np.array_equal(out, ref[shape]), shape
+        st = srv.stats()
+        log("stats", st)
+        assert st["completed"] == len(order) and st["contexts_evicted"] == 0 and st["contexts_created"] == 2
