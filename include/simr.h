@@ -276,8 +276,11 @@ int simr_enable_timing(simr_ctx* ctx, int on);
  * overlaps on the device.  Contexts are cached per (setup, shape) and reused,
  * which keeps the OTFs resident; a request is admitted when a cached idle
  * context of its key exists or a new one fits in hbm_budget bytes (idle
- * contexts of other keys are evicted, least recently used first).  Requests
- * are taken in arrival order, skipping over ones that cannot be admitted yet.
+ * contexts of other keys are evicted, least recently used first; a second
+ * context of a key whose context is running is made only when it fits
+ * without evicting, otherwise the request waits for the running one).
+ * Requests are taken in arrival order, skipping over ones that cannot be
+ * admitted yet.
  * Time series: a request with series >= 0 and t > 0 waits for t = 0 of its
  * series, then runs on any context with that series' time-0 state, so its
  * result is the one a single sequential context gives.
