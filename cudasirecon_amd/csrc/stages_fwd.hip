@@ -195,8 +195,12 @@ __global__ __launch_bounds__(kMaxBands* fft_tpt(N)) void k_rows_fwd(FwdParams p,
   constexpr int Q = N / 4;
   constexpr int NQ = kMaxBands * 2 * Q;
   constexpr int QPT = (NQ + NT - 1) / NT;
-  __shared__ float2 cz[kMaxBands * P];
-  __shared__ __align__(16) float rv[kMaxBands * 2 * N];
+  // the flat-fielded phase rows (rv, 2 kMaxBands rows of N floats) share
+  // their LDS with the packed band rows (cz) that the separation writes:
+  // every thread reads its columns' phase values before any band is written
+  __shared__ __align__(16) float2 cz[kMaxBands * P];
+  static_assert(2 * kMaxBands * N <= 2 * kMaxBands * P, "phase rows fit in the band rows");
+  float* const rv = reinterpret_cast<float*>(cz);
   __shared__ float dy[kMaxBands * 2];
   const int tid = threadIdx.x;
   FftTw<N> twr;
@@ -271,24 +275,29 @@ __global__ __launch_bounds__(kMaxBands* fft_tpt(N)) void k_rows_fwd(FwdParams p,
   // a thread reads the ten phase values of its column x once and forms all
   // bands there
   constexpr int XPT = (N + NT - 1) / NT;
+  float r0[XPT][kMaxBands], r1[XPT][kMaxBands];
+#pragma unroll
+  for (int i = 0; i < XPT; ++i) {
+    const int x = tid + i * NT < N ? tid + i * NT : N - 1;
+#pragma unroll
+    for (int ph = 0; ph < kMaxBands; ++ph) {
+      r0[i][ph] = rv[(ph * 2 + 0) * N + x];
+      r1[i][ph] = rv[(ph * 2 + 1) * N + x];
+    }
+  }
+  __syncthreads();  // rv is overwritten by cz from here
 #pragma unroll
   for (int i = 0; i < XPT; ++i) {
     const int x = tid + i * NT;
     if (N % NT == 0 || x < N) {
-      float r0[kMaxBands], r1[kMaxBands];
-#pragma unroll
-      for (int ph = 0; ph < kMaxBands; ++ph) {
-        r0[ph] = rv[(ph * 2 + 0) * N + x];
-        r1[ph] = rv[(ph * 2 + 1) * N + x];
-      }
 #pragma unroll
       for (int b = 0; b < kMaxBands; ++b) {
         float a0 = 0.f, a1 = 0.f;
 #pragma unroll
         for (int ph = 0; ph < kMaxBands; ++ph) {
           const float m = p.sep[b * kMaxBands + ph];  // zero outside nbands x nphases
-          a0 += m * r0[ph];
-          a1 += m * r1[ph];
+          a0 += m * r0[i][ph];
+          a1 += m * r1[i][ph];
         }
         cz[b * P + lds_swz<N>(x)] = make_float2(a0, a1);
       }
