@@ -151,34 +151,41 @@ __global__ __launch_bounds__(2 * kOvRows * fft_tpt(N)) void k_overlaps_x(OvParam
   __syncthreads();
   const int i0 = blockIdx.x * kOvRows;
   constexpr int EPT = (2 * kOvRows * N + NT - 1) / NT;
-  // sources of all elements first, then every band load (clamped to a valid
-  // index for zero elements) at once, then the products
-  OvSrc src[EPT];
+  // In chunks of CH elements per thread: the sources of a chunk first, then
+  // all its band loads (clamped to a valid index for zero elements) at
+  // once, then the products.  Chunks keep the registers (and so the
+  // workgroups per CU) within bounds.
+  constexpr int CH = EPT < 4 ? EPT : 4;
 #pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int idx = threadIdx.x + q * NT;
-    const int t = idx / N, j = idx - (idx / N) * N;  // t = which * kOvRows + row
-    const int which = t / kOvRows, i = i0 + (t - which * kOvRows);
-    src[q] = (idx < 2 * kOvRows * N && i < p.ny) ? ov_source(p, prof, prof + nxotf, which, j, i, z0)
-                                                 : OvSrc{0, false, false, make_float2(0.f, 0.f)};
-  }
-  float2 vre[EPT], vim[EPT];
+  for (int q0 = 0; q0 < EPT; q0 += CH) {
+    OvSrc src[CH];
 #pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int idx = threadIdx.x + q * NT;
-    const int which = (idx / N) / kOvRows;
-    const float2* bre = which == 0 ? p.b1re : p.b2re;
-    const float2* bim = which == 0 ? (p.order1 > 0 ? p.b1im : p.b1re) : p.b2im;
-    vre[q] = bre[src[q].ind];
-    vim[q] = bim[src[q].ind];
-  }
+    for (int c = 0; c < CH; ++c) {
+      const int idx = threadIdx.x + (q0 + c) * NT;
+      const int t = idx / N, j = idx - (idx / N) * N;  // t = which * kOvRows + row
+      const int which = t / kOvRows, i = i0 + (t - which * kOvRows);
+      src[c] = (q0 + c < EPT && idx < 2 * kOvRows * N && i < p.ny)
+                   ? ov_source(p, prof, prof + nxotf, which, j, i, z0)
+                   : OvSrc{0, false, false, make_float2(0.f, 0.f)};
+    }
+    float2 vre[CH], vim[CH];
 #pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int idx = threadIdx.x + q * NT;
-    if (idx < 2 * kOvRows * N) {
-      const int t = idx / N, j = idx - (idx / N) * N;
-      const bool has_im = t >= kOvRows || p.order1 > 0;
-      lds[t * P + lds_swz<N>(j)] = ov_combine(src[q], vre[q], vim[q], has_im);
+    for (int c = 0; c < CH; ++c) {
+      const int idx = threadIdx.x + (q0 + c) * NT;
+      const int which = (idx / N) / kOvRows < 2 ? (idx / N) / kOvRows : 1;
+      const float2* bre = which == 0 ? p.b1re : p.b2re;
+      const float2* bim = which == 0 ? (p.order1 > 0 ? p.b1im : p.b1re) : p.b2im;
+      vre[c] = bre[src[c].ind];
+      vim[c] = bim[src[c].ind];
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int idx = threadIdx.x + (q0 + c) * NT;
+      if (q0 + c < EPT && idx < 2 * kOvRows * N) {
+        const int t = idx / N, j = idx - (idx / N) * N;
+        const bool has_im = t >= kOvRows || p.order1 > 0;
+        lds[t * P + lds_swz<N>(j)] = ov_combine(src[c], vre[c], vim[c], has_im);
+      }
     }
   }
   __syncthreads();
