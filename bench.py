@@ -87,6 +87,50 @@ def cpu_baseline(p, tab, raw):
     }
 
 
+def bench_2d(steps=50, warmup=5, with_cpu=True):
+    """The reference's CPU-runnable case: plain 2D SIM, 256x256, 3 directions x
+    3 phases, uint16 (seeded beads + filaments, Poisson noise, background 90),
+    wiener 0.001, zoom 2 -> 512x512 f32, raw resident in HBM.  The OTF is the
+    synthetic 2D model table (not a makeotf product).  With with_cpu, the FP32
+    CPU oracle reconstructs the same stack (timed, the CPU baseline of this
+    case) and the GPU result is compared with it."""
+    from cudasirecon_amd import engine, synth
+
+    p, tab, raw, _ = synth.make_dataset(256, 256, 1, nphases=3, seed=20262201, wiener=0.001)
+    r = engine.Reconstructor(p, tab)
+    r.set_raw(raw)
+
+    def step():
+        r.load_and_rescale(0)
+        r.set_time(0)
+        r.process_volume()
+
+    for _ in range(warmup):
+        step()
+    r.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    r.sync()
+    dt = (time.perf_counter() - t0) / steps
+    vol = r.result()
+    r.close()
+    res = {"workload": "plain 2D SIM 256x256, 3 dirs x 3 phases uint16 -> 512x512 f32, raw resident in HBM",
+           "ms_per_stack": round(dt * 1e3, 4), "voxels_per_s": round(vol.size / dt, 1)}
+    if with_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        from oracle import binding as ob
+
+        t0 = time.perf_counter()
+        ref, _ = ob.reconstruct(p, tab, raw.astype(np.float32), 32, False)
+        tc = time.perf_counter() - t0
+        res["cpu_oracle_ms_per_stack"] = round(tc * 1e3, 2)
+        res["cpu_threads"] = threads
+        res["rel_l2_vs_fp32_oracle"] = float(np.linalg.norm((vol - ref).ravel()) / np.linalg.norm(ref.ravel()))
+    return res
+
+
 def profiled_traffic(kernel_prefix):
     """HBM bytes per dispatch of a kernel from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (calibrated against a plain copy); None when absent."""
@@ -354,6 +398,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 1 GiB device-to-device copy kernel three times, the known-size "
                          "reference that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
+    ap.add_argument("--no-2d", action="store_true", help="skip the plain 2D 256x256 measurement")
     ap.add_argument("--no-series", action="store_true", help="skip the time-series batching measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream measurement")
     ap.add_argument("--serve-requests", type=int, default=200,
@@ -475,6 +520,9 @@ def main():
                            "ms_per_stack": round(pcie_dt * 1e3, 3),
                            "note": "host uint16 stack copied in and the float32 volume copied out each stack"},
     }
+    if not args.no_2d:
+        res["plain_2d"] = bench_2d(with_cpu=not args.no_cpu)
+        print("bench: 2D %s" % json.dumps(res["plain_2d"]), file=sys.stderr, flush=True)
     if not args.no_series:
         res["batched"] = bench_series(slots=args.slots)
         print("bench: series %s" % json.dumps(res["batched"]), file=sys.stderr, flush=True)
