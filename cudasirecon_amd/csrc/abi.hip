@@ -238,6 +238,13 @@ int simr_debug_overlaps(simr_ctx* ctx, int dir, int o1, int o2, float k0x, float
                         double* sy) {
   return guard([&] { E(ctx).debug_overlaps(dir, o1, o2, k0x, k0y, P, sx, sy); });
 }
+int simr_debug_overlap_volumes(simr_ctx* ctx, int dir, int o1, int o2, float k0x, float k0y, float* ov0,
+                               float* ov1) {
+  return guard([&] {
+    if (!ov0 || !ov1) throw simr::ArgError("overlap volume pointers are null");
+    E(ctx).overlap_volumes(dir, o1, o2, k0x, k0y, ov0, ov1);
+  });
+}
 
 int simr_sync(simr_ctx* ctx) { return guard([&] { E(ctx).sync(); }); }
 void* simr_stream(simr_ctx* ctx) { return ctx && ctx->eng ? (void*)ctx->eng->stream() : nullptr; }

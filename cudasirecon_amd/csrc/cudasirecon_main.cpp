@@ -26,7 +26,12 @@ int main(int argc, char** argv) {
       if ((rc = SR_setFile(sr, it, iw)) != 0) break;
       if ((rc = SR_loadAndRescaleImage(sr, it, iw)) != 0) break;
       if ((rc = SR_setCurTimeIdx(sr, it)) != 0) break;
-      if ((rc = SR_processOneVolume(sr)) != 0) break;
+      // 1: a debug dump stopped processing before the filter; nothing to write
+      if ((rc = SR_processOneVolume(sr)) == 1) {
+        rc = 0;
+        continue;
+      }
+      if (rc != 0) break;
       rc = SR_writeResult(sr, it, iw);
     }
   }

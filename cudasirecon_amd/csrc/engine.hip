@@ -589,6 +589,7 @@ void Engine::make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float
   p.f2 = p.order0_2_factor * p.order0_2_factor;
   p.inv_dkr2_1 = (float)(1.0 / ((double)p.otf1.dkrotf * (double)p.otf1.dkrotf));
   p.inv_dkr2_2 = (float)(1.0 / ((double)p.otf2.dkrotf * (double)p.otf2.dkrotf));
+  w.zd = p.zd;
   const int st = stage_begin("overlaps", w.stream);
   launch_overlaps_xy(p, w.ovx, w.stream);
   launch_overlaps_z_reduce(p, w.ovx, w.P, w.SX, w.SY, w.sumparts, fit_fp32_, per_elem, w.stream);
@@ -1242,6 +1243,26 @@ void Engine::debug_overlaps(int dir, int o1, int o2, float k0x, float k0y, doubl
   SIMR_HIP(hipMemcpyAsync(Ph, w.P, L.nxy * 16, hipMemcpyDeviceToHost, stream_));
   SIMR_HIP(hipMemcpyAsync(sx, w.SX, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
   SIMR_HIP(hipMemcpyAsync(sy, w.SY, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
+  sync();
+}
+
+// The overlap pair in real space, [nz][ny][nx] complex each (the reference's
+// overlap0/overlap1 after makeoverlaps' inverse 3D FFT).  Built with
+// direction 0's fit scratch into the assembly staging region, which no fit
+// or assembly uses between find_modulation and filter_assemble.
+void Engine::overlap_volumes(int dir, int o1, int o2, float k0x, float k0y, float* ov0, float* ov1) {
+  SIMR_REQUIRE(dir >= 0 && dir < L.ndirs, "direction out of range");
+  SIMR_REQUIRE(o1 >= 0 && o2 > o1 && o2 < L.norders, "overlap orders out of range");
+  const size_t vbytes = (size_t)L.nz * L.nxy * 8;
+  const size_t stage_bytes = align256((size_t)L.nasm * L.zdim * L.nxy * 8) +
+                             align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * L.nx * 8);
+  SIMR_REQUIRE(2 * vbytes <= stage_bytes, "overlap volumes do not fit the staging region");
+  sync();  // every direction's fit has finished with the staging region
+  FitWork& w = fw_[0];
+  make_overlaps(w, dir, o1, o2, k0x, k0y);
+  launch_overlap_volumes(w.ovx, w.zd, L.nz, L.nxy, stageZ_, stream_);
+  SIMR_HIP(hipMemcpyAsync(ov0, stageZ_, vbytes, hipMemcpyDeviceToHost, stream_));
+  SIMR_HIP(hipMemcpyAsync(ov1, (const char*)stageZ_ + vbytes, vbytes, hipMemcpyDeviceToHost, stream_));
   sync();
 }
 

@@ -68,6 +68,7 @@ class Engine {
   void get_band(int dir, int band, float* host);
   void set_band(int dir, int band, const float* host);
   void debug_overlaps(int dir, int o1, int o2, float k0x, float k0y, double* P, double* sx, double* sy);
+  void overlap_volumes(int dir, int o1, int o2, float k0x, float k0y, float* ov0, float* ov1);
   void sync();
   hipStream_t stream() const { return stream_; }
   const simr_params& params() const { return P; }
@@ -102,6 +103,7 @@ class Engine {
     double* sumparts = nullptr;
     void* hstage = nullptr;  // pinned host staging
     hipEvent_t done = nullptr;  // read-back completion (polled by wait_fit)
+    int zd = 0;  // z support half-width of the overlaps in ovx
     bool ov_totals_valid = false;
     double ov_sx = 0, ov_sy = 0;
     std::map<std::pair<uint32_t, uint32_t>, Eval> cache;

@@ -91,7 +91,11 @@ int SR_loadAndRescaleImage(SIM_Reconstructor* sr, int it, int iw) {
 
 int SR_setCurTimeIdx(SIM_Reconstructor* sr, int it) { return guard([&] { R(sr).set_cur_time(it); }); }
 
-int SR_processOneVolume(SIM_Reconstructor* sr) { return guard([&] { R(sr).process_one_volume(); }); }
+int SR_processOneVolume(SIM_Reconstructor* sr) {
+  int done = 1;
+  const int rc = guard([&] { done = R(sr).process_one_volume(); });
+  return rc != 0 ? rc : (done ? 0 : 1);
+}
 
 int SR_getResult(SIM_Reconstructor* sr, float* result) { return guard([&] { R(sr).get_result(result); }); }
 

@@ -131,6 +131,7 @@ Reconstructor::Reconstructor(int argc, char** argv) {
   }
   std::printf("nx_raw=%d, ny=%d, nz=%d\n", I_.nx_raw, I_.ny, I_.nz);
   setup_common();
+  open_dumps();
 
   if (!P_.bTIFF) {
     // setOutputHeader (src/cudaSirecon/mrc.h:47-60)
@@ -208,8 +209,6 @@ void Reconstructor::check_supported() const {
     throw UnsupportedError(
         "--usecorr together with --deskew or --cropXY is not supported (the reference indexes the "
         "calibration maps with the raw width but sizes them with the resampled one)");
-  if (P_.bSaveSeparated || P_.bSaveAlignedRaw || P_.bSaveOverlaps)
-    throw UnsupportedError("--saveprefiltered/--savealignedraw/--saveoverlaps are not supported by this engine");
   if (!P_.bRadAvgOTF)
     throw UnsupportedError(
         "OTFs that are not rotationally averaged are not supported (pass --otfRA); the reference's GPU path "
@@ -507,23 +506,180 @@ void Reconstructor::set_cur_time(int it) {
   if (ctx_) check(simr_set_time_index(ctx_, it));
 }
 
+// processOneVolume (cudaSirecon.cpp:1264-1321).  Returns 1 when the volume
+// was reconstructed, 0 when a debug dump was requested: the dumps are written
+// and processing stops before the Wiener filter, as saveIntermediateDataForDebugging
+// makes it (src/cudaSirecon/mrc.h:76-96; cudaSirecon.cpp:1286-1288), and
+// there is no result to write.
 int Reconstructor::process_one_volume() {
   if (!loaded_) throw StateError("no raw data loaded: call SR_setRaw or SR_setFile first");
   sync_engine();
-  check(simr_process_volume(ctx_));
-  const float dkx = 1.f / ((float)I_.nx * I_.dxy), dky = 1.f / ((float)I_.ny * I_.dxy);
-  for (int d = 0; d < P_.ndirs; d++) {
-    simr_fit f;
-    check(simr_get_fit(ctx_, d, &f));
-    std::printf("direction %d: k0 = (%f, %f) pixels, guess (%f, %f)\n", d, f.k0x / dkx, f.k0y / dky,
-                f.k0guess_x / dkx, f.k0guess_y / dky);
-    for (int o = 1; o < P_.norders; o++)
-      std::printf("  order %d: modamp mag=%f, phase=%f, correlation coeff=%f\n", o,
-                  std::sqrt(f.amp_re[o] * f.amp_re[o] + f.amp_im[o] * f.amp_im[o]), std::atan2(f.amp_im[o], f.amp_re[o]),
-                  f.corr[o]);
+  const bool dumps = P_.bSaveSeparated || P_.bSaveAlignedRaw || P_.bSaveOverlaps;
+  if (P_.bSaveSeparated && !P_.bTIFF) {
+    // the separated bands replace the k0 search of every direction (cudaSirecon.cpp:269-287)
+    write_separated();
+  } else {
+    if (P_.bSaveSeparated)
+      for (int d = 0; d < P_.ndirs; d++) std::printf("No unmixed raw images were saved in TIFF mode\n");
+    check(dumps ? simr_find_modulation(ctx_) : simr_process_volume(ctx_));
+    const float dkx = 1.f / ((float)I_.nx * I_.dxy), dky = 1.f / ((float)I_.ny * I_.dxy);
+    for (int d = 0; d < P_.ndirs; d++) {
+      simr_fit f;
+      check(simr_get_fit(ctx_, d, &f));
+      std::printf("direction %d: k0 = (%f, %f) pixels, guess (%f, %f)\n", d, f.k0x / dkx, f.k0y / dky,
+                  f.k0guess_x / dkx, f.k0guess_y / dky);
+      for (int o = 1; o < P_.norders; o++)
+        std::printf("  order %d: modamp mag=%f, phase=%f, correlation coeff=%f\n", o,
+                    std::sqrt(f.amp_re[o] * f.amp_re[o] + f.amp_im[o] * f.amp_im[o]),
+                    std::atan2(f.amp_im[o], f.amp_re[o]), f.corr[o]);
+    }
+    if (P_.bSaveOverlaps) write_overlaps();
+  }
+  if (dumps) {
+    std::printf("\nQuit processing because either bSaveSeparated, bSaveAlignedRaw, or bSaveOverlaps is TRUE\n");
+    processed_ = false;
+    return 0;
   }
   processed_ = true;
   return 1;
+}
+
+// Debug dump files, MRC mode (setup_common, cudaSirecon.cpp:1179-1209): each
+// header is the input file's with the changes below; no extended header.
+//   savealignedraw:  float, nx = the reconstruction width.  Drift correction
+//                    is unreachable in the reference, so no section is ever
+//                    written to it: the file holds the header only.
+//   saveprefiltered: complex, nx = nx/2+1: per time point, per direction,
+//                    nphases separated 3D spectra (band 0, 1re, 1im, 2re, 2im),
+//                    nz sections each.
+//   saveoverlaps:    complex, nx = nx, nz = nz*2*ndirs*ntimes*nwaves, two
+//                    waves (overlap0, overlap1) interleaved WZT.
+// TIFF mode writes only the overlaps, as one float TIFF per call (below).
+void Reconstructor::open_dumps() {
+  if (P_.bTIFF || !in_mrc_) return;
+  const mrc::Header& in = in_mrc_->header();
+  auto open = [&](const char* path, mrc::Header h) {
+    h.next = 0;
+    try {
+      return std::make_unique<mrc::Writer>(path, h);
+    } catch (const IoError&) {
+      throw IoError(std::string("Error creating debug output file ") + path);
+    }
+  };
+  if (P_.bSaveAlignedRaw) {
+    mrc::Header h = in;
+    h.mode = mrc::kFloat;
+    h.nx = I_.nx;
+    aligned_mrc_ = open(P_.fileRawAligned, h);
+  }
+  if (P_.bSaveSeparated) {
+    mrc::Header h = in;
+    h.nx = I_.nx / 2 + 1;
+    h.mode = mrc::kComplex;
+    sep_mrc_ = open(P_.fileSeparated, h);
+  }
+  if (P_.bSaveOverlaps) {
+    mrc::Header h = in;
+    h.nz = I_.nz * 2 * P_.ndirs * I_.ntimes * I_.nwaves;
+    h.nx = I_.nx;
+    h.num_waves = 2;
+    h.interleaved = mrc::kWZT;
+    h.mode = mrc::kComplex;
+    ovl_mrc_ = open(P_.fileOverlaps, h);
+  }
+}
+
+// The separated bands of every direction, each [z][y][nx/2+1] complex
+// (cudaSirecon.cpp:275-286; there is no z padding, so zoffset is 0).  A
+// phase index past the last band (even nphases) gets zero sections.
+void Reconstructor::write_separated() {
+  if (!sep_mrc_) throw StateError("the separated-bands file is not open");
+  const int nb = 2 * P_.norders - 1;
+  const size_t plane = (size_t)(I_.nx / 2 + 1) * I_.ny * 2;
+  std::vector<float> band(plane * I_.nz0, 0.f);
+  for (int d = 0; d < P_.ndirs; d++)
+    for (int ph = 0; ph < P_.nphases; ph++) {
+      if (ph < nb)
+        check(simr_get_band_host(ctx_, d, ph, band.data()));
+      else
+        std::fill(band.begin(), band.end(), 0.f);
+      for (int z = 0; z < I_.nz; z++) sep_mrc_->write_c64(band.data() + plane * z);
+    }
+}
+
+// The overlap pair of each direction in real space.  With a k0 search, the
+// pair findk0 builds: order 0 against order 2 (3D) or 1 (2D, Bessel) at the
+// k0 guess (cudaSirecon.cpp:318-352; gpuFunctionsImpl.cu:306-330); MRC:
+// sections overlap0 z, overlap1 z for every z; TIFF: one float file of width
+// 2*nx (re, im), height ny, 2*nz pages (overlap0's planes, then
+// overlap1's), rewritten by every direction.  With a known k0 (no search,
+// or a later time point reusing time 0's k0), the order 0 / order 1 pair at
+// that k0, MRC only (cudaSirecon.cpp:427-445; the reference steps through
+// those buffers by nx*ny floats, half a complex plane, per section; here
+// every section is one whole plane).
+void Reconstructor::write_overlaps() {
+  const bool search = P_.bSearchforvector && !(I_.ntimes > 1 && I_.curTimeIdx > 0 && P_.bUseTime0k0);
+  const size_t plane = (size_t)I_.nx * I_.ny * 2;
+  const size_t vol = plane * I_.nz0;
+  std::vector<float> ov(2 * vol);
+  auto put = [&]() {
+    if (P_.bTIFF) {
+      tiff::write_f32(P_.fileOverlaps, ov.data(), 2 * I_.nx, I_.ny, 2 * I_.nz0);
+      return;
+    }
+    if (!ovl_mrc_) throw StateError("the overlaps file is not open");
+    for (int z = 0; z < I_.nz0; z++) {
+      ovl_mrc_->write_c64(ov.data() + plane * z);
+      ovl_mrc_->write_c64(ov.data() + vol + plane * z);
+    }
+  };
+  for (int d = 0; d < P_.ndirs; d++) {
+    simr_fit f;
+    check(simr_get_fit(ctx_, d, &f));
+    if (search) {
+      const int o2 = (I_.nz0 > 1 && !P_.bBessel) ? 2 : 1;
+      check(simr_debug_overlap_volumes(ctx_, d, 0, o2, f.k0guess_x, f.k0guess_y, ov.data(), ov.data() + vol));
+      put();
+    } else {
+      for (int o = 1; o < P_.norders; o++) {
+        if (P_.bTIFF) {
+          std::printf("No overlaps were saved in TIFF mode");
+        } else if (o == 1) {
+          check(simr_debug_overlap_volumes(ctx_, d, 0, 1, f.k0x, f.k0y, ov.data(), ov.data() + vol));
+          put();
+        }
+      }
+    }
+  }
+}
+
+// saveIntermediateDataForDebugging (src/cudaSirecon/mrc.h:76-89): each dump
+// gets its title appended to the labels (IMWrHdr with ntflag 1) and the
+// header's min/max/mean set to 0/1/0 (the aligned-raw file keeps the input's).
+// Closed once, after the last time point.
+void Reconstructor::close_dumps() {
+  auto finish = [](std::unique_ptr<mrc::Writer>& w, const char* title, bool range) {
+    if (!w) return;
+    mrc::Header& h = w->header();
+    int n = std::max(0, std::min<int>(h.nlab, 10));
+    if (n == 10) {
+      for (int l = 0; l < 9; l++) std::memcpy(h.label[l], h.label[l + 1], 80);
+      n = 9;
+    }
+    std::memset(h.label[n], ' ', 80);
+    std::memcpy(h.label[n], title, std::min<size_t>(80, std::strlen(title)));
+    h.nlab = n + 1;
+    if (range) {
+      h.amin = 0.f;
+      h.amax = 1.f;
+      h.amean = 0.f;
+    }
+    w->close();
+    w.reset();
+  };
+  finish(sep_mrc_, "separated bands of all directions", true);
+  finish(aligned_mrc_, "drift-corrected raw images", false);
+  finish(ovl_mrc_, "overlaps in real space", true);
 }
 
 size_t Reconstructor::result_elems() const {
@@ -589,6 +745,7 @@ void Reconstructor::write_result(int it, int iw) {
 // title; with --writeTitle it replaces the labels.  IVE's IMWrHdr(..., title
 // = header.label, ntflag = 1, ...) then appends the header's first label.
 void Reconstructor::close_files() {
+  close_dumps();
   if (P_.bTIFF || !out_mrc_) return;
   in_mrc_.reset();
   mrc::Header& h = out_mrc_->header();

@@ -55,6 +55,10 @@ class Reconstructor {
   void sync_engine();  // create the context, or bring it up to date with P_/I_
   void upload_raw(const float* f32, const uint16_t* u16);
   std::string output_path(const std::string& input) const;
+  void open_dumps();
+  void write_separated();
+  void write_overlaps();
+  void close_dumps();
 
   SR_ReconParams P_{};
   SR_ImageParams I_{};
@@ -68,6 +72,8 @@ class Reconstructor {
   std::string out_dir_;
   std::unique_ptr<mrc::Reader> in_mrc_;
   std::unique_ptr<mrc::Writer> out_mrc_;
+  // debug dumps (--saveprefiltered, --savealignedraw, --saveoverlaps), MRC mode
+  std::unique_ptr<mrc::Writer> sep_mrc_, aligned_mrc_, ovl_mrc_;
   mrc::Header io_header_;
   std::unique_ptr<tiff::Reader> otf_tiff_;
   std::unique_ptr<mrc::Reader> otf_mrc_;

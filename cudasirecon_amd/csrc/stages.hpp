@@ -72,6 +72,9 @@ constexpr int kZsumMaxBlocks = 2048;
 int zsum_blocks(long long nxy);
 void launch_overlaps_z_reduce(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, double* totals,
                               bool fp32, bool per_elem, hipStream_t s);
+// Inverse z DFT of the stored kz planes of both overlaps into real-space
+// volumes vol[2][nz][nx*ny] (debug dump only).
+void launch_overlap_volumes(const float2* ovx, int zd, int nz, long long nxy, float2* vol, hipStream_t s);
 // crosscorr = conj(P) as float, then its forward 2D FFT and |.|^2 argmax.
 // peak_out (host-visible after sync): [0] = index of the first maximum,
 // values: int peak index and the 5 intensities (center, x-1, x+1, y-1, y+1).

@@ -532,4 +532,39 @@ void launch_modamp_fp32(const double2* P, const double* SX, const double* SY, co
   SIMR_LAUNCH_CHECK();
 }
 
+// Real-space overlap volumes (the debug dump of the overlap pair,
+// cudaSirecon.cpp:330-352,433-445): the inverse DFT along z, unnormalised
+// and exp(+i) like the reference's inverse 3D transform, of the 2zd+1 stored
+// kz planes (plane j holds kz = j - zd; every other kz is zero) of both
+// overlaps: vol[which][z][y][x].  A direct sum over the stored planes; this
+// runs once per dumped direction, not in the reconstruction.
+__global__ __launch_bounds__(256) void k_overlap_volumes(const float2* __restrict__ ovx, int zd, int nz, long long nxy,
+                                                         float2* __restrict__ vol) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= nxy) return;
+  const int z = blockIdx.y, which = blockIdx.z;
+  const int nzs = 2 * zd + 1;
+  const float2* src = ovx + (long long)which * nzs * nxy + e;
+  float re = 0.f, im = 0.f;
+  for (int j = 0; j < nzs; ++j) {
+    int m = ((j - zd) * z) % nz;  // exp(2 pi i kz z / nz) depends on kz*z mod nz only
+    if (m < 0) m += nz;
+    double sd, cd;
+    sincospi(2.0 * (double)m / (double)nz, &sd, &cd);
+    const float c = (float)cd, sn = (float)sd;
+    const float2 v = src[(long long)j * nxy];
+    re += v.x * c - v.y * sn;
+    im += v.x * sn + v.y * c;
+  }
+  vol[((long long)which * nz + z) * nxy + e] = make_float2(re, im);
+}
+
+void launch_overlap_volumes(const float2* ovx, int zd, int nz, long long nxy, float2* vol, hipStream_t s) {
+  if (zd < 0 || 2 * zd + 1 > nz) throw ArgError("overlap z support wider than the stack");
+  if (nz > 65535) throw ArgError("overlap volumes: too many z planes");
+  hipLaunchKernelGGL(k_overlap_volumes, dim3((unsigned)ceil_div(nxy, 256), nz, 2), dim3(256), 0, s, ovx, zd, nz, nxy,
+                     vol);
+  SIMR_LAUNCH_CHECK();
+}
+
 }  // namespace simr

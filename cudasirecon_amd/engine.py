@@ -66,6 +66,7 @@ def lib():
         "simr_set_band_host": (ip, [vp, ip, ip, fp]),
         "simr_filter_dir": (ip, [vp, ip]),
         "simr_debug_overlaps": (ip, [vp, ip, ip, ip, C.c_float, C.c_float, dp, dp, dp]),
+        "simr_debug_overlap_volumes": (ip, [vp, ip, ip, ip, C.c_float, C.c_float, fp, fp]),
         "simr_sync": (ip, [vp]),
         "simr_stream": (vp, [vp]),
         "simr_enable_timing": (ip, [vp, ip]),
@@ -272,6 +273,15 @@ class Reconstructor:
         _check(lib().simr_debug_overlaps(self.h, d, o1, o2, k0x, k0y, _dp(P), _dp(sx), _dp(sy)))
         shp = (self.p.ny, self.p.nx)
         return dict(P=P.view(np.complex128).reshape(shp), sx=sx.reshape(shp), sy=sy.reshape(shp))
+
+    def overlap_volumes(self, d, o1, o2, k0x, k0y):
+        """(overlap0, overlap1) in real space, complex64 [nz][ny][nx] each."""
+        n = 2 * self.p.nx * self.p.ny * self.p.nz
+        ov0 = np.empty(n, np.float32)
+        ov1 = np.empty(n, np.float32)
+        _check(lib().simr_debug_overlap_volumes(self.h, d, o1, o2, k0x, k0y, _fp(ov0), _fp(ov1)))
+        shp = (self.p.nz, self.p.ny, self.p.nx)
+        return ov0.view(np.complex64).reshape(shp), ov1.view(np.complex64).reshape(shp)
 
     def enable_timing(self, on=True):
         _check(lib().simr_enable_timing(self.h, int(on)))

@@ -205,7 +205,9 @@ class SIMReconstructor:
         _check(lib().SR_setCurTimeIdx(self._h, it))
 
     def process_one_volume(self):
-        _check(lib().SR_processOneVolume(self._h))
+        """True when a volume was reconstructed; False when a debug dump
+        (saveprefiltered / savealignedraw / saveoverlaps) stopped processing."""
+        return _check(lib().SR_processOneVolume(self._h)) == 0
 
     def get_result(self):
         out = np.empty(self.output_shape(), np.float32)

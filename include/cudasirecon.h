@@ -102,7 +102,11 @@ int SR_loadAndRescaleImage(SIM_Reconstructor* sr, int it, int iw);
 /* interface.cpp:31-34. */
 int SR_setCurTimeIdx(SIM_Reconstructor* sr, int it);
 
-/* interface.cpp:35-38.  k0/modamp fit, Wiener filter and assembly. */
+/* interface.cpp:35-38.  k0/modamp fit, Wiener filter and assembly.  Returns 0
+ * after a reconstruction, or 1 when --saveprefiltered, --savealignedraw or
+ * --saveoverlaps wrote their debug dumps and stopped before the filter (the
+ * reference's processOneVolume returns 0 then and its driver skips
+ * writeResult, cudaSirecon.cpp:1286-1288, cudaSireconDriver.cpp:15-16). */
 int SR_processOneVolume(SIM_Reconstructor* sr);
 
 /* interface.cpp:39-42.  result: caller-allocated

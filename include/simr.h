@@ -191,6 +191,14 @@ int simr_filter_dir(simr_ctx* ctx, int dir);
  * [ny][nx]), SX = sum_z |X|^2, SY = sum_z |Y|^2 (double [ny][nx]). */
 int simr_debug_overlaps(simr_ctx* ctx, int dir, int order1, int order2, float k0x, float k0y, double* P,
                         double* sx, double* sy);
+/* The same overlap pair in real space, for the --saveoverlaps dump
+ * (cudaSirecon.cpp:330-352,433-445; makeoverlaps' inverse 3D FFT,
+ * gpuFunctionsImpl.cu:639-669): ov0, ov1 receive overlap0 and overlap1 as
+ * complex float [nz][ny][nx] (2*nx*ny*nz floats each, unnormalised inverse
+ * transform).  Call after simr_find_modulation and before
+ * simr_filter_assemble: the volumes are built in the assembly staging memory. */
+int simr_debug_overlap_volumes(simr_ctx* ctx, int dir, int order1, int order2, float k0x, float k0y, float* ov0,
+                               float* ov1);
 
 /*
  * Batched complex FFT utility (the engine's own LDS Stockham kernels; replaces

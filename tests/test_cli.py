@@ -13,7 +13,7 @@ import subprocess
 import pytest
 
 from cases import GOLDEN
-from cudasirecon_amd import sr
+from cudasirecon_amd import mrc, sr
 from cudasirecon_amd.engine import SimrError
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -190,7 +190,7 @@ def test_deskew_to_an_unsupported_width_is_refused(tmp_path):
 
 
 def test_unsupported_branches_are_refused(tmp_path):
-    for extra in (["--bessel", "--noRecon"], ["--2lenses"], ["--saveoverlaps", "x.dv"], []):
+    for extra in (["--bessel", "--noRecon"], ["--2lenses"], []):
         r = subprocess.run([EXE, os.path.join(GOLDEN, "raw.dv"), str(tmp_path / "o.dv"),
                             os.path.join(GOLDEN, "otf.dv")] + extra + (["--otfRA"] if extra else []),
                            capture_output=True, text=True)
@@ -207,3 +207,25 @@ def test_tiff_folder_discovery_creates_output_folder(tmp_path):
                         os.path.join(GOLDEN, "config-tiff")], capture_output=True, text=True)
     assert (d / "GPUsirecon").is_dir()
     assert "nx_raw=64, ny=64, nz=9" in r.stdout
+
+
+def test_debug_dump_files_get_the_reference_headers(tmp_path):
+    """--saveprefiltered / --saveoverlaps / --savealignedraw open their MRC files at
+    setup with the input header changed as setup_common does
+    (src/cudaSirecon/cudaSirecon.cpp:1179-1209).  Without a GPU the run stops at
+    the device, after setup."""
+    sep, ov, al = (str(tmp_path / n) for n in ("sep.dv", "ov.dv", "al.dv"))
+    r = subprocess.run([EXE, os.path.join(GOLDEN, "raw.dv"), str(tmp_path / "o.dv"), os.path.join(GOLDEN, "otf.dv"),
+                        "-c", os.path.join(GOLDEN, "config"), "--saveprefiltered", sep, "--saveoverlaps", ov,
+                        "--savealignedraw", al], capture_output=True, text=True)
+    assert "nx_raw=64, ny=64, nz=9" in r.stdout, r.stdout + r.stderr
+    hin = mrc.read_header(open(os.path.join(GOLDEN, "raw.dv"), "rb").read(1024))
+    hs = mrc.read_header(open(sep, "rb").read(1024))
+    assert (hs["nx"], hs["ny"], hs["nz"], hs["mode"], hs["next"]) == (33, 64, hin["nz"], 4, 0)
+    ho = mrc.read_header(open(ov, "rb").read(1024))
+    assert (ho["nx"], ho["ny"], ho["nz"], ho["mode"]) == (64, 64, 9 * 2 * 3, 4)
+    assert (ho["num_waves"], ho["interleaved"]) == (2, 1)
+    ha = mrc.read_header(open(al, "rb").read(1024))
+    assert (ha["nx"], ha["ny"], ha["nz"], ha["mode"]) == (64, 64, hin["nz"], 2)
+    for h in (hs, ho, ha):
+        assert (h["xlen"], h["zlen"], h["iwav1"]) == (hin["xlen"], hin["zlen"], hin["iwav1"])

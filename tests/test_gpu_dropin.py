@@ -280,3 +280,91 @@ def test_multichannel_driver_matches_oracle(tmp_path, oracle_mrc):
         log("channel %d rel-L2 %.3e" % (c, err))
         assert err < TOL
     assert rel_l2(chans[0], chans[1]) > 1e-2  # the two channels did differ
+
+
+def _preprocessed_oracle(p, otf, raw):
+    o = ob.Oracle(p, otf, 32, False)
+    o.load(raw)
+    o.rescale(0)
+    o.set_time(0)
+    o.preprocess_bands()
+    return o
+
+
+def test_cli_debug_dumps_match_oracle(tmp_path):
+    """--saveprefiltered: every direction's separated 3D spectra, no k0 search;
+    --saveoverlaps: findk0's overlap pair (order 0 / order 2 at the k0 guess) of
+    every direction in real space; --savealignedraw: the header only.  Each
+    stops before the filter and writes no result (cudaSirecon.cpp:269-287,
+    330-352, 1179-1209, 1286-1288; src/cudaSirecon/mrc.h:76-96)."""
+    p, otf, raw = load_testdata()
+    o = _preprocessed_oracle(p, otf, raw)
+    args = [os.path.join(GOLDEN, "raw.dv"), None, os.path.join(GOLDEN, "otf.dv"), "-c", os.path.join(GOLDEN, "config")]
+
+    def run(tag, opt, path):
+        out = str(tmp_path / ("%s_out.dv" % tag))
+        r = run_exe(args[:1] + [out] + args[2:] + [opt, path])
+        assert "Quit processing because either bSaveSeparated, bSaveAlignedRaw, or bSaveOverlaps is TRUE" in r.stdout
+        assert os.path.getsize(out) == 1024  # the output file holds its header only
+        return r
+
+    sep = str(tmp_path / "sep.dv")
+    r = run("sep", "--saveprefiltered", sep)
+    assert "direction 0: k0" not in r.stdout  # no k0 search
+    h, d = mrc.read(sep)
+    assert (h["nx"], h["ny"], h["nz"], h["mode"]) == (33, 64, 135, 4)
+    assert h["labels"][h["nlab"] - 1].startswith("separated bands of all directions")
+    assert (h["amin"], h["amax"], h["amean"]) == (0.0, 1.0, 0.0)
+    got = d.reshape(3, 5, 9, 64, 33)
+    for dd in range(3):
+        for b in range(5):
+            err = rel_l2(got[dd, b], o.band(dd, b))
+            assert err < TOL, (dd, b, err)
+    log("saveprefiltered ok")
+
+    ov = str(tmp_path / "ov.dv")
+    r = run("ov", "--saveoverlaps", ov)
+    assert "direction 2: k0" in r.stdout  # the fit ran
+    h, d = mrc.read(ov)
+    assert (h["nx"], h["ny"], h["nz"], h["mode"], h["num_waves"], h["interleaved"]) == (64, 64, 54, 4, 2, 1)
+    assert h["labels"][h["nlab"] - 1].startswith("overlaps in real space")
+    got = d.reshape(3, 9, 2, 64, 64)
+    for dd in range(3):
+        f = o.fit(dd)
+        ref = o.makeoverlaps(dd, 0, 2, f.k0guess_x, f.k0guess_y, volumes=True)
+        for w, key in ((0, "ov0"), (1, "ov1")):
+            err = rel_l2(got[dd, :, w], ref[key])
+            log("overlaps dir %d %s rel-L2 %.3e" % (dd, key, err))
+            assert err < TOL
+
+    al = str(tmp_path / "al.dv")
+    run("al", "--savealignedraw", al)
+    assert os.path.getsize(al) == 1024
+    h = mrc.read_header(open(al, "rb").read(1024))
+    assert (h["nx"], h["mode"]) == (64, 2)
+    assert h["labels"][h["nlab"] - 1].startswith("drift-corrected raw images")
+
+
+def test_cli_tiff_saveoverlaps_matches_oracle(tmp_path):
+    """TIFF mode: --saveoverlaps writes one float TIFF (width 2*nx, 2*nz pages:
+    overlap0's planes, then overlap1's), rewritten per direction, so the last
+    direction's pair is left (cudaSirecon.cpp:330-339); no result file."""
+    p, otf, raw = load_testdata_tiff()
+    o = _preprocessed_oracle(p, otf, raw)
+    dd = tmp_path / "data"
+    dd.mkdir()
+    shutil.copy(os.path.join(GOLDEN, "raw.tif"), dd / "raw.tif")
+    ov = str(tmp_path / "ov.tif")
+    r = run_exe([str(dd), "raw", os.path.join(GOLDEN, "otf.tif"), "-c", os.path.join(GOLDEN, "config-tiff"),
+                 "--saveoverlaps", ov])
+    assert "Quit processing" in r.stdout
+    assert not (dd / "GPUsirecon" / "raw_proc.tif").exists()
+    vol = tiff_frames(ov)
+    assert vol.shape == (18, 64, 128) and vol.dtype == np.float32
+    got = np.ascontiguousarray(vol).view(np.complex64)
+    f = o.fit(2)
+    ref = o.makeoverlaps(2, 0, 2, f.k0guess_x, f.k0guess_y, volumes=True)
+    for sl, key in ((slice(0, 9), "ov0"), (slice(9, 18), "ov1")):
+        err = rel_l2(got[sl], ref[key])
+        log("tiff overlaps %s rel-L2 %.3e" % (key, err))
+        assert err < TOL

@@ -107,6 +107,30 @@ def test_overlap_products_match_oracle(case):
                 assert e < 1e-5, (case, o1, o2, key, e)
 
 
+@pytest.mark.parametrize("case", _cases())
+def test_overlap_volumes_match_oracle(case):
+    """The overlap pair in real space (the --saveoverlaps dump): the GPU's
+    stored kz planes transformed along z against the oracle's makeoverlaps
+    volumes after its inverse 3D FFT."""
+    log("overlap volumes", case)
+    p, otf, raw = _case(case)
+    o = _oracle_preprocessed(p, otf, raw.astype(np.float32))
+    fo2 = 2 if p.nz > 1 and not p.bBessel else 1
+    with _gpu(p, otf) as g:
+        for d in range(p.ndirs):
+            for b in range(p.nphases):
+                g.set_band(d, b, o.band(d, b))
+        d = p.ndirs - 1
+        f = o.fit(d)
+        for (o1, o2) in sorted({(0, fo2), (0, 1)}):
+            ref = o.makeoverlaps(d, o1, o2, f.k0guess_x, f.k0guess_y, volumes=True)
+            v0, v1 = g.overlap_volumes(d, o1, o2, f.k0guess_x, f.k0guess_y)
+            for got, key in ((v0, "ov0"), (v1, "ov1")):
+                e = rel_l2(got, ref[key])
+                assert e < 1e-5, (case, o1, o2, key, e)
+            assert np.abs(v0).max() > 0 and np.abs(v1).max() > 0
+
+
 def _fitted_oracle(p, otf, raw):
     o = _oracle_preprocessed(p, otf, raw.astype(np.float32), 32, True)
     for d in range(p.ndirs):
