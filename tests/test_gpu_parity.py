@@ -57,7 +57,7 @@ def _oracle_preprocessed(p, otf, raw, prec=32, fit64=False):
 
 
 def _cases():
-    return ["testdata", "syn3d", "syn2d", "bessel3d"]
+    return ["testdata", "syn3d", "syn2d", "bessel3d", "opts3d", "opts2d"]
 
 
 def _case(name):
@@ -69,6 +69,14 @@ def _case(name):
         return p, otf, raw
     if name == "bessel3d":  # --bessel: order-1 fit, Bessel axial cutoffs 15/14/10, anisotropic apodization
         p, otf, raw, _ = synthetic(64, 64, 32, nphases=5, dz=0.1, bBessel=1, BesselNA=0.65, BesselLambdaEx=0.488)
+        return p, otf, raw
+    if name == "opts3d":  # non-square, zoom 1.5, z_zoom 2, cosine apodization, no suppression, noKz0, equalizez
+        p, otf, raw, _ = synthetic(96, 128, 12, nphases=5, zoomfact=1.5, z_zoom=2, apodizeoutput=1,
+                                   bSuppress_singularities=0, bNoKz0=1, equalizez=1, wiener=0.01, napodize=6)
+        return p, otf, raw
+    if name == "opts2d":  # non-square 2D, gamma apodization 0.8, suppression radius 5
+        p, otf, raw, _ = synthetic(120, 100, 1, nphases=3, apodizeoutput=2, apoGamma=0.8, suppression_radius=5,
+                                   wiener=0.005)
         return p, otf, raw
     p, otf, raw, _ = synthetic(128, 128, 1, nphases=3)
     return p, otf, raw
