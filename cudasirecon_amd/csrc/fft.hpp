@@ -7,7 +7,7 @@
 //     v[r] = a[j + r N/R] * w_N^{r k N/(Ns R)},  r = 0..R-1
 //     v    = DFT_R(v)
 //     b[(j/Ns) Ns R + k + q Ns] = v[q]
-// run for the radix sequence of N (8s first, then 4, 2, 5, 3), Ns = product of
+// run for the radix sequence of N (8s first, then 4, 2, 5, 3, 7), Ns = product of
 // the radices already applied.  Forward uses w_N = exp(-2 pi i / N), inverse
 // its conjugate; both unnormalised (as cuFFT).
 //
@@ -53,6 +53,10 @@ constexpr Radices make_radices(int N) {
   while (m % 3 == 0) {
     out.r[out.n++] = 3;
     m /= 3;
+  }
+  while (m % 7 == 0) {
+    out.r[out.n++] = 7;
+    m /= 7;
   }
   if (m != 1) out.n = -1;  // unsupported factor
   return out;
@@ -209,6 +213,52 @@ template <>
 __device__ __forceinline__ void dft<5, false>(float2* v) { dft5<false>(v); }
 template <>
 __device__ __forceinline__ void dft<5, true>(float2* v) { dft5<true>(v); }
+
+// 7-point DFT by the symmetric pairs t_k = v_k + v_{7-k}, u_k = v_k - v_{7-k}
+// (k = 1..3): X_0 = v_0 + t_1 + t_2 + t_3 and, for m = 1..3,
+// a_m = v_0 + sum_k cos(2 pi k m / 7) t_k, b_m = sum_k sin(2 pi k m / 7) u_k,
+// X_m = a_m - i b_m, X_{7-m} = a_m + i b_m (forward; the inverse swaps them).
+// k m mod 7 folds every angle onto 2 pi j / 7, j = 1..3.
+template <bool INV>
+__device__ __forceinline__ void dft7(float2* v) {
+  const float c1 = 0.62348980185873353053f;   // cos(2 pi/7)
+  const float c2 = -0.22252093395631440429f;  // cos(4 pi/7)
+  const float c3 = -0.90096886790241912624f;  // cos(6 pi/7)
+  const float s1 = 0.78183148246802980871f;   // sin(2 pi/7)
+  const float s2 = 0.97492791218182360702f;   // sin(4 pi/7)
+  const float s3 = 0.43388373911755812048f;   // sin(6 pi/7)
+  const float2 a0 = v[0];
+  const float2 t1 = cadd(v[1], v[6]), t2 = cadd(v[2], v[5]), t3 = cadd(v[3], v[4]);
+  const float2 u1 = csub(v[1], v[6]), u2 = csub(v[2], v[5]), u3 = csub(v[3], v[4]);
+  const float2 a1 = make_float2(a0.x + c1 * t1.x + c2 * t2.x + c3 * t3.x, a0.y + c1 * t1.y + c2 * t2.y + c3 * t3.y);
+  const float2 a2 = make_float2(a0.x + c2 * t1.x + c3 * t2.x + c1 * t3.x, a0.y + c2 * t1.y + c3 * t2.y + c1 * t3.y);
+  const float2 a3 = make_float2(a0.x + c3 * t1.x + c1 * t2.x + c2 * t3.x, a0.y + c3 * t1.y + c1 * t2.y + c2 * t3.y);
+  const float2 b1 = make_float2(s1 * u1.x + s2 * u2.x + s3 * u3.x, s1 * u1.y + s2 * u2.y + s3 * u3.y);
+  const float2 b2 = make_float2(s2 * u1.x - s3 * u2.x - s1 * u3.x, s2 * u1.y - s3 * u2.y - s1 * u3.y);
+  const float2 b3 = make_float2(s3 * u1.x - s1 * u2.x + s2 * u3.x, s3 * u1.y - s1 * u2.y + s2 * u3.y);
+  // i b
+  const float2 ib1 = make_float2(-b1.y, b1.x), ib2 = make_float2(-b2.y, b2.x), ib3 = make_float2(-b3.y, b3.x);
+  v[0] = make_float2(a0.x + t1.x + t2.x + t3.x, a0.y + t1.y + t2.y + t3.y);
+  if (INV) {
+    v[1] = cadd(a1, ib1);
+    v[6] = csub(a1, ib1);
+    v[2] = cadd(a2, ib2);
+    v[5] = csub(a2, ib2);
+    v[3] = cadd(a3, ib3);
+    v[4] = csub(a3, ib3);
+  } else {
+    v[1] = csub(a1, ib1);
+    v[6] = cadd(a1, ib1);
+    v[2] = csub(a2, ib2);
+    v[5] = cadd(a2, ib2);
+    v[3] = csub(a3, ib3);
+    v[4] = cadd(a3, ib3);
+  }
+}
+template <>
+__device__ __forceinline__ void dft<7, false>(float2* v) { dft7<false>(v); }
+template <>
+__device__ __forceinline__ void dft<7, true>(float2* v) { dft7<true>(v); }
 
 // ------------------------------------------------------------- LDS layout
 // Element i of a transform lives at slot lds_swz<N>(i) of its LDS row.  The
@@ -393,7 +443,7 @@ __device__ __forceinline__ void fft_stage_t(float2* buf, int t, const float2* __
 
 template <int N, bool INV, int TPT>
 __device__ __forceinline__ void fft_lds_t(float2* buf, int t, const float2* __restrict__ tab) {
-  static_assert(fft_supported(N), "FFT length must factor into 2, 3, 5");
+  static_assert(fft_supported(N), "FFT length must factor into 2, 3, 5, 7");
   if constexpr (N > 1) fft_stage_t<N, INV, TPT, 0>(buf, t, tab);
 }
 
@@ -402,7 +452,7 @@ __device__ __forceinline__ void fft_lds_t(float2* buf, int t, const float2* __re
 // (it contains barriers).
 template <int N, bool INV, int TPT = fft_tpt(N)>
 __device__ __forceinline__ void fft_lds(float2* buf, int t, const FftTw<N, TPT>& tw) {
-  static_assert(fft_supported(N), "FFT length must factor into 2, 3, 5");
+  static_assert(fft_supported(N), "FFT length must factor into 2, 3, 5, 7");
   if constexpr (N > 1) fft_stage<N, INV, TPT, 0>(buf, t, tw);
 }
 
@@ -413,6 +463,8 @@ __device__ __forceinline__ void fft_lds(float2* buf, int t, const FftTw<N, TPT>&
   X(90) X(96) X(100) X(120) X(128) X(144) X(150) X(160) X(180) X(192) X(200) X(240) X(256)   \
   X(288) X(300) X(320) X(360) X(384) X(400) X(480) X(500) X(512) X(576) X(600) X(640)        \
   X(720) X(768) X(800) X(960) X(1000) X(1024) X(1152) X(1200) X(1280) X(1440) X(1536)        \
-  X(1600) X(1920) X(2000) X(2048) X(2400) X(2560) X(3072) X(4096)
+  X(1600) X(1920) X(2000) X(2048) X(2400) X(2560) X(3072) X(4096)                              \
+  X(7) X(14) X(21) X(28) X(35) X(42) X(49) X(56) X(63) X(70) X(84) X(98) X(105) X(112) X(126)  \
+  X(140) X(168) X(196) X(210) X(224)
 
 }  // namespace simr

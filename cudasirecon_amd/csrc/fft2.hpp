@@ -178,6 +178,7 @@ struct F2Plan {
 };
 // N1 = the largest supported factor with N1 | N2 = N/N1 and N2 <= 32.
 constexpr F2Plan fft2_plan(int N) {
+  if (N % 7 == 0) return {0, 0};  // lengths with a factor 7 run on the radix-stage core
   const int cands[] = {32, 16, 12, 10, 8, 6, 5, 4, 3, 2};
   for (int c : cands) {
     if (N % c) continue;

@@ -7,7 +7,7 @@
 // from the textbook definition X[k] = sum_n x[n] exp(-+2 pi i n k / N)
 // (forward = minus sign, inverse = plus sign, both unnormalised, as cuFFT).
 // Twiddles are evaluated in long double and rounded once to T.
-// Checked against numpy.fft and a brute-force DFT by tests/test_oracle_fft.py.
+// Checked against numpy.fft and a brute-force DFT by tests/test_oracle.py.
 #pragma once
 #include <cmath>
 #include <cstddef>

@@ -27,7 +27,10 @@ def log(*a):
 FFT_SHAPES = [(64,), (48,), (9,), (512,), (1024,), (2048,), (384,), (640,), (1280,), (4096,),
               (4,), (8,), (12,), (20,), (24,), (60,), (80,), (128,), (192,), (256,),
               (16, 8), (64, 64), (96, 128), (40, 24), (1024, 16), (512, 12), (10, 96, 128), (9, 64, 64),
-              (48, 32, 16), (12, 20, 8), (60, 4, 32)]
+              (48, 32, 16), (12, 20, 8), (60, 4, 32),
+              # lengths with factors of 7 (radix-stage core)
+              (7,), (14,), (21,), (49,), (56,), (112,), (196,), (224,), (28, 16), (14, 64, 64), (35, 16, 8),
+              (63, 8, 8), (210, 4, 4)]
 
 
 @pytest.mark.parametrize("shape", FFT_SHAPES)
@@ -57,7 +60,7 @@ def _oracle_preprocessed(p, otf, raw, prec=32, fit64=False):
 
 
 def _cases():
-    return ["testdata", "syn3d", "syn2d", "bessel3d", "opts3d", "opts2d"]
+    return ["testdata", "syn3d", "syn2d", "bessel3d", "opts3d", "opts2d", "z14", "z21zz"]
 
 
 def _case(name):
@@ -73,6 +76,12 @@ def _case(name):
     if name == "opts3d":  # non-square, zoom 1.5, z_zoom 2, cosine apodization, no suppression, noKz0, equalizez
         p, otf, raw, _ = synthetic(96, 128, 12, nphases=5, zoomfact=1.5, z_zoom=2, apodizeoutput=1,
                                    bSuppress_singularities=0, bNoKz0=1, equalizez=1, wiener=0.01, napodize=6)
+        return p, otf, raw
+    if name == "z14":  # axial length with a factor 7
+        p, otf, raw, _ = synthetic(64, 64, 14, nphases=5)
+        return p, otf, raw
+    if name == "z21zz":  # nz 21, z_zoom 2: assembly depth 42
+        p, otf, raw, _ = synthetic(48, 64, 21, nphases=5, z_zoom=2)
         return p, otf, raw
     if name == "opts2d":  # non-square 2D, gamma apodization 0.8, suppression radius 5
         p, otf, raw, _ = synthetic(120, 100, 1, nphases=3, apodizeoutput=2, apoGamma=0.8, suppression_radius=5,

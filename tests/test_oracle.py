@@ -15,7 +15,8 @@ from oracle import binding as ob
 
 
 # ------------------------------------------------------------------ FFT
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 24, 25, 27, 40, 48, 64, 100, 384, 640, 1024])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16, 21, 24, 25, 27, 40, 48, 49, 64, 100, 384,
+                               640, 1024])
 def test_fft_matches_numpy_1d(n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
