@@ -162,9 +162,10 @@ Layout Layout::of(const simr_params& p) {
   L.otfsize = (size_t)p.nzotf * p.nxotf * p.nyotf;
   SIMR_REQUIRE(in_list_xy(p.nx), "image width " + std::to_string(p.nx) + " is not a supported size");
   SIMR_REQUIRE(fft_size_supported(p.ny) && p.ny <= 4096, "image height " + std::to_string(p.ny) + " is not a supported size");
-  SIMR_REQUIRE(in_list_z(p.nz), "stack depth " + std::to_string(p.nz) + " is not a supported size");
+  // depths outside the transform-kernel list take the direct-DFT z passes
+  SIMR_REQUIRE(p.nz >= 1 && p.nz <= 512, "stack depth " + std::to_string(p.nz) + " is not a supported size (1 to 512)");
   SIMR_REQUIRE(in_list_zoom(L.xdim) && in_list_zoom(L.ydim), "zoomed lateral size is not a supported size");
-  SIMR_REQUIRE(in_list_z(L.zdim), "zoomed stack depth is not a supported size");
+  SIMR_REQUIRE(L.zdim >= 1 && L.zdim <= 1024, "zoomed stack depth is not a supported size (1 to 1024)");
   SIMR_REQUIRE(2 * L.zd_ov_max + 1 <= p.nz, "overlap z support wider than the stack");
   const int ncand = kMaxCand;
   size_t b = 0;
@@ -205,8 +206,9 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   char* fit0 = (char*)arena_.take(fitb);
   const size_t zb = align256((size_t)L.nasm * L.zdim * L.nxy * 8);
   const size_t yb = align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * L.nx * 8);
-  char* stage = (char*)arena_.take(std::max(std::max(zb + yb, (size_t)(L.ndirs - 1) * fitb),
-                                            L.resample ? (size_t)L.nimg * (size_t)L.nxy * 4 : (size_t)0));
+  stage_bytes_ = std::max(std::max(zb + yb, (size_t)(L.ndirs - 1) * fitb),
+                          L.resample ? (size_t)L.nimg * (size_t)L.nxy * 4 : (size_t)0);
+  char* stage = (char*)arena_.take(stage_bytes_);
   stageZ_ = (float2*)stage;
   resampled_ = (float*)stage;
   stageY_ = (float2*)(stage + zb);
@@ -512,8 +514,24 @@ void Engine::load_and_rescale(int it) {
   stage_end(cur_stage_, stream_);
   if (L.nz > 1) {
     cur_stage_ = stage_begin("cols_fwd_z", stream_);
-    fft_cols(L.nz, bands_, (long long)L.nxh * L.ny, (long long)L.nxh * L.ny, (long long)L.ndirs * L.nbands, L.H,
-             false, stream_);
+    const long long plane = (long long)L.nxh * L.ny;
+    if (in_list_z(L.nz)) {
+      fft_cols(L.nz, bands_, plane, plane, (long long)L.ndirs * L.nbands, L.H, false, stream_);
+    } else {
+      // depths without a transform kernel: direct DFT along z (double
+      // accumulation), (direction, band) volumes in chunks through the
+      // assembly staging region, which is idle until the fit and the assembly
+      const long long vols = (long long)L.ndirs * L.nbands;
+      const long long chunk =
+          std::max<long long>(1, std::min<long long>(vols, (long long)(stage_bytes_ / ((size_t)L.H * 8))));
+      SIMR_REQUIRE((size_t)L.H * 8 <= stage_bytes_, "staging region smaller than one band volume");
+      for (long long v0 = 0; v0 < vols; v0 += chunk) {
+        const long long nv = std::min(chunk, vols - v0);
+        float2* src = bands_ + v0 * L.H;
+        dft_lines(src, stageZ_, L.nz, plane, plane, L.H, nv, false, stream_);
+        SIMR_HIP(hipMemcpyAsync(src, stageZ_, (size_t)nv * L.H * 8, hipMemcpyDeviceToDevice, stream_));
+      }
+    }
     stage_end(cur_stage_, stream_);
   }
   preprocessed_ = true;

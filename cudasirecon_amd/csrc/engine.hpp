@@ -154,6 +154,7 @@ class Engine {
   float* out_ = nullptr;
 
   size_t hstage_bytes_ = 0;
+  size_t stage_bytes_ = 0;  // bytes of the assembly staging region
 
   // reconstruction state
   int cur_t_ = 0;

@@ -622,6 +622,65 @@ static bool launch_asm_z_pair(const AsmParams& p, hipStream_t s) {
   return false;
 }
 
+// ---- z pass for depths without a transform kernel (any zdim): the
+// move_kernel gather of k_asm_z, one element per thread, into dst
+// [a][zout][y][x] (the y-pass staging buffer, idle until the y pass), then a
+// direct inverse DFT along z (double accumulation) from dst into stageZ.
+__global__ __launch_bounds__(256) void k_asm_z_gather(AsmParams p, int N, float2* __restrict__ dst) {
+#pragma clang fp contract(off)
+  const long long nxy = (long long)p.nx * p.ny;
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (long long)N * nxy) return;
+  const int a = blockIdx.y;
+  const int zout = (int)(e / nxy);
+  const long long r = e - (long long)zout * nxy;
+  const int yc = (int)(r / p.nx), xc = (int)(r - (long long)yc * p.nx);
+  bool valid;
+  int z = z_of_zout(zout, p.nz, N, valid);
+  int x = xc - (p.nx / 2 - 1), yy = yc - (p.ny / 2 - 1);
+  const bool cj = x < 0;
+  if (cj) {
+    x = -x;
+    yy = -yy;
+    z = -z;
+  }
+  if (yy < 0) yy += p.ny;
+  if (z < 0) z += p.nz;
+  float2 v = make_float2(0.f, 0.f);
+  if (valid) {
+    const long long ind = ((long long)z * p.ny + yy) * p.nxh + x;
+    const long long dirstride = (long long)p.nbands * p.band_stride;
+    if (a == 0) {  // band 0: sum over directions
+      for (int d = 0; d < p.ndirs; ++d) {
+        const float2 b = p.bands[d * dirstride + ind];
+        v.x += b.x;
+        v.y += b.y;
+      }
+      if (cj) v.y = -v.y;
+    } else {  // side band: re + i im of the order's two real-space bands
+      const int no = p.norders - 1;
+      const int dir = (a - 1) / no, order = 1 + (a - 1) % no;
+      float2 re = p.bands[dir * dirstride + (2 * order - 1) * p.band_stride + ind];
+      float2 im = p.bands[dir * dirstride + (2 * order) * p.band_stride + ind];
+      if (cj) {
+        re.y = -re.y;
+        im.y = -im.y;
+      }
+      v = make_float2(re.x - im.y, re.y + im.x);
+    }
+  }
+  dst[((long long)a * N + zout) * nxy + r] = v;
+}
+
+static void launch_asm_z_any(const AsmParams& p, hipStream_t s) {
+  const long long nxy = (long long)p.nx * p.ny;
+  const long long n = (long long)p.zdim * nxy;
+  if (ceil_div(n, 256) > 2147483647LL || p.nasm > 65535) throw ArgError("assembly z pass: stack too large");
+  hipLaunchKernelGGL(k_asm_z_gather, dim3((unsigned)ceil_div(n, 256), p.nasm), dim3(256), 0, s, p, p.zdim, p.stageY);
+  SIMR_LAUNCH_CHECK();
+  dft_lines(p.stageY, p.stageZ, p.zdim, nxy, nxy, n, p.nasm, true, s);
+}
+
 void launch_asm_z(const AsmParams& p, hipStream_t s) {
   switch (p.zdim) {
 #define SIMR_CASE(N)                                                                              \
@@ -636,7 +695,7 @@ void launch_asm_z(const AsmParams& p, hipStream_t s) {
     SIMR_Z_SIZES(SIMR_CASE)
 #undef SIMR_CASE
     default:
-      throw ArgError("zoomed stack depth " + std::to_string(p.zdim) + " is not a supported size");
+      launch_asm_z_any(p, s);
   }
 }
 

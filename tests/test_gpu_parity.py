@@ -60,7 +60,7 @@ def _oracle_preprocessed(p, otf, raw, prec=32, fit64=False):
 
 
 def _cases():
-    return ["testdata", "syn3d", "syn2d", "bessel3d", "opts3d", "opts2d", "z14", "z21zz"]
+    return ["testdata", "syn3d", "syn2d", "bessel3d", "opts3d", "opts2d", "z14", "z21zz", "z11", "z13zz"]
 
 
 def _case(name):
@@ -82,6 +82,12 @@ def _case(name):
         return p, otf, raw
     if name == "z21zz":  # nz 21, z_zoom 2: assembly depth 42
         p, otf, raw, _ = synthetic(48, 64, 21, nphases=5, z_zoom=2)
+        return p, otf, raw
+    if name == "z11":  # prime depth: the direct-DFT z passes
+        p, otf, raw, _ = synthetic(64, 64, 11, nphases=5)
+        return p, otf, raw
+    if name == "z13zz":  # prime depth 13 with z_zoom 2: assembly depth 26, both z passes by direct DFT
+        p, otf, raw, _ = synthetic(48, 48, 13, nphases=5, z_zoom=2)
         return p, otf, raw
     if name == "opts2d":  # non-square 2D, gamma apodization 0.8, suppression radius 5
         p, otf, raw, _ = synthetic(120, 100, 1, nphases=3, apodizeoutput=2, apoGamma=0.8, suppression_radius=5,
