@@ -518,8 +518,8 @@ void Engine::load_and_rescale(int it) {
     if (in_list_z(L.nz)) {
       fft_cols(L.nz, bands_, plane, plane, (long long)L.ndirs * L.nbands, L.H, false, stream_);
     } else {
-      // depths without a transform kernel: direct DFT along z (double
-      // accumulation), (direction, band) volumes in chunks through the
+      // depths without a transform kernel: direct DFT along z (dft_cols),
+      // (direction, band) volumes in chunks through the
       // assembly staging region, which is idle until the fit and the assembly
       const long long vols = (long long)L.ndirs * L.nbands;
       const long long chunk =
@@ -528,7 +528,7 @@ void Engine::load_and_rescale(int it) {
       for (long long v0 = 0; v0 < vols; v0 += chunk) {
         const long long nv = std::min(chunk, vols - v0);
         float2* src = bands_ + v0 * L.H;
-        dft_lines(src, stageZ_, L.nz, plane, plane, L.H, nv, false, stream_);
+        dft_cols(src, stageZ_, L.nz, plane, plane, L.H, nv, false, stream_);
         SIMR_HIP(hipMemcpyAsync(src, stageZ_, (size_t)nv * L.H * 8, hipMemcpyDeviceToDevice, stream_));
       }
     }

@@ -22,6 +22,11 @@ void fft3d(float2* data, int nx, int ny, int nz, bool inverse, hipStream_t s);
 // o*outer_stride + n*n_stride + i, i < ninner, o < nouter; out-of-place.
 void dft_lines(const float2* in, float2* out, int n, long long n_stride, long long ninner, long long outer_stride,
                long long nouter, bool inverse, hipStream_t s);
+// Direct DFT along a strided axis over ninner contiguous columns per batch
+// (nouter batches <= 65535), out of place, float accumulation, any length:
+// the z passes for depths without a transform kernel.
+void dft_cols(const float2* in, float2* out, int n, long long n_stride, long long ninner, long long outer_stride,
+              long long nouter, bool inverse, hipStream_t s);
 // 3D transform of any size: Stockham passes where the length is supported,
 // direct DFT passes otherwise (scratch: nx*ny*nz elements).
 void fft3d_any(float2* data, float2* scratch, int nx, int ny, int nz, bool inverse, hipStream_t s);

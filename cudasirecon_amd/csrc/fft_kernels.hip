@@ -357,6 +357,63 @@ void dft_lines(const float2* in, float2* out, int n, long long n_stride, long lo
   SIMR_LAUNCH_CHECK();
 }
 
+// Direct DFT along a strided axis for lengths without a transform kernel,
+// over contiguous columns: thread i of batch o transforms column i, KB
+// outputs at a time with the accumulators in registers, so each input is read
+// once per block of KB outputs; the twiddles exp(-+2 pi i m / N) come from an
+// LDS table, indexed by k z mod N stepped incrementally (the same index for
+// every lane: a broadcast read).  Out of place.
+constexpr int kDftColsKB = 16;
+__global__ __launch_bounds__(256) void k_dft_cols(const float2* __restrict__ in, float2* __restrict__ out, int N,
+                                                  long long sn, long long ninner, long long so,
+                                                  const float2* __restrict__ tw, int inverse) {
+  extern __shared__ float2 tws[];
+  for (int m = threadIdx.x; m < N; m += 256) {
+    float2 w = tw[m];
+    if (inverse) w.y = -w.y;
+    tws[m] = w;
+  }
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= ninner) return;
+  const float2* src = in + (long long)blockIdx.y * so + i;
+  float2* dst = out + (long long)blockIdx.y * so + i;
+  for (int k0 = 0; k0 < N; k0 += kDftColsKB) {
+    float2 acc[kDftColsKB];
+    int m[kDftColsKB], step[kDftColsKB];
+#pragma unroll
+    for (int j = 0; j < kDftColsKB; ++j) {
+      acc[j] = make_float2(0.f, 0.f);
+      m[j] = 0;
+      step[j] = k0 + j < N ? k0 + j : 0;  // outputs past N stay at index 0 and are not stored
+    }
+    for (int z = 0; z < N; ++z) {
+      const float2 x = src[(long long)z * sn];
+#pragma unroll
+      for (int j = 0; j < kDftColsKB; ++j) {
+        const float2 w = tws[m[j]];
+        acc[j].x += x.x * w.x - x.y * w.y;
+        acc[j].y += x.x * w.y + x.y * w.x;
+        m[j] += step[j];
+        if (m[j] >= N) m[j] -= N;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kDftColsKB; ++j)
+      if (k0 + j < N) dst[(long long)(k0 + j) * sn] = acc[j];
+  }
+}
+
+void dft_cols(const float2* in, float2* out, int n, long long n_stride, long long ninner, long long outer_stride,
+              long long nouter, bool inverse, hipStream_t s) {
+  if (ninner <= 0 || nouter <= 0 || n <= 0) return;
+  if (nouter > 65535 || ceil_div(ninner, 256) > 2147483647LL) throw ArgError("dft_cols: problem too large");
+  hipLaunchKernelGGL(k_dft_cols, dim3((unsigned)ceil_div(ninner, 256), (unsigned)nouter), dim3(256),
+                     (size_t)n * sizeof(float2), s, in, out, n, n_stride, ninner, outer_stride, twiddles(n),
+                     inverse ? 1 : 0);
+  SIMR_LAUNCH_CHECK();
+}
+
 void fft3d_any(float2* data, float2* scratch, int nx, int ny, int nz, bool inverse, hipStream_t s) {
   const long long nxy = (long long)nx * ny, n = nxy * nz;
   auto dft_pass = [&](int len, long long sn, long long ninner, long long so, long long nouter) {

@@ -625,7 +625,7 @@ static bool launch_asm_z_pair(const AsmParams& p, hipStream_t s) {
 // ---- z pass for depths without a transform kernel (any zdim): the
 // move_kernel gather of k_asm_z, one element per thread, into dst
 // [a][zout][y][x] (the y-pass staging buffer, idle until the y pass), then a
-// direct inverse DFT along z (double accumulation) from dst into stageZ.
+// direct inverse DFT along z (dft_cols) from dst into stageZ.
 __global__ __launch_bounds__(256) void k_asm_z_gather(AsmParams p, int N, float2* __restrict__ dst) {
 #pragma clang fp contract(off)
   const long long nxy = (long long)p.nx * p.ny;
@@ -678,7 +678,7 @@ static void launch_asm_z_any(const AsmParams& p, hipStream_t s) {
   if (ceil_div(n, 256) > 2147483647LL || p.nasm > 65535) throw ArgError("assembly z pass: stack too large");
   hipLaunchKernelGGL(k_asm_z_gather, dim3((unsigned)ceil_div(n, 256), p.nasm), dim3(256), 0, s, p, p.zdim, p.stageY);
   SIMR_LAUNCH_CHECK();
-  dft_lines(p.stageY, p.stageZ, p.zdim, nxy, nxy, n, p.nasm, true, s);
+  dft_cols(p.stageY, p.stageZ, p.zdim, nxy, nxy, n, p.nasm, true, s);
 }
 
 void launch_asm_z(const AsmParams& p, hipStream_t s) {
