@@ -138,15 +138,20 @@ __global__ __launch_bounds__(256) void k_asm_z(AsmParams p, const float2* __rest
 // transformed separately: direct R + iI, mirror conj(R) + i conj(I).
 // Band 0 (sum over directions): direct B, mirror conj(B).
 // threads per workgroup of the paired z pass: W = threads / (slots * TPT)
-// stored columns, so each plane's read of a stored row is W * 8 bytes
-constexpr int kAsmZpThreads = 512;
+// stored columns, so each plane's read of a stored row is W * 8 bytes.
+// Side bands (four slots) take 512 threads, eight columns; band 0 (two
+// slots) 256 threads, also eight columns.
+template <bool B0>
+constexpr int asm_zp_threads() {
+  return B0 ? 256 : 512;
+}
 template <int N, bool B0>
 constexpr int asm_zp_slots() {
   return B0 ? 2 : 4;  // transforms per stored column: rows s, -s (x re, im)
 }
 template <int N, bool B0>
 constexpr int asm_zp_w() {
-  return kAsmZpThreads / (asm_zp_slots<N, B0>() * fft_tpt(N));
+  return asm_zp_threads<B0>() / (asm_zp_slots<N, B0>() * fft_tpt(N));
 }
 template <int N>
 constexpr bool asm_zp_ok() {
@@ -154,12 +159,12 @@ constexpr bool asm_zp_ok() {
 }
 
 template <int N, bool B0>
-__global__ __launch_bounds__(kAsmZpThreads) void k_asm_z_pair(AsmParams p, const float2* __restrict__ tw) {
+__global__ __launch_bounds__(asm_zp_threads<B0>()) void k_asm_z_pair(AsmParams p, const float2* __restrict__ tw) {
   constexpr int TPT = fft_tpt(N);
   constexpr int S = asm_zp_slots<N, B0>();
   constexpr int W = asm_zp_w<N, B0>();
   constexpr int NT = S * W * TPT;
-  static_assert(NT == kAsmZpThreads, "one transform slot per TPT threads");
+  static_assert(NT == asm_zp_threads<B0>(), "one transform slot per TPT threads");
   constexpr int LP = lds_pitch(N);
   __shared__ float2 lds[S * W * LP];
   FftTw<N> twr;
@@ -612,11 +617,11 @@ static bool launch_asm_z_pair(const AsmParams& p, hipStream_t s) {
     if (p.zdim != p.nz || p.nz == 1 || (p.nx & 1) || (p.ny & 1)) return false;
     const int nstored = p.nx / 2 + 1;
     hipLaunchKernelGGL((k_asm_z_pair<N, true>), dim3(ceil_div(nstored, asm_zp_w<N, true>()), p.ny / 2 + 1, 1),
-                       dim3(kAsmZpThreads), 0, s, p, twiddles(N));
+                       dim3(asm_zp_threads<true>()), 0, s, p, twiddles(N));
     SIMR_LAUNCH_CHECK();
     if (p.nasm > 1) {
       hipLaunchKernelGGL((k_asm_z_pair<N, false>),
-                         dim3(ceil_div(nstored, asm_zp_w<N, false>()), p.ny / 2 + 1, p.nasm - 1), dim3(kAsmZpThreads), 0, s, p,
+                         dim3(ceil_div(nstored, asm_zp_w<N, false>()), p.ny / 2 + 1, p.nasm - 1), dim3(asm_zp_threads<false>()), 0, s, p,
                          twiddles(N));
       SIMR_LAUNCH_CHECK();
     }
