@@ -407,8 +407,9 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="stacks in flight in the batched/serving forms")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; the multi-GPU sharding is described in DESIGN.md "
-              "(--gpus %d not supported)" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU (--gpus %d not supported): stacks are independent and would shard one "
+              "process per GPU, and within a stack by SIM direction with one reduce of the output volume"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from cudasirecon_amd import engine, synth
