@@ -64,6 +64,26 @@ simr::Engine& E(simr_ctx* c) {
 }
 }  // namespace
 
+namespace {
+void check_fft_dims(const float* data, int nx, int ny, int nz) {
+  if (!data) throw simr::ArgError("FFT data pointer is null");
+  if (nx < 1 || ny < 1 || nz < 1 || nx > 4096 || ny > 4096 || nz > 4096)
+    throw simr::ArgError("FFT lengths must be 1 to 4096");
+}
+
+// Device memory that stays allocated until the scope ends (the synchronous
+// entry points' staging and direct-DFT scratch).
+struct DevMem {
+  void* p = nullptr;
+  explicit DevMem(size_t bytes) {
+    if (bytes) SIMR_HIP(hipMalloc(&p, bytes));
+  }
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+};
+}  // namespace
+
 extern "C" {
 
 void simr_set_last_error(const char* msg) { g_err = msg ? msg : ""; }
@@ -259,28 +279,29 @@ int simr_fft_size_supported(int n) { return simr::fft_size_supported(n) ? 1 : 0;
 
 int simr_fft3d_dev(float* data, int nx, int ny, int nz, int inverse, void* stream) {
   return guard([&] {
-    if (!simr::fft_size_supported(nx) || !simr::fft_size_supported(ny) || !simr::fft_size_supported(nz))
-      throw simr::ArgError("unsupported FFT size");
-    simr::fft3d((float2*)data, nx, ny, nz, inverse != 0, (hipStream_t)stream);
+    check_fft_dims(data, nx, ny, nz);
+    hipStream_t s = (hipStream_t)stream;
+    if (!simr::fft3d_needs_scratch(nx, ny, nz)) {
+      simr::fft3d((float2*)data, nx, ny, nz, inverse != 0, s);
+      return;
+    }
+    // a length without a transform kernel: direct-DFT passes through a
+    // scratch volume allocated for the call; the call then waits for them
+    DevMem scratch((size_t)nx * ny * nz * sizeof(float2));
+    simr::fft3d_any((float2*)data, (float2*)scratch.p, nx, ny, nz, inverse != 0, s);
+    SIMR_HIP(hipStreamSynchronize(s));
   });
 }
 
 int simr_fft3d_host(float* data, int nx, int ny, int nz, int inverse) {
   return guard([&] {
-    if (!simr::fft_size_supported(nx) || !simr::fft_size_supported(ny) || !simr::fft_size_supported(nz))
-      throw simr::ArgError("unsupported FFT size");
-    const size_t bytes = (size_t)nx * ny * nz * 8;
-    float2* d = nullptr;
-    SIMR_HIP(hipMalloc(&d, bytes));
-    try {
-      SIMR_HIP(hipMemcpy(d, data, bytes, hipMemcpyHostToDevice));
-      simr::fft3d(d, nx, ny, nz, inverse != 0, nullptr);
-      SIMR_HIP(hipMemcpy(data, d, bytes, hipMemcpyDeviceToHost));
-    } catch (...) {
-      (void)hipFree(d);
-      throw;
-    }
-    SIMR_HIP(hipFree(d));
+    check_fft_dims(data, nx, ny, nz);
+    const size_t bytes = (size_t)nx * ny * nz * sizeof(float2);
+    DevMem d(bytes);
+    DevMem scratch(simr::fft3d_needs_scratch(nx, ny, nz) ? bytes : 0);
+    SIMR_HIP(hipMemcpy(d.p, data, bytes, hipMemcpyHostToDevice));
+    simr::fft3d_any((float2*)d.p, (float2*)scratch.p, nx, ny, nz, inverse != 0, nullptr);
+    SIMR_HIP(hipMemcpy(data, d.p, bytes, hipMemcpyDeviceToHost));
   });
 }
 

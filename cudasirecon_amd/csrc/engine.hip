@@ -192,6 +192,10 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   SIMR_HIP(hipSetDevice(device_));
   SIMR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   fit_fp32_ = P.fit_fp64 == 0;
+  // every twiddle table this context's transforms use (rows, columns, z, the
+  // zoomed assembly lengths), so that no launch path allocates one
+  for (int n : {L.nx, L.ny, L.nz, L.xdim, L.ydim, L.zdim})
+    if (n > 1) (void)twiddles(n);
   arena_.reserve(L.bytes_total);
   raw_ = arena_.take((size_t)L.nimg * L.nxy_raw * 4);
   bg_ = (float*)arena_.take((size_t)L.nxy_raw * 4);
@@ -952,6 +956,17 @@ void Engine::fitk0andmodamps(FitWork& w, int dir) {
   }
 }
 
+// A non-finite k0 (the reference carries it on into NaN output) only comes
+// from non-finite or empty input: a NaN/Inf pixel spreads through the
+// transforms to every overlap product, and a stack with no modulated signal
+// makes the modamp ratios 0/0.  It is reported here, before any kernel takes
+// it as a frequency shift.
+void Engine::require_finite_k0(int dir, const char* where) const {
+  if (std::isfinite(k0_[dir].x) && std::isfinite(k0_[dir].y)) return;
+  throw ArgError(std::string(where) + " of direction " + std::to_string(dir) +
+                 " gave a non-finite modulation vector: the stack holds NaN/Inf pixels or no modulated signal");
+}
+
 static float2 cmulh(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 static float cmagh(float2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
 
@@ -962,7 +977,9 @@ void Engine::modulation(FitWork& w, int dir) {
   if (!reuse) k0_[dir] = k0guess_[dir];
   if (P.bSearchforvector && !reuse) {
     findk0(w, dir);
+    require_finite_k0(dir, "the k0 search");
     fitk0andmodamps(w, dir);
+    require_finite_k0(dir, "the k0 fit");
     if (cur_t_ == 0) k0_time0_[dir] = k0_[dir];
     const float dx = k0_[dir].x - k0guess_[dir].x, dy = k0_[dir].y - k0guess_[dir].y;
     const float dist = sqrtf(dx * dx + dy * dy);
@@ -1214,6 +1231,7 @@ simr_fit Engine::fit(int dir) const {
 
 void Engine::set_fit(int dir, const simr_fit& f) {
   SIMR_REQUIRE(dir >= 0 && dir < L.ndirs, "direction out of range");
+  SIMR_REQUIRE(std::isfinite(f.k0x) && std::isfinite(f.k0y), "k0 must be finite");
   k0_[dir] = make_float2(f.k0x, f.k0y);
   for (int o = 0; o < L.norders; ++o) amp_[dir][o] = make_float2(f.amp_re[o], f.amp_im[o]);
   fitted_ = true;

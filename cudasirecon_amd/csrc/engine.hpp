@@ -126,6 +126,7 @@ class Engine {
   void findk0(FitWork& w, int dir);
   void fitk0andmodamps(FitWork& w, int dir);
   void modulation(FitWork& w, int dir);
+  void require_finite_k0(int dir, const char* where) const;
   FilterParams filter_params(int dir, int order) const;
   void assemble();
   int stage_begin(const char* name, hipStream_t s);

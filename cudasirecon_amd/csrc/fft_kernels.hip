@@ -37,6 +37,22 @@ const float2* twiddles(int N) {
   return d;
 }
 
+// exp(-2 pi i m / n) in double (sincospi: the argument is an exact multiple
+// of pi up to the rounding of 2m/n), rounded to float.
+__global__ __launch_bounds__(256) void k_twiddle_table(float2* __restrict__ out, int n) {
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= n) return;
+  double sn, cs;
+  sincospi(-2.0 * (double)m / (double)n, &sn, &cs);
+  out[m] = make_float2((float)cs, (float)sn);
+}
+
+void twiddles_into(float2* out, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_twiddle_table, dim3(ceil_div(n, 256)), dim3(256), 0, s, out, n);
+  SIMR_LAUNCH_CHECK();
+}
+
 // W transforms per workgroup so that a workgroup has at most 1024 threads and
 // at most 80 KB of LDS (two workgroups per CU within gfx950's 160 KB).
 template <int N>
@@ -226,18 +242,17 @@ __global__ __launch_bounds__(W* fft2_tpt(N)) void k_fft2_rows(float2* __restrict
 
 template <int N>
 static void launch_cols(float2* data, long long pitch, long long ncols, long long nplanes,
-                        long long plane_stride, bool inverse, hipStream_t s) {
+                        long long plane_stride, bool inverse, hipStream_t s, const float2* tw) {
+  if (!tw) tw = twiddles(N);
   if constexpr (use_fft2<N>()) {
     constexpr int W = tile_w2<N>();
     if constexpr (W * fft2_tpt(N) % 64 == 0) {
       dim3 grid(ceil_div(ncols, W), (unsigned)nplanes);
       dim3 block(W * fft2_tpt(N));
       if (inverse)
-        hipLaunchKernelGGL((k_fft2_cols<N, true, W>), grid, block, 0, s, data, pitch, ncols, plane_stride,
-                           twiddles(N));
+        hipLaunchKernelGGL((k_fft2_cols<N, true, W>), grid, block, 0, s, data, pitch, ncols, plane_stride, tw);
       else
-        hipLaunchKernelGGL((k_fft2_cols<N, false, W>), grid, block, 0, s, data, pitch, ncols, plane_stride,
-                           twiddles(N));
+        hipLaunchKernelGGL((k_fft2_cols<N, false, W>), grid, block, 0, s, data, pitch, ncols, plane_stride, tw);
       SIMR_LAUNCH_CHECK();
       return;
     }
@@ -246,23 +261,25 @@ static void launch_cols(float2* data, long long pitch, long long ncols, long lon
   dim3 grid(ceil_div(ncols, W), (unsigned)nplanes);
   dim3 block(W * fft_tpt(N));
   if (inverse)
-    hipLaunchKernelGGL((k_fft_cols<N, true, W>), grid, block, 0, s, data, pitch, ncols, plane_stride, twiddles(N));
+    hipLaunchKernelGGL((k_fft_cols<N, true, W>), grid, block, 0, s, data, pitch, ncols, plane_stride, tw);
   else
-    hipLaunchKernelGGL((k_fft_cols<N, false, W>), grid, block, 0, s, data, pitch, ncols, plane_stride, twiddles(N));
+    hipLaunchKernelGGL((k_fft_cols<N, false, W>), grid, block, 0, s, data, pitch, ncols, plane_stride, tw);
   SIMR_LAUNCH_CHECK();
 }
 
 template <int N>
-static void launch_rows(float2* data, long long pitch, long long nrows, bool inverse, hipStream_t s) {
+static void launch_rows(float2* data, long long pitch, long long nrows, bool inverse, hipStream_t s,
+                        const float2* tw) {
+  if (!tw) tw = twiddles(N);
   if constexpr (use_fft2<N>()) {
     constexpr int W = tile_w2<N>();
     if constexpr (W * fft2_tpt(N) % 64 == 0) {
       dim3 grid(ceil_div(nrows, W));
       dim3 block(W * fft2_tpt(N));
       if (inverse)
-        hipLaunchKernelGGL((k_fft2_rows<N, true, W>), grid, block, 0, s, data, pitch, nrows, twiddles(N));
+        hipLaunchKernelGGL((k_fft2_rows<N, true, W>), grid, block, 0, s, data, pitch, nrows, tw);
       else
-        hipLaunchKernelGGL((k_fft2_rows<N, false, W>), grid, block, 0, s, data, pitch, nrows, twiddles(N));
+        hipLaunchKernelGGL((k_fft2_rows<N, false, W>), grid, block, 0, s, data, pitch, nrows, tw);
       SIMR_LAUNCH_CHECK();
       return;
     }
@@ -271,9 +288,9 @@ static void launch_rows(float2* data, long long pitch, long long nrows, bool inv
   dim3 grid(ceil_div(nrows, W));
   dim3 block(W * fft_tpt(N));
   if (inverse)
-    hipLaunchKernelGGL((k_fft_rows<N, true, W>), grid, block, 0, s, data, pitch, nrows, twiddles(N));
+    hipLaunchKernelGGL((k_fft_rows<N, true, W>), grid, block, 0, s, data, pitch, nrows, tw);
   else
-    hipLaunchKernelGGL((k_fft_rows<N, false, W>), grid, block, 0, s, data, pitch, nrows, twiddles(N));
+    hipLaunchKernelGGL((k_fft_rows<N, false, W>), grid, block, 0, s, data, pitch, nrows, tw);
   SIMR_LAUNCH_CHECK();
 }
 
@@ -292,13 +309,13 @@ bool fft_size_supported(int n) {
 }
 
 void fft_cols(int n, float2* data, long long pitch, long long ncols, long long nplanes, long long plane_stride,
-              bool inverse, hipStream_t s) {
+              bool inverse, hipStream_t s, const float2* tw) {
   if (n == 1 || ncols <= 0 || nplanes <= 0) return;
   if (nplanes > 65535) throw ArgError("fft_cols: too many planes");
   switch (n) {
 #define SIMR_CASE(N)                                                          \
   case N:                                                                     \
-    launch_cols<N>(data, pitch, ncols, nplanes, plane_stride, inverse, s);    \
+    launch_cols<N>(data, pitch, ncols, nplanes, plane_stride, inverse, s, tw); \
     return;
     SIMR_FFT_SIZES(SIMR_CASE)
 #undef SIMR_CASE
@@ -307,12 +324,13 @@ void fft_cols(int n, float2* data, long long pitch, long long ncols, long long n
   }
 }
 
-void fft_rows(int n, float2* data, long long pitch, long long nrows, bool inverse, hipStream_t s) {
+void fft_rows(int n, float2* data, long long pitch, long long nrows, bool inverse, hipStream_t s,
+              const float2* tw) {
   if (n == 1 || nrows <= 0) return;
   switch (n) {
 #define SIMR_CASE(N)                                     \
   case N:                                                \
-    launch_rows<N>(data, pitch, nrows, inverse, s);      \
+    launch_rows<N>(data, pitch, nrows, inverse, s, tw);  \
     return;
     SIMR_FFT_SIZES(SIMR_CASE)
 #undef SIMR_CASE
@@ -348,12 +366,12 @@ __global__ __launch_bounds__(256) void k_dft_lines(const float2* __restrict__ in
 }
 
 void dft_lines(const float2* in, float2* out, int n, long long n_stride, long long ninner, long long outer_stride,
-               long long nouter, bool inverse, hipStream_t s) {
+               long long nouter, bool inverse, hipStream_t s, const float2* tw) {
   const long long total = ninner * n * nouter;
   if (total <= 0) return;
   if (ceil_div(total, 256) > 2147483647LL) throw ArgError("dft_lines: problem too large");
   hipLaunchKernelGGL(k_dft_lines, dim3(ceil_div(total, 256)), dim3(256), 0, s, in, out, n, n_stride, ninner,
-                     outer_stride, nouter, twiddles(n), inverse ? 1 : 0);
+                     outer_stride, nouter, tw ? tw : twiddles(n), inverse ? 1 : 0);
   SIMR_LAUNCH_CHECK();
 }
 
@@ -414,29 +432,39 @@ void dft_cols(const float2* in, float2* out, int n, long long n_stride, long lon
   SIMR_LAUNCH_CHECK();
 }
 
-void fft3d_any(float2* data, float2* scratch, int nx, int ny, int nz, bool inverse, hipStream_t s) {
+bool fft3d_needs_scratch(int nx, int ny, int nz) {
+  return (nx > 1 && !fft_size_supported(nx)) || (ny > 1 && !fft_size_supported(ny)) ||
+         (nz > 1 && !fft_size_supported(nz));
+}
+
+void fft3d_any(float2* data, float2* scratch, int nx, int ny, int nz, bool inverse, hipStream_t s,
+               const float2* const* tw) {
   const long long nxy = (long long)nx * ny, n = nxy * nz;
-  auto dft_pass = [&](int len, long long sn, long long ninner, long long so, long long nouter) {
-    dft_lines(data, scratch, len, sn, ninner, so, nouter, inverse, s);
+  const float2* twx = tw ? tw[0] : nullptr;
+  const float2* twy = tw ? tw[1] : nullptr;
+  const float2* twz = tw ? tw[2] : nullptr;
+  auto dft_pass = [&](int len, long long sn, long long ninner, long long so, long long nouter, const float2* t) {
+    if (!scratch) throw ArgError("fft3d_any: a direct-DFT pass needs scratch memory");
+    dft_lines(data, scratch, len, sn, ninner, so, nouter, inverse, s, t);
     SIMR_HIP(hipMemcpyAsync(data, scratch, (size_t)n * sizeof(float2), hipMemcpyDeviceToDevice, s));
   };
   if (nx > 1) {
     if (fft_size_supported(nx))
-      fft_rows(nx, data, nx, (long long)ny * nz, inverse, s);
+      fft_rows(nx, data, nx, (long long)ny * nz, inverse, s, twx);
     else
-      dft_pass(nx, 1, 1, nx, (long long)ny * nz);
+      dft_pass(nx, 1, 1, nx, (long long)ny * nz, twx);
   }
   if (ny > 1) {
     if (fft_size_supported(ny))
-      fft_cols(ny, data, nx, nx, nz, nxy, inverse, s);
+      fft_cols(ny, data, nx, nx, nz, nxy, inverse, s, twy);
     else
-      dft_pass(ny, nx, nx, nxy, nz);
+      dft_pass(ny, nx, nx, nxy, nz, twy);
   }
   if (nz > 1) {
     if (fft_size_supported(nz))
-      fft_cols(nz, data, nxy, nxy, 1, 0, inverse, s);
+      fft_cols(nz, data, nxy, nxy, 1, 0, inverse, s, twz);
     else
-      dft_pass(nz, nxy, nxy, 0, 1);
+      dft_pass(nz, nxy, nxy, 0, 1, twz);
   }
 }
 

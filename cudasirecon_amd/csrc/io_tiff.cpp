@@ -30,7 +30,8 @@ struct Cursor {
   const std::string& path;
   template <class T>
   T at(uint64_t off) const {
-    if (off + sizeof(T) > f.size()) throw FormatError("TIFF structure points past the end of " + path);
+    if (off > f.size() || sizeof(T) > f.size() - off)
+      throw FormatError("TIFF structure points past the end of " + path);
     T v;
     std::memcpy(&v, f.data() + off, sizeof(T));
     return swap ? bswap(v) : v;
@@ -200,9 +201,11 @@ Reader::Reader(const std::string& path) : path_(path) {
       const size_t ts = type_size(type);
       const uint64_t inline_cap = big_ ? 8 : 4;
       uint64_t voff = eo + (big_ ? 12 : 8);
+      // a count whose byte size exceeds the file cannot be valid (and ts * count could wrap)
+      if (ts != 0 && count > file_.size() / ts) throw FormatError("TIFF entry count past the end of " + path);
       if (ts * count > inline_cap) voff = big_ ? c.at<uint64_t>(voff) : c.at<uint32_t>(voff);
       if (tag == 270 && type == 2) {
-        if (voff + count <= file_.size())
+        if (voff <= file_.size() && count <= file_.size() - voff)
           p.description.assign((const char*)file_.data() + voff, (size_t)count);
         continue;
       }
@@ -272,7 +275,7 @@ void Reader::decode(int page, std::vector<unsigned char>& pix) const {
   std::vector<unsigned char> blk;
   for (size_t b = 0; b < p.offsets.size(); b++) {
     const uint64_t off = p.offsets[b], cnt = p.counts[b];
-    if (off + cnt > file_.size()) throw FormatError("TIFF data past the end of " + path_);
+    if (off > file_.size() || cnt > file_.size() - off) throw FormatError("TIFF data past the end of " + path_);
     const unsigned char* src = file_.data() + off;
     blk.clear();
     switch (p.compression) {

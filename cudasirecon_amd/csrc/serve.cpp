@@ -43,6 +43,9 @@ struct Key {
 struct Slot {
   simr_ctx* ctx = nullptr;
   Key key{};
+  // the key's setup, taken under the server lock when the slot is made: the
+  // request path reads it without the lock while setups may still be added
+  std::shared_ptr<const Setup> setup;
   size_t bytes = 0;
   bool busy = false;
   uint64_t last_use = 0;
@@ -73,7 +76,7 @@ struct simr_server {
   size_t budget = 0;
   std::mutex m;
   std::condition_variable cv;
-  std::vector<Setup> setups;
+  std::vector<std::shared_ptr<const Setup>> setups;  // read and grown under m only
   std::deque<Request*> queue;
   std::map<int64_t, std::unique_ptr<Request>> reqs;
   std::vector<std::unique_ptr<Slot>> slots;
@@ -86,8 +89,8 @@ struct simr_server {
   bool stop = false;
   std::vector<std::thread> workers;
 
-  simr_params params_of(const Key& k) const {
-    simr_params p = setups[k.setup].p;
+  static simr_params params_of(const Setup& su, const Key& k) {
+    simr_params p = su.p;
     p.nx = k.nx;
     p.ny = k.ny;
     p.nz = k.nz;
@@ -150,7 +153,7 @@ struct simr_server {
         }
         key_busy = key_busy || s->key == key;
       }
-      const simr_params p = params_of(key);
+      const simr_params p = params_of(*setups[key.setup], key);
       const size_t need = simr_footprint(&p);
       if (need == 0 || need > budget) {
         if (need == 0)
@@ -185,6 +188,7 @@ struct simr_server {
       if (used + need <= budget) {
         auto s = std::make_unique<Slot>();
         s->key = key;
+        s->setup = setups[key.setup];
         s->bytes = need;
         s->busy = true;
         used += need;
@@ -213,8 +217,8 @@ struct simr_server {
     };
     int rc;
     if (!s->ctx) {
-      const simr_params p = params_of(s->key);
-      rc = simr_create(&p, setups[s->key.setup].otf.data(), device, &s->ctx);
+      const simr_params p = params_of(*s->setup, s->key);
+      rc = simr_create(&p, s->setup->otf.data(), device, &s->ctx);
       if (rc) return fail(rc);
       std::lock_guard<std::mutex> lk(m);
       ++st.contexts_created;
@@ -254,7 +258,7 @@ struct simr_server {
     } else if ((rc = simr_sync(s->ctx))) {
       return fail(rc);
     }
-    const int ndirs = setups[s->key.setup].p.ndirs;
+    const int ndirs = s->setup->p.ndirs;
     for (int d = 0; d < ndirs && d < SIMR_MAX_DIRS; ++d)
       if ((rc = simr_get_fit(s->ctx, d, &info.fit[d]))) return fail(rc);
     info.status = SIMR_OK;
@@ -309,9 +313,9 @@ int simr_server_add_setup(simr_server* srv, const simr_params* p, const float* o
     simr_set_last_error("simr_server_add_setup: empty OTF for these parameters");
     return SIMR_ERR_ARG;
   }
-  Setup su;
-  su.p = *p;
-  su.otf.assign(otf, otf + 2 * n);
+  auto su = std::make_shared<Setup>();
+  su->p = *p;
+  su->otf.assign(otf, otf + 2 * n);
   std::lock_guard<std::mutex> lk(srv->m);
   srv->setups.push_back(std::move(su));
   *setup_id = (int)srv->setups.size() - 1;

@@ -393,6 +393,10 @@ __global__ __launch_bounds__(256) void k_argmax_final(const float2* __restrict__
   }
   if (threadIdx.x != 0) return;
   idx = si[0];
+  // No element won (every intensity NaN, e.g. from one NaN input pixel spread
+  // by the transforms): findpeak keeps its initial centre (0, 0)
+  // (gpuFunctionsImpl.cu:443-455).
+  if (idx < 0 || idx >= nx * ny) idx = 0;
   const int xc = idx % nx, yc = idx / nx;
   auto I = [&](int y, int x) { return intensity(cc[(long long)y * nx + x]); };
   vals[0] = I(yc, xc);

@@ -203,8 +203,13 @@ int simr_debug_overlap_volumes(simr_ctx* ctx, int dir, int order1, int order2, f
 /*
  * Batched complex FFT utility (the engine's own LDS Stockham kernels; replaces
  * the cufftPlan3d/cufftExecC2C pairs at src/cudaSirecon/gpuFunctionsImpl.cu:639-669).
- * data: interleaved complex float [nz][ny][nx], unnormalised, forward = exp(-i).
+ * data: interleaved complex float [nz][ny][nx], unnormalised, forward = exp(-i),
+ * every length 1 to 4096.  Lengths of the form 2^a 3^b 5^c (and 7) run on the
+ * transform kernels (simr_fft_size_supported); any other length runs as a
+ * direct DFT along that axis through a scratch volume of nx*ny*nz complex
+ * values that the library allocates for the call.
  * _dev runs in place on device memory on `stream` (NULL = default stream);
+ * with a direct-DFT length it also waits for the stream before returning.
  * _host copies to the current device, transforms and copies back.
  */
 int simr_fft3d_dev(float* data, int nx, int ny, int nz, int inverse, void* stream);

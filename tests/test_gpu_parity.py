@@ -60,7 +60,8 @@ def _oracle_preprocessed(p, otf, raw, prec=32, fit64=False):
 
 
 def _cases():
-    return ["testdata", "syn3d", "syn2d", "bessel3d", "opts3d", "opts2d", "z14", "z21zz", "z11", "z13zz"]
+    return ["testdata", "syn3d", "syn2d", "bessel3d", "opts3d", "opts2d", "z14", "z21zz", "z11", "z13zz", "z61",
+            "z131zz"]
 
 
 def _case(name):
@@ -88,6 +89,12 @@ def _case(name):
         return p, otf, raw
     if name == "z13zz":  # prime depth 13 with z_zoom 2: assembly depth 26, both z passes by direct DFT
         p, otf, raw, _ = synthetic(48, 48, 13, nphases=5, z_zoom=2)
+        return p, otf, raw
+    if name == "z61":  # prime depth above 32: both z passes by direct DFT
+        p, otf, raw, _ = synthetic(64, 64, 61, nphases=5)
+        return p, otf, raw
+    if name == "z131zz":  # prime depth 131 with z_zoom 2: assembly depth 262, above the transform kernels' 256
+        p, otf, raw, _ = synthetic(32, 32, 131, nphases=5, z_zoom=2)
         return p, otf, raw
     if name == "opts2d":  # non-square 2D, gamma apodization 0.8, suppression radius 5
         p, otf, raw, _ = synthetic(120, 100, 1, nphases=3, apodizeoutput=2, apoGamma=0.8, suppression_radius=5,
@@ -240,3 +247,31 @@ def test_end_to_end_matches_oracle(case):
     log("  fp32-fit: k0 err %.2e px, modamp err %.2e, rel-L2 vs FP32 oracle %.2e" % (k0e, ame, e))
     assert k0e < 1e-3 and ame < 1e-3, (k0e, ame)
     assert e < 1e-5, e
+
+
+def test_nan_pixel_reports_error_and_context_recovers():
+    """One NaN pixel in a float stack spreads through the transforms to every
+    cross-correlation value.  The peak search then keeps findpeak's initial
+    centre (gpuFunctionsImpl.cu:443-455) - the oracle's k0 turns NaN there -
+    and the engine reports the non-finite k0 as an error instead of taking it
+    into the overlap and filter kernels.  The same context then reconstructs a
+    clean stack as the oracle does."""
+    log("case: testdata with raw[7, 20, 30] = NaN, then the clean stack")
+    p, otf, raw = load_testdata()
+    bad = raw.astype(np.float32).copy()
+    bad[7, 20, 30] = np.nan
+    _, fbad = ob.reconstruct(p, otf, bad, 32, True)
+    assert not np.isfinite(fbad[0].k0x)
+    with _gpu(p, otf) as g:
+        g.set_raw(bad)
+        g.load_and_rescale(0)
+        g.set_time(0)
+        with pytest.raises(engine.SimrError, match="non-finite"):
+            g.process_volume()
+        g.set_raw(raw)
+        g.load_and_rescale(0)
+        g.set_time(0)
+        g.process_volume()
+        vol = g.result()
+    ref, _ = ob.reconstruct(p, otf, raw, 32, True)
+    assert rel_l2(vol, ref) < 1e-5

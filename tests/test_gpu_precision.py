@@ -3,14 +3,16 @@ FP64 k0/modamp fit and with an all-FP32 fit, against the FP64 CPU oracle and
 the simulation's ground truth, at low (peak 300 photons) and high (5000)
 SNR, with one OTF per direction.
 
-By default only the 128x128x16 stack runs (seconds).  SIMR_PRECISION_STUDY=1
-adds 384x384x40 and 640x640x24 (minutes, dominated by the FP64 oracle).
-Every case prints one line: k0 error (px) against the truth and the FP64
-oracle, the largest modamp difference from the FP64 oracle, and the output's
-relative L2 against the FP64 oracle, for both fits.
+Sizes 128x128x16, 384x384x40 and 640x640x24.  Every case prints one line:
+k0 error (px) against the truth and the FP64 oracle, the largest modamp
+difference from the FP64 oracle, and the output's relative L2 against the
+FP64 oracle, for both fits.  Both fits are also held to the parity bars
+against the FP32 oracle of their own precision plan (the FP64-fit GPU path
+against the FP32 oracle with FP64 fit reductions; the FP32-fit GPU path
+against the FP32 oracle with the reference's float reductions): output
+rel-L2 <= 1e-5, k0 within 1e-3 px, modamp within 1e-3.
 """
 import math
-import os
 import sys
 
 import numpy as np
@@ -22,9 +24,7 @@ from oracle import binding as ob
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(128, 128, 16)]
-if os.environ.get("SIMR_PRECISION_STUDY"):
-    SIZES += [(384, 384, 40), (640, 640, 24)]
+SIZES = [(128, 128, 16), (384, 384, 40), (640, 640, 24)]
 
 
 def log(*a):
@@ -67,7 +67,8 @@ def amp_diff(fits, ref_fits, norders):
 def test_precision_fp64_fit_vs_fp32_fit(shape, photons):
     nx, ny, nz = shape
     log("case: %dx%dx%d peak %g photons, otfPerAngle" % (nx, ny, nz, photons))
-    p, tab, raw, truth = synth.make_dataset(nx, ny, nz, seed=20265202 + nx + nz, photons=photons)
+    p, tab, raw, truth = synth.make_dataset(nx, ny, nz, seed=20265202 + nx + nz, photons=photons,
+                                            modamps=(0.8, 0.4), gpu=nx > 128)
     p, otf = per_angle(p, tab)
     o = ob.Oracle(p, otf, 64, True)
     o.load(raw)
@@ -80,9 +81,14 @@ def test_precision_fp64_fit_vs_fp32_fit(shape, photons):
     row = {}
     for fit64 in (True, False):
         out, fits = gpu_run(p, otf, raw, fit64)
+        r32, f32 = ob.reconstruct(p, otf, raw.astype(np.float32), 32, fit64)
         row[fit64] = dict(k0_truth=k0_err_px(fits, truth["k0"], p),
                           k0_oracle=k0_err_px(fits, [(f.k0x, f.k0y) for f in ref_fits], p),
-                          amp=amp_diff(fits, ref_fits, norders), l2=rel_l2(out, ref))
+                          amp=amp_diff(fits, ref_fits, norders), l2=rel_l2(out, ref),
+                          k0_o32=k0_err_px(fits, [(f.k0x, f.k0y) for f in f32], p),
+                          amp_o32=amp_diff(fits, f32, norders), l2_o32=rel_l2(out, r32))
+        log("  %s fit vs the FP32 oracle of the same plan: rel-L2 %.2e, k0 %.2e px, modamp %.2e"
+            % ("fp64" if fit64 else "fp32", row[fit64]["l2_o32"], row[fit64]["k0_o32"], row[fit64]["amp_o32"]))
     log("precision %dx%dx%d photons=%g oracle64 k0err_truth=%.2e px | fp64 fit: k0err_truth=%.2e k0err_oracle=%.2e "
         "amp_diff=%.2e relL2=%.2e | fp32 fit: k0err_truth=%.2e k0err_oracle=%.2e amp_diff=%.2e relL2=%.2e" % (
             nx, ny, nz, photons, k0_err_px(ref_fits, truth["k0"], p),
@@ -92,5 +98,10 @@ def test_precision_fp64_fit_vs_fp32_fit(shape, photons):
     assert row[True]["k0_oracle"] < 1e-3
     assert row[True]["amp"] < 1e-3
     assert row[True]["l2"] < 1e-4
-    # the FP32 fit is reported, not bounded as tightly (float partial sums along the line search)
-    assert row[False]["k0_oracle"] < 0.05
+    # both fits against the FP32 oracle of their own precision plan: the parity bars
+    for fit64 in (True, False):
+        assert row[fit64]["k0_o32"] < 1e-3, (fit64, row[fit64]["k0_o32"])
+        assert row[fit64]["amp_o32"] < 1e-3, (fit64, row[fit64]["amp_o32"])
+        assert row[fit64]["l2_o32"] < 1e-5, (fit64, row[fit64]["l2_o32"])
+    # the FP32 fit against the FP64 oracle is reported; its k0 stays far inside the data's own error
+    assert row[False]["k0_oracle"] < 1e-3

@@ -219,3 +219,34 @@ def test_tiff_rejects_truncated(codec):
     p.write_bytes(open(os.path.join(GOLDEN, "otf.tif"), "rb").read()[:5000])
     with pytest.raises(RuntimeError, match="TIFF"):
         codec("tiff-read", p, codec.dir / "x.f32")
+
+
+def _bigtiff_u16(path, strip_off, strip_cnt, width=4, height=4, extra=()):
+    """A one-page BigTIFF (little endian) whose strip offset and byte count are
+    given as 64-bit values, plus 16 bytes of pixel data after the IFD."""
+    entries = [(256, 3, 1, width), (257, 3, 1, height), (258, 3, 1, 16), (259, 3, 1, 1), (277, 3, 1, 1),
+               (278, 3, 1, height), (273, 16, 1, strip_off), (279, 16, 1, strip_cnt)] + list(extra)
+    entries.sort()
+    ifd = struct.pack("<Q", len(entries))
+    for tag, typ, cnt, val in entries:
+        ifd += struct.pack("<HHQQ", tag, typ, cnt, val)
+    ifd += struct.pack("<Q", 0)
+    data = struct.pack("<2sHHHQ", b"II", 43, 8, 0, 16) + ifd + bytes(range(32))
+    path.write_bytes(data)
+
+
+def test_tiff_offsets_near_2_64_are_refused(codec):
+    """Offsets and counts chosen so that offset + count wraps past 2^64 to a
+    small number: the reader must refuse them, not read out of bounds."""
+    good = codec.dir / "big_ok.tif"
+    n_hdr = 16 + 8 + 8 * 20 + 8
+    _bigtiff_u16(good, n_hdr, 32)
+    assert codec("tiff-u16", good, codec.dir / "ok.u16") == (4, 4, 1)
+    bad = codec.dir / "big_wrap.tif"
+    _bigtiff_u16(bad, 2 ** 64 - 16, 32)  # offset + count wraps to 16
+    with pytest.raises(RuntimeError, match="past the end"):
+        codec("tiff-u16", bad, codec.dir / "x.u16")
+    bad2 = codec.dir / "big_count.tif"  # an extra entry whose count * 8 wraps to 0 bytes
+    _bigtiff_u16(bad2, n_hdr + 20, 32, extra=[(324, 16, 2 ** 61, 0)])
+    with pytest.raises(RuntimeError, match="past the end"):
+        codec("tiff-u16", bad2, codec.dir / "y.u16")

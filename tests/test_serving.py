@@ -119,3 +119,33 @@ def test_otf_total_counts_orders_and_directions():
     assert L.simr_otf_total(C.byref(p)) == 3 * per
     p.bOneOTFperAngle = 1
     assert L.simr_otf_total(C.byref(p)) == 3 * 3 * per
+
+
+def test_adding_setups_while_requests_run():
+    """simr_server_add_setup may be called while workers run requests: the
+    request path holds its own reference to the setup it runs, so growing the
+    setup table under it is safe.  Without a GPU every request fails at
+    context creation (and with one, the all-zero stack has no modulation to
+    fit); either way every request completes and every setup id is new."""
+    import threading
+
+    p = _setup_params()
+    otf = np.zeros(3 * engine.lib().simr_otf_size(C.byref(p)), np.complex64)
+    raw = np.zeros((8 * 15, 64, 64), np.uint16)
+    with serving.Server(hbm_budget=1 << 30, max_inflight=4) as srv:
+        first = srv.add_setup(p, otf)
+        ids = []
+
+        def adder():
+            for _ in range(200):
+                ids.append(srv.add_setup(p, otf))
+
+        th = threading.Thread(target=adder)
+        th.start()
+        tks = [srv.submit(first, raw) for _ in range(40)]
+        res = [srv.wait(tk, check=False) for tk in tks]
+        th.join()
+        assert ids == list(range(first + 1, first + 201))
+        st = srv.stats()
+        assert st["submitted"] == 40 and st["completed"] + st["failed"] == 40
+        assert len(res) == 40
