@@ -32,7 +32,7 @@ def lib():
         for name, args in [
             ("orc_load", [vp, fp]), ("orc_rescale", [vp, ip]), ("orc_set_time", [vp, ip]),
             ("orc_process_volume", [vp]), ("orc_preprocess_bands", [vp]), ("orc_modulation", [vp, ip]),
-            ("orc_filter_assemble", [vp]), ("orc_filter_dir", [vp, ip]),
+            ("orc_filter_assemble", [vp]), ("orc_filter_dir", [vp, ip]), ("orc_assemble", [vp]),
             ("orc_get_band", [vp, ip, ip, fp]), ("orc_set_band", [vp, ip, ip, fp]),
             ("orc_get_band_d", [vp, ip, ip, dp]),
             ("orc_get_result", [vp, fp]), ("orc_get_result_d", [vp, dp]),
@@ -141,6 +141,10 @@ class Oracle:
 
     def filter_assemble(self):
         _check(lib().orc_filter_assemble(self.h))
+
+    def assemble(self):
+        """Assembly of the current bands of every direction, without filtering."""
+        _check(lib().orc_assemble(self.h))
 
     def filter_dir(self, d):
         _check(lib().orc_filter_dir(self.h, d))

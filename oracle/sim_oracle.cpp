@@ -1194,12 +1194,12 @@ struct Oracle {
     }
     filter_and_assemble();
   }
-  void filter_and_assemble() {
+  void filter_and_assemble(bool filter = true) {
     const long long nout = (long long)std::rint((double)P.zoomfact * nx) *
                            (long long)std::rint((double)P.zoomfact * ny) * P.z_zoom * nz;
     outbuffer.assign(nout, (T)0);
     for (int dir = 0; dir < ndirs; ++dir) {
-      filterbands(dir);
+      if (filter) filterbands(dir);
       assemble(dir);
     }
   }
@@ -1310,6 +1310,8 @@ int orc_preprocess_bands(void* h) {
 int orc_modulation(void* h, int dir) { return DISPATCH(h, o.modulation(dir)); }
 int orc_filter_assemble(void* h) { return DISPATCH(h, o.filter_and_assemble()); }
 int orc_filter_dir(void* h, int dir) { return DISPATCH(h, o.filterbands(dir)); }
+// assemblerealspacebands of every direction's current bands into a zeroed output (no filtering)
+int orc_assemble(void* h) { return DISPATCH(h, o.filter_and_assemble(false)); }
 
 // band: complex float [H] (converted from T)
 int orc_get_band(void* h, int dir, int b, float* out) {
