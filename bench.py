@@ -11,8 +11,11 @@ resident in HBM: flat-field + bleach correction + apodisation + separation +
 3D FFT, the k0 search and fit, Wiener filtering and zoomed assembly
 (simr_load_and_rescale + simr_process_volume).
 
-Prints one JSON line.  Kernel-level HIP-event timings are recorded on the
-context's stream over the timed steps for the roofline figure.
+Prints one JSON line.  An untimed calibration pass records HIP events around
+every stage kernel (on the stream each runs on) and picks the kernel with the
+most device time per step; the timed steps then carry events around that
+kernel's launches only, which give its average launch duration for the
+roofline figure.
 """
 import argparse
 import json
@@ -27,63 +30,152 @@ import numpy as np  # noqa: E402
 
 METRIC = "output voxels/s, 3D-SIM 512²×48×15 imgs; p50/p99 latency on mixed stream"
 HBM_PEAK_GBS = 8000.0
-# stage -> kernel name (templated on the transform length) as rocprofv3 reports it
-KERNEL_OF_STAGE = {"asm_x": "simr::k_asm_x<%d>", "asm_y": "simr::k_asm_y<%d>", "asm_z": "simr::k_asm_z<%d>",
-                   "rows_fwd": "simr::k_rows_fwd<%d,", "cols_fwd_y": "simr::k_fft_cols<%d,",
-                   "cols_fwd_z": "simr::k_fft_cols<%d,"}
-XDIM_OF_STAGE = {"asm_x": "xdim", "asm_y": "ydim", "asm_z": "zdim", "rows_fwd": "nx", "cols_fwd_y": "ny",
-                 "cols_fwd_z": "nz"}
+FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X published dense FP32 vector peak
+# stage -> the kernel(s) it launches, as rocprofv3 names them (length-templated)
+KERNEL_OF_STAGE = {
+    "section_sums": "simr::k_section_sums + simr::k_ratios",
+    "rows_fwd": "simr::k_rows_fwd<{nx}, unsigned short>",
+    "cols_fwd_y": "simr::k_fft2_cols<{ny}, false, ...>",
+    "cols_fwd_z": "simr::k_fft2_cols<{nz}, false, ...>",
+    "ov_x": "simr::k_overlaps_x<{nx}>",
+    "ov_y": "simr::k_fft2_cols<{ny}, true, ...>",
+    "ov_zsum": "simr::k_overlaps_zsum<false, false>",
+    "modamp": "simr::k_modamp_fp64",
+    "findk0_peak": "simr::k_cc_prep + 2D FFT + simr::k_argmax_*",
+    "filter0": "simr::k_filter<...> (order 0)",
+    "filter": "simr::k_filter<...> (side band pair)",
+    "asm_z": "simr::k_asm_z_pair<{zdim}, true> + simr::k_asm_z_pair<{zdim}, false>",
+    "asm_y": "simr::k_asm_y<{ydim}>",
+    "asm_x": "simr::k_asm_x<{xdim}>",
+}
+
+
+def geometry(p):
+    import math
+
+    nx, ny, nz = p.nx, p.ny, p.nz
+    norders = p.nphases // 2 + 1
+    L = dict(nx=nx, ny=ny, nz=nz, nxh=nx // 2 + 1, nxy=nx * ny, nb=2 * norders - 1, norders=norders,
+             nimg=nz * p.ndirs * p.nphases, xdim=int(round(p.zoomfact * nx)), ydim=int(round(p.zoomfact * ny)),
+             zdim=p.z_zoom * nz, nasm=1 + p.ndirs * (norders - 1))
+    L["H"] = L["nxh"] * ny * nz
+    # makeoverlaps' axial support half-width (gpuFunctionsImpl.cu:529-548)
+    lam = (p.wave / p.nimm) / 1e3
+    dkz = 1.0 / (nz * p.dz) if nz > 1 else 1.0
+    zd = int(math.ceil((1 - math.cos(math.asin(p.na / p.nimm))) / lam / dkz))
+    L["zd_ov"] = (nz // 2 - 1 if zd > nz // 2 else zd) if nz > 1 else 0
+    L["nzs"] = 2 * L["zd_ov"] + 1
+    return L
 
 
 def stage_bytes(p, L):
-    """Algorithmic HBM bytes of one launch of each stage kernel (reads + writes once)."""
-    nx, ny, nz = p.nx, p.ny, p.nz
-    nxh = nx // 2 + 1
-    H = nxh * ny * nz
-    nxy = nx * ny
-    norders = p.nphases // 2 + 1
-    nb = 2 * norders - 1
-    nimg = nz * p.ndirs * p.nphases
-    xdim, ydim, zdim = L["xdim"], L["ydim"], L["zdim"]
-    nasm = 1 + p.ndirs * (norders - 1)
-    zd = L["zd_ov"]
-    nzs = 2 * zd + 1
+    """Algorithmic HBM bytes of one launch of each stage kernel: every input
+    element the kernel needs read once, every output element written once
+    (complex float 8 B, raw uint16 2 B, output float 4 B)."""
+    H, nxy, nb, nzs = L["H"], L["nxy"], L["nb"], L["nzs"]
+    nread = 3 if p.nz > 1 else 4  # overlap x pass: band 0 + the other order's re/im (2D fit of 1 vs 2: 4)
     return {
-        "section_sums": nimg * nxy * 2,
-        "rows_fwd": nimg * nxy * 2 + p.ndirs * nb * H * 8,
+        "section_sums": L["nimg"] * nxy * 2,
+        "rows_fwd": L["nimg"] * nxy * 2 + p.ndirs * nb * H * 8,
         "cols_fwd_y": 2 * p.ndirs * nb * H * 8,
         "cols_fwd_z": 2 * p.ndirs * nb * H * 8,
-        # x: band reads over the support planes + overlap rows; y: r+w; z: read + P/SX/SY
-        "overlaps": 3 * H * 8 * nzs // nz + 2 * nzs * nxy * 8 + 2 * 2 * nzs * nxy * 8 + 2 * nzs * nxy * 8 + nxy * 32,
-        "asm_z": p.ndirs * nb * H * 8 + nasm * zdim * nxy * 8,
-        "asm_y": nasm * zdim * nxy * 8 + nasm * zdim * ydim * nx * 8,
-        "asm_x": nasm * zdim * ydim * nx * 8 + zdim * ydim * xdim * 4,
-        "filter": 2 * p.ndirs * nb * H * 8,  # whole stage: every band read and written once
+        # the bands over the 2zd+1 support planes in, both overlaps' planes out
+        "ov_x": nread * nzs * L["nxh"] * p.ny * 8 + 2 * nzs * nxy * 8,
+        "ov_y": 2 * 2 * nzs * nxy * 8,
+        "ov_zsum": 2 * nzs * nxy * 8 + 16 * nxy,
+        "filter0": 2 * H * 8,
+        "filter": 4 * H * 8,
+        "asm_z": p.ndirs * nb * H * 8 + L["nasm"] * L["zdim"] * nxy * 8,
+        "asm_y": L["nasm"] * L["zdim"] * nxy * 8 + L["nasm"] * L["zdim"] * L["ydim"] * p.nx * 8,
+        "asm_x": L["nasm"] * L["zdim"] * L["ydim"] * p.nx * 8 + L["zdim"] * L["ydim"] * L["xdim"] * 4,
     }
 
 
-def cpu_baseline(p, tab, raw):
-    """The FP32 oracle on a bounded sample of the same workload: the same 512x512x48
-    stack, reconstructed once or more until about 10 s of CPU work is done."""
-    threads = min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+def stage_fft_flops(p, L):
+    """FFT flops of one launch of each transform stage: 5 N log2 N per complex
+    transform of N points, 2.5 N log2 N per real one (the usual FFT count)."""
+    import math
+
+    def c(n):
+        return 5.0 * n * math.log2(n) if n > 1 else 0.0
+
+    nb, nzs = L["nb"], L["nzs"]
+    return {
+        "rows_fwd": p.ndirs * nb * p.nz * p.ny * 0.5 * c(p.nx),
+        "cols_fwd_y": p.ndirs * nb * p.nz * L["nxh"] * c(p.ny),
+        "cols_fwd_z": p.ndirs * nb * p.ny * L["nxh"] * c(p.nz),
+        "ov_x": 2 * nzs * p.ny * c(p.nx),
+        "ov_y": 2 * nzs * p.nx * c(p.ny),
+        "asm_z": L["nasm"] * p.nx * p.ny * c(L["zdim"]),
+        "asm_y": L["nasm"] * L["zdim"] * p.nx * c(L["ydim"]),
+        "asm_x": L["nasm"] * L["zdim"] * L["ydim"] * c(L["xdim"]),
+    }
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(p, tab, raw, gpu_vol):
+    """The CPU oracle on the same 512x512x48 stack: FP32 (with the FP64 fit
+    reductions the GPU uses) on every host thread of the GPU's share (at most
+    16), median of 3 runs; FP64 once; and one thread on the plain 2D 256x256
+    stack (the bounded one-core sample).  Also the GPU volume's distance from
+    both oracle volumes."""
     from oracle import binding as ob
 
+    nproc = os.cpu_count() or 1
+    threads = min(16, nproc)
+    ob.set_threads(threads)
     raw32 = raw.astype(np.float32)
-    runs, total, vox = 0, 0.0, 0
-    while runs < 3 and (runs == 0 or total < 10.0):
+    times, ref = [], None
+    for _ in range(3):
         t0 = time.perf_counter()
-        vol, _ = ob.reconstruct(p, tab, raw32, 32, False)
-        total += time.perf_counter() - t0
-        vox += vol.size
-        runs += 1
+        vol, _ = ob.reconstruct(p, tab, raw32, 32, True)
+        times.append(time.perf_counter() - t0)
+        if ref is None:
+            ref = vol
+    t32 = float(np.median(times))
+    e32 = float(np.linalg.norm(gpu_vol.ravel().astype(np.float64) - ref.ravel()) /
+                np.linalg.norm(ref.ravel().astype(np.float64)))
+    del ref
+    t0 = time.perf_counter()
+    vol64, _ = ob.reconstruct(p, tab, raw32, 64)
+    t64 = time.perf_counter() - t0
+    e64 = float(np.linalg.norm(gpu_vol.ravel().astype(np.float64) - vol64.ravel()) / np.linalg.norm(vol64.ravel()))
+    del vol64
+    # one thread: the plain 2D 256x256 stack (the 3D stack would take minutes on one core)
+    from cudasirecon_amd import synth
+
+    p2, t2, raw2, _ = synth.make_dataset(256, 256, 1, nphases=3, seed=20262201, wiener=0.001)
+    ob.set_threads(1)
+    t0 = time.perf_counter()
+    v2, _ = ob.reconstruct(p2, t2, raw2.astype(np.float32), 32, True)
+    t1c = time.perf_counter() - t0
+    ob.set_threads(threads)
+    out_vox = gpu_vol.size
     return {
-        "value": round(vox / total, 1),
+        "value": round(out_vox / t32, 1),
         "unit": "output voxels/s",
         "cores": threads,
         "kind": "port",
-        "sample": "FP32 CPU oracle (OpenMP, %d threads), the same 512x512x48 stack (3 dirs x 5 phases) "
-                  "-> 1024x1024x48, %d run(s) in %.1f s" % (threads, runs, total),
+        "sample": "FP32 CPU oracle (FP64 fit reductions, as the GPU), OpenMP on %d threads, the same 512x512x48 "
+                  "stack (3 dirs x 5 phases) -> 1024x1024x48, median of 3 runs (%.2f, %.2f, %.2f s)"
+                  % ((threads,) + tuple(times)),
+        "host": {"nproc": nproc, "cpu": cpu_model()},
+        "fp32_s_per_stack": round(t32, 3),
+        "fp64_s_per_stack": round(t64, 3),
+        "fp64_voxels_per_s": round(out_vox / t64, 1),
+        "one_thread_2d_256": {"s_per_stack": round(t1c, 4), "voxels_per_s": round(v2.size / t1c, 1),
+                              "workload": "plain 2D 256x256, 3 dirs x 3 phases -> 512x512, FP32 oracle, 1 thread"},
+        "gpu_rel_l2_vs_fp32_oracle": e32,
+        "gpu_rel_l2_vs_fp64_oracle": e64,
     }
 
 
@@ -131,15 +223,20 @@ def bench_2d(steps=50, warmup=5, with_cpu=True):
     return res
 
 
-def profiled_traffic(kernel_prefix):
-    """HBM bytes per dispatch of a kernel from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (calibrated against a plain copy); None when absent."""
-    path = os.path.join(ROOT, "profiles", "c2_plain_hbm_traffic.md")
+TRAFFIC_PROFILE = "profiles/c2_plain_hbm_traffic.md"
+
+
+def profiled_traffic(stage):
+    """HBM bytes per dispatch of a stage's kernel, looked up in the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the
+    copy calibration stated there); None when the summary has no line for it.
+    Counters cannot be read inside this process, so this is a lookup."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
     if not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) == 3 and parts[0] == "traffic_bytes_per_dispatch" and parts[1].startswith(kernel_prefix):
+        if len(parts) >= 3 and parts[0] == "traffic_bytes_per_dispatch" and parts[1] == stage:
             return int(parts[2])
     return None
 
@@ -151,7 +248,9 @@ def bench_series(T=64, distinct=4, slots=4):
     cycled over the T time points, and frame t is scaled by the 2 %/frame
     bleach decay 0.98^t; every stack is staged in HBM before timing and every
     result is copied into its own device buffer.  Plain: one context, one
-    stack after another.  Batched: the serving loop with `slots` stacks in
+    stack after another, the time points after the first as captured graphs
+    (also reported: the same with every kernel launched one by one and the
+    modamps fitted on the host).  Batched: the serving loop with `slots` stacks in
     flight as one series (t = 0 first, then all others on any context with the
     time-0 k0).  The two outputs are compared element for element.
     """
@@ -174,22 +273,31 @@ def bench_series(T=64, distinct=4, slots=4):
     torch.cuda.synchronize()
     vox = T * out_elems
 
-    r = engine.Reconstructor(p, tab)
+    def run_plain(use_graphs, dst):
+        q = p.copy()
+        q.use_graphs = use_graphs
+        r = engine.Reconstructor(q, tab)
 
-    def plain(n):
-        for t in range(n):
-            r.set_raw_device(d_raw[t].data_ptr(), u16=True)
-            r.load_and_rescale(t)
-            r.set_time(t)
-            r.process_volume()
-            r.result_to_device(d_plain[t].data_ptr())
-        r.sync()
+        def plain(n):
+            for t in range(n):
+                r.set_raw_device(d_raw[t].data_ptr(), u16=True)
+                r.load_and_rescale(t)
+                r.set_time(t)
+                r.process_volume()
+                r.result_to_device(dst[t].data_ptr())
+            r.sync()
 
-    plain(min(T, 4))  # warm-up
-    t0 = time.perf_counter()
-    plain(T)
-    t_plain = time.perf_counter() - t0
-    r.close()
+        plain(min(T, 4))  # warm-up (t > 0 captures the graph)
+        t0 = time.perf_counter()
+        plain(T)
+        dt = time.perf_counter() - t0
+        r.close()
+        return dt
+
+    # the same kernels launched one by one with the modamps on the host, then graphed
+    t_eager = run_plain(0, d_batch)
+    t_plain = run_plain(1, d_plain)
+    graph_diff = float((d_batch - d_plain).abs().max().item())
 
     budget = (slots + 1) * engine.footprint(p)
     with serving.Server(hbm_budget=budget, max_inflight=slots) as srv:
@@ -213,6 +321,8 @@ def bench_series(T=64, distinct=4, slots=4):
         "time_points": T,
         "plain_voxels_per_s": round(vox / t_plain, 1),
         "plain_ms_per_stack": round(t_plain / T * 1e3, 3),
+        "plain_eager_ms_per_stack": round(t_eager / T * 1e3, 3),
+        "graphed_vs_eager_max_abs_diff": graph_diff,
         "batched_voxels_per_s": round(vox / t_batch, 1),
         "batched_ms_per_stack": round(t_batch / T * 1e3, 3),
         "speedup_vs_plain": round(t_plain / t_batch, 3),
@@ -401,8 +511,8 @@ def main():
     ap.add_argument("--no-2d", action="store_true", help="skip the plain 2D 256x256 measurement")
     ap.add_argument("--no-series", action="store_true", help="skip the time-series batching measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream measurement")
-    ap.add_argument("--serve-requests", type=int, default=200,
-                    help="requests in the serving stream (2000 gives the full 100 s stream at 20 req/s)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests in the serving stream (2000: the full 100 s stream at 20 req/s)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--slots", type=int, default=4, help="stacks in flight in the batched/serving forms")
     args = ap.parse_args()
@@ -425,18 +535,11 @@ def main():
         del src, dst
 
     print("bench: generating the synthetic 512x512x48 stack", file=sys.stderr, flush=True)
-    p, tab, raw, truth = synth.make_dataset(512, 512, 48, seed=20262202)
+    p, tab, raw, truth = synth.make_dataset(512, 512, 48, seed=20262202, modamps=(0.8, 0.4))
     p.fit_fp64 = 0 if args.fit_fp32 else 1
     r = engine.Reconstructor(p, tab)
     r.set_raw(raw)  # resident in HBM from here on
-    L = dict(xdim=int(round(p.zoomfact * p.nx)), ydim=int(round(p.zoomfact * p.ny)), zdim=p.z_zoom * p.nz,
-             nx=p.nx, ny=p.ny, nz=p.nz)
-    # overlap support half-width (makeoverlaps), needed for the byte count
-    import math
-    lam = (p.wave / p.nimm) / 1e3
-    dkz = 1.0 / (p.nz * p.dz)
-    zd = int(math.ceil((1 - math.cos(math.asin(p.na / p.nimm))) / lam / dkz))
-    L["zd_ov"] = min(zd, p.nz // 2 - 1) if zd > p.nz // 2 else zd
+    L = geometry(p)
 
     def step():
         r.load_and_rescale(0)
@@ -447,30 +550,51 @@ def main():
     for _ in range(args.warmup):
         step()
     r.sync()
+
+    # calibration (untimed): every stage's device time, from HIP events on the
+    # stream each stage runs on, to find the kernel that takes the most time
+    cal_steps = 2
     r.enable_timing(True)
+    r.time_stage(None)
     r.timings()  # clear
+    for _ in range(cal_steps):
+        step()
+    r.sync()
+    agg = {}
+    for name, ms in r.timings(cap=100000):
+        a = agg.setdefault(name, [0.0, 0])
+        a[0] += ms
+        a[1] += 1
+    per_step = {k: v[0] / cal_steps for k, v in agg.items()}
+    launches = {k: v[1] // cal_steps for k, v in agg.items()}
+    sb = stage_bytes(p, L)
+    dom = max((k for k in per_step if k in sb), key=lambda k: per_step[k])
+
+    # timed region: events around the dominant kernel's launches only
+    r.time_stage(dom)
+    r.timings()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     r.sync()
     dt = time.perf_counter() - t0
-    stages = r.timings(cap=100000)
+    dom_ms = [ms for name, ms in r.timings(cap=100000) if name == dom]
     r.enable_timing(False)
-    out_vox = p.z_zoom * p.nz * int(round(p.zoomfact * p.ny)) * int(round(p.zoomfact * p.nx))
+    r.time_stage(None)
+    gpu_vol = r.result()
+    out_vox = gpu_vol.size
     value = args.steps * out_vox / dt
-
-    # per-stage device time (HIP events on the context stream)
-    agg = {}
-    for name, ms in stages:
-        a = agg.setdefault(name, [0.0, 0])
-        a[0] += ms
-        a[1] += 1
-    per_step = {k: v[0] / args.steps for k, v in agg.items()}
-    sb = stage_bytes(p, L)
-    launch_stages = [k for k in ("asm_x", "asm_y", "asm_z", "rows_fwd", "cols_fwd_y", "cols_fwd_z") if k in agg]
-    dom = max(launch_stages, key=lambda k: per_step[k])
-    avg_ms = agg[dom][0] / agg[dom][1]
+    avg_ms = float(np.mean(dom_ms))
     achieved = sb[dom] / (avg_ms * 1e-3) / 1e9
+
+    # FFT FP32 rate of each transform kernel (calibration pass) and of the whole step
+    fl = stage_fft_flops(p, L)
+    fft = {}
+    for k, f in fl.items():
+        if k in agg:
+            fft[k] = round(f / (agg[k][0] / agg[k][1] * 1e-3) / 1e9, 1)
+    step_flops = sum(fl[k] * launches.get(k, 0) for k in fl)
+
     # PCIe-inclusive rate (not the headline value): host uint16 stack in, host float32 volume out
     pcie_n = 3
     t1 = time.perf_counter()
@@ -480,6 +604,8 @@ def main():
         host_out = r.result()
     pcie_dt = (time.perf_counter() - t1) / pcie_n
     del host_out
+    import math
+
     dk = 1.0 / (p.nx * p.dxy)
     k0err = max(math.hypot(r.fit(d).k0x - truth["k0"][d][0], r.fit(d).k0y - truth["k0"][d][1]) / dk
                 for d in range(p.ndirs))
@@ -497,7 +623,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded beads+filaments, Poisson noise, uint16)",
+        "data": "synthetic (seeded beads+filaments, modamps 0.8/0.4, Poisson noise, uint16)",
         "config": {
             "workload": "plain 3D-SIM single stack: 512x512x48, 3 dirs x 5 phases (720 uint16 images) "
                         "-> 1024x1024x48 f32, one stack per step, raw resident in HBM",
@@ -507,19 +633,34 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
+            "kernel_symbol": KERNEL_OF_STAGE.get(dom, dom).format(**L),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": profiled_traffic(KERNEL_OF_STAGE[dom] % L[XDIM_OF_STAGE.get(dom, "xdim")]),
+            "traffic": profiled_traffic(dom),
+            "traffic_source": "lookup: %s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this kernel on this workload)"
+                              % TRAFFIC_PROFILE,
             "launch_ms": round(avg_ms, 4),
+            "launches_timed": len(dom_ms),
+            "launches_per_step": launches.get(dom, 0),
             "algorithmic_bytes": int(sb[dom]),
+            "picked_by": "largest device time per step over every stage kernel (calibration pass)",
         },
         "stage_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items(), key=lambda kv: -kv[1])},
+        "stage_launches_per_step": launches,
+        "stage_hbm_frac": {k: round(sb[k] / (agg[k][0] / agg[k][1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                           for k in agg if k in sb},
+        "fft_fp32": {"unit": "GFLOP/s", "count": "5 N log2 N per complex transform of N points",
+                     "per_kernel": fft, "step_gflop": round(step_flops / 1e9, 2),
+                     "step_rate": round(step_flops / (dt / args.steps) / 1e9, 1),
+                     "peak": FP32_VECTOR_PEAK_TFLOPS * 1e3,
+                     "step_frac": round(step_flops / (dt / args.steps) / 1e12 / FP32_VECTOR_PEAK_TFLOPS, 4)},
         "k0_err_vs_truth_px": round(k0err, 5),
         "pcie_inclusive": {"value": round(out_vox / pcie_dt, 1), "unit": "output voxels/s",
                            "ms_per_stack": round(pcie_dt * 1e3, 3),
-                           "note": "host uint16 stack copied in and the float32 volume copied out each stack"},
+                           "note": "host uint16 stack copied in and the float32 volume copied out each stack "
+                                   "(pageable host memory)"},
     }
     if not args.no_2d:
         res["plain_2d"] = bench_2d(with_cpu=not args.no_cpu)
@@ -534,7 +675,7 @@ def main():
         print("bench: serving %s" % json.dumps(sv), file=sys.stderr, flush=True)
     if not args.no_cpu:
         print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
-        res["cpu_baseline"] = cpu_baseline(p, tab, raw)
+        res["cpu_baseline"] = cpu_baseline(p, tab, raw, gpu_vol)
     else:
         res["cpu_baseline"] = None
     print(json.dumps(res), flush=True)

@@ -41,7 +41,7 @@ class SimrParams(C.Structure):
         ("fit_fp64", C.c_int32),
         ("nx_raw", C.c_int32), ("deskew_factor", C.c_float), ("deskew_shift", C.c_int32),
         ("bBessel", C.c_int32), ("BesselNA", C.c_float), ("BesselLambdaEx", C.c_float),
-        ("reserved", C.c_int32 * 9),
+        ("use_graphs", C.c_int32), ("reserved", C.c_int32 * 8),
     ]
 
     def copy(self):
@@ -198,6 +198,7 @@ def default_params():
     p.bBessel = 0
     p.BesselNA = 0.144
     p.BesselLambdaEx = 0.488
+    p.use_graphs = 1
     return p
 
 

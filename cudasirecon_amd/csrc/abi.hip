@@ -129,6 +129,7 @@ void simr_params_default(simr_params* p) {
   p->fit_fp64 = 1;
   p->BesselNA = 0.144f;       // --besselNA default (cudaSirecon.cpp:983)
   p->BesselLambdaEx = 0.488f;  // --besselExWave default (cudaSirecon.cpp:981)
+  p->use_graphs = 1;
 }
 
 size_t simr_otf_size(const simr_params* p) {
@@ -210,12 +211,7 @@ int simr_load_and_rescale(simr_ctx* ctx, int it) { return guard([&] { E(ctx).loa
 int simr_set_time_index(simr_ctx* ctx, int it) { return guard([&] { E(ctx).set_time(it); }); }
 int simr_find_modulation(simr_ctx* ctx) { return guard([&] { E(ctx).find_modulation(); }); }
 int simr_filter_assemble(simr_ctx* ctx) { return guard([&] { E(ctx).filter_assemble(); }); }
-int simr_process_volume(simr_ctx* ctx) {
-  return guard([&] {
-    E(ctx).find_modulation();
-    E(ctx).filter_assemble();
-  });
-}
+int simr_process_volume(simr_ctx* ctx) { return guard([&] { E(ctx).process_volume(); }); }
 
 size_t simr_result_elems(const simr_ctx* ctx) { return ctx && ctx->eng ? ctx->eng->result_elems() : 0; }
 int simr_get_result_host(simr_ctx* ctx, float* out) { return guard([&] { E(ctx).get_result(out); }); }
@@ -269,6 +265,7 @@ int simr_debug_overlap_volumes(simr_ctx* ctx, int dir, int o1, int o2, float k0x
 int simr_sync(simr_ctx* ctx) { return guard([&] { E(ctx).sync(); }); }
 void* simr_stream(simr_ctx* ctx) { return ctx && ctx->eng ? (void*)ctx->eng->stream() : nullptr; }
 int simr_enable_timing(simr_ctx* ctx, int on) { return guard([&] { E(ctx).enable_timing(on != 0); }); }
+int simr_time_stage(simr_ctx* ctx, const char* stage) { return guard([&] { E(ctx).time_only(stage); }); }
 int simr_stage_timings(simr_ctx* ctx, const char** names, float* ms, int cap) {
   int n = 0;
   int rc = guard([&] { n = E(ctx).timings(names, ms, cap); });
@@ -276,6 +273,7 @@ int simr_stage_timings(simr_ctx* ctx, const char** names, float* ms, int cap) {
 }
 
 int simr_fft_size_supported(int n) { return simr::fft_size_supported(n) ? 1 : 0; }
+
 
 int simr_fft3d_dev(float* data, int nx, int ny, int nz, int inverse, void* stream) {
   return guard([&] {
@@ -292,6 +290,7 @@ int simr_fft3d_dev(float* data, int nx, int ny, int nz, int inverse, void* strea
     SIMR_HIP(hipStreamSynchronize(s));
   });
 }
+
 
 int simr_fft3d_host(float* data, int nx, int ny, int nz, int inverse) {
   return guard([&] {

@@ -174,6 +174,7 @@ Layout Layout::of(const simr_params& p) {
   b += align256((size_t)L.nimg * kSumSplit * 8) + align256((size_t)L.nimg * 4) + align256((size_t)p.nz * 8) + align256(256);
   b += align256((size_t)L.ndirs * L.nbands * L.H * 8);   // bands
   b += align256((size_t)L.nDirsOTF * L.norders * L.otfsize * 8);
+  b += 3 * align256((size_t)L.ndirs * SIMR_MAX_ORDERS * 8);  // device modamps of graphed time points
   const size_t fit = fit_set_bytes(L, ncand);  // direction 0's fit scratch
   b += fit;
   // assembly staging; the fit scratch of directions 1.. aliases it
@@ -206,6 +207,9 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   apofact_ = (float*)arena_.take(256);
   bands_ = (float2*)arena_.take((size_t)L.ndirs * L.nbands * L.H * 8);
   otf_ = (float2*)arena_.take((size_t)L.nDirsOTF * L.norders * L.otfsize * 8);
+  kc_dev_ = (float2*)arena_.take((size_t)L.ndirs * SIMR_MAX_ORDERS * 8);
+  amps_dev_ = (float2*)arena_.take((size_t)L.ndirs * SIMR_MAX_ORDERS * 8);
+  corr_dev_ = (float*)arena_.take((size_t)L.ndirs * SIMR_MAX_ORDERS * 8);
   const size_t fitb = fit_set_bytes(L, kMaxCand);
   char* fit0 = (char*)arena_.take(fitb);
   const size_t zb = align256((size_t)L.nasm * L.zdim * L.nxy * 8);
@@ -235,6 +239,18 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
     }
     SIMR_HIP(hipHostMalloc(&w.hstage, hstage_bytes_, hipHostMallocDefault));
     SIMR_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
+  }
+
+  SIMR_HIP(hipHostMalloc(&fit_host_, (size_t)L.ndirs * L.norders * 12, hipHostMallocDefault));
+  SIMR_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+  join_ev_.assign(L.ndirs, nullptr);
+  for (auto& e : join_ev_) SIMR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  {
+    // order 0's modamp is 1 (setup_common, cudaSirecon.cpp:1257-1260); the
+    // graphed fit writes the others
+    std::vector<float2> one((size_t)L.ndirs * L.norders, make_float2(1.f, 0.f));
+    SIMR_HIP(hipMemcpyAsync(amps_dev_, one.data(), one.size() * 8, hipMemcpyHostToDevice, stream_));
+    SIMR_HIP(hipStreamSynchronize(stream_));
   }
 
   // OTF tables
@@ -309,13 +325,21 @@ void Engine::update_params(const simr_params& p) {
                    N.nDirsOTF == L.nDirsOTF && N.otfsize == L.otfsize && N.zd_ov_max <= L.zd_ov_max,
                "parameter change alters the context's buffer layout; create a new context");
   sync();
+  refresh_fit();
   P = p;
   fit_fp32_ = P.fit_fp64 == 0;
   derive();
+  ++param_version_;  // a captured graph holds the old parameters
 }
 
 Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+  for (hipEvent_t e : join_ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (fit_host_) (void)hipHostFree(fit_host_);
   for (auto& e : events_) {
     (void)hipEventDestroy(e.second.first);
     (void)hipEventDestroy(e.second.second);
@@ -368,7 +392,7 @@ void Engine::enable_timing(bool on) {
 // Stage timing: a pair of events around the stage's launches on its stream
 // (the fit's directions record on their own streams, from their threads).
 int Engine::stage_begin(const char* name, hipStream_t s) {
-  if (!timing_) return -1;
+  if (!timing_ || (!timing_only_.empty() && timing_only_ != name)) return -1;
   hipEvent_t a, b;
   int id;
   {
@@ -612,8 +636,13 @@ void Engine::make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float
   p.inv_dkr2_1 = (float)(1.0 / ((double)p.otf1.dkrotf * (double)p.otf1.dkrotf));
   p.inv_dkr2_2 = (float)(1.0 / ((double)p.otf2.dkrotf * (double)p.otf2.dkrotf));
   w.zd = p.zd;
-  const int st = stage_begin("overlaps", w.stream);
-  launch_overlaps_xy(p, w.ovx, w.stream);
+  int st = stage_begin("ov_x", w.stream);
+  launch_overlaps_x(p, w.ovx, w.stream);
+  stage_end(st, w.stream);
+  st = stage_begin("ov_y", w.stream);
+  launch_overlaps_y(p, w.ovx, w.stream);
+  stage_end(st, w.stream);
+  st = stage_begin("ov_zsum", w.stream);
   launch_overlaps_z_reduce(p, w.ovx, w.P, w.SX, w.SY, w.sumparts, fit_fp32_, per_elem, w.stream);
   stage_end(st, w.stream);
   w.ov_totals_valid = false;
@@ -1032,6 +1061,7 @@ void Engine::modulation(FitWork& w, int dir) {
 // its line-search read-back.  Each thread ends with its stream drained.
 void Engine::find_modulation() {
   if (!preprocessed_) throw StateError("find_modulation before load_and_rescale");
+  refresh_fit();
   if (L.ndirs > 1) {
     hipEvent_t ready;
     SIMR_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
@@ -1164,7 +1194,12 @@ FilterParams Engine::filter_params(int dir, int order) const {
 }
 
 void Engine::filter_dir(int dir) {
-  for (int order = 0; order < L.norders; ++order) launch_filter(filter_params(dir, order), stream_);
+  refresh_fit();
+  for (int order = 0; order < L.norders; ++order) {
+    const int st = stage_begin(order == 0 ? "filter0" : "filter", stream_);
+    launch_filter(filter_params(dir, order), stream_);
+    stage_end(st, stream_);
+  }
 }
 
 void Engine::assemble() {
@@ -1201,11 +1236,129 @@ void Engine::assemble() {
 
 void Engine::filter_assemble() {
   if (!fitted_) throw StateError("filter_assemble before find_modulation");
-  cur_stage_ = stage_begin("filter", stream_);
   for (int dir = 0; dir < L.ndirs; ++dir) filter_dir(dir);
-  stage_end(cur_stage_, stream_);
   assemble();
   processed_ = true;
+}
+
+void Engine::process_volume() {
+  if (graph_eligible()) {
+    run_graphed();
+    return;
+  }
+  find_modulation();
+  filter_assemble();
+}
+
+// ------------------------------------------------------------ graphed time points
+// A time point that takes time 0's k0 (cudaSirecon.cpp:296-331) needs no
+// host decision: its modamps are findrealspacemodamp at the known k0 for
+// every order (realspacemodamp with order1 = 0, then the policy of 450-491),
+// so the fit, the filter and the assembly go on the device as one graph:
+// every direction's overlaps and modamp reduction on its own stream (fork
+// and join events), the modamp result arithmetic and the policy in one-thread
+// kernels, and the filter reading the modamps from device memory.  The
+// graph is captured once per context and replayed while k0 and the
+// parameters stay the same.
+bool Engine::graph_eligible() const {
+  return P.use_graphs != 0 && !timing_ && preprocessed_ && P.ntimes > 1 && cur_t_ > 0 && P.bUseTime0k0 != 0;
+}
+
+void Engine::build_graph() {
+  if (graph_exec_) SIMR_HIP(hipGraphExecDestroy(graph_exec_));
+  if (graph_) SIMR_HIP(hipGraphDestroy(graph_));
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+  // candidate vectors of findrealspacemodamp (gpuFunctionsImpl.cu:1093-1094), order1 = 0
+  std::vector<float2> kc((size_t)L.ndirs * L.norders, make_float2(0.f, 0.f));
+  for (int d = 0; d < L.ndirs; ++d)
+    for (int o = 1; o < L.norders; ++o)
+      kc[(size_t)d * L.norders + o] = make_float2(k0_[d].x * (float)o, k0_[d].y * (float)o);
+  SIMR_HIP(hipMemcpyAsync(kc_dev_, kc.data(), kc.size() * 8, hipMemcpyHostToDevice, stream_));
+  sync();
+  SIMR_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  try {
+    if (L.ndirs > 1) {
+      SIMR_HIP(hipEventRecord(fork_ev_, stream_));
+      for (int d = 1; d < L.ndirs; ++d) SIMR_HIP(hipStreamWaitEvent(fw_[d].stream, fork_ev_, 0));
+    }
+    const int nparts = fit_fp32_ ? modamp_partials_fp32(L.nx, L.ny) : modamp_partials_fp64(L.nx, L.ny);
+    for (int d = 0; d < L.ndirs; ++d) {
+      FitWork& w = fw_[d];
+      for (int o = 1; o < L.norders; ++o) {
+        const size_t idx = (size_t)d * L.norders + o;
+        make_overlaps(w, d, 0, o, k0_[d].x, k0_[d].y);
+        if (fit_fp32_)
+          launch_modamp_fp32(w.P, w.SX, w.SY, kc_dev_ + idx, 1, L.nx, L.ny, P.dxy, w.part32, w.stream);
+        else
+          launch_modamp_fp64(w.P, kc_dev_ + idx, 1, L.nx, L.ny, P.dxy, w.part64, w.stream);
+        launch_modamp_finish(fit_fp32_, w.part64, w.part32, nparts, w.sumparts, zsum_blocks(L.nxy), amps_dev_ + idx,
+                             corr_dev_ + idx, w.stream);
+      }
+    }
+    if (L.ndirs > 1)
+      for (int d = 1; d < L.ndirs; ++d) {
+        SIMR_HIP(hipEventRecord(join_ev_[d], fw_[d].stream));
+        SIMR_HIP(hipStreamWaitEvent(stream_, join_ev_[d], 0));
+      }
+    AmpPolicy a{};
+    a.ndirs = L.ndirs;
+    a.norders = L.norders;
+    a.chain = L.nz == 1;
+    a.nforceamp = P.nforceamp;
+    for (int o = 0; o < SIMR_MAX_ORDERS; ++o) a.forceamp[o] = P.forceamp[o];
+    a.fit_all_phases = P.bFitallphases;
+    launch_amp_policy(a, amps_dev_, stream_);
+    for (int d = 0; d < L.ndirs; ++d)
+      for (int o = 0; o < L.norders; ++o) {
+        FilterParams f = filter_params(d, o);
+        f.amps = amps_dev_;
+        launch_filter(f, stream_);
+      }
+    assemble();
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(stream_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  SIMR_HIP(hipStreamEndCapture(stream_, &graph_));
+  SIMR_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  graph_k0_ = k0_;
+  graph_version_ = param_version_;
+}
+
+void Engine::run_graphed() {
+  refresh_fit();
+  bool same_k0 = graph_k0_.size() == k0_.size();
+  for (size_t d = 0; same_k0 && d < k0_.size(); ++d)
+    same_k0 = std::memcmp(&graph_k0_[d], &k0_[d], sizeof(float2)) == 0;
+  if (!graph_exec_ || graph_version_ != param_version_ || !same_k0) build_graph();
+  SIMR_HIP(hipGraphLaunch(graph_exec_, stream_));
+  const size_t na = (size_t)L.ndirs * L.norders;
+  SIMR_HIP(hipMemcpyAsync(fit_host_, amps_dev_, na * 8, hipMemcpyDeviceToHost, stream_));
+  SIMR_HIP(hipMemcpyAsync((char*)fit_host_ + na * 8, corr_dev_, na * 4, hipMemcpyDeviceToHost, stream_));
+  fit_pending_ = true;
+  for (int d = 0; d < L.ndirs; ++d) {
+    stats_[d] = simr_fit{};
+    stats_[d].n_reductions = L.norders - 1;
+  }
+  fitted_ = processed_ = true;
+}
+
+// Moves the modamps of the last graphed time point into the host state.
+void Engine::refresh_fit() {
+  if (!fit_pending_) return;
+  SIMR_HIP(hipStreamSynchronize(stream_));
+  const size_t na = (size_t)L.ndirs * L.norders;
+  const float2* a = (const float2*)fit_host_;
+  const float* c = (const float*)((const char*)fit_host_ + na * 8);
+  for (int d = 0; d < L.ndirs; ++d)
+    for (int o = 1; o < L.norders; ++o) {
+      amp_[d][o] = a[(size_t)d * L.norders + o];
+      corr_[d][o] = c[(size_t)d * L.norders + o];
+    }
+  fit_pending_ = false;
 }
 
 void Engine::get_result(float* host) {
@@ -1214,8 +1367,9 @@ void Engine::get_result(float* host) {
   sync();
 }
 
-simr_fit Engine::fit(int dir) const {
+simr_fit Engine::fit(int dir) {
   SIMR_REQUIRE(dir >= 0 && dir < L.ndirs, "direction out of range");
+  refresh_fit();
   simr_fit f = stats_[dir];
   f.k0x = k0_[dir].x;
   f.k0y = k0_[dir].y;
@@ -1232,6 +1386,7 @@ simr_fit Engine::fit(int dir) const {
 void Engine::set_fit(int dir, const simr_fit& f) {
   SIMR_REQUIRE(dir >= 0 && dir < L.ndirs, "direction out of range");
   SIMR_REQUIRE(std::isfinite(f.k0x) && std::isfinite(f.k0y), "k0 must be finite");
+  refresh_fit();
   k0_[dir] = make_float2(f.k0x, f.k0y);
   for (int o = 0; o < L.norders; ++o) amp_[dir][o] = make_float2(f.amp_re[o], f.amp_im[o]);
   fitted_ = true;

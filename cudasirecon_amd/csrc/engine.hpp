@@ -56,11 +56,14 @@ class Engine {
   void set_time(int it) { cur_t_ = it; }
   void find_modulation();
   void filter_assemble();
+  // find_modulation + filter_assemble; a time point that reuses time 0's k0
+  // runs as one captured graph (use_graphs)
+  void process_volume();
   void filter_dir(int dir);
   void get_result(float* host);
   const float* result_dev() const { return out_; }
   size_t result_elems() const { return (size_t)L.zdim * L.ydim * L.xdim; }
-  simr_fit fit(int dir) const;
+  simr_fit fit(int dir);
   void set_fit(int dir, const simr_fit& f);
   size_t time0_state_size() const;
   void get_time0_state(void* host, size_t bytes);
@@ -73,6 +76,8 @@ class Engine {
   hipStream_t stream() const { return stream_; }
   const simr_params& params() const { return P; }
   void enable_timing(bool on);
+  // Restricts stage timing to the stages of one name ("" = every stage).
+  void time_only(const char* name) { timing_only_ = name ? name : ""; }
   int timings(const char** names, float* ms, int cap);
 
  private:
@@ -127,6 +132,10 @@ class Engine {
   void fitk0andmodamps(FitWork& w, int dir);
   void modulation(FitWork& w, int dir);
   void require_finite_k0(int dir, const char* where) const;
+  bool graph_eligible() const;
+  void build_graph();
+  void run_graphed();
+  void refresh_fit();
   FilterParams filter_params(int dir, int order) const;
   void assemble();
   int stage_begin(const char* name, hipStream_t s);
@@ -165,8 +174,22 @@ class Engine {
   std::vector<std::vector<float>> corr_;
   std::vector<simr_fit> stats_;
 
+  // device-side modamps of the graphed time points
+  float2* kc_dev_ = nullptr;    // [ndirs][norders] candidate vectors k0 * order
+  float2* amps_dev_ = nullptr;  // [ndirs][norders] modamps (order 0: 1)
+  float* corr_dev_ = nullptr;   // [ndirs][norders] correlation coefficients
+  void* fit_host_ = nullptr;    // pinned read-back of amps_dev_ and corr_dev_
+  bool fit_pending_ = false;    // fit_host_ holds a read-back not yet moved into amp_ / corr_
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  std::vector<float2> graph_k0_;  // k0 the graph was captured with
+  int graph_version_ = -1, param_version_ = 0;
+  hipEvent_t fork_ev_ = nullptr;
+  std::vector<hipEvent_t> join_ev_;
+
   // timing
   bool timing_ = false;
+  std::string timing_only_;
   int cur_stage_ = -1;  // open stage on the context stream
   std::mutex ev_mu_;
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> events_;

@@ -62,6 +62,9 @@ struct OvParams {
 };
 // ovx: [2][2zd+1][ny][nx]; builds both overlaps and inverse-transforms along x and y.
 void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s);
+// its two halves: the overlap formation with the x inverse transform, and the y inverse transform
+void launch_overlaps_x(const OvParams& p, float2* ovx, hipStream_t s);
+void launch_overlaps_y(const OvParams& p, float2* ovx, hipStream_t s);
 // The per-(x,y) z sums of the inverse-transformed overlap pair,
 // P = sum_z conj(X) Y, SX = sum |X|^2, SY = sum |Y|^2 (double storage).
 // fp32: accumulated in float as the reference's reductionKernel threads, P,
@@ -90,6 +93,22 @@ void launch_modamp_fp64(const double2* P, const float2* kcand, int ncand, int nx
                         double2* partials, hipStream_t s);
 void launch_modamp_fp32(const double2* P, const double* SX, const double* SY, const float2* kcand, int ncand,
                         int nx, int ny, float dxy, float4* partials, hipStream_t s);
+// findrealspacemodamp's result (gpuFunctionsImpl.cu:1128-1147) on the device,
+// for one candidate: the partials summed in block order as the host does
+// (cpuReduce, gpuFunctionsImpl_hh.cu:162-180), then modamp = XStarY / sumX
+// and corr = |XStarY| / sqrt(sumX sumY), written to *amp and *corr.
+// fp64: part64 [nparts] and the z-sum totals [2][zblocks]; fp32: part32 [nparts].
+void launch_modamp_finish(bool fp32, const double2* part64, const float4* part32, int nparts, const double* totals,
+                          int zblocks, float2* amp, float* corr, hipStream_t s);
+// The modamp policy after every order is fitted at a known k0
+// (cudaSirecon.cpp:450-491): 2D chaining amp[o] *= amp[o-1], forcemodamp
+// (raise magnitudes below the forced ones), and with fit_all_phases == 0 the
+// phases of orders >= 2 tied to order 1's.  amps: [ndirs][norders], in place.
+struct AmpPolicy {
+  int ndirs, norders, chain, nforceamp, fit_all_phases;
+  float forceamp[kMaxOrders];
+};
+void launch_amp_policy(const AmpPolicy& a, float2* amps, hipStream_t s);
 
 // ---- Wiener filter ------------------------------------------------------------
 struct FilterParams {
@@ -104,6 +123,9 @@ struct FilterParams {
   float ampmag2_all[kMaxDirs * kMaxOrders];
   float noiseVar[kMaxDirs * kMaxOrders];
   float2 conjamp;  // conj(amp[dir][order])
+  // non-null: the modamps [ndirs][norders] on the device (fitted there);
+  // ampmag2_all and conjamp are then formed from them in the kernel
+  const float2* amps;
   OtfRA otf[kMaxOrders];
   float2 *bp, *bp2;
   // host-derived constants (same float/double expressions as the per-thread ones)

@@ -58,6 +58,15 @@ __device__ __forceinline__ float2 prof_pos(const float2* prof, int nxotf, float 
 // support.
 // UNV: every noiseVar is 1 (known at compile time; the division would
 // otherwise be computed and discarded by a select in the unrolled loop).
+// |modamp of (dir, order)|^2: the host's value, or formed from the device
+// modamps with the host's float expression
+__device__ __forceinline__ float amp_mag2(const FilterParams& p, int dir, int order) {
+#pragma clang fp contract(off)
+  if (p.amps == nullptr) return p.ampmag2_all[dir * p.norders + order];
+  const float2 a = p.amps[dir * p.norders + order];
+  return a.x * a.x + a.y * a.y;
+}
+
 template <bool UNV>
 __device__ __forceinline__ float filter_term(const FilterParams& p, const float2* prof, int nxotf, int order, int dir2,
                                              int order2, float xabs, float yabs, int z0, float zf3) {
@@ -80,7 +89,7 @@ __device__ __forceinline__ float filter_term(const FilterParams& p, const float2
   const float2 otf2 = prof_pos(prof + ao2 * nxotf, nxotf, __builtin_amdgcn_sqrtf(d2 * p.inv_dkr2));
   float w = __builtin_fmaf(otf2.x, otf2.x, otf2.y * otf2.y);
   if (!UNV && !p.unit_nv) w = w / p.noiseVar[dir2 * p.norders + ao2];
-  if (order2 != 0) w *= p.ampmag2_all[dir2 * p.norders + ao2];
+  if (order2 != 0) w *= amp_mag2(p, dir2, ao2);
   if (p.bSuppress && order2 != 0 && d2 <= p.supp2)
     w *= dev_suppress(__builtin_amdgcn_sqrtf(d2 * p.inv_min_dkr2));
   else if (!p.bDampenOrder0 && p.bSuppress && order2 == 0 && d2 <= p.supp2)
@@ -116,7 +125,7 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
   }
   const float2 otf1 = prof_pos(prof + order * nxotf, nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2));
   float weight = otf1.x * otf1.x + otf1.y * otf1.y;
-  if (order != 0) weight *= p.ampmag2_all[p.dir * p.norders + order];
+  if (order != 0) weight *= amp_mag2(p, p.dir, order);
   float dampfact = p.inv_nv[p.dir * p.norders + order];
   if (p.bSuppress && order != 0 && d1 <= p.supp2)
     dampfact *= dev_suppress(__builtin_amdgcn_sqrtf(d1 * p.inv_min_dkr2));
@@ -165,7 +174,12 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
   if (order == 0) {
     re = cmul_nc(re, scale);
   } else {
-    scale = cmul_nc(scale, p.conjamp);
+    float2 ca = p.conjamp;
+    if (p.amps != nullptr) {
+      const float2 a = p.amps[p.dir * p.norders + order];
+      ca = make_float2(a.x, -a.y);
+    }
+    scale = cmul_nc(scale, ca);
     if (conj) {
       re.y = -re.y;
       im.y = -im.y;

@@ -209,7 +209,7 @@ static void launch_ov_x(const OvParams& p, float2* ovx, hipStream_t s) {
   SIMR_LAUNCH_CHECK();
 }
 
-void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s) {
+void launch_overlaps_x(const OvParams& p, float2* ovx, hipStream_t s) {
   switch (p.nx) {
 #define SIMR_CASE(N)               \
   case N:                          \
@@ -220,8 +220,16 @@ void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s) {
     default:
       throw ArgError("image width " + std::to_string(p.nx) + " is not a supported size");
   }
+}
+
+void launch_overlaps_y(const OvParams& p, float2* ovx, hipStream_t s) {
   const long long nxy = (long long)p.nx * p.ny;
   fft_cols(p.ny, ovx, p.nx, p.nx, 2LL * (2 * p.zd + 1), nxy, true, s);
+}
+
+void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s) {
+  launch_overlaps_x(p, ovx, s);
+  launch_overlaps_y(p, ovx, s);
 }
 
 // z pass, reduced to what the fit reads.  The modamp and cross-correlation
@@ -475,6 +483,84 @@ void launch_modamp_fp64(const double2* P, const float2* kcand, int ncand, int nx
                         double2* partials, hipStream_t s) {
   dim3 grid(modamp_partials_fp64(nx, ny), ncand);
   hipLaunchKernelGGL(k_modamp_fp64, grid, dim3(256), sizeof(double2) * nx, s, P, kcand, nx, ny, dxy, partials);
+  SIMR_LAUNCH_CHECK();
+}
+
+// The result arithmetic of one modamp candidate, in the host's order and
+// precision (one thread: the sums are sequential in block order).
+__global__ void k_modamp_finish(int fp32, const double2* __restrict__ part64, const float4* __restrict__ part32,
+                                int nparts, const double* __restrict__ totals, int zblocks, float2* __restrict__ amp,
+                                float* __restrict__ corr) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (fp32) {
+    float xr = 0.f, xi = 0.f, sx = 0.f, sy = 0.f;
+    for (int b = 0; b < nparts; ++b) {
+      const float4 v = part32[b];
+      xr += v.x;
+      xi += v.y;
+      sx += v.z;
+      sy += v.w;
+    }
+    const float mx = xr / sx, my = xi / sx;
+    *amp = make_float2(mx, my);
+    *corr = sqrtf((xr * xr + xi * xi) / (sx * sy));
+  } else {
+    double sx = 0.0, sy = 0.0;
+    for (int b = 0; b < zblocks; ++b) sx += totals[b];
+    for (int b = 0; b < zblocks; ++b) sy += totals[zblocks + b];
+    double xr = 0.0, xi = 0.0;
+    for (int b = 0; b < nparts; ++b) {
+      xr += part64[b].x;
+      xi += part64[b].y;
+    }
+    *amp = make_float2((float)(xr / sx), (float)(xi / sx));
+    *corr = (float)sqrt((xr * xr + xi * xi) / (sx * sy));
+  }
+}
+
+void launch_modamp_finish(bool fp32, const double2* part64, const float4* part32, int nparts, const double* totals,
+                          int zblocks, float2* amp, float* corr, hipStream_t s) {
+  hipLaunchKernelGGL(k_modamp_finish, dim3(1), dim3(64), 0, s, fp32 ? 1 : 0, part64, part32, nparts, totals,
+                     zblocks, amp, corr);
+  SIMR_LAUNCH_CHECK();
+}
+
+__device__ __forceinline__ float2 cmul_dev(float2 a, float2 b) {
+#pragma clang fp contract(off)
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ void k_amp_policy(AmpPolicy a, float2* __restrict__ amps) {
+#pragma clang fp contract(off)
+  const int dir = threadIdx.x;
+  if (dir >= a.ndirs) return;
+  float2* am = amps + dir * a.norders;
+  if (a.chain)
+    for (int order = 2; order < a.norders; ++order) am[order] = cmul_dev(am[order], am[order - 1]);
+  if (a.nforceamp > 0 && a.forceamp[0] > 0.0f) {
+    for (int order = 1; order < a.norders; ++order) {
+      const float m = sqrtf(am[order].x * am[order].x + am[order].y * am[order].y);
+      if (order - 1 < a.nforceamp && m < a.forceamp[order - 1]) {
+        const float f = a.forceamp[order - 1] / m;
+        am[order].x *= f;
+        am[order].y *= f;
+      }
+    }
+  }
+  if (!a.fit_all_phases) {
+    const float base_phase = atan2f(am[1].y, am[1].x);
+    for (int order = 2; order < a.norders; ++order) {
+      const float2 amplitude = make_float2(sqrtf(am[order].x * am[order].x + am[order].y * am[order].y), 0.f);
+      const float phi = (float)order * base_phase;
+      am[order] = cmul_dev(amplitude, make_float2((float)cos((double)phi), (float)sin((double)phi)));
+    }
+  }
+}
+
+void launch_amp_policy(const AmpPolicy& a, float2* amps, hipStream_t s) {
+  if (a.ndirs > 64) throw ArgError("amp policy: too many directions");
+  hipLaunchKernelGGL(k_amp_policy, dim3(1), dim3(64), 0, s, a, amps);
   SIMR_LAUNCH_CHECK();
 }
 

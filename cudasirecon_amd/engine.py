@@ -70,6 +70,7 @@ def lib():
         "simr_sync": (ip, [vp]),
         "simr_stream": (vp, [vp]),
         "simr_enable_timing": (ip, [vp, ip]),
+        "simr_time_stage": (ip, [vp, C.c_char_p]),
         "simr_stage_timings": (ip, [vp, C.POINTER(C.c_char_p), fp, ip]),
         "simr_fft3d_dev": (ip, [vp, ip, ip, ip, ip, vp]),
         "simr_fft3d_host": (ip, [fp, ip, ip, ip, ip]),
@@ -285,6 +286,10 @@ class Reconstructor:
 
     def enable_timing(self, on=True):
         _check(lib().simr_enable_timing(self.h, int(on)))
+
+    def time_stage(self, name=None):
+        """Restrict stage timing to one stage name (None: every stage)."""
+        _check(lib().simr_time_stage(self.h, None if name is None else name.encode()))
 
     def timings(self, cap=256):
         names = (C.c_char_p * cap)()

@@ -94,7 +94,13 @@ typedef struct simr_params {
   int32_t bBessel;
   float BesselNA;              /* excitation NA of the Bessel beam */
   float BesselLambdaEx;        /* excitation wavelength in microns */
-  int32_t reserved[9];
+  /* 1 (default): a time point that takes time 0's k0 (ntimes > 1, t > 0,
+     bUseTime0k0) runs its modamp fit, filter and assembly as one captured
+     HIP graph per context, with the modamps computed on the device; 0: the
+     same kernels launched one by one, the modamps computed on the host.
+     Both give the same result bit for bit.  (No counterpart in the reference.) */
+  int32_t use_graphs;
+  int32_t reserved[8];
 } simr_params;
 
 /* Per-direction fit results, the observable outputs the reference prints. */
@@ -278,6 +284,12 @@ void* simr_stream(simr_ctx* ctx);
  * the context stream.  names/ms arrays of capacity cap; returns count. */
 int simr_stage_timings(simr_ctx* ctx, const char** names, float* ms, int cap);
 int simr_enable_timing(simr_ctx* ctx, int on);
+/* Restricts the timing to the stages named `stage` (NULL or "": every stage),
+ * so that a timed run carries the events of one kernel only.  Stage names:
+ * resample, section_sums, rows_fwd, cols_fwd_y, cols_fwd_z, ov_x, ov_y,
+ * ov_zsum, modamp, findk0_peak, filter0 (order 0), filter (side bands),
+ * asm_z, asm_y, asm_x. */
+int simr_time_stage(simr_ctx* ctx, const char* stage);
 
 /*
  * Serving: a request loop over many stacks of mixed sizes on one GPU (no

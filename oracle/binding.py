@@ -41,6 +41,7 @@ def lib():
             ("orc_get_sepmatrix", [vp, dp]),
             ("orc_makeoverlaps", [vp, ip, ip, ip, C.c_float, C.c_float, dp, dp, dp, fp, fp]),
             ("orc_fft3d_f", [fp, ip, ip, ip, ip]), ("orc_fft3d_d", [dp, ip, ip, ip, ip]),
+            ("orc_set_flatfield", [vp, fp, fp]),
         ]:
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = args
@@ -56,6 +57,10 @@ def lib():
         L.orc_makeotf.restype = C.c_int
         L.orc_makeotf.argtypes = [C.POINTER(SimrOtfParams), fp, ip, ip, ip, fp, fp, fp, C.POINTER(SimrOtfInfo)]
         L.orc_makeotf_error.restype = C.c_char_p
+        L.orc_set_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
+        L.orc_get_threads.argtypes = []
         _lib = L
     return _lib
 
@@ -71,6 +76,15 @@ def _dp(a):
 def _check(rc):
     if rc != 0:
         raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops from now on."""
+    _check(lib().orc_set_threads(int(n)))
+
+
+def get_threads():
+    return lib().orc_get_threads()
 
 
 def fitparabola(a1, a2, a3):
@@ -123,6 +137,12 @@ class Oracle:
     def load(self, raw):
         raw = np.ascontiguousarray(raw, dtype=np.float32)
         _check(lib().orc_load(self.h, _fp(raw)))
+
+    def set_flatfield(self, background=None, slope=None):
+        """Per-pixel background and slope maps ([ny][nx] floats), or None for the constants."""
+        bg = None if background is None else np.ascontiguousarray(background, dtype=np.float32)
+        sl = None if slope is None else np.ascontiguousarray(slope, dtype=np.float32)
+        _check(lib().orc_set_flatfield(self.h, None if bg is None else _fp(bg), None if sl is None else _fp(sl)))
 
     def rescale(self, it=0):
         _check(lib().orc_rescale(self.h, it))

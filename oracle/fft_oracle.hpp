@@ -36,6 +36,7 @@ inline C2<T> cmulc(C2<T> a, C2<T> b) {
 template <typename T>
 struct FFTPlan {
   int n = 0;
+  int max_radix = 1;
   std::vector<int> radix;
   std::vector<C2<T>> tw;  // tw[m] = exp(-2 pi i m / n)
 
@@ -60,6 +61,7 @@ struct FFTPlan {
         m /= p;
       }
     }
+    for (int r : radix) max_radix = r > max_radix ? r : max_radix;
     tw.resize(n);
     const long double twopi = 6.283185307179586476925286766559L;
     for (int k = 0; k < n; ++k) {
@@ -75,8 +77,8 @@ struct FFTPlan {
     C2<T>* src = data;
     C2<T>* dst = work;
     int Ns = 1;
-    C2<T> v[64];
-    C2<T> o[64];
+    // one butterfly's inputs and outputs (a prime radix can be as long as n)
+    std::vector<C2<T>> v(max_radix), o(max_radix);
     for (int R : radix) {
       const int m = n / R;
       const int twstep = n / (Ns * R);

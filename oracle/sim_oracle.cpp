@@ -27,6 +27,8 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 #include "../include/simr.h"
 #include "fft_oracle.hpp"
 
@@ -1349,6 +1351,27 @@ int orc_get_result_d(void* h, double* out) {
 }
 int orc_get_fit(void* h, int dir, simr_fit* out) { return DISPATCH(h, copy_fit(o, dir, out)); }
 int orc_set_fit(void* h, int dir, const simr_fit* in) { return DISPATCH(h, set_fit(o, dir, in)); }
+// Flat-field maps of bgAndSlope (cudaSirecon.cpp:139-161): background and
+// slope per raw pixel (max(nx, nx_raw) * ny floats each); NULL restores the
+// constant background and unit slope.
+int orc_set_flatfield(void* h, const float* bg, const float* slope) {
+  return DISPATCH(h, {
+    const size_t n = o.background.size();
+    for (size_t e = 0; e < n; ++e) {
+      o.background[e] = bg ? (decltype(o.inscale))bg[e] : (decltype(o.inscale))o.P.constbkgd;
+      o.slope[e] = slope ? (decltype(o.inscale))slope[e] : (decltype(o.inscale))1;
+    }
+  });
+}
+
+// OpenMP threads of the oracle's parallel loops from here on (timing runs).
+int orc_set_threads(int n) {
+  if (n < 1) return -1;
+  omp_set_num_threads(n);
+  return 0;
+}
+int orc_get_threads(void) { return omp_get_max_threads(); }
+
 int orc_get_inscale(void* h, double* out) { return DISPATCH(h, *out = (double)o.inscale); }
 int orc_get_zdist(void* h, int* out) {
   return DISPATCH(h, {

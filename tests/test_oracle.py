@@ -15,8 +15,8 @@ from oracle import binding as ob
 
 
 # ------------------------------------------------------------------ FFT
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16, 21, 24, 25, 27, 40, 48, 49, 64, 100, 384,
-                               640, 1024])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16, 21, 24, 25, 27, 40, 48, 49, 64, 100, 131,
+                               262, 384, 640, 1024])
 def test_fft_matches_numpy_1d(n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
@@ -410,3 +410,44 @@ def test_deskew_interpolation_reproduces_a_linear_ramp():
         got = secs[1][2][z]
         assert np.allclose(got[:, ok], exp[ok], rtol=2e-6, atol=0), z
         assert (got[:, xin < 0] == 0).all()
+
+
+# ------------------------------------------------------------------ flat field
+@pytest.mark.parametrize("prec", [32, 64])
+def test_flatfield_maps_undo_a_known_camera_response(prec):
+    """--usecorr (bgAndSlope, cudaSirecon.cpp:139-192; load_and_flatfield
+    1715-1731): each raw pixel becomes (raw - background) * slope.  A stack
+    recorded through a camera with offset b(x, y) and gain 1/s(x, y), i.e.
+    raw = b + clean / s, reconstructs with the maps (b, s) as the clean stack
+    does with background 0 and unit slope."""
+    p, otf, raw, _ = synthetic(64, 64, 8, seed=21)
+    clean = raw.astype(np.float64) - p.constbkgd
+    rng = np.random.default_rng(4)
+    b = rng.uniform(80.0, 120.0, (p.ny, p.nx))
+    s = rng.uniform(0.8, 1.25, (p.ny, p.nx))
+    recorded = (b + clean / s).astype(np.float32)
+    q = p.copy()
+    q.constbkgd = 0.0
+    o_ref = ob.Oracle(q, otf, prec, True)
+    o_ref.load(clean.astype(np.float32))
+    o_ref.rescale(0)
+    o_ref.set_time(0)
+    o_ref.process_volume()
+    o = ob.Oracle(q, otf, prec, True)
+    o.set_flatfield(b.astype(np.float32), s.astype(np.float32))
+    o.load(recorded)
+    o.rescale(0)
+    o.set_time(0)
+    o.process_volume()
+    for d in range(p.ndirs):
+        f, g = o.fit(d), o_ref.fit(d)
+        assert abs(f.k0x - g.k0x) < 1e-4 and abs(f.k0y - g.k0y) < 1e-4
+    assert rel_l2(o.result(double=prec == 64), o_ref.result(double=prec == 64)) < 2e-6
+    # a wrong map (background left out) does not pass
+    o_bad = ob.Oracle(q, otf, prec, True)
+    o_bad.set_flatfield(np.zeros_like(b, np.float32), s.astype(np.float32))
+    o_bad.load(recorded)
+    o_bad.rescale(0)
+    o_bad.set_time(0)
+    o_bad.process_volume()
+    assert rel_l2(o_bad.result(), o_ref.result()) > 1e-3
