@@ -1,6 +1,7 @@
 // extern "C" stage-level ABI (include/simr.h).  Nothing throws across it:
 // every entry point maps exceptions to a simr_status and a thread-local message.
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -82,6 +83,39 @@ struct DevMem {
     if (p) (void)hipFree(p);
   }
 };
+}  // namespace
+
+// ---- FFT without allocation (simr_fft3d_dev_noalloc)
+//
+// The device memory simr_fft3d_dev takes from the library (its twiddle tables
+// and the direct-DFT scratch volume) comes here from one file-scope device
+// array per GPU, so the call makes no allocation at all.  Its layout: three
+// twiddle tables of up to 4096 entries (x, y, z), then the scratch volume.
+// Every call on a GPU uses that GPU's one array: calls that overlap on one
+// GPU (other streams or threads) race.
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr size_t kNoAllocTwBytes = (size_t)4096 * sizeof(float2);
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The current device's address of `scratch` (the host-side name is only a
+// stand-in), looked up on the first call on each device.
+unsigned char* noalloc_scratch() {
+  int dev = 0;
+  SIMR_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) throw simr::ArgError("device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SIMR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw simr::HipError("hipGetSymbolAddress returned no address for the FFT scratch");
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  return g_scratch_ptr[dev];
+}
 }  // namespace
 
 extern "C" {
@@ -291,6 +325,31 @@ int simr_fft3d_dev(float* data, int nx, int ny, int nz, int inverse, void* strea
   });
 }
 
+
+size_t simr_fft3d_noalloc_max_elems(void) { return (kNoAllocScratchBytes - 3 * kNoAllocTwBytes) / sizeof(float2); }
+
+int simr_fft3d_dev_noalloc(float* data, int nx, int ny, int nz, int inverse, void* stream) {
+  return guard([&] {
+    check_fft_dims(data, nx, ny, nz);
+    const bool need = simr::fft3d_needs_scratch(nx, ny, nz);
+    const size_t elems = (size_t)nx * ny * nz;
+    if (need && elems > simr_fft3d_noalloc_max_elems())
+      throw simr::ArgError("simr_fft3d_dev_noalloc: " + std::to_string(elems) +
+                           " complex values exceed its scratch of " + std::to_string(simr_fft3d_noalloc_max_elems()) +
+                           " (a length without a transform kernel needs a scratch volume of the array's size)");
+    hipStream_t s = (hipStream_t)stream;
+    unsigned char* base = noalloc_scratch();
+    float2* tw[3];
+    const int len[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) {
+      tw[a] = (float2*)(base + a * kNoAllocTwBytes);
+      if (len[a] > 1) simr::twiddles_into(tw[a], len[a], s);  // rewritten every call
+    }
+    const float2* twc[3] = {tw[0], tw[1], tw[2]};
+    simr::fft3d_any((float2*)data, need ? (float2*)(base + 3 * kNoAllocTwBytes) : nullptr, nx, ny, nz,
+                    inverse != 0, s, twc);
+  });
+}
 
 int simr_fft3d_host(float* data, int nx, int ny, int nz, int inverse) {
   return guard([&] {

@@ -221,6 +221,19 @@ int simr_debug_overlap_volumes(simr_ctx* ctx, int dir, int order1, int order2, f
 int simr_fft3d_dev(float* data, int nx, int ny, int nz, int inverse, void* stream);
 int simr_fft3d_host(float* data, int nx, int ny, int nz, int inverse);
 int simr_fft_size_supported(int n);
+/*
+ * simr_fft3d_dev without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments,
+ * lengths and results as simr_fft3d_dev; asynchronous on `stream` in every
+ * case.  Its twiddle tables and direct-DFT scratch live in one 64 MiB device
+ * array per GPU, held until the process ends: with a length that is not
+ * 2^a 3^b 5^c (or 7) the array holds at most simr_fft3d_noalloc_max_elems()
+ * complex values (larger arrays are refused with SIMR_ERR_ARG).  Calls that
+ * overlap on one GPU share that array and race; serialise them.
+ */
+int simr_fft3d_dev_noalloc(float* data, int nx, int ny, int nz, int inverse, void* stream);
+size_t simr_fft3d_noalloc_max_elems(void);
 
 /*
  * makeotf: PSF -> rotationally averaged OTF (replaces the reference's
