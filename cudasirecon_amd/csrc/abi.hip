@@ -14,7 +14,7 @@ namespace simr {
 void otf_params_default(simr_otf_params* p);
 size_t otf_elems(const simr_otf_params& p, int nx, int ny, int nsections);
 void make_otf(const simr_otf_params& p, const float* psf, int nx, int ny, int nsec, const float* section_bg,
-              float* otf_out, float* band0_out, simr_otf_info* info);
+              float* otf_out, float* band0_out, simr_otf_info* info, bool i2m = false);
 }  // namespace simr
 
 struct simr_ctx {
@@ -226,6 +226,14 @@ int simr_makeotf(const simr_otf_params* p, const float* psf, int nx, int ny, int
   });
 }
 
+int simr_makeotf_i2m(const simr_otf_params* p, const float* i2m, int nx, int ny, int nz, float* otf,
+                     simr_otf_info* info) {
+  return guard([&] {
+    if (!p) throw simr::ArgError("null makeotf parameter block");
+    simr::make_otf(*p, i2m, nx, ny, nz, nullptr, otf, nullptr, info, true);
+  });
+}
+
 int simr_update_params(simr_ctx* ctx, const simr_params* p) {
   return guard([&] {
     if (!p) throw simr::ArgError("null parameter block");
@@ -248,6 +256,21 @@ int simr_filter_assemble(simr_ctx* ctx) { return guard([&] { E(ctx).filter_assem
 int simr_process_volume(simr_ctx* ctx) { return guard([&] { E(ctx).process_volume(); }); }
 
 size_t simr_result_elems(const simr_ctx* ctx) { return ctx && ctx->eng ? ctx->eng->result_elems() : 0; }
+void* simr_host_alloc(size_t bytes) {
+  void* p = nullptr;
+  const int rc = guard([&] {
+    if (bytes == 0) throw simr::ArgError("simr_host_alloc: zero bytes");
+    SIMR_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+  });
+  return rc == SIMR_OK ? p : nullptr;
+}
+
+int simr_host_free(void* p) {
+  return guard([&] {
+    if (p) SIMR_HIP(hipHostFree(p));
+  });
+}
+
 int simr_get_result_host(simr_ctx* ctx, float* out) { return guard([&] { E(ctx).get_result(out); }); }
 int simr_get_result_dev(simr_ctx* ctx, float* out_dev) {
   return guard([&] {

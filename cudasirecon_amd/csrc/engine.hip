@@ -1,11 +1,9 @@
 // Host orchestration of one reconstruction context.  See engine.hpp.
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
-#include <thread>
 
 #include "engine.hpp"
 #include "fft_api.hpp"
@@ -18,7 +16,15 @@ constexpr double kPI = 3.14159265358979323846;
 constexpr float K0_WARNING_THRESH = 0.05f;  // cudaSireconImpl.h:54
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
-constexpr int kMaxCand = 64;  // line-search candidates reduced per launch
+constexpr int kMaxCand = 128;  // modamp candidates (of every direction) reduced per launch
+// the fit's zero-copy host region: candidate vectors, their directions, the
+// modamp sums (4 doubles each), the peak values (8 floats per direction) and
+// peak indices
+constexpr size_t kZcSlot = ((size_t)kMaxCand * 8 + 255) & ~(size_t)255;
+constexpr size_t kZcEval = kZcSlot + (((size_t)kMaxCand * 4 + 255) & ~(size_t)255);
+constexpr size_t kZcPeakV = kZcEval + (size_t)kMaxCand * 32;
+constexpr size_t kZcPeakI = kZcPeakV + (size_t)kMaxDirs * 32;
+constexpr size_t kZcBytes = kZcPeakI + 256;
 
 int norders_of(const simr_params& p) { return p.norders_output != 0 ? p.norders_output : p.nphases / 2 + 1; }
 
@@ -113,17 +119,30 @@ void Arena::release() {
 }
 
 // ------------------------------------------------------------------- layout
-// Bytes of one direction's fit scratch, in the order Engine::carve_fit lays it out.
-static size_t fit_set_bytes(const Layout& L, int ncand) {
-  size_t b = 0;
-  b += align256((size_t)2 * (2 * L.zd_ov_max + 1) * L.nxy * 8);  // overlap x/y stage
-  b += align256((size_t)L.nxy * 16) + 2 * align256((size_t)L.nxy * 8) + align256((size_t)L.nxy * 8);  // P, SX, SY, cc
-  b += 4 * align256(4096) + align256((size_t)ncand * 8);  // peak search, candidates
-  b += align256((size_t)ncand * modamp_partials_fp64(L.nx, L.ny) * 16);
-  b += align256((size_t)ncand * modamp_partials_fp32(L.nx, L.ny) * 16);
-  b += align256((size_t)2 * kZsumMaxBlocks * 8);
+namespace {
+// The fit's device scratch (aliasing the assembly staging region), in the
+// order the constructor carves it: every direction's overlap stage, one after
+// another; every direction's cross-correlation, one after another; per
+// direction P, SX, SY and the z-sum partials; the batch buffers.
+size_t ovx_elems(const Layout& L) { return (size_t)2 * (2 * L.zd_ov_max + 1) * L.nxy; }
+size_t fit_block_bytes(const Layout& L) {
+  size_t b = align256((size_t)L.ndirs * ovx_elems(L) * 8);
+  b += align256((size_t)L.ndirs * L.nxy * 8);
+  b += (size_t)L.ndirs * (align256((size_t)L.nxy * 16) + 2 * align256((size_t)L.nxy * 8) +
+                          align256((size_t)2 * kZsumMaxBlocks * 8));
+  b += align256((size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16);
+  b += align256((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
+  b += 2 * align256((size_t)kMaxDirs * kArgBlocks * 4);
+  b += align256(kZcEval);  // the candidates' device copy
   return b;
 }
+// The overlap dumps' scratch, in its own region: one overlap stage, P, SX,
+// SY and the z-sum partials.
+size_t dbg_set_bytes(const Layout& L) {
+  return align256(ovx_elems(L) * 8) + align256((size_t)L.nxy * 16) + 2 * align256((size_t)L.nxy * 8) +
+         align256((size_t)2 * kZsumMaxBlocks * 8);
+}
+}  // namespace
 
 Layout Layout::of(const simr_params& p) {
   Layout L{};
@@ -132,8 +151,10 @@ Layout Layout::of(const simr_params& p) {
   L.norders = norders_of(p);
   SIMR_REQUIRE(L.norders >= 1 && L.norders <= SIMR_MAX_ORDERS, "norders must be in 1..3");
   SIMR_REQUIRE(2 * L.norders - 1 == p.nphases, "nphases must equal 2*norders-1");
-  SIMR_REQUIRE(p.bRadAvgOTF, "only rotationally averaged OTFs are supported (otfRA)");
-  SIMR_REQUIRE(p.nxotf >= 2 && p.nzotf >= 1 && p.nyotf == 1, "invalid radially averaged OTF geometry");
+  if (p.bRadAvgOTF)
+    SIMR_REQUIRE(p.nxotf >= 2 && p.nzotf >= 1 && p.nyotf == 1, "invalid radially averaged OTF geometry");
+  else
+    SIMR_REQUIRE(p.nxotf >= 2 && p.nyotf >= 1 && p.nzotf >= 1, "invalid OTF geometry");
   SIMR_REQUIRE(p.dkrotf > 0 && p.dkzotf > 0, "OTF pixel sizes must be positive");
   SIMR_REQUIRE(p.dxy > 0, "dxy must be positive");
   SIMR_REQUIRE(p.zoomfact >= 1 && p.z_zoom >= 1, "zoom factors must be >= 1");
@@ -167,7 +188,6 @@ Layout Layout::of(const simr_params& p) {
   SIMR_REQUIRE(in_list_zoom(L.xdim) && in_list_zoom(L.ydim), "zoomed lateral size is not a supported size");
   SIMR_REQUIRE(L.zdim >= 1 && L.zdim <= 1024, "zoomed stack depth is not a supported size (1 to 1024)");
   SIMR_REQUIRE(2 * L.zd_ov_max + 1 <= p.nz, "overlap z support wider than the stack");
-  const int ncand = kMaxCand;
   size_t b = 0;
   b += align256((size_t)L.nimg * L.nxy_raw * 4);        // raw (float or uint16)
   b += 2 * align256((size_t)L.nxy_raw * 4);              // bg, slope
@@ -175,13 +195,13 @@ Layout Layout::of(const simr_params& p) {
   b += align256((size_t)L.ndirs * L.nbands * L.H * 8);   // bands
   b += align256((size_t)L.nDirsOTF * L.norders * L.otfsize * 8);
   b += 3 * align256((size_t)L.ndirs * SIMR_MAX_ORDERS * 8);  // device modamps of graphed time points
-  const size_t fit = fit_set_bytes(L, ncand);  // direction 0's fit scratch
-  b += fit;
-  // assembly staging; the fit scratch of directions 1.. aliases it
+  b += align256((size_t)kMaxDirs * 4);                       // their candidates' direction slots
+  b += dbg_set_bytes(L);  // the overlap dumps' scratch
+  // assembly staging; the fit scratch aliases it
   const size_t stage = align256((size_t)L.nasm * L.zdim * L.nxy * 8) +            // stageZ
                        align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * p.nx * 8);  // stageY
   // (and the resampled float planes, which live there while the bands are formed)
-  b += std::max(std::max(stage, (size_t)(L.ndirs - 1) * fit), L.resample ? (size_t)L.nimg * (size_t)L.nxy * 4 : (size_t)0);
+  b += std::max(std::max(stage, fit_block_bytes(L)), L.resample ? (size_t)L.nimg * (size_t)L.nxy * 4 : (size_t)0);
   b += align256((size_t)L.zdim * L.ydim * (size_t)L.xdim * 4);              // output
   L.bytes_total = b;
   return L;
@@ -210,11 +230,11 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   kc_dev_ = (float2*)arena_.take((size_t)L.ndirs * SIMR_MAX_ORDERS * 8);
   amps_dev_ = (float2*)arena_.take((size_t)L.ndirs * SIMR_MAX_ORDERS * 8);
   corr_dev_ = (float*)arena_.take((size_t)L.ndirs * SIMR_MAX_ORDERS * 8);
-  const size_t fitb = fit_set_bytes(L, kMaxCand);
-  char* fit0 = (char*)arena_.take(fitb);
+  slot_ids_ = (int*)arena_.take((size_t)kMaxDirs * 4);
+  char* dbg = (char*)arena_.take(dbg_set_bytes(L));
   const size_t zb = align256((size_t)L.nasm * L.zdim * L.nxy * 8);
   const size_t yb = align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * L.nx * 8);
-  stage_bytes_ = std::max(std::max(zb + yb, (size_t)(L.ndirs - 1) * fitb),
+  stage_bytes_ = std::max(std::max(zb + yb, fit_block_bytes(L)),
                           L.resample ? (size_t)L.nimg * (size_t)L.nxy * 4 : (size_t)0);
   char* stage = (char*)arena_.take(stage_bytes_);
   stageZ_ = (float2*)stage;
@@ -222,29 +242,64 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   stageY_ = (float2*)(stage + zb);
   out_ = (float*)arena_.take((size_t)L.zdim * L.ydim * (size_t)L.xdim * 4);
 
-  hstage_bytes_ = std::max<size_t>((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16,
-                                   (size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16) +
-                  (size_t)2 * kZsumMaxBlocks * 8 + 8192;
-  fw_.resize(L.ndirs);
-  for (int d = 0; d < L.ndirs; ++d) {
-    FitWork& w = fw_[d];
-    char* at = d == 0 ? fit0 : stage + (size_t)(d - 1) * fitb;
-    size_t unused = 0;
-    carve_fit(w, at, false, unused);
-    if (d == 0) {
-      w.stream = stream_;
-    } else {
-      SIMR_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
-      w.own_stream = true;
+  // fit scratch: the layout fit_block_bytes counts, at the start of the staging region
+  nzs_stride_ = 2 * L.zd_ov_max + 1;
+  {
+    char* at = stage;
+    auto take = [&](size_t n) {
+      char* q = at;
+      at += align256(n);
+      return q;
+    };
+    fw_.resize(L.ndirs);
+    float2* ovx = (float2*)take((size_t)L.ndirs * ovx_elems(L) * 8);
+    float2* cc = (float2*)take((size_t)L.ndirs * L.nxy * 8);
+    for (int d = 0; d < L.ndirs; ++d) {
+      FitWork& w = fw_[d];
+      w.ovx = ovx + (size_t)d * ovx_elems(L);
+      w.cc = cc + (size_t)d * L.nxy;
+      w.P = (double2*)take((size_t)L.nxy * 16);
+      w.SX = (double*)take((size_t)L.nxy * 8);
+      w.SY = (double*)take((size_t)L.nxy * 8);
+      w.sumparts = (double*)take((size_t)2 * kZsumMaxBlocks * 8);
     }
-    SIMR_HIP(hipHostMalloc(&w.hstage, hstage_bytes_, hipHostMallocDefault));
-    SIMR_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
+    part64_ = (double2*)take((size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16);
+    part32_ = (float4*)take((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
+    pv_ = (float*)take((size_t)kMaxDirs * kArgBlocks * 4);
+    pi_ = (int*)take((size_t)kMaxDirs * kArgBlocks * 4);
+    kdev_ = take(kZcEval);
+    SIMR_REQUIRE((size_t)(at - stage) <= stage_bytes_, "fit scratch larger than the staging region");
+  }
+  {
+    char* at = dbg;
+    auto take = [&](size_t n) {
+      char* q = at;
+      at += align256(n);
+      return q;
+    };
+    dbg_.ovx = (float2*)take(ovx_elems(L) * 8);
+    dbg_.P = (double2*)take((size_t)L.nxy * 16);
+    dbg_.SX = (double*)take((size_t)L.nxy * 8);
+    dbg_.SY = (double*)take((size_t)L.nxy * 8);
+    dbg_.sumparts = (double*)take((size_t)2 * kZsumMaxBlocks * 8);
+  }
+  {
+    void* h = nullptr;
+    void* d = nullptr;
+    SIMR_HIP(hipHostMalloc(&h, kZcBytes, hipHostMallocMapped | hipHostMallocCoherent));
+    hzc_ = (char*)h;
+    SIMR_HIP(hipHostGetDevicePointer(&d, h, 0));
+    dzc_ = (char*)d;
+  }
+  SIMR_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+  {
+    int ids[kMaxDirs];
+    for (int k = 0; k < kMaxDirs; ++k) ids[k] = k;
+    SIMR_HIP(hipMemcpyAsync(slot_ids_, ids, sizeof(ids), hipMemcpyHostToDevice, stream_));
+    SIMR_HIP(hipStreamSynchronize(stream_));
   }
 
   SIMR_HIP(hipHostMalloc(&fit_host_, (size_t)L.ndirs * L.norders * 12, hipHostMallocDefault));
-  SIMR_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
-  join_ev_.assign(L.ndirs, nullptr);
-  for (auto& e : join_ev_) SIMR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   {
     // order 0's modamp is 1 (setup_common, cudaSirecon.cpp:1257-1260); the
     // graphed fit writes the others
@@ -336,45 +391,36 @@ Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
-  if (fork_ev_) (void)hipEventDestroy(fork_ev_);
-  for (hipEvent_t e : join_ev_)
-    if (e) (void)hipEventDestroy(e);
   if (fit_host_) (void)hipHostFree(fit_host_);
+  if (hzc_) (void)hipHostFree(hzc_);
+  if (done_) (void)hipEventDestroy(done_);
   for (auto& e : events_) {
     (void)hipEventDestroy(e.second.first);
     (void)hipEventDestroy(e.second.second);
   }
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
-  for (auto& w : fw_) {
-    if (w.own_stream) {
-      (void)hipStreamSynchronize(w.stream);
-      (void)hipStreamDestroy(w.stream);
-    }
-    if (w.done) (void)hipEventDestroy(w.done);
-    if (w.hstage) (void)hipHostFree(w.hstage);
-  }
   arena_.release();
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 void Engine::sync() { SIMR_HIP(hipStreamSynchronize(stream_)); }
 
-// Waits for the work queued on a fit stream.  The fit's read-backs are small
-// and on its critical path, so the event is polled: the wait ends within
-// microseconds of the copy, where a blocking synchronize adds tens of them.
-// After 2 ms of polling (a busy device, e.g. other contexts' work) it falls
-// back to the blocking wait.
-void Engine::wait_fit(FitWork& w) {
-  SIMR_HIP(hipEventRecord(w.done, w.stream));
+// Waits for the work queued on the context stream.  The fit's read-backs are
+// small and on its critical path, so the event is polled: the wait ends
+// within microseconds of the copy, where a blocking synchronize adds tens of
+// them.  After 2 ms of polling (a busy device, e.g. other contexts' work) it
+// falls back to the blocking wait.
+void Engine::wait_fit() {
+  SIMR_HIP(hipEventRecord(done_, stream_));
   const auto t0 = std::chrono::steady_clock::now();
   for (int i = 0;; ++i) {
-    const hipError_t e = hipEventQuery(w.done);
+    const hipError_t e = hipEventQuery(done_);
     if (e == hipSuccess) return;
     if (e != hipErrorNotReady) SIMR_HIP(e);
     if ((i & 15) == 15 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
     for (int k = 0; k < 32; ++k) __builtin_ia32_pause();  // a few microseconds between queries
   }
-  SIMR_HIP(hipEventSynchronize(w.done));
+  SIMR_HIP(hipEventSynchronize(done_));
 }
 
 void Engine::enable_timing(bool on) {
@@ -389,8 +435,7 @@ void Engine::enable_timing(bool on) {
   }
 }
 
-// Stage timing: a pair of events around the stage's launches on its stream
-// (the fit's directions record on their own streams, from their threads).
+// Stage timing: a pair of events around the stage's launches on its stream.
 int Engine::stage_begin(const char* name, hipStream_t s) {
   if (!timing_ || (!timing_only_.empty() && timing_only_ != name)) return -1;
   hipEvent_t a, b;
@@ -419,29 +464,6 @@ void Engine::stage_end(int id, hipStream_t s) {
     b = events_[id].second.second;
   }
   SIMR_HIP(hipEventRecord(b, s));
-}
-
-// Fit scratch of one direction at `at` (the layout fit_set_bytes counts).
-void Engine::carve_fit(FitWork& w, char*& at, bool count_only, size_t& bytes) const {
-  auto take = [&](size_t n) {
-    char* p = at;
-    at += align256(n);
-    bytes += align256(n);
-    return count_only ? nullptr : p;
-  };
-  w.ovx = (float2*)take((size_t)2 * (2 * L.zd_ov_max + 1) * L.nxy * 8);
-  w.P = (double2*)take((size_t)L.nxy * 16);
-  w.SX = (double*)take((size_t)L.nxy * 8);
-  w.SY = (double*)take((size_t)L.nxy * 8);
-  w.cc = (float2*)take((size_t)L.nxy * 8);
-  w.pv = (float*)take(4096);
-  w.pi = (int*)take(4096);
-  w.peakvals = (float*)take(4096);
-  w.peakidx = (int*)take(4096);
-  w.kcand = (float2*)take((size_t)kMaxCand * 8);
-  w.part64 = (double2*)take((size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16);
-  w.part32 = (float4*)take((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
-  w.sumparts = (double*)take((size_t)2 * kZsumMaxBlocks * 8);
 }
 
 int Engine::timings(const char** names, float* ms, int cap) {
@@ -586,18 +608,20 @@ int Engine::zd_overlaps() const { return L.zd_ov_max; }
 // fitorder2 of findk0 / fitk0andmodamps (gpuFunctionsImpl.cu:316-323, 880-887)
 int Engine::fit_order2() const { return L.nz > 1 && !P.bBessel ? 2 : 1; }
 
-OtfRA Engine::otf_table(int dir, int order) const {
+OtfTab Engine::otf_table(int dir, int order) const {
   const int d = P.bOneOTFperAngle ? dir : 0;
-  OtfRA o;
+  OtfTab o;
   o.t = otf_ + ((size_t)d * L.norders + order) * L.otfsize;
   o.nzotf = P.nzotf;
   o.nxotf = P.nxotf;
   o.dkrotf = P.dkrotf;
+  o.nyotf = P.bRadAvgOTF ? 1 : P.nyotf;
+  o.ra = P.bRadAvgOTF ? 1 : 0;
   return o;
 }
 
 // ------------------------------------------------------------ overlaps / fit
-void Engine::make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float k0y, bool per_elem) {
+OvParams Engine::ov_params(int dir, int o1, int o2, float k0x, float k0y) const {
   const Geo g = geometry();
   OvParams p{};
   p.nx = L.nx;
@@ -635,75 +659,109 @@ void Engine::make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float
   p.f2 = p.order0_2_factor * p.order0_2_factor;
   p.inv_dkr2_1 = (float)(1.0 / ((double)p.otf1.dkrotf * (double)p.otf1.dkrotf));
   p.inv_dkr2_2 = (float)(1.0 / ((double)p.otf2.dkrotf * (double)p.otf2.dkrotf));
-  w.zd = p.zd;
-  int st = stage_begin("ov_x", w.stream);
-  launch_overlaps_x(p, w.ovx, w.stream);
-  stage_end(st, w.stream);
-  st = stage_begin("ov_y", w.stream);
-  launch_overlaps_y(p, w.ovx, w.stream);
-  stage_end(st, w.stream);
-  st = stage_begin("ov_zsum", w.stream);
-  launch_overlaps_z_reduce(p, w.ovx, w.P, w.SX, w.SY, w.sumparts, fit_fp32_, per_elem, w.stream);
-  stage_end(st, w.stream);
-  w.ov_totals_valid = false;
-  w.cache.clear();
-  w.cache_o1 = o1;
-  w.cache_o2 = o2;
+  return p;
 }
 
-std::vector<Engine::Eval> Engine::eval_batch(FitWork& w, const std::vector<float2>& kvecs) {
-  std::vector<Eval> res;
-  for (size_t c0 = 0; c0 < kvecs.size(); c0 += kMaxCand) {
-    const int nc = (int)std::min<size_t>(kMaxCand, kvecs.size() - c0);
-    SIMR_HIP(hipMemcpyAsync(w.kcand, kvecs.data() + c0, nc * 8, hipMemcpyHostToDevice, w.stream));
-    const int st = stage_begin("modamp", w.stream);
-    if (fit_fp32_) {
-      const int nb = modamp_partials_fp32(L.nx, L.ny);
-      launch_modamp_fp32(w.P, w.SX, w.SY, w.kcand, nc, L.nx, L.ny, P.dxy, w.part32, w.stream);
-      stage_end(st, w.stream);
-      SIMR_HIP(hipMemcpyAsync(w.hstage, w.part32, (size_t)nc * nb * 16, hipMemcpyDeviceToHost, w.stream));
-      wait_fit(w);
-      const float4* h = (const float4*)w.hstage;
-      for (int c = 0; c < nc; ++c) {
-        float xr = 0, xi = 0, sx = 0, sy = 0;  // cpuReduce order (gpuFunctionsImpl_hh.cu:162-180)
-        for (int b = 0; b < nb; ++b) {
-          const float4 v = h[(size_t)c * nb + b];
-          xr += v.x;
-          xi += v.y;
-          sx += v.z;
-          sy += v.w;
-        }
-        res.push_back({xr, xi, sx, sy});
-      }
-    } else {
-      const int nb = modamp_partials_fp64(L.nx, L.ny);
-      launch_modamp_fp64(w.P, w.kcand, nc, L.nx, L.ny, P.dxy, w.part64, w.stream);
-      stage_end(st, w.stream);
-      char* h = (char*)w.hstage;
-      SIMR_HIP(hipMemcpyAsync(h, w.part64, (size_t)nc * nb * 16, hipMemcpyDeviceToHost, w.stream));
-      const size_t soff = (size_t)kMaxCand * nb * 16;
-      const int zb = zsum_blocks(L.nxy);
-      if (!w.ov_totals_valid)
-        SIMR_HIP(hipMemcpyAsync(h + soff, w.sumparts, (size_t)2 * zb * 8, hipMemcpyDeviceToHost, w.stream));
-      wait_fit(w);
-      if (!w.ov_totals_valid) {
-        const double* sp = (const double*)(h + soff);
-        double sx = 0, sy = 0;
-        for (int b = 0; b < zb; ++b) sx += sp[b];
-        for (int b = 0; b < zb; ++b) sy += sp[zb + b];
-        w.ov_sx = sx;
-        w.ov_sy = sy;
-        w.ov_totals_valid = true;
-      }
-      const double2* hp = (const double2*)h;
-      for (int c = 0; c < nc; ++c) {
-        double xr = 0, xi = 0;
-        for (int b = 0; b < nb; ++b) {
-          xr += hp[(size_t)c * nb + b].x;
-          xi += hp[(size_t)c * nb + b].y;
-        }
-        res.push_back({xr, xi, w.ov_sx, w.ov_sy});
-      }
+// makeoverlaps (gpuFunctionsImpl.cu:497-675) for every request, at most one
+// per direction, each in its direction's slot (or all in `only`, the dumps'
+// scratch, for a single request): one launch per pass covers every request.
+void Engine::make_overlaps(const std::vector<OvReq>& reqs, bool per_elem, FitWork* only) {
+  SIMR_REQUIRE(!reqs.empty() && (int)reqs.size() <= kMaxDirs && (!only || reqs.size() == 1),
+               "overlap batch: 1..8 requests");
+  OvBatch b{};
+  ZsumBatch z{};
+  b.n = z.n = (int)reqs.size();
+  b.nzs_stride = z.nzs_stride = nzs_stride_;
+  z.nz = L.nz;
+  z.nxy = L.nxy;
+  int dmin = L.ndirs, dmax = -1;
+  for (int k = 0; k < b.n; ++k) {
+    const OvReq& r = reqs[k];
+    SIMR_REQUIRE(r.dir >= 0 && r.dir < L.ndirs, "direction out of range");
+    FitWork& w = only ? *only : fw_[r.dir];
+    for (int j = 0; j < k; ++j) SIMR_REQUIRE(reqs[j].dir != r.dir, "overlap batch: one request per direction");
+    b.p[k] = ov_params(r.dir, r.o1, r.o2, r.k0x, r.k0y);
+    b.p[k].ovx = w.ovx;
+    z.ovx[k] = w.ovx;
+    z.P[k] = w.P;
+    z.SX[k] = w.SX;
+    z.SY[k] = w.SY;
+    z.totals[k] = w.sumparts;
+    z.nzs[k] = 2 * b.p[k].zd + 1;
+    w.zd = b.p[k].zd;
+    w.cache.clear();
+    dmin = std::min(dmin, r.dir);
+    dmax = std::max(dmax, r.dir);
+  }
+  int st = stage_begin("ov_x", stream_);
+  launch_overlaps_x(b, stream_);
+  stage_end(st, stream_);
+  st = stage_begin("ov_y", stream_);
+  // the directions' stages lie one after another: one batch from the first
+  // requested to the last (a direction in between is transformed unused)
+  if (only)
+    launch_overlaps_y(L.nx, L.ny, nzs_stride_, only->ovx, 1, stream_);
+  else
+    launch_overlaps_y(L.nx, L.ny, nzs_stride_, fw_[dmin].ovx, dmax - dmin + 1, stream_);
+  stage_end(st, stream_);
+  st = stage_begin("ov_zsum", stream_);
+  launch_overlaps_z_reduce(z, fit_fp32_, per_elem, stream_);
+  stage_end(st, stream_);
+}
+
+ModampSlots Engine::modamp_slots() const {
+  ModampSlots m{};
+  for (int d = 0; d < L.ndirs; ++d) {
+    m.P[d] = fw_[d].P;
+    m.SX[d] = fw_[d].SX;
+    m.SY[d] = fw_[d].SY;
+    m.totals[d] = fw_[d].sumparts;
+  }
+  return m;
+}
+
+// The modamp reductions of every direction's candidates (req: direction and
+// its candidate vectors), in launches of up to kMaxCand candidates.  The
+// candidates reach the device in one copy, and the partials are summed per
+// candidate in block order on the device as the host does (cpuReduce,
+// gpuFunctionsImpl_hh.cu:162-180), straight into the zero-copy region: a
+// round is one copy, two kernels and one wait.
+std::vector<std::vector<Engine::Eval>> Engine::eval_multi(
+    const std::vector<std::pair<int, std::vector<float2>>>& req) {
+  std::vector<std::vector<Eval>> res(req.size());
+  std::vector<std::pair<int, int>> flat;  // (request, candidate)
+  for (size_t r = 0; r < req.size(); ++r) {
+    SIMR_REQUIRE(req[r].first >= 0 && req[r].first < L.ndirs, "direction out of range");
+    for (size_t c = 0; c < req[r].second.size(); ++c) flat.push_back({(int)r, (int)c});
+    res[r].resize(req[r].second.size());
+  }
+  const ModampSlots m = modamp_slots();
+  const int nb = fit_fp32_ ? modamp_partials_fp32(L.nx, L.ny) : modamp_partials_fp64(L.nx, L.ny);
+  float2* kv = (float2*)hzc_;
+  int* ks = (int*)(hzc_ + kZcSlot);
+  const double* ev = (const double*)(hzc_ + kZcEval);
+  const float2* dkv = (const float2*)kdev_;
+  const int* dks = (const int*)(kdev_ + kZcSlot);
+  double* dev = (double*)(dzc_ + kZcEval);
+  for (size_t c0 = 0; c0 < flat.size(); c0 += kMaxCand) {
+    const int nc = (int)std::min<size_t>(kMaxCand, flat.size() - c0);
+    for (int c = 0; c < nc; ++c) {
+      const auto& f = flat[c0 + c];
+      kv[c] = req[f.first].second[f.second];
+      ks[c] = req[f.first].first;
+    }
+    SIMR_HIP(hipMemcpyAsync(kdev_, hzc_, kZcSlot + (size_t)nc * 4, hipMemcpyHostToDevice, stream_));
+    const int st = stage_begin("modamp", stream_);
+    if (fit_fp32_)
+      launch_modamp_fp32(m, dkv, dks, nc, L.nx, L.ny, P.dxy, part32_, stream_);
+    else
+      launch_modamp_fp64(m, dkv, dks, nc, L.nx, L.ny, P.dxy, part64_, stream_);
+    launch_modamp_evals(fit_fp32_, m, dks, nc, part64_, part32_, nb, zsum_blocks(L.nxy), dev, stream_);
+    stage_end(st, stream_);
+    wait_fit();
+    for (int c = 0; c < nc; ++c) {
+      const auto& f = flat[c0 + c];
+      res[f.first][f.second] = {ev[4 * c], ev[4 * c + 1], ev[4 * c + 2], ev[4 * c + 3]};
     }
   }
   return res;
@@ -740,40 +798,53 @@ static float2 kvec_of(float angle, float mag, int o1, int o2) {
   return make_float2(kx * (float)(o2 - o1), ky * (float)(o2 - o1));
 }
 
-void Engine::prefetch(FitWork& w, int o1, int o2, const std::vector<std::pair<float, float>>& am) {
-  std::vector<float2> kv;
-  std::vector<std::pair<uint32_t, uint32_t>> keys;
-  for (auto& x : am) {
-    auto k = key_of(x.first, x.second);
-    if (w.cache.count(k) || std::find(keys.begin(), keys.end(), k) != keys.end()) continue;
-    keys.push_back(k);
-    kv.push_back(kvec_of(x.first, x.second, o1, o2));
+// Evaluates the (angle, mag) candidates of each direction that its cache
+// does not hold yet, all in one batch.
+void Engine::prefetch_multi(const std::vector<std::pair<int, Walk>>& req, int o1, int o2) {
+  std::vector<std::pair<int, std::vector<float2>>> kv;
+  std::vector<std::vector<std::pair<uint32_t, uint32_t>>> keys;
+  for (auto& r : req) {
+    FitWork& w = fw_[r.first];
+    std::vector<float2> v;
+    std::vector<std::pair<uint32_t, uint32_t>> ks;
+    for (auto& x : r.second) {
+      auto k = key_of(x.first, x.second);
+      if (w.cache.count(k) || std::find(ks.begin(), ks.end(), k) != ks.end()) continue;
+      ks.push_back(k);
+      v.push_back(kvec_of(x.first, x.second, o1, o2));
+    }
+    if (v.empty()) continue;
+    kv.push_back({r.first, v});
+    keys.push_back(ks);
   }
   if (kv.empty()) return;
-  auto ev = eval_batch(w, kv);
-  for (size_t i = 0; i < keys.size(); ++i) w.cache[keys[i]] = ev[i];
+  auto ev = eval_multi(kv);
+  for (size_t r = 0; r < kv.size(); ++r)
+    for (size_t i = 0; i < keys[r].size(); ++i) fw_[kv[r].first].cache[keys[r][i]] = ev[r][i];
 }
 
-// getmodamp (gpuFunctionsImpl.cu:1046-1072) on the current overlap set.
-double Engine::getmodamp(FitWork& w, int dir, float angle, float mag, int o1, int o2, float2* modamp,
-                         float* corr) {
-  auto k = key_of(angle, mag);
-  if (!w.cache.count(k)) prefetch(w, o1, o2, {{angle, mag}});
-  double a2;
-  modamp_of(w.cache[k], modamp, &a2, corr);
-  stats_[dir].n_reductions++;
-  return a2;
-}
-
-double Engine::realspacemodamp(FitWork& w, int dir, int o1, int o2, float2 kv, bool redo, float2* modamp,
-                               float* corr) {
-  if (redo) make_overlaps(w, dir, o1, o2, kv.x, kv.y);
-  const float2 k = make_float2(kv.x * (float)(o2 - o1), kv.y * (float)(o2 - o1));
-  auto ev = eval_batch(w, {k});
-  stats_[dir].n_reductions++;
-  double a2;
-  modamp_of(ev[0], modamp, &a2, corr);
-  return a2;
+// findrealspacemodamp at each direction's k0_ with overlaps of orders
+// (o1, o2) built there (redoarrays): amp_[dir][o2] and corr_[dir][o2].
+void Engine::realspacemodamp_multi(const std::vector<int>& dirs, int o1, int o2) {
+  if (dirs.empty()) return;
+  std::vector<OvReq> reqs;
+  std::vector<std::pair<int, std::vector<float2>>> kv;
+  for (int d : dirs) {
+    reqs.push_back({d, o1, o2, k0_[d].x, k0_[d].y});
+    kv.push_back({d, {make_float2(k0_[d].x * (float)(o2 - o1), k0_[d].y * (float)(o2 - o1))}});
+  }
+  make_overlaps(reqs);
+  auto ev = eval_multi(kv);
+  for (size_t i = 0; i < dirs.size(); ++i) {
+    const int d = dirs[i];
+    stats_[d].n_reductions++;
+    double a2;
+    float2 m;
+    float cc;
+    modamp_of(ev[i][0], &m, &a2, &cc);
+    amp_[d][o2] = m;
+    corr_[d][o2] = cc;
+  }
 }
 
 // findk0 (gpuFunctionsImpl.cu:306-372) with findpeak/fitparabola (443-495)
@@ -786,31 +857,44 @@ static float fitparabola(float a1, float a2, float a3) {
   return peak;
 }
 
-void Engine::findk0(FitWork& w, int dir) {
+void Engine::findk0_all(const std::vector<int>& dirs) {
+  if (dirs.empty()) return;
   const int fo2 = fit_order2();
-  make_overlaps(w, dir, 0, fo2, k0_[dir].x, k0_[dir].y);
-  const int st = stage_begin("findk0_peak", w.stream);
-  launch_findpeak(w.P, w.cc, w.pv, w.pi, w.peakvals, w.peakidx, L.nx, L.ny, w.stream);
-  stage_end(st, w.stream);
-  char* h = (char*)w.hstage;
-  SIMR_HIP(hipMemcpyAsync(h, w.peakvals, 5 * 4, hipMemcpyDeviceToHost, w.stream));
-  SIMR_HIP(hipMemcpyAsync(h + 64, w.peakidx, 4, hipMemcpyDeviceToHost, w.stream));
-  wait_fit(w);
-  const float* v = (const float*)h;
-  const int idx = *(const int*)(h + 64);
-  const int xc = idx % L.nx, yc = idx / L.nx;
-  float2 pk;
-  pk.x = fitparabola(v[1], v[0], v[2]) + (float)xc;
-  pk.y = fitparabola(v[3], v[0], v[4]) + (float)yc;
+  std::vector<OvReq> reqs;
+  int dmin = L.ndirs, dmax = -1;
+  for (int d : dirs) {
+    reqs.push_back({d, 0, fo2, k0_[d].x, k0_[d].y});
+    dmin = std::min(dmin, d);
+    dmax = std::max(dmax, d);
+  }
+  make_overlaps(reqs);
+  // the cross-correlations of directions dmin..dmax, one after another
+  PeakBatch pb{};
+  pb.n = dmax - dmin + 1;
+  for (int k = 0; k < pb.n; ++k) pb.P[k] = fw_[dmin + k].P;
+  const int st = stage_begin("findk0_peak", stream_);
+  // the peak values and indices land in the zero-copy region
+  launch_findpeak(pb, fw_[dmin].cc, pv_, pi_, (float*)(dzc_ + kZcPeakV), (int*)(dzc_ + kZcPeakI), L.nx, L.ny,
+                  stream_);
+  stage_end(st, stream_);
+  wait_fit();
   const Geo g = geometry();
-  const float2 old = k0_[dir];
-  if (old.x / g.dkx < pk.x - (float)(L.nx / 2)) pk.x -= (float)L.nx;
-  if (old.x / g.dkx > pk.x + (float)(L.nx / 2)) pk.x += (float)L.nx;
-  if (old.y / g.dky < pk.y - (float)(L.ny / 2)) pk.y -= (float)L.ny;
-  if (old.y / g.dky > pk.y + (float)(L.ny / 2)) pk.y += (float)L.ny;
-  pk.x *= g.dkx / (float)fo2;
-  pk.y *= g.dky / (float)fo2;
-  k0_[dir] = pk;
+  for (int d : dirs) {
+    const float* v = (const float*)(hzc_ + kZcPeakV) + (size_t)(d - dmin) * 8;
+    const int idx = ((const int*)(hzc_ + kZcPeakI))[d - dmin];
+    const int xc = idx % L.nx, yc = idx / L.nx;
+    float2 pk;
+    pk.x = fitparabola(v[1], v[0], v[2]) + (float)xc;
+    pk.y = fitparabola(v[3], v[0], v[4]) + (float)yc;
+    const float2 old = k0_[d];
+    if (old.x / g.dkx < pk.x - (float)(L.nx / 2)) pk.x -= (float)L.nx;
+    if (old.x / g.dkx > pk.x + (float)(L.nx / 2)) pk.x += (float)L.nx;
+    if (old.y / g.dky < pk.y - (float)(L.ny / 2)) pk.y -= (float)L.ny;
+    if (old.y / g.dky > pk.y + (float)(L.ny / 2)) pk.y += (float)L.ny;
+    pk.x *= g.dkx / (float)fo2;
+    pk.y *= g.dky / (float)fo2;
+    k0_[d] = pk;
+  }
 }
 
 template <typename U>
@@ -825,60 +909,53 @@ static U fitxyparabola(U x1, U y1, U x2, U y2, U x3, U y3) {
   return xbar2 - slope2 / curve;
 }
 
-// fitk0andmodamps (gpuFunctionsImpl.cu:874-1020).  The reference evaluates the
-// line-search steps one at a time with a synchronous reduction each; the
-// step values it will visit are deterministic (float accumulation of the
-// step), so each walk's next kLook steps in both directions are reduced in
-// one launch and the reference's step rule is replayed on the results.
-void Engine::fitk0andmodamps(FitWork& w, int dir) {
-  constexpr int kLook = 6;
-  const int fo1 = 0, fo2 = fit_order2();
-  const float2 kv = k0_[dir];
-  const float k0mag = sqrtf(kv.x * kv.x + kv.y * kv.y);
-  const float k0angle = atan2f(kv.y, kv.x);
-  auto parab = [&](float x1, double y1, float x2, double y2, float x3, double y3) -> float {
-    if (!fit_fp32_) return (float)fitxyparabola<double>(x1, y1, x2, y2, x3, y3);
-    return fitxyparabola<float>(x1, (float)y1, x2, (float)y2, x3, (float)y3);
-  };
-  const float deltaangle = 0.001f;
-  const float deltamag = (float)(0.1 / (double)((float)std::max(L.nx, L.ny) * P.dxy));
-  auto walk = [&](float start, float mag_or_angle, bool is_angle, float delta, int sign) {
-    std::vector<std::pair<float, float>> v;
-    float a = start;
-    for (int m = 0; m < kLook; ++m) {
-      v.push_back(is_angle ? std::make_pair(a, mag_or_angle) : std::make_pair(mag_or_angle, a));
-      a = sign > 0 ? a + delta : a - delta;
-    }
-    return v;
-  };
-  auto ensure = [&](float angle, float mag, bool is_angle, float delta, int sign) {
-    if (w.cache.count(key_of(angle, mag))) return;
-    prefetch(w, fo1, fo2, is_angle ? walk(angle, mag, true, delta, sign) : walk(mag, angle, false, delta, sign));
-  };
+namespace {
+constexpr int kLook = 6;  // line-search steps evaluated ahead in each direction
 
-  // first evaluation recomputes the overlaps (redoarrays = recalcarrays >= 1)
-  {
-    const float2 k1 = make_float2(k0mag * cosf(k0angle), k0mag * sinf(k0angle));
-    make_overlaps(w, dir, fo1, fo2, k1.x, k1.y);
+// The next kLook points of a walk from `start` (angle walks vary the angle at
+// a fixed magnitude, magnitude walks the magnitude at a fixed angle), by the
+// reference's float accumulation of the step.
+std::vector<std::pair<float, float>> walk_from(float start, float other, bool is_angle, float delta, int sign) {
+  std::vector<std::pair<float, float>> v;
+  float a = start;
+  for (int m = 0; m < kLook; ++m) {
+    v.push_back(is_angle ? std::make_pair(a, other) : std::make_pair(other, a));
+    a = sign > 0 ? a + delta : a - delta;
   }
-  // angle walk: both directions' first kLook steps in one batch
-  {
-    auto up = walk(k0angle + deltaangle, k0mag, true, deltaangle, +1);
-    auto dn = walk(k0angle - deltaangle, k0mag, true, deltaangle, -1);
-    std::vector<std::pair<float, float>> all{{k0angle, k0mag}};
-    all.insert(all.end(), up.begin(), up.end());
-    all.insert(all.end(), dn.begin(), dn.end());
-    prefetch(w, fo1, fo2, all);
-  }
-  float2 modamp;
-  float x2 = k0angle;
-  double amp2 = getmodamp(w, dir, k0angle, k0mag, fo1, fo2, &modamp, nullptr);
-  stats_[dir].n_angle_evals++;
-  float angle = k0angle + deltaangle;
-  float x3 = angle;
-  double amp3 = getmodamp(w, dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
-  stats_[dir].n_angle_evals++;
-  double amp1 = 0;
+  return v;
+}
+}  // namespace
+
+// One line-search walk of fitk0andmodamps (gpuFunctionsImpl.cu:912-983):
+// from `start` step by `delta` while the modamp magnitude grows, then the
+// parabola through the last three points.  The steps the reference visits
+// are deterministic, so the walk is replayed over the direction's cache of
+// evaluated candidates: true (and the vertex, and the number of steps) when
+// every value it visits is cached; otherwise false, with `need` the next
+// kLook steps from the first value missing.
+bool Engine::replay_walk(int dir, bool is_angle, float start, float other, float delta, int o1, int o2, float* out,
+                         int* nevals, Walk* need) {
+  FitWork& w = fw_[dir];
+  int n = 0;
+  int sign = +1;
+  auto f = [&](float a, double* amp) {
+    auto k = is_angle ? key_of(a, other) : key_of(other, a);
+    auto it = w.cache.find(k);
+    if (it == w.cache.end()) {
+      *need = walk_from(a, other, is_angle, delta, sign);
+      return false;
+    }
+    float2 m;
+    modamp_of(it->second, &m, amp, nullptr);
+    ++n;
+    return true;
+  };
+  float x2 = start;
+  double amp2, amp3, amp1 = 0;
+  if (!f(start, &amp2)) return false;
+  float a = start + delta;
+  float x3 = a;
+  if (!f(a, &amp3)) return false;
   float x1 = 0;
   if (amp3 > amp2) {
     while (amp3 > amp2) {
@@ -886,102 +963,121 @@ void Engine::fitk0andmodamps(FitWork& w, int dir) {
       x1 = x2;
       amp2 = amp3;
       x2 = x3;
-      angle += deltaangle;
-      x3 = angle;
-      ensure(angle, k0mag, true, deltaangle, +1);
-      amp3 = getmodamp(w, dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
-      stats_[dir].n_angle_evals++;
+      a += delta;
+      x3 = a;
+      if (!f(a, &amp3)) return false;
     }
   } else {
-    angle = k0angle;
-    double a = amp3;
+    a = start;
+    double t = amp3;
     amp3 = amp2;
-    amp2 = a;
-    float ax = x3;
+    amp2 = t;
+    float tx = x3;
     x3 = x2;
-    x2 = ax;
+    x2 = tx;
+    sign = -1;
     while (amp3 > amp2) {
       amp1 = amp2;
       x1 = x2;
       amp2 = amp3;
       x2 = x3;
-      angle -= deltaangle;
-      x3 = angle;
-      ensure(angle, k0mag, true, deltaangle, -1);
-      amp3 = getmodamp(w, dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
-      stats_[dir].n_angle_evals++;
+      a -= delta;
+      x3 = a;
+      if (!f(a, &amp3)) return false;
     }
   }
-  angle = parab(x1, amp1, x2, amp2, x3, amp3);
+  if (!fit_fp32_)
+    *out = (float)fitxyparabola<double>(x1, amp1, x2, amp2, x3, amp3);
+  else
+    *out = fitxyparabola<float>(x1, (float)amp1, x2, (float)amp2, x3, (float)amp3);
+  *nevals = n;
+  return true;
+}
 
-  // magnitude walk at the fitted angle
+// fitk0andmodamps (gpuFunctionsImpl.cu:874-1020) for every direction in
+// `dirs` at once.  The reference evaluates the line-search steps one at a
+// time with a synchronous reduction each; here each walk's next kLook steps
+// in both directions are reduced together, every direction's in the same
+// launch, and each direction's walk is replayed on the results; a round
+// fetches what the walks that are not done yet need next.
+void Engine::fitk0_all(const std::vector<int>& dirs) {
+  if (dirs.empty()) return;
+  const int fo1 = 0, fo2 = fit_order2();
+  const float deltaangle = 0.001f;
+  const float deltamag = (float)(0.1 / (double)((float)std::max(L.nx, L.ny) * P.dxy));
+  std::vector<float> k0mag(L.ndirs), k0angle(L.ndirs), angle(L.ndirs), mag(L.ndirs);
+  // first evaluation recomputes the overlaps (redoarrays = recalcarrays >= 1)
   {
-    auto up = walk(k0mag + deltamag, angle, false, deltamag, +1);
-    auto dn = walk(k0mag - deltamag, angle, false, deltamag, -1);
-    std::vector<std::pair<float, float>> all{{angle, k0mag}};
-    all.insert(all.end(), up.begin(), up.end());
-    all.insert(all.end(), dn.begin(), dn.end());
-    prefetch(w, fo1, fo2, all);
-  }
-  x2 = k0mag;
-  amp2 = getmodamp(w, dir, angle, k0mag, fo1, fo2, &modamp, nullptr);
-  stats_[dir].n_mag_evals++;
-  float mag = k0mag + deltamag;
-  x3 = mag;
-  amp3 = getmodamp(w, dir, angle, mag, fo1, fo2, &modamp, nullptr);
-  stats_[dir].n_mag_evals++;
-  if (amp3 > amp2) {
-    while (amp3 > amp2) {
-      amp1 = amp2;
-      x1 = x2;
-      amp2 = amp3;
-      x2 = x3;
-      mag += deltamag;
-      x3 = mag;
-      ensure(angle, mag, false, deltamag, +1);
-      amp3 = getmodamp(w, dir, angle, mag, fo1, fo2, &modamp, nullptr);
-      stats_[dir].n_mag_evals++;
+    std::vector<OvReq> reqs;
+    for (int d : dirs) {
+      const float2 kv = k0_[d];
+      k0mag[d] = sqrtf(kv.x * kv.x + kv.y * kv.y);
+      k0angle[d] = atan2f(kv.y, kv.x);
+      reqs.push_back({d, fo1, fo2, k0mag[d] * cosf(k0angle[d]), k0mag[d] * sinf(k0angle[d])});
     }
-  } else {
-    mag = k0mag;
-    double a = amp3;
-    amp3 = amp2;
-    amp2 = a;
-    float ax = x3;
-    x3 = x2;
-    x2 = ax;
-    while (amp3 > amp2) {
-      amp1 = amp2;
-      x1 = x2;
-      amp2 = amp3;
-      x2 = x3;
-      mag -= deltamag;
-      x3 = mag;
-      ensure(angle, mag, false, deltamag, -1);
-      amp3 = getmodamp(w, dir, angle, mag, fo1, fo2, &modamp, nullptr);
-      stats_[dir].n_mag_evals++;
-    }
+    make_overlaps(reqs);
   }
-  mag = parab(x1, amp1, x2, amp2, x3, amp3);
-
-  float cc = 0;
-  getmodamp(w, dir, angle, mag, fo1, fo2, &modamp, &cc);
-  corr_[dir][fo2] = cc;
-  k0_[dir] = make_float2(mag * cosf(angle), mag * sinf(angle));
-  amp_[dir][fo2] = modamp;
+  auto run_walks = [&](bool is_angle) {
+    std::vector<std::pair<int, Walk>> first;
+    for (int d : dirs) {
+      const float s0 = is_angle ? k0angle[d] : k0mag[d];
+      const float other = is_angle ? k0mag[d] : angle[d];
+      const float delta = is_angle ? deltaangle : deltamag;
+      // the start and the first kLook steps both ways in one batch
+      Walk all{is_angle ? std::make_pair(s0, other) : std::make_pair(other, s0)};
+      auto up = walk_from(s0 + delta, other, is_angle, delta, +1);
+      auto dn = walk_from(s0 - delta, other, is_angle, delta, -1);
+      all.insert(all.end(), up.begin(), up.end());
+      all.insert(all.end(), dn.begin(), dn.end());
+      first.push_back({d, all});
+    }
+    prefetch_multi(first, fo1, fo2);
+    std::vector<bool> done(L.ndirs, false);
+    for (;;) {
+      std::vector<std::pair<int, Walk>> more;
+      for (int d : dirs) {
+        if (done[d]) continue;
+        const float s0 = is_angle ? k0angle[d] : k0mag[d];
+        const float other = is_angle ? k0mag[d] : angle[d];
+        float r;
+        int ne = 0;
+        Walk need;
+        if (replay_walk(d, is_angle, s0, other, is_angle ? deltaangle : deltamag, fo1, fo2, &r, &ne, &need)) {
+          (is_angle ? angle : mag)[d] = r;
+          (is_angle ? stats_[d].n_angle_evals : stats_[d].n_mag_evals) += ne;
+          stats_[d].n_reductions += ne;
+          done[d] = true;
+        } else {
+          more.push_back({d, need});
+        }
+      }
+      if (more.empty()) break;
+      prefetch_multi(more, fo1, fo2);
+    }
+  };
+  run_walks(true);   // angle walk at the start magnitude
+  run_walks(false);  // magnitude walk at the fitted angle
+  {
+    std::vector<std::pair<int, Walk>> fin;
+    for (int d : dirs) fin.push_back({d, {{angle[d], mag[d]}}});
+    prefetch_multi(fin, fo1, fo2);
+  }
+  for (int d : dirs) {
+    float2 modamp;
+    double a2;
+    float cc = 0;
+    modamp_of(fw_[d].cache.at(key_of(angle[d], mag[d])), &modamp, &a2, &cc);
+    stats_[d].n_reductions++;
+    corr_[d][fo2] = cc;
+    k0_[d] = make_float2(mag[d] * cosf(angle[d]), mag[d] * sinf(angle[d]));
+    amp_[d][fo2] = modamp;
+  }
+  // the other orders at the fitted k0
   if (L.nz == 1) {
-    for (int order = 2; order < L.norders; ++order) {
-      realspacemodamp(w, dir, order - 1, order, make_float2(mag * cosf(angle), mag * sinf(angle)), true, &modamp, &cc);
-      amp_[dir][order] = modamp;
-      corr_[dir][order] = cc;
-    }
+    for (int order = 2; order < L.norders; ++order) realspacemodamp_multi(dirs, order - 1, order);
   } else {
     for (int order = 1; order < L.norders; ++order)
-      if (order != fo2) {
-        realspacemodamp(w, dir, 0, order, make_float2(mag * cosf(angle), mag * sinf(angle)), true, &modamp, &cc);
-        amp_[dir][order] = modamp;
-        corr_[dir][order] = cc;
-      }
+      if (order != fo2) realspacemodamp_multi(dirs, 0, order);
   }
 }
 
@@ -999,104 +1095,62 @@ void Engine::require_finite_k0(int dir, const char* where) const {
 static float2 cmulh(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 static float cmagh(float2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
 
-// k0 policy of findModulationVectorsAndPhasesForAllDirections (cudaSirecon.cpp:296-492)
-void Engine::modulation(FitWork& w, int dir) {
-  const bool reuse = P.ntimes > 1 && cur_t_ > 0 && P.bUseTime0k0;
-  stats_[dir] = simr_fit{};
-  if (!reuse) k0_[dir] = k0guess_[dir];
-  if (P.bSearchforvector && !reuse) {
-    findk0(w, dir);
-    require_finite_k0(dir, "the k0 search");
-    fitk0andmodamps(w, dir);
-    require_finite_k0(dir, "the k0 fit");
-    if (cur_t_ == 0) k0_time0_[dir] = k0_[dir];
-    const float dx = k0_[dir].x - k0guess_[dir].x, dy = k0_[dir].y - k0guess_[dir].y;
-    const float dist = sqrtf(dx * dx + dy * dy);
-    if (P.ntimes > 1 && cur_t_ > 0 && dist > 2 * K0_WARNING_THRESH) {
-      k0_[dir] = k0_time0_[dir];
-      for (int order = 1; order < L.norders; ++order) {
-        float2 m;
-        float cc;
-        const int o1 = L.nz > 1 ? 0 : order - 1;
-        realspacemodamp(w, dir, o1, order, k0_[dir], true, &m, &cc);
-        amp_[dir][order] = m;
-        corr_[dir][order] = cc;
-      }
-    }
-  } else {
-    for (int order = 1; order < L.norders; ++order) {
-      float2 m;
-      float cc;
-      realspacemodamp(w, dir, 0, order, k0_[dir], true, &m, &cc);
-      amp_[dir][order] = m;
-      corr_[dir][order] = cc;
-    }
-  }
-  if (L.nz == 1)
-    for (int order = 2; order < L.norders; ++order) amp_[dir][order] = cmulh(amp_[dir][order], amp_[dir][order - 1]);
-  if (P.nforceamp > 0 && P.forceamp[0] > 0.0f) {
-    for (int order = 1; order < L.norders; ++order) {
-      const float a = cmagh(amp_[dir][order]);
-      if (order - 1 < P.nforceamp && a < P.forceamp[order - 1]) {
-        const float f = P.forceamp[order - 1] / a;
-        amp_[dir][order].x *= f;
-        amp_[dir][order].y *= f;
-      }
-    }
-  }
-  if (!P.bFitallphases) {
-    const float base_phase = atan2f(amp_[dir][1].y, amp_[dir][1].x);
-    for (int order = 2; order < L.norders; ++order) {
-      const float2 amplitude = make_float2(cmagh(amp_[dir][order]), 0);
-      const float phi = (float)order * base_phase;
-      amp_[dir][order] = cmulh(amplitude, make_float2((float)std::cos((double)phi), (float)std::sin((double)phi)));
-    }
-  }
-}
-
-// The directions' fits are independent (findModulationVectorsAndPhasesForAllDirections
-// loops over them): direction 0 runs on the calling thread and the context
-// stream, every other one on a thread and stream of its own after the bands
-// are ready, so one direction's kernels fill the GPU while another waits for
-// its line-search read-back.  Each thread ends with its stream drained.
+// The k0 policy of findModulationVectorsAndPhasesForAllDirections
+// (cudaSirecon.cpp:296-492).  The reference fits the directions one after
+// another; they are independent, so here they advance together and share
+// every launch and read-back.
 void Engine::find_modulation() {
   if (!preprocessed_) throw StateError("find_modulation before load_and_rescale");
   refresh_fit();
-  if (L.ndirs > 1) {
-    hipEvent_t ready;
-    SIMR_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-    SIMR_HIP(hipEventRecord(ready, stream_));
-    for (int dir = 1; dir < L.ndirs; ++dir) SIMR_HIP(hipStreamWaitEvent(fw_[dir].stream, ready, 0));
-    SIMR_HIP(hipEventDestroy(ready));
+  const bool reuse = P.ntimes > 1 && cur_t_ > 0 && P.bUseTime0k0;
+  std::vector<int> all(L.ndirs);
+  for (int d = 0; d < L.ndirs; ++d) {
+    all[d] = d;
+    stats_[d] = simr_fit{};
+    if (!reuse) k0_[d] = k0guess_[d];
   }
-  std::vector<std::exception_ptr> err(L.ndirs);
-  std::vector<std::thread> th;
-  std::atomic<int> running(L.ndirs - 1);
-  for (int dir = 1; dir < L.ndirs; ++dir)
-    th.emplace_back([this, dir, &err, &running] {
-      try {
-        SIMR_HIP(hipSetDevice(device_));
-        modulation(fw_[dir], dir);
-        wait_fit(fw_[dir]);
-      } catch (...) {
-        err[dir] = std::current_exception();
+  if (P.bSearchforvector && !reuse) {
+    findk0_all(all);
+    for (int d : all) require_finite_k0(d, "the k0 search");
+    fitk0_all(all);
+    std::vector<int> fallback;
+    for (int d : all) {
+      require_finite_k0(d, "the k0 fit");
+      if (cur_t_ == 0) k0_time0_[d] = k0_[d];
+      const float dx = k0_[d].x - k0guess_[d].x, dy = k0_[d].y - k0guess_[d].y;
+      const float dist = sqrtf(dx * dx + dy * dy);
+      if (P.ntimes > 1 && cur_t_ > 0 && dist > 2 * K0_WARNING_THRESH) {
+        k0_[d] = k0_time0_[d];
+        fallback.push_back(d);
       }
-      running.fetch_sub(1, std::memory_order_release);
-    });
-  try {
-    modulation(fw_[0], 0);
-  } catch (...) {
-    err[0] = std::current_exception();
+    }
+    for (int order = 1; order < L.norders; ++order)
+      realspacemodamp_multi(fallback, L.nz > 1 ? 0 : order - 1, order);
+  } else {
+    for (int order = 1; order < L.norders; ++order) realspacemodamp_multi(all, 0, order);
   }
-  // the filter that follows needs every direction's fit: wait for the last
-  // one by polling (a blocking join wakes up tens of microseconds late)
-  const auto t0 = std::chrono::steady_clock::now();
-  while (running.load(std::memory_order_acquire) > 0 &&
-         std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(2))
-    for (int k = 0; k < 32; ++k) __builtin_ia32_pause();
-  for (auto& t : th) t.join();
-  for (auto& e : err)
-    if (e) std::rethrow_exception(e);
+  for (int dir = 0; dir < L.ndirs; ++dir) {
+    if (L.nz == 1)
+      for (int order = 2; order < L.norders; ++order) amp_[dir][order] = cmulh(amp_[dir][order], amp_[dir][order - 1]);
+    if (P.nforceamp > 0 && P.forceamp[0] > 0.0f) {
+      for (int order = 1; order < L.norders; ++order) {
+        const float a = cmagh(amp_[dir][order]);
+        if (order - 1 < P.nforceamp && a < P.forceamp[order - 1]) {
+          const float f = P.forceamp[order - 1] / a;
+          amp_[dir][order].x *= f;
+          amp_[dir][order].y *= f;
+        }
+      }
+    }
+    if (!P.bFitallphases) {
+      const float base_phase = atan2f(amp_[dir][1].y, amp_[dir][1].x);
+      for (int order = 2; order < L.norders; ++order) {
+        const float2 amplitude = make_float2(cmagh(amp_[dir][order]), 0);
+        const float phi = (float)order * base_phase;
+        amp_[dir][order] = cmulh(amplitude, make_float2((float)std::cos((double)phi), (float)std::sin((double)phi)));
+      }
+    }
+  }
   fitted_ = true;
 }
 
@@ -1255,9 +1309,10 @@ void Engine::process_volume() {
 // host decision: its modamps are findrealspacemodamp at the known k0 for
 // every order (realspacemodamp with order1 = 0, then the policy of 450-491),
 // so the fit, the filter and the assembly go on the device as one graph:
-// every direction's overlaps and modamp reduction on its own stream (fork
-// and join events), the modamp result arithmetic and the policy in one-thread
-// kernels, and the filter reading the modamps from device memory.  The
+// per order, every direction's overlaps in one launch per pass and their
+// modamp reductions in one launch, the modamp result arithmetic and the
+// policy in small kernels, and the filter reading the modamps from device
+// memory.  The
 // graph is captured once per context and replayed while k0 and the
 // parameters stay the same.
 bool Engine::graph_eligible() const {
@@ -1269,38 +1324,30 @@ void Engine::build_graph() {
   if (graph_) SIMR_HIP(hipGraphDestroy(graph_));
   graph_exec_ = nullptr;
   graph_ = nullptr;
-  // candidate vectors of findrealspacemodamp (gpuFunctionsImpl.cu:1093-1094), order1 = 0
-  std::vector<float2> kc((size_t)L.ndirs * L.norders, make_float2(0.f, 0.f));
-  for (int d = 0; d < L.ndirs; ++d)
-    for (int o = 1; o < L.norders; ++o)
-      kc[(size_t)d * L.norders + o] = make_float2(k0_[d].x * (float)o, k0_[d].y * (float)o);
+  // candidate vectors of findrealspacemodamp (gpuFunctionsImpl.cu:1093-1094),
+  // order1 = 0: [order][dir] = k0[dir] * order
+  std::vector<float2> kc((size_t)L.norders * L.ndirs, make_float2(0.f, 0.f));
+  for (int o = 1; o < L.norders; ++o)
+    for (int d = 0; d < L.ndirs; ++d)
+      kc[(size_t)o * L.ndirs + d] = make_float2(k0_[d].x * (float)o, k0_[d].y * (float)o);
   SIMR_HIP(hipMemcpyAsync(kc_dev_, kc.data(), kc.size() * 8, hipMemcpyHostToDevice, stream_));
   sync();
   SIMR_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
   try {
-    if (L.ndirs > 1) {
-      SIMR_HIP(hipEventRecord(fork_ev_, stream_));
-      for (int d = 1; d < L.ndirs; ++d) SIMR_HIP(hipStreamWaitEvent(fw_[d].stream, fork_ev_, 0));
-    }
     const int nparts = fit_fp32_ ? modamp_partials_fp32(L.nx, L.ny) : modamp_partials_fp64(L.nx, L.ny);
-    for (int d = 0; d < L.ndirs; ++d) {
-      FitWork& w = fw_[d];
-      for (int o = 1; o < L.norders; ++o) {
-        const size_t idx = (size_t)d * L.norders + o;
-        make_overlaps(w, d, 0, o, k0_[d].x, k0_[d].y);
-        if (fit_fp32_)
-          launch_modamp_fp32(w.P, w.SX, w.SY, kc_dev_ + idx, 1, L.nx, L.ny, P.dxy, w.part32, w.stream);
-        else
-          launch_modamp_fp64(w.P, kc_dev_ + idx, 1, L.nx, L.ny, P.dxy, w.part64, w.stream);
-        launch_modamp_finish(fit_fp32_, w.part64, w.part32, nparts, w.sumparts, zsum_blocks(L.nxy), amps_dev_ + idx,
-                             corr_dev_ + idx, w.stream);
-      }
+    const ModampSlots m = modamp_slots();
+    for (int o = 1; o < L.norders; ++o) {
+      std::vector<OvReq> reqs;
+      for (int d = 0; d < L.ndirs; ++d) reqs.push_back({d, 0, o, k0_[d].x, k0_[d].y});
+      make_overlaps(reqs);
+      const float2* kc_o = kc_dev_ + (size_t)o * L.ndirs;
+      if (fit_fp32_)
+        launch_modamp_fp32(m, kc_o, slot_ids_, L.ndirs, L.nx, L.ny, P.dxy, part32_, stream_);
+      else
+        launch_modamp_fp64(m, kc_o, slot_ids_, L.ndirs, L.nx, L.ny, P.dxy, part64_, stream_);
+      launch_modamp_finish(fit_fp32_, m, slot_ids_, L.ndirs, part64_, part32_, nparts, zsum_blocks(L.nxy),
+                           amps_dev_, corr_dev_, L.norders, o, stream_);
     }
-    if (L.ndirs > 1)
-      for (int d = 1; d < L.ndirs; ++d) {
-        SIMR_HIP(hipEventRecord(join_ev_[d], fw_[d].stream));
-        SIMR_HIP(hipStreamWaitEvent(stream_, join_ev_[d], 0));
-      }
     AmpPolicy a{};
     a.ndirs = L.ndirs;
     a.norders = L.norders;
@@ -1429,29 +1476,25 @@ void Engine::set_band(int dir, int band, const float* host) {
 }
 
 void Engine::debug_overlaps(int dir, int o1, int o2, float k0x, float k0y, double* Ph, double* sx, double* sy) {
-  FitWork& w = fw_[0];
-  make_overlaps(w, dir, o1, o2, k0x, k0y, true);
-  SIMR_HIP(hipMemcpyAsync(Ph, w.P, L.nxy * 16, hipMemcpyDeviceToHost, stream_));
-  SIMR_HIP(hipMemcpyAsync(sx, w.SX, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
-  SIMR_HIP(hipMemcpyAsync(sy, w.SY, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
+  make_overlaps({{dir, o1, o2, k0x, k0y}}, true, &dbg_);
+  SIMR_HIP(hipMemcpyAsync(Ph, dbg_.P, L.nxy * 16, hipMemcpyDeviceToHost, stream_));
+  SIMR_HIP(hipMemcpyAsync(sx, dbg_.SX, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
+  SIMR_HIP(hipMemcpyAsync(sy, dbg_.SY, L.nxy * 8, hipMemcpyDeviceToHost, stream_));
   sync();
 }
 
 // The overlap pair in real space, [nz][ny][nx] complex each (the reference's
-// overlap0/overlap1 after makeoverlaps' inverse 3D FFT).  Built with
-// direction 0's fit scratch into the assembly staging region, which no fit
-// or assembly uses between find_modulation and filter_assemble.
+// overlap0/overlap1 after makeoverlaps' inverse 3D FFT).  Built in the dumps'
+// own scratch, then transformed into the assembly staging region, which no
+// fit or assembly uses between find_modulation and filter_assemble.
 void Engine::overlap_volumes(int dir, int o1, int o2, float k0x, float k0y, float* ov0, float* ov1) {
   SIMR_REQUIRE(dir >= 0 && dir < L.ndirs, "direction out of range");
   SIMR_REQUIRE(o1 >= 0 && o2 > o1 && o2 < L.norders, "overlap orders out of range");
   const size_t vbytes = (size_t)L.nz * L.nxy * 8;
-  const size_t stage_bytes = align256((size_t)L.nasm * L.zdim * L.nxy * 8) +
-                             align256((size_t)L.nasm * L.zdim * (size_t)L.ydim * L.nx * 8);
-  SIMR_REQUIRE(2 * vbytes <= stage_bytes, "overlap volumes do not fit the staging region");
-  sync();  // every direction's fit has finished with the staging region
-  FitWork& w = fw_[0];
-  make_overlaps(w, dir, o1, o2, k0x, k0y);
-  launch_overlap_volumes(w.ovx, w.zd, L.nz, L.nxy, stageZ_, stream_);
+  SIMR_REQUIRE(2 * vbytes <= stage_bytes_, "overlap volumes do not fit the staging region");
+  sync();
+  make_overlaps({{dir, o1, o2, k0x, k0y}}, false, &dbg_);
+  launch_overlap_volumes(dbg_.ovx, dbg_.zd, nzs_stride_, L.nz, L.nxy, stageZ_, stream_);
   SIMR_HIP(hipMemcpyAsync(ov0, stageZ_, vbytes, hipMemcpyDeviceToHost, stream_));
   SIMR_HIP(hipMemcpyAsync(ov1, (const char*)stageZ_ + vbytes, vbytes, hipMemcpyDeviceToHost, stream_));
   sync();

@@ -87,50 +87,47 @@ class Engine {
   struct Eval {
     double xr, xi, sx, sy;
   };
-  // Scratch and state of one direction's k0/modamp fit.  Directions are
-  // fitted concurrently, each on its own stream and host thread; direction
-  // 0 uses the context stream, the others alias the assembly staging
-  // buffers, which are idle while the fit runs.
+  // Scratch and state of one direction's k0/modamp fit.  The directions are
+  // fitted together: every overlap, cross-correlation and modamp launch
+  // covers all of them (one slot per direction), and one read-back serves
+  // every direction's line-search step.  The directions' overlap stages and
+  // cross-correlations sit one after another (ovx, cc), so a transform pass
+  // takes them as one batch; all of it aliases the assembly staging region,
+  // which is idle while the fit runs.
   struct FitWork {
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    float2* ovx = nullptr;
+    float2* ovx = nullptr;  // [2][nzs_stride][ny][nx]
     double2* P = nullptr;
     double *SX = nullptr, *SY = nullptr;
-    float2* cc = nullptr;
-    float* pv = nullptr;
-    int* pi = nullptr;
-    float* peakvals = nullptr;
-    int* peakidx = nullptr;
-    float2* kcand = nullptr;
-    double2* part64 = nullptr;
-    float4* part32 = nullptr;
+    float2* cc = nullptr;  // [ny][nx]
     double* sumparts = nullptr;
-    void* hstage = nullptr;  // pinned host staging
-    hipEvent_t done = nullptr;  // read-back completion (polled by wait_fit)
     int zd = 0;  // z support half-width of the overlaps in ovx
-    bool ov_totals_valid = false;
-    double ov_sx = 0, ov_sy = 0;
     std::map<std::pair<uint32_t, uint32_t>, Eval> cache;
-    int cache_o1 = -1, cache_o2 = -1;
   };
+  struct OvReq {
+    int dir, o1, o2;
+    float k0x, k0y;
+  };
+  using Walk = std::vector<std::pair<float, float>>;  // (angle, mag) candidates
 
   void derive();
   Geo geometry() const;
   int zd_overlaps() const;
   int fit_order2() const;
-  OtfRA otf_table(int dir, int order) const;
-  void carve_fit(FitWork& w, char*& at, bool count_only, size_t& bytes) const;
-  void wait_fit(FitWork& w);
-  void make_overlaps(FitWork& w, int dir, int o1, int o2, float k0x, float k0y, bool per_elem = false);
-  std::vector<Eval> eval_batch(FitWork& w, const std::vector<float2>& kvecs);
+  OtfTab otf_table(int dir, int order) const;
+  void wait_fit();
+  OvParams ov_params(int dir, int o1, int o2, float k0x, float k0y) const;
+  // builds the requested overlap pairs (each in its direction's slot, or
+  // every one in `only` when given) and their z sums
+  void make_overlaps(const std::vector<OvReq>& reqs, bool per_elem = false, FitWork* only = nullptr);
+  ModampSlots modamp_slots() const;
+  std::vector<std::vector<Eval>> eval_multi(const std::vector<std::pair<int, std::vector<float2>>>& req);
   void modamp_of(const Eval& e, float2* modamp, double* amp2, float* corr) const;
-  double getmodamp(FitWork& w, int dir, float angle, float mag, int o1, int o2, float2* modamp, float* corr);
-  double realspacemodamp(FitWork& w, int dir, int o1, int o2, float2 kv, bool redo, float2* modamp, float* corr);
-  void prefetch(FitWork& w, int o1, int o2, const std::vector<std::pair<float, float>>& am);
-  void findk0(FitWork& w, int dir);
-  void fitk0andmodamps(FitWork& w, int dir);
-  void modulation(FitWork& w, int dir);
+  void prefetch_multi(const std::vector<std::pair<int, Walk>>& req, int o1, int o2);
+  bool replay_walk(int dir, bool is_angle, float start, float other, float delta, int o1, int o2, float* out,
+                   int* nevals, Walk* need);
+  void realspacemodamp_multi(const std::vector<int>& dirs, int o1, int o2);
+  void findk0_all(const std::vector<int>& dirs);
+  void fitk0_all(const std::vector<int>& dirs);
   void require_finite_k0(int dir, const char* where) const;
   bool graph_eligible() const;
   void build_graph();
@@ -158,12 +155,28 @@ class Engine {
   double *sums_ = nullptr, *sdp_ = nullptr;
   float2* bands_ = nullptr;
   float2* otf_ = nullptr;
-  std::vector<FitWork> fw_;  // one per direction
+  std::vector<FitWork> fw_;  // one per direction, in the staging region
+  FitWork dbg_;              // overlap dumps (its own region: they write the staging region)
+  int nzs_stride_ = 1;       // planes reserved per overlap: 2 zd_max + 1
+  // batch buffers shared by the directions
+  int* slot_ids_ = nullptr;   // 0, 1, ..., kMaxDirs-1 (the graph's candidates)
+  double2* part64_ = nullptr;
+  float4* part32_ = nullptr;
+  float* pv_ = nullptr;
+  int* pi_ = nullptr;
+  // Coherent pinned host memory: the candidate vectors and their directions
+  // are written there and reach the device in one copy (kdev_, same layout);
+  // the modamp sums and the cross-correlation peaks are written there by the
+  // kernels directly, so a line-search round has one copy and one wait.
+  // Host and device views of the same bytes.
+  char* hzc_ = nullptr;
+  char* dzc_ = nullptr;
+  char* kdev_ = nullptr;  // device copy of the candidate part of hzc_
+  hipEvent_t done_ = nullptr;  // round completion (polled by wait_fit)
   float2 *stageZ_ = nullptr, *stageY_ = nullptr;
   float* resampled_ = nullptr;  // [nimg][ny][nx] resampled raw planes (aliases the staging region)
   float* out_ = nullptr;
 
-  size_t hstage_bytes_ = 0;
   size_t stage_bytes_ = 0;  // bytes of the assembly staging region
 
   // reconstruction state
@@ -184,8 +197,6 @@ class Engine {
   hipGraphExec_t graph_exec_ = nullptr;
   std::vector<float2> graph_k0_;  // k0 the graph was captured with
   int graph_version_ = -1, param_version_ = 0;
-  hipEvent_t fork_ev_ = nullptr;
-  std::vector<hipEvent_t> join_ev_;
 
   // timing
   bool timing_ = false;

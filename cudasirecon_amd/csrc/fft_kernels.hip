@@ -81,8 +81,10 @@ __global__ __launch_bounds__(W* fft_tpt(N)) void k_fft_cols(float2* __restrict__
   FftTw<N> twr;
   __shared__ float2 twstage[N > 1 ? N : 1];
   load_twiddles(twr, twstage, tid % TPT, tw, threadIdx.x, blockDim.x);
-  const long long c0 = (long long)blockIdx.x * W;
-  float2* base = data + (long long)blockIdx.y * plane_stride;
+  unsigned bx, by;
+  xcd_tile(bx, by);
+  const long long c0 = (long long)bx * W;
+  float2* base = data + (long long)by * plane_stride;
   // all loads of a thread are issued before the first LDS write (clamped
   // addresses, no branches), so they are in flight together
   constexpr int EPT = (N * W + NT - 1) / NT;
@@ -174,8 +176,10 @@ __global__ __launch_bounds__(W* fft2_tpt(N)) void k_fft2_cols(float2* __restrict
   const int tid = threadIdx.x;
   Fft2Tw<N> tw2;
   tw2.load(tid % TPT, tw);
-  const long long c0 = (long long)blockIdx.x * W;
-  float2* base = data + (long long)blockIdx.y * plane_stride;
+  unsigned bx, by;
+  xcd_tile(bx, by);
+  const long long c0 = (long long)bx * W;
+  float2* base = data + (long long)by * plane_stride;
   constexpr int EPT = N * W / NT;
   float2 v[EPT];
 #pragma unroll

@@ -46,6 +46,7 @@ struct DevBuf {
 
 struct PrepParams {
   int nx, ny, nz, nphases, napod, estimate;
+  int fix_minus1;  // camera fix when the last pixel is exactly -1 (the I2M sections), else when it is < 0
 };
 
 // One workgroup per section.
@@ -60,7 +61,9 @@ __global__ __launch_bounds__(1024) void k_psf_prep(PrepParams p, const float* __
   const long long nxy = (long long)p.nx * p.ny;
   const float* src = psf + (long long)s * nxy;
   float* dst = vol + ((long long)phase * p.nz + z) * nxy;
-  const float lastv = src[nxy - 1] < 0 ? src[nxy - 2] : src[nxy - 1];  // camera fix of the last pixel
+  const float last = src[nxy - 1];
+  const bool fix = p.fix_minus1 ? last == -1.0f : last < 0;
+  const float lastv = fix ? src[nxy - 2] : last;  // camera fix of the last pixel
   const int tid = threadIdx.x, nt = blockDim.x;
   if (p.estimate) {
     const int border = 20;
@@ -579,8 +582,26 @@ size_t otf_elems(const simr_otf_params& p, int nx, int ny, int nsections) {
   return (size_t)nbo * (std::min(nx, ny) / 2 + 1) * (size_t)nz;
 }
 
-void make_otf(const simr_otf_params& p, const float* psf, int nx, int ny, int nsec, const float* section_bg,
-              float* otf_out, float* band0_out, simr_otf_info* info) {
+// i2m: the I2M OTF of radialft.cpp:267-286, 289-307, 345-357, 383-386,
+// 425-430 from one I2M section per z (nsec = nz): camera fix only for an
+// exact -1, no apodization, its own bead centre, one background (the border
+// estimate of section nz/2) for every section, no bead-size compensation, no
+// modify or cleanup, fixorigin and the order-0 rescale.
+void make_otf(const simr_otf_params& p0, const float* psf, int nx, int ny, int nsec, const float* section_bg,
+              float* otf_out, float* band0_out, simr_otf_info* info, bool i2m) {
+  simr_otf_params p = p0;
+  if (i2m) {
+    p.nphases = 1;
+    p.napodize = 0;
+    p.background = -1.f;
+    p.do_compen = 0;
+    p.ifixz = p.ifixr = 0;
+    p.leavekz[0] = p.leavekz[1] = p.leavekz[2] = 0;
+    p.five_bands = 0;
+    p.conjugate = 0;
+    section_bg = nullptr;
+    band0_out = nullptr;
+  }
   SIMR_REQUIRE(psf != nullptr && otf_out != nullptr, "null PSF or output pointer");
   SIMR_REQUIRE(p.nphases >= 1 && p.nphases <= SIMR_MAX_PHASES, "makeotf: nphases must be 1..5");
   SIMR_REQUIRE(nx >= 4 && ny >= 4, "makeotf: PSF sections must be at least 4x4");
@@ -635,10 +656,19 @@ void make_otf(const simr_otf_params& p, const float* psf, int nx, int ny, int ns
   SIMR_HIP(hipMemcpyAsync(d_bg.p, bgh.data(), (size_t)nsec * 4, hipMemcpyHostToDevice, s));
 
   const int nused = nz * nphases;  // trailing sections beyond whole z planes are ignored
-  const PrepParams pp{nx, ny, nz, nphases, p.napodize, estimate};
+  const PrepParams pp{nx, ny, nz, nphases, p.napodize, estimate, i2m ? 1 : 0};
   hipLaunchKernelGGL(k_psf_prep, dim3(nused), dim3(1024), 0, s, pp, d_psf.as<float>(), d_vol.as<float>(),
                      d_bg.as<float>(), d_tab.as<float>(), d_tab.as<float>() + 64, d_tab.as<float>() + 64 + nx);
   SIMR_LAUNCH_CHECK();
+  if (i2m) {
+    // background_i2m: section nz/2's border estimate, subtracted from every section
+    float b;
+    SIMR_HIP(hipMemcpyAsync(&b, d_bg.as<float>() + nz / 2, 4, hipMemcpyDeviceToHost, s));
+    SIMR_HIP(hipStreamSynchronize(s));
+    std::fill(bgh.begin(), bgh.end(), b);
+    SIMR_HIP(hipMemcpyAsync(d_bg.p, bgh.data(), (size_t)nsec * 4, hipMemcpyHostToDevice, s));
+    SIMR_HIP(hipStreamSynchronize(s));
+  }
 
   // bead centre
   const int nparts = (int)std::min<long long>(1024, (nvox + 255) / 256);

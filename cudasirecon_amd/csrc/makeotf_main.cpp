@@ -7,10 +7,13 @@
 //           [-leavekz a b c] [-nphases n] [-beaddiam d] [-background b]
 //           [-bgInExtHdr] [-5bands] [-nocompen] [-conj] [-rescale] [-2lenses]
 //           [-bessel] [-PIshift] [-ifixkz n] [-ifixkr n] [-wavelength nm]
-//           [-xyres um] [-zres um] [-gen_order0 file]
+//           [-xyres um] [-zres um] [-gen_order0 file] [-I2M file]
 // The input is read as TIFF when it is one, else as MRC.  MRC input takes the
 // pixel sizes and wavelength from its header; TIFF input from -xyres, -zres
-// and -wavelength.  The output format follows the input's.
+// and -wavelength.  The output format follows the input's.  With -I2M the
+// MRC input holds one extra I2M section after the phases of every z, whose
+// OTF (simr_makeotf_i2m) is written to the -I2M file as an MRC complex
+// [kr][kz] section (radialft.cpp:425-430, mrc_file_write mode 2).
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -55,6 +58,8 @@ void usage() {
       "  -xyres um              x-y pixel size (0.106; TIFF input)\n"
       "  -zres um               z step (0.1; TIFF input)\n"
       "  -gen_order0 file       also write the separated order-0 band\n"
+      "  -I2M file              the input holds an I2M section after each z's phases (MRC);\n"
+      "                         write its OTF to this file\n"
       "  -help or -h            this message\n");
 }
 
@@ -80,7 +85,7 @@ bool getint(const char* s, int* out) {
 
 struct Cli {
   simr_otf_params p;
-  std::string in, out, order0;
+  std::string in, out, order0, i2m_out;
   bool in_given = false, out_given = false, bg_in_ext = false, i2m = false;
   float user_dr = 0, user_dz = 0;
   int wavelength = 525;
@@ -158,7 +163,12 @@ bool commandline(int argc, char** argv, Cli& c) {
     } else if (a == "-PIshift") {
       c.p.forced_pi_shift = 1, n++;
     } else if (a == "-I2M") {
+      if (!need(1)) {
+        std::printf("Invalid input for switch -I2M\n");
+        return false;
+      }
       c.i2m = true;
+      c.i2m_out = argv[n + 1];
       n += 2;
     } else if (a == "-gen_order0") {
       if (!need(1)) {
@@ -202,10 +212,6 @@ int run(int argc, char** argv) {
   Cli c;
   simr_otf_params_default(&c.p);
   if (!commandline(argc, argv, c)) return -1;
-  if (c.i2m) {
-    std::fprintf(stderr, "-I2M is not supported by this engine\n");
-    return -1;
-  }
   std::printf("ileavekz[0]=%d, ifixz[0]=%d\n", c.p.leavekz[0], c.p.ifixz);
   if (!c.in_given) {
     std::cout << " PSF dataset file name: ";
@@ -250,22 +256,40 @@ int run(int argc, char** argv) {
     std::fprintf(stderr, "nphases must be positive\n");
     return -1;
   }
-  const int nz = nsec / c.p.nphases;
+  if (c.i2m && tiff) {
+    // the reference reads no I2M section from TIFF input (radialft.cpp:268-271)
+    std::fprintf(stderr, "-I2M needs MRC input\n");
+    return -1;
+  }
+  if (c.i2m && nx != ny) {
+    std::fprintf(stderr, "-I2M needs square sections (its OTF is written with nx/2+1 radial samples)\n");
+    return -1;
+  }
+  // sections per z: the phases, then the I2M section
+  const int per_z = c.p.nphases + (c.i2m ? 1 : 0);
+  const int nz = nsec / per_z;
   std::printf("nx=%d, ny=%d, nz=%d, dr=%f, dz=%f, wavelength=%d\n", nx, ny, nz, c.p.dr, c.p.dz, c.p.wavelength);
   const int used = nz * c.p.nphases;
   const size_t nxy = (size_t)nx * ny;
   std::vector<float> psf(nxy * used);
   std::vector<float> bg;
-  for (int s = 0; s < used; s++) {
-    if (tiff)
-      tiff_in->read_f32(s, psf.data() + nxy * s);
-    else
-      mrc_in->read_f32(s, psf.data() + nxy * s);
+  std::vector<float> i2m(c.i2m ? nxy * nz : 0);
+  for (int z = 0; z < nz; z++) {
+    for (int ph = 0; ph < c.p.nphases; ph++) {
+      const int s = z * c.p.nphases + ph;
+      if (tiff)
+        tiff_in->read_f32(s, psf.data() + nxy * s);
+      else
+        mrc_in->read_f32((long long)z * per_z + ph, psf.data() + nxy * s);
+    }
+    if (c.i2m) mrc_in->read_f32((long long)z * per_z + c.p.nphases, i2m.data() + nxy * z);
   }
   const float* bgp = nullptr;
   if (!tiff) {
     // MRC input: the per-section background comes from the extended header
-    // with -bgInExtHdr and is 0 otherwise (-background applies to TIFF input)
+    // with -bgInExtHdr and is 0 otherwise (-background applies to TIFF input).
+    // Phase section s takes the extended header of file section s, as the
+    // reference's section counter skips the I2M sections (radialft.cpp:244-252).
     bg.assign(used, 0.f);
     if (c.bg_in_ext) {
       std::vector<float> ext;
@@ -285,6 +309,16 @@ int run(int argc, char** argv) {
     return -1;
   }
   std::printf("Center of mass is (%.3f, %.3f, %.3f)\n\n", info.xc, info.yc, info.zc);
+  std::vector<float> otf_i2m;
+  simr_otf_info info_i2m;
+  if (c.i2m) {
+    otf_i2m.assign((size_t)2 * info.nkr * nz, 0.f);
+    if (simr_makeotf_i2m(&c.p, i2m.data(), nx, ny, nz, otf_i2m.data(), &info_i2m) != SIMR_OK) {
+      std::fprintf(stderr, "makeotf -I2M failed: %s\n", simr_last_error());
+      return -1;
+    }
+    std::printf("I2M psf's center of mass is (%.3f, %.3f, %.3f)\n\n", info_i2m.xc, info_i2m.yc, info_i2m.zc);
+  }
   for (int o = 1; o < info.norders && !c.p.five_bands; o++) std::printf("  phi=%f\n", info.phi[o]);
 
   const int nbo = info.nbands_out, nkr = info.nkr;
@@ -336,6 +370,32 @@ int run(int argc, char** argv) {
     h.nlab = 1;
     simr::mrc::Writer wr(c.out, h);
     for (int b = 0; b < nbo; b++) wr.write_c64(otf.data() + (size_t)2 * nkr * nz * b);
+    wr.close();
+  }
+  if (c.i2m) {
+    // mrc_file_write(..., nz, nx/2+1, 1, dkz, dkx, mode 2, wavelength, I2Mfiles) (radialft.cpp:1556-1633)
+    std::printf("Writing output file: %s\n", c.i2m_out.c_str());
+    simr::mrc::Header h;
+    h.nx = h.mx = nz;
+    h.ny = h.my = nkr;
+    h.nz = h.mz = 1;
+    h.mode = simr::mrc::kComplex;
+    h.xlen = h.ylen = info.dkz;
+    h.zlen = info.dkr;
+    h.alpha = h.beta = h.gamma = 0;
+    h.num_waves = 1;
+    h.num_times = 1;
+    h.iwav1 = (int16_t)c.p.wavelength;
+    h.nres = 1;
+    h.nzfact = 1;
+    h.interleaved = simr::mrc::kZWT;
+    h.amin = 0.f;
+    h.amax = 1.f;
+    h.amean = 0.1f;
+    h.nlab = 1;
+    std::memcpy(h.label[0], "Processed", 9);
+    simr::mrc::Writer wr(c.i2m_out, h);
+    wr.write_c64(otf_i2m.data());
     wr.close();
   }
   return 0;

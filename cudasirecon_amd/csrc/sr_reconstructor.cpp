@@ -209,10 +209,6 @@ void Reconstructor::check_supported() const {
     throw UnsupportedError(
         "--usecorr together with --deskew or --cropXY is not supported (the reference indexes the "
         "calibration maps with the raw width but sizes them with the resampled one)");
-  if (!P_.bRadAvgOTF)
-    throw UnsupportedError(
-        "OTFs that are not rotationally averaged are not supported (pass --otfRA); the reference's GPU path "
-        "evaluates them as zero");
 }
 
 // setup_common (cudaSirecon.cpp:1139-1261): the host part.  The separation
@@ -295,12 +291,17 @@ void Reconstructor::determine_otf_dimensions() {
   }
   std::printf("nzotf=%d, dkzotf=%f, nxotf=%d, nyotf=%d, dkrotf=%f\n", P_.nzotf, P_.dkzotf, P_.nxotf, P_.nyotf,
               P_.dkrotf);
+  if (I_.nz > 1 && !P_.bRadAvgOTF && (P_.nzotf < 1 || !(P_.dkzotf > 0) || !(P_.dkrotf > 0)))
+    throw ArgError("the OTF file does not hold a full 3D OTF (norders x nzotf sections with positive kx and kz "
+                   "steps); pass --otfRA for a rotationally averaged OTF");
 }
 
-// loadOTFs (cudaSirecon.cpp:1532-1617), rotationally averaged / 2D case:
+// loadOTFs (cudaSirecon.cpp:1532-1617).  Rotationally averaged or 2D:
 // section (dir*norders + order) of an MRC file, or page `order` of a TIFF
 // (the TIFF branch reads page `order` for every direction); when the file has
-// fewer sections the order-0 OTF of direction 0 is repeated.
+// fewer sections the order-0 OTF of direction 0 is repeated.  A full 3D OTF
+// (no --otfRA): nzotf consecutive MRC sections per (direction, order), or
+// TIFF pages order*nzotf .. order*nzotf + nzotf - 1 for every direction.
 void Reconstructor::load_otfs() {
   const int ndo = P_.bOneOTFperAngle ? P_.ndirs : 1;
   const size_t size = (size_t)P_.nzotf * P_.nxotf * P_.nyotf;
@@ -308,8 +309,28 @@ void Reconstructor::load_otfs() {
   auto slot = [&](int dir, int order) { return otf_.data() + ((size_t)dir * P_.norders + order) * size * 2; };
   std::vector<float> page;
   long long next_sec = 0;
+  const bool full3d = I_.nz > 1 && !P_.bRadAvgOTF;
   for (int dir = 0; dir < ndo; dir++) {
     for (int i = 0; i < P_.norders; i++) {
+      if (full3d) {
+        const size_t plane = (size_t)P_.nxotf * P_.nyotf;
+        for (int z = 0; z < P_.nzotf; z++) {
+          if (P_.bTIFF) {
+            const long long pg = (long long)i * P_.nzotf + z;
+            if (pg >= otf_tiff_->pages()) throw FormatError("OTF TIFF has fewer pages than norders * nzotf");
+            page.resize((size_t)otf_tiff_->width() * otf_tiff_->height());
+            if (page.size() < 2 * plane) throw FormatError("OTF TIFF page smaller than an OTF plane");
+            otf_tiff_->read_f32(pg, page.data());
+          } else {
+            const mrc::Header& h = otf_mrc_->header();
+            if (next_sec >= h.nz) throw FormatError("OTF MRC file has fewer sections than the OTF it should hold");
+            page.resize((size_t)h.nx * h.ny * 2);
+            otf_mrc_->read_c64(next_sec++, page.data());
+          }
+          std::memcpy(slot(dir, i) + z * plane * 2, page.data(), plane * 2 * sizeof(float));
+        }
+        continue;
+      }
       const long long nsec = P_.bTIFF ? otf_tiff_->pages() : otf_mrc_->header().nz;
       if (nsec > i + dir * P_.norders) {
         if (P_.bTIFF) {

@@ -10,11 +10,18 @@ constexpr int kMaxDirs = 8;
 constexpr int kMaxOrders = 3;
 constexpr int kMaxBands = 5;
 
-// Rotationally averaged OTF table: [kr][kz], kz fastest (nzotf entries per kr).
-struct OtfRA {
+// OTF table of one (direction, order).
+//   ra = 1: rotationally averaged, [kr][kz], kz fastest (nzotf entries per
+//           kr, nxotf radial samples; nyotf = 1).
+//   ra = 0: the full OTF over kx >= 0, [kz][ky][kx] (nxotf x nyotf per kz
+//           plane, nzotf planes, ky and kz in FFT order), read by trilinear
+//           interpolation (the reference's GPU path loads such a table but
+//           evaluates it as zero, gpuFunctionsImpl.cu:1680-1714).
+struct OtfTab {
   const float2* t;
   int nzotf, nxotf;
   float dkrotf;
+  int nyotf, ra;
 };
 
 // ---- forward path -------------------------------------------------------
@@ -47,6 +54,9 @@ void launch_rows_fwd(const FwdParams& p, const void* raw, bool u16, const float*
                      const float* ratio, const float* apofact, float2* bands, hipStream_t s);
 
 // ---- k0 search --------------------------------------------------------------
+// The directions' overlap pairs, cross-correlations and modamp candidates are
+// processed together: every launch below covers up to kMaxDirs slots (one per
+// direction being fitted), each with its own parameters and buffers.
 struct OvParams {
   int nx, ny, nz, nxh;
   int zd;  // z support half-width in planes
@@ -55,17 +65,23 @@ struct OvParams {
   float rdistcutoff, otfcutoff, order0_2_factor;
   float dkx, dky, kzscale;
   int bNoKz0;
-  OtfRA otf1, otf2;
+  OtfTab otf1, otf2;
   const float2 *b1re, *b1im, *b2re, *b2im;  // b1im null for order1 == 0
   // host-derived: rdistcutoff^2, otfcutoff^2, order0_2_factor^2, 1/dkrotf^2 of otf1 and otf2
   float rc2, cut2, f2, inv_dkr2_1, inv_dkr2_2;
+  float2* ovx;  // [2][nzs_stride][ny][nx]: overlap0 planes, then overlap1 planes
 };
-// ovx: [2][2zd+1][ny][nx]; builds both overlaps and inverse-transforms along x and y.
-void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s);
-// its two halves: the overlap formation with the x inverse transform, and the y inverse transform
-void launch_overlaps_x(const OvParams& p, float2* ovx, hipStream_t s);
-void launch_overlaps_y(const OvParams& p, float2* ovx, hipStream_t s);
-// The per-(x,y) z sums of the inverse-transformed overlap pair,
+struct OvBatch {
+  OvParams p[kMaxDirs];
+  int n;           // slots
+  int nzs_stride;  // planes reserved per overlap (2 zd_max + 1 >= 2 zd + 1)
+};
+// Builds both overlaps of every slot and inverse-transforms them along x.
+void launch_overlaps_x(const OvBatch& b, hipStream_t s);
+// The y inverse transforms of nslots ovx sets laid out one after another from
+// ovx (each [2][nzs_stride][ny][nx]).
+void launch_overlaps_y(int nx, int ny, int nzs_stride, float2* ovx, int nslots, hipStream_t s);
+// The per-(x,y) z sums of each slot's inverse-transformed overlap pair,
 // P = sum_z conj(X) Y, SX = sum |X|^2, SY = sum |Y|^2 (double storage).
 // fp32: accumulated in float as the reference's reductionKernel threads, P,
 // SX, SY stored per (x, y).  Otherwise in double, P per (x, y) and the totals
@@ -73,33 +89,64 @@ void launch_overlaps_y(const OvParams& p, float2* ovx, hipStream_t s);
 // (per_elem: SX and SY per (x, y) as well).
 constexpr int kZsumMaxBlocks = 2048;
 int zsum_blocks(long long nxy);
-void launch_overlaps_z_reduce(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, double* totals,
-                              bool fp32, bool per_elem, hipStream_t s);
+struct ZsumBatch {
+  const float2* ovx[kMaxDirs];
+  double2* P[kMaxDirs];
+  double* SX[kMaxDirs];
+  double* SY[kMaxDirs];
+  double* totals[kMaxDirs];
+  int nzs[kMaxDirs];
+  int n, nzs_stride, nz;
+  long long nxy;
+};
+void launch_overlaps_z_reduce(const ZsumBatch& b, bool fp32, bool per_elem, hipStream_t s);
 // Inverse z DFT of the stored kz planes of both overlaps into real-space
 // volumes vol[2][nz][nx*ny] (debug dump only).
-void launch_overlap_volumes(const float2* ovx, int zd, int nz, long long nxy, float2* vol, hipStream_t s);
-// crosscorr = conj(P) as float, then its forward 2D FFT and |.|^2 argmax.
-// peak_out (host-visible after sync): [0] = index of the first maximum,
-// values: int peak index and the 5 intensities (center, x-1, x+1, y-1, y+1).
-void launch_findpeak(const double2* P, float2* cc, float* partial_val, int* partial_idx, float* peak_vals,
+void launch_overlap_volumes(const float2* ovx, int zd, int nzs_stride, int nz, long long nxy, float2* vol,
+                            hipStream_t s);
+// crosscorr = conj(P) as float, then its forward 2D FFT and |.|^2 argmax, for
+// n slots: cc [n][nx*ny] (contiguous), partials [n][kArgBlocks],
+// peak_vals [n][8] (the 5 intensities centre, x-1, x+1, y-1, y+1),
+// peak_idx [n] (index of the first maximum).
+constexpr int kArgBlocks = 256;
+struct PeakBatch {
+  const double2* P[kMaxDirs];
+  int n;
+};
+void launch_findpeak(const PeakBatch& b, float2* cc, float* partial_val, int* partial_idx, float* peak_vals,
                      int* peak_idx, int nx, int ny, hipStream_t s);
-// Modamp reductions for a batch of candidate vectors.
+// Modamp reductions for a batch of candidate vectors kcand[c] on slot kslot[c].
 //   fp64: out[c] = sum P e^{i phi_c} over the image in double; partials [ncand][nparts] double2.
 //   fp32: reference reductionKernel on 64x4 blocks; partials [ncand][nblocks] of
 //         (xsy.x, xsy.y, sx, sy) float4, summed on the host in block order.
+struct ModampSlots {
+  const double2* P[kMaxDirs];
+  const double* SX[kMaxDirs];
+  const double* SY[kMaxDirs];
+  const double* totals[kMaxDirs];  // the z-sum totals (fp64 finish)
+};
 int modamp_partials_fp64(int nx, int ny);
 int modamp_partials_fp32(int nx, int ny);
-void launch_modamp_fp64(const double2* P, const float2* kcand, int ncand, int nx, int ny, float dxy,
-                        double2* partials, hipStream_t s);
-void launch_modamp_fp32(const double2* P, const double* SX, const double* SY, const float2* kcand, int ncand,
-                        int nx, int ny, float dxy, float4* partials, hipStream_t s);
+void launch_modamp_fp64(const ModampSlots& m, const float2* kcand, const int* kslot, int ncand, int nx, int ny,
+                        float dxy, double2* partials, hipStream_t s);
+void launch_modamp_fp32(const ModampSlots& m, const float2* kcand, const int* kslot, int ncand, int nx, int ny,
+                        float dxy, float4* partials, hipStream_t s);
+// The sums the host makes of the modamp partials (cpuReduce,
+// gpuFunctionsImpl_hh.cu:162-180), on the device: for candidate c, its
+// partials summed in block order and, fp64, its slot's z-sum totals summed in
+// block order; out[4c..4c+3] = (xr, xi, sx, sy) (fp32: the float sums).
+void launch_modamp_evals(bool fp32, const ModampSlots& m, const int* kslot, int ncand, const double2* part64,
+                         const float4* part32, int nparts, int zblocks, double* out, hipStream_t s);
 // findrealspacemodamp's result (gpuFunctionsImpl.cu:1128-1147) on the device,
-// for one candidate: the partials summed in block order as the host does
-// (cpuReduce, gpuFunctionsImpl_hh.cu:162-180), then modamp = XStarY / sumX
-// and corr = |XStarY| / sqrt(sumX sumY), written to *amp and *corr.
-// fp64: part64 [nparts] and the z-sum totals [2][zblocks]; fp32: part32 [nparts].
-void launch_modamp_finish(bool fp32, const double2* part64, const float4* part32, int nparts, const double* totals,
-                          int zblocks, float2* amp, float* corr, hipStream_t s);
+// for each of ncand candidates: the partials summed in block order as the
+// host does (cpuReduce, gpuFunctionsImpl_hh.cu:162-180), then
+// modamp = XStarY / sumX and corr = |XStarY| / sqrt(sumX sumY), written to
+// amp[kslot[c] * stride + order] and corr[same].
+// fp64: part64 [ncand][nparts] and the slot's z-sum totals [2][zblocks];
+// fp32: part32 [ncand][nparts].
+void launch_modamp_finish(bool fp32, const ModampSlots& m, const int* kslot, int ncand, const double2* part64,
+                          const float4* part32, int nparts, int zblocks, float2* amp, float* corr, int stride,
+                          int order, hipStream_t s);
 // The modamp policy after every order is fitted at a known k0
 // (cudaSirecon.cpp:450-491): 2D chaining amp[o] *= amp[o-1], forcemodamp
 // (raise magnitudes below the forced ones), and with fit_all_phases == 0 the
@@ -126,7 +173,7 @@ struct FilterParams {
   // non-null: the modamps [ndirs][norders] on the device (fitted there);
   // ampmag2_all and conjamp are then formed from them in the kernel
   const float2* amps;
-  OtfRA otf[kMaxOrders];
+  OtfTab otf[kMaxOrders];
   float2 *bp, *bp2;
   // host-derived constants (same float/double expressions as the per-thread ones)
   float inv_nv[kMaxDirs * kMaxOrders];              // (float)(1.0 / (double)noiseVar)

@@ -170,9 +170,11 @@ __global__ __launch_bounds__(asm_zp_threads<B0>()) void k_asm_z_pair(AsmParams p
   FftTw<N> twr;
   __shared__ float2 twstage[N > 1 ? N : 1];
   load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
-  const int a = B0 ? 0 : 1 + blockIdx.z;
-  const int s = blockIdx.y, sp = s == 0 ? 0 : p.ny - s;  // stored rows s and -s
-  const int x0 = blockIdx.x * W;
+  unsigned bx, by, bz;
+  xcd_tile3(bx, by, bz);
+  const int a = B0 ? 0 : 1 + (int)bz;
+  const int s = (int)by, sp = s == 0 ? 0 : p.ny - s;  // stored rows s and -s
+  const int x0 = (int)bx * W;
   const int xmax = p.nx / 2;  // stored columns 0 .. nx/2 feed the output
   const long long dirstride = (long long)p.nbands * p.band_stride;
   const float2* b0;
@@ -528,6 +530,54 @@ __device__ __forceinline__ void asm_x_load_rows(const AsmParams& p, float2* dst,
     for (int X = p.nx - half + 1 + lane; X < N - (half - 1); X += 64) dst[r * LP + asm_x_pos<N>(X)] = make_float2(0.f, 0.f);
 }
 
+// The same rows when the source row fits one batch (nx <= 64 B, every 2x
+// zoom): asm_x_fetch issues the global loads into registers, asm_x_put
+// writes them (and the zero columns) to LDS.  The kernel fetches the next
+// (plane, round) right after putting the current one, so the loads are in
+// flight during the transform and accumulation instead of stalling the
+// start of every plane.
+template <int N>
+constexpr int asm_x_batch() {
+  return N >= 256 ? N / 128 : 2;
+}
+template <int N, int R>
+struct AsmXRows {
+  float2 v[R][asm_x_batch<N>()];
+};
+template <int N, int R>
+__device__ __forceinline__ void asm_x_fetch(const AsmParams& p, AsmXRows<N, R>& b, int a0, int zout, int Y,
+                                            int lane) {
+  constexpr int B = asm_x_batch<N>();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int a = a0 + r < p.nasm ? a0 + r : p.nasm - 1;
+    const float2* src = p.stageY + (((long long)a * p.zdim + zout) * p.ydim + Y) * p.nx;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int sc = lane + 64 * k;
+      b.v[r][k] = src[sc < p.nx ? sc : p.nx - 1];
+    }
+  }
+}
+template <int N, int R>
+__device__ __forceinline__ void asm_x_put(const AsmParams& p, float2* dst, const AsmXRows<N, R>& b, int a0,
+                                          int lane) {
+  constexpr int LP = asm_x_pitch<N>();
+  constexpr int B = asm_x_batch<N>();
+  const int half = p.nx / 2;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int sc = lane + 64 * k;
+      const int X = sc >= half - 1 ? sc - (half - 1) : sc + N - (half - 1);
+      if (sc < p.nx) dst[r * LP + asm_x_pos<N>(X)] = a0 + r < p.nasm ? b.v[r][k] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    for (int X = p.nx - half + 1 + lane; X < N - (half - 1); X += 64) dst[r * LP + asm_x_pos<N>(X)] = make_float2(0.f, 0.f);
+}
+
 template <int N>
 __global__ __launch_bounds__(asm_x_threads<N>(), 2) void k_asm_x(AsmParams p,
                                                                                    const float2* __restrict__ tw) {
@@ -555,6 +605,47 @@ __global__ __launch_bounds__(asm_x_threads<N>(), 2) void k_asm_x(AsmParams p,
       lut[e][b] = (LUT && X < N && b + 1 < p.nasm) ? asm_phasor<N>(p, b + 1, X, Y) : make_float2(1.f, 0.f);
     }
   __syncthreads();
+  if constexpr (asm_x_fft2<N>()) {
+    if (p.nx <= 64 * asm_x_batch<N>()) {
+      constexpr int TPW = 64 / TPT;
+      const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+      AsmXRows<N, TPW> rows;
+      asm_x_fetch<N, TPW>(p, rows, wave * TPW, 0, Y, lane);
+      for (int zout = 0; zout < p.zdim; ++zout) {
+        float acc[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) acc[e] = 0.f;
+#pragma nounroll
+        for (int a0 = 0; a0 < p.nasm; a0 += G) {
+          asm_x_put<N, TPW>(p, lds + wave * TPW * LP, rows, a0 + wave * TPW, lane);
+          // the next (plane, round) this wavefront loads
+          int na0 = a0 + G, nz = zout;
+          if (na0 >= p.nasm) {
+            na0 = 0;
+            ++nz;
+          }
+          if (nz < p.zdim) asm_x_fetch<N, TPW>(p, rows, na0 + wave * TPW, nz, Y, lane);
+          wave_lds_sync();
+          fft2_lds<N, true, true>(lds + g * LP, t, twt);
+          __syncthreads();
+          if (a0 == 0)
+            asm_x_round<N, NT, G, E, LB, 0>(p, lds, a0, Y, acc, lut);
+          else if (a0 == G)
+            asm_x_round<N, NT, G, E, LB, 1>(p, lds, a0, Y, acc, lut);
+          else
+            asm_x_round<N, NT, G, E, LB, -1>(p, lds, a0, Y, acc, lut);
+          __syncthreads();
+        }
+        float* orow = p.out + ((long long)zout * p.ydim + Y) * N;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int X = threadIdx.x + e * NT;
+          if (X < N) orow[X] = acc[e];
+        }
+      }
+      return;
+    }
+  }
   for (int zout = 0; zout < p.zdim; ++zout) {
     float acc[E];
 #pragma unroll

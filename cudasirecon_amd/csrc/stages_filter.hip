@@ -67,7 +67,7 @@ __device__ __forceinline__ float amp_mag2(const FilterParams& p, int dir, int or
   return a.x * a.x + a.y * a.y;
 }
 
-template <bool UNV>
+template <bool UNV, bool RA>
 __device__ __forceinline__ float filter_term(const FilterParams& p, const float2* prof, int nxotf, int order, int dir2,
                                              int order2, float xabs, float yabs, int z0, float zf3) {
 #pragma clang fp contract(off)
@@ -86,7 +86,11 @@ __device__ __forceinline__ float filter_term(const FilterParams& p, const float2
   // divisions, far inside the output tolerance)
   if (!(d2 < p.cut2)) return 0.f;
   const int ao2 = order2 < 0 ? -order2 : order2;
-  const float2 otf2 = prof_pos(prof + ao2 * nxotf, nxotf, __builtin_amdgcn_sqrtf(d2 * p.inv_dkr2));
+  float2 otf2;
+  if constexpr (RA)
+    otf2 = prof_pos(prof + ao2 * nxotf, nxotf, __builtin_amdgcn_sqrtf(d2 * p.inv_dkr2));
+  else
+    otf2 = otf3d_interp(p.otf[ao2], x2, y2, z0, p.kzscale);
   float w = __builtin_fmaf(otf2.x, otf2.x, otf2.y * otf2.y);
   if (!UNV && !p.unit_nv) w = w / p.noiseVar[dir2 * p.norders + ao2];
   if (order2 != 0) w *= amp_mag2(p, dir2, ao2);
@@ -103,7 +107,7 @@ __device__ __forceinline__ float filter_term(const FilterParams& p, const float2
 // ND, NO > 0: the direction and order counts as compile-time constants (the
 // (dir2, order2) loop is unrolled, so its per-term constants are read once
 // and the OTF lookups of different terms overlap); 0: taken from p.
-template <int ND, int NO, bool UNV>
+template <int ND, int NO, bool UNV, bool RA>
 __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2* prof, int x1, int y1, int z0,
                                              bool conj, float2& re, float2& im) {
 #pragma clang fp contract(off)
@@ -123,7 +127,11 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
     const float zf = fabsf((float)z0 * p.inv_zd0);
     zf3 = zf * zf * zf;
   }
-  const float2 otf1 = prof_pos(prof + order * nxotf, nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2));
+  float2 otf1;
+  if constexpr (RA)
+    otf1 = prof_pos(prof + order * nxotf, nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2));
+  else
+    otf1 = otf3d_interp(p.otf[order], x1f, y1f, z0, p.kzscale);
   float weight = otf1.x * otf1.x + otf1.y * otf1.y;
   if (order != 0) weight *= amp_mag2(p, p.dir, order);
   float dampfact = p.inv_nv[p.dir * p.norders + order];
@@ -141,11 +149,11 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
     for (int dir2 = 0; dir2 < ND; dir2++)
 #pragma unroll
       for (int order2 = -(NO - 1); order2 < NO; order2++)
-        sumweight += filter_term<UNV>(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
+        sumweight += filter_term<UNV, RA>(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
   } else {
     for (int dir2 = 0; dir2 < p.ndirs; dir2++)
       for (int order2 = -(p.norders - 1); order2 < p.norders; order2++)
-        sumweight += filter_term<false>(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
+        sumweight += filter_term<false, RA>(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
   }
   sumweight += p.wiener;
   const float rs = __builtin_amdgcn_rcpf(sumweight);
@@ -204,7 +212,7 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
 // Planes whose z lies outside +-zd are zeroed (kernel3).
 constexpr int kFiltPer = 4;
 
-template <int ND, int NO, bool UNV>
+template <int ND, int NO, bool UNV, bool RA>
 __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
 #pragma clang fp contract(off)
   extern __shared__ float2 prof[];  // [2][norders][nxotf]
@@ -234,12 +242,14 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
   }
   const int nxotf = p.otf[0].nxotf;
   const int nprof = p.norders * nxotf;
-  for (int i = threadIdx.x; i < 2 * nprof; i += 256) {
-    const int side = i / nprof, rem = i - side * nprof;
-    const int o = rem / nxotf, ir = rem - o * nxotf;
-    prof[i] = otf_profile_entry(p.otf[o], ir, side ? -z0n : z0n, p.kzscale);
+  if constexpr (RA) {
+    for (int i = threadIdx.x; i < 2 * nprof; i += 256) {
+      const int side = i / nprof, rem = i - side * nprof;
+      const int o = rem / nxotf, ir = rem - o * nxotf;
+      prof[i] = otf_profile_entry(p.otf[o], ir, side ? -z0n : z0n, p.kzscale);
+    }
+    __syncthreads();
   }
-  __syncthreads();
   for (int k = 0; k < kFiltPer; ++k) {
     const long long ep = ep0 + k * 256;
     if (ep >= nplane) break;
@@ -271,31 +281,40 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
     const int nvis = (two && jin >= 1 && jin <= p.nx / 2 - 1) ? 2 : 1;
     const int yc = -((iin <= p.ny / 2) ? iin : iin - p.ny);
     for (int v = 0; v < nvis; ++v)
-      filter_visit<ND, NO, UNV>(p, v ? prof + nprof : prof, v ? -jin : jin, v ? yc : y1n, v ? -z0n : z0n, v != 0,
-                                re, im);
+      filter_visit<ND, NO, UNV, RA>(p, v ? prof + nprof : prof, v ? -jin : jin, v ? yc : y1n, v ? -z0n : z0n,
+                                    v != 0, re, im);
     bp[e] = re;
     if (two) bp2[e] = im;
   }
 }
 
-void launch_filter(const FilterParams& p, hipStream_t s) {
-  const long long nplane = (long long)p.nxh * p.ny;
-  const size_t lds = (size_t)2 * p.norders * p.otf[0].nxotf * sizeof(float2);
-  if (lds > 64 * 1024) throw ArgError("OTF table too long for the filter's LDS profile");
-  const dim3 grid(ceil_div(nplane, 256 * kFiltPer), p.nz);
+template <bool RA>
+static void launch_filter_kind(const FilterParams& p, size_t lds, dim3 grid, hipStream_t s) {
   const bool unv = p.unit_nv != 0;
   if (p.ndirs == 3 && p.norders == 3) {  // the 3-beam 3D case
     if (unv)
-      hipLaunchKernelGGL((k_filter<3, 3, true>), grid, dim3(256), lds, s, p);
+      hipLaunchKernelGGL((k_filter<3, 3, true, RA>), grid, dim3(256), lds, s, p);
     else
-      hipLaunchKernelGGL((k_filter<3, 3, false>), grid, dim3(256), lds, s, p);
+      hipLaunchKernelGGL((k_filter<3, 3, false, RA>), grid, dim3(256), lds, s, p);
   } else if (p.ndirs == 3 && p.norders == 2) {  // 2D, 3 phases
     if (unv)
-      hipLaunchKernelGGL((k_filter<3, 2, true>), grid, dim3(256), lds, s, p);
+      hipLaunchKernelGGL((k_filter<3, 2, true, RA>), grid, dim3(256), lds, s, p);
     else
-      hipLaunchKernelGGL((k_filter<3, 2, false>), grid, dim3(256), lds, s, p);
+      hipLaunchKernelGGL((k_filter<3, 2, false, RA>), grid, dim3(256), lds, s, p);
   } else {
-    hipLaunchKernelGGL((k_filter<0, 0, false>), grid, dim3(256), lds, s, p);
+    hipLaunchKernelGGL((k_filter<0, 0, false, RA>), grid, dim3(256), lds, s, p);
+  }
+}
+
+void launch_filter(const FilterParams& p, hipStream_t s) {
+  const long long nplane = (long long)p.nxh * p.ny;
+  const dim3 grid(ceil_div(nplane, 256 * kFiltPer), p.nz);
+  if (p.otf[0].ra) {
+    const size_t lds = (size_t)2 * p.norders * p.otf[0].nxotf * sizeof(float2);
+    if (lds > 64 * 1024) throw ArgError("OTF table too long for the filter's LDS profile");
+    launch_filter_kind<true>(p, lds, grid, s);
+  } else {
+    launch_filter_kind<false>(p, 0, grid, s);
   }
   SIMR_LAUNCH_CHECK();
 }

@@ -14,7 +14,10 @@
 //    which Parseval's identity along z forms from the stored kz planes, so
 //    there is no z transform, and the 3D overlap volumes are never re-read:
 //    every later line-search step is a 2D reduction of P against a phase ramp;
-//  * a whole batch of line-search candidates is reduced in one launch.
+//  * a whole batch of line-search candidates is reduced in one launch;
+//  * every launch covers the overlap pairs (or candidates) of all directions
+//    being fitted, one slot per direction, so a launch fills the GPU and the
+//    directions share each host read-back.
 #include <type_traits>
 
 #include "device_math.hpp"
@@ -60,6 +63,7 @@ __device__ __forceinline__ float2 prof_lerp(const float2* prof, int nxotf, float
   return make_float2(__builtin_fmaf(ar, t1.x - t0.x, t0.x), __builtin_fmaf(ar, t1.y - t0.y, t0.y));
 }
 
+template <bool RA>
 __device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof1, const float2* prof2, int which,
                                            int j, int i, int z0) {
 #pragma clang fp contract(off)
@@ -92,14 +96,24 @@ __device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof
   float sa2, sb2; // their cutoff scales, squared
   if (which == 0) {
     if (!(d12 <= p.rc2)) return o;
-    oa = prof_lerp(prof1, p.otf1.nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2_1));
-    ob = prof_lerp(prof2, p.otf2.nxotf, __builtin_amdgcn_sqrtf(d12 * p.inv_dkr2_2));
+    if constexpr (RA) {
+      oa = prof_lerp(prof1, p.otf1.nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2_1));
+      ob = prof_lerp(prof2, p.otf2.nxotf, __builtin_amdgcn_sqrtf(d12 * p.inv_dkr2_2));
+    } else {
+      oa = otf3d_interp(p.otf1, x1f, y1f, z0, p.kzscale);
+      ob = otf3d_interp(p.otf2, x12, y12, z0, p.kzscale);
+    }
     sa2 = 1.f;
     sb2 = p.f2;
   } else {
     if (!(d21 <= p.rc2)) return o;
-    oa = prof_lerp(prof2, p.otf2.nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2_2));
-    ob = prof_lerp(prof1, p.otf1.nxotf, __builtin_amdgcn_sqrtf(d21 * p.inv_dkr2_1));
+    if constexpr (RA) {
+      oa = prof_lerp(prof2, p.otf2.nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2_2));
+      ob = prof_lerp(prof1, p.otf1.nxotf, __builtin_amdgcn_sqrtf(d21 * p.inv_dkr2_1));
+    } else {
+      oa = otf3d_interp(p.otf2, x1f, y1f, z0, p.kzscale);
+      ob = otf3d_interp(p.otf1, x21, y21, z0, p.kzscale);
+    }
     sa2 = p.f2;
     sb2 = 1.f;
   }
@@ -128,12 +142,17 @@ __device__ __forceinline__ float2 ov_combine(const OvSrc& o, float2 vre, float2 
 }
 
 // x pass: rows [kOvRows*blockIdx.x, +kOvRows) of plane zb (blockIdx.y) of both
-// overlaps; the plane's two radial OTF profiles are built once in LDS.
+// overlaps of slot blockIdx.z; the plane's two radial OTF profiles are built
+// once in LDS.
 constexpr int kOvRows = 4;
 
-template <int N>
-__global__ __launch_bounds__(2 * kOvRows * fft_tpt(N)) void k_overlaps_x(OvParams p, float2* __restrict__ ovx,
-                                                                         const float2* __restrict__ tw) {
+template <int N, bool RA>
+__global__ __launch_bounds__(2 * kOvRows * fft_tpt(N)) void k_overlaps_x(OvBatch b, const float2* __restrict__ tw) {
+  const OvParams& p = b.p[blockIdx.z];
+  const int zb = blockIdx.y;
+  if (zb > 2 * p.zd) return;  // a slot with a narrower z support (whole block)
+  float2* __restrict__ ovx = p.ovx;
+  const int nzs = b.nzs_stride;
   constexpr int TPT = fft_tpt(N);
   constexpr int NT = 2 * kOvRows * TPT;
   constexpr int P = lds_pitch(N);
@@ -142,12 +161,11 @@ __global__ __launch_bounds__(2 * kOvRows * fft_tpt(N)) void k_overlaps_x(OvParam
   FftTw<N> twr;
   __shared__ float2 twstage[N > 1 ? N : 1];
   load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
-  const int zb = blockIdx.y;
   const int z0 = zb - p.zd;
-  const int nzs = 2 * p.zd + 1;
   const int nxotf = p.otf1.nxotf;
-  for (int k = threadIdx.x; k < 2 * nxotf; k += NT)
-    prof[k] = otf_profile_entry(k < nxotf ? p.otf1 : p.otf2, k < nxotf ? k : k - nxotf, z0, p.kzscale);
+  if constexpr (RA)
+    for (int k = threadIdx.x; k < 2 * nxotf; k += NT)
+      prof[k] = otf_profile_entry(k < nxotf ? p.otf1 : p.otf2, k < nxotf ? k : k - nxotf, z0, p.kzscale);
   __syncthreads();
   const int i0 = blockIdx.x * kOvRows;
   constexpr int EPT = (2 * kOvRows * N + NT - 1) / NT;
@@ -165,7 +183,7 @@ __global__ __launch_bounds__(2 * kOvRows * fft_tpt(N)) void k_overlaps_x(OvParam
       const int t = idx / N, j = idx - (idx / N) * N;  // t = which * kOvRows + row
       const int which = t / kOvRows, i = i0 + (t - which * kOvRows);
       src[c] = (q0 + c < EPT && idx < 2 * kOvRows * N && i < p.ny)
-                   ? ov_source(p, prof, prof + nxotf, which, j, i, z0)
+                   ? ov_source<RA>(p, prof, prof + nxotf, which, j, i, z0)
                    : OvSrc{0, false, false, make_float2(0.f, 0.f)};
     }
     float2 vre[CH], vim[CH];
@@ -202,34 +220,46 @@ __global__ __launch_bounds__(2 * kOvRows * fft_tpt(N)) void k_overlaps_x(OvParam
 }
 
 template <int N>
-static void launch_ov_x(const OvParams& p, float2* ovx, hipStream_t s) {
-  const size_t lds = (size_t)2 * p.otf1.nxotf * sizeof(float2);
-  hipLaunchKernelGGL(k_overlaps_x<N>, dim3(ceil_div(p.ny, kOvRows), 2 * p.zd + 1), dim3(2 * kOvRows * fft_tpt(N)),
-                     lds, s, p, ovx, twiddles(N));
+static void launch_ov_x(const OvBatch& b, int nzs_max, hipStream_t s) {
+  const OvParams& p = b.p[0];
+  const dim3 grid(ceil_div(p.ny, kOvRows), nzs_max, b.n), block(2 * kOvRows * fft_tpt(N));
+  if (p.otf1.ra) {
+    const size_t lds = (size_t)2 * p.otf1.nxotf * sizeof(float2);
+    hipLaunchKernelGGL((k_overlaps_x<N, true>), grid, block, lds, s, b, twiddles(N));
+  } else {
+    hipLaunchKernelGGL((k_overlaps_x<N, false>), grid, block, 0, s, b, twiddles(N));
+  }
   SIMR_LAUNCH_CHECK();
 }
 
-void launch_overlaps_x(const OvParams& p, float2* ovx, hipStream_t s) {
-  switch (p.nx) {
-#define SIMR_CASE(N)               \
-  case N:                          \
-    launch_ov_x<N>(p, ovx, s);     \
+void launch_overlaps_x(const OvBatch& b, hipStream_t s) {
+  if (b.n < 1 || b.n > kMaxDirs) throw ArgError("overlap batch: 1..8 slots");
+  int nzs_max = 0;
+  for (int k = 0; k < b.n; ++k) {
+    const OvParams& p = b.p[k];
+    if (p.nx != b.p[0].nx || p.ny != b.p[0].ny || p.otf1.nxotf != b.p[0].otf1.nxotf ||
+        p.otf2.nxotf != b.p[0].otf1.nxotf || p.otf1.ra != b.p[0].otf1.ra || p.otf2.ra != b.p[0].otf1.ra)
+      throw ArgError("overlap batch: slots of different geometry");
+    if (p.zd < 0 || 2 * p.zd + 1 > b.nzs_stride) throw ArgError("overlap z support wider than its planes");
+    nzs_max = std::max(nzs_max, 2 * p.zd + 1);
+  }
+  switch (b.p[0].nx) {
+#define SIMR_CASE(N)                   \
+  case N:                              \
+    launch_ov_x<N>(b, nzs_max, s);     \
     break;
     SIMR_XY_SIZES(SIMR_CASE)
 #undef SIMR_CASE
     default:
-      throw ArgError("image width " + std::to_string(p.nx) + " is not a supported size");
+      throw ArgError("image width " + std::to_string(b.p[0].nx) + " is not a supported size");
   }
 }
 
-void launch_overlaps_y(const OvParams& p, float2* ovx, hipStream_t s) {
-  const long long nxy = (long long)p.nx * p.ny;
-  fft_cols(p.ny, ovx, p.nx, p.nx, 2LL * (2 * p.zd + 1), nxy, true, s);
-}
-
-void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s) {
-  launch_overlaps_x(p, ovx, s);
-  launch_overlaps_y(p, ovx, s);
+// Every reserved plane is transformed (planes beyond a slot's own 2zd+1 hold
+// stale values that nothing reads), so the slots form one batch of columns.
+void launch_overlaps_y(int nx, int ny, int nzs_stride, float2* ovx, int nslots, hipStream_t s) {
+  const long long nxy = (long long)nx * ny;
+  fft_cols(ny, ovx, nx, nx, 2LL * nzs_stride * nslots, nxy, true, s);
 }
 
 // z pass, reduced to what the fit reads.  The modamp and cross-correlation
@@ -248,18 +278,24 @@ void launch_overlaps_xy(const OvParams& p, float2* ovx, hipStream_t s) {
 int zsum_blocks(long long nxy) { return (int)std::min<long long>(ceil_div(nxy, 256), kZsumMaxBlocks); }
 
 template <bool FP32, bool ELEM>
-__global__ __launch_bounds__(256) void k_overlaps_zsum(const float2* __restrict__ ovx, int nzs, int nz, long long nxy,
-                                                       double2* __restrict__ P, double* __restrict__ SX,
-                                                       double* __restrict__ SY, double* __restrict__ totals) {
+__global__ __launch_bounds__(256) void k_overlaps_zsum(ZsumBatch b) {
 #pragma clang fp contract(off)
   using A = typename std::conditional<FP32, float, double>::type;
   constexpr int KB = 8;
-  const A n = (A)nz;
+  const int slot = blockIdx.y;
+  const float2* __restrict__ ovx = b.ovx[slot];
+  double2* __restrict__ P = b.P[slot];
+  double* __restrict__ SX = b.SX[slot];
+  double* __restrict__ SY = b.SY[slot];
+  double* __restrict__ totals = b.totals[slot];
+  const int nzs = b.nzs[slot];
+  const long long nxy = b.nxy;
+  const A n = (A)b.nz;
   double tsx = 0.0, tsy = 0.0;
   for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < nxy; e += (long long)gridDim.x * 256) {
     A sxr = 0, sxi = 0, sx = 0, sy = 0;
     const float2* X = ovx + e;
-    const float2* Y = ovx + (long long)nzs * nxy + e;
+    const float2* Y = ovx + (long long)b.nzs_stride * nxy + e;
     for (int s = 0; s < nzs; s += KB) {
       const int cnt = nzs - s < KB ? nzs - s : KB;  // uniform
       float2 xa[KB], yb[KB];
@@ -310,24 +346,25 @@ __global__ __launch_bounds__(256) void k_overlaps_zsum(const float2* __restrict_
   }
 }
 
-void launch_overlaps_z_reduce(const OvParams& p, float2* ovx, double2* P, double* SX, double* SY, double* totals,
-                              bool fp32, bool per_elem, hipStream_t s) {
-  if (2 * p.zd + 1 > p.nz) throw ArgError("overlap z support wider than the stack");
-  const long long nxy = (long long)p.nx * p.ny;
-  const int nzs = 2 * p.zd + 1;
-  const dim3 grid(zsum_blocks(nxy));
+void launch_overlaps_z_reduce(const ZsumBatch& b, bool fp32, bool per_elem, hipStream_t s) {
+  if (b.n < 1 || b.n > kMaxDirs) throw ArgError("z-sum batch: 1..8 slots");
+  for (int k = 0; k < b.n; ++k)
+    if (b.nzs[k] < 1 || b.nzs[k] > b.nz || b.nzs[k] > b.nzs_stride)
+      throw ArgError("overlap z support wider than the stack");
+  const dim3 grid(zsum_blocks(b.nxy), b.n);
   if (fp32)
-    hipLaunchKernelGGL((k_overlaps_zsum<true, true>), grid, dim3(256), 0, s, ovx, nzs, p.nz, nxy, P, SX, SY, totals);
+    hipLaunchKernelGGL((k_overlaps_zsum<true, true>), grid, dim3(256), 0, s, b);
   else if (per_elem)
-    hipLaunchKernelGGL((k_overlaps_zsum<false, true>), grid, dim3(256), 0, s, ovx, nzs, p.nz, nxy, P, SX, SY, totals);
+    hipLaunchKernelGGL((k_overlaps_zsum<false, true>), grid, dim3(256), 0, s, b);
   else
-    hipLaunchKernelGGL((k_overlaps_zsum<false, false>), grid, dim3(256), 0, s, ovx, nzs, p.nz, nxy, P, SX, SY,
-                       totals);
+    hipLaunchKernelGGL((k_overlaps_zsum<false, false>), grid, dim3(256), 0, s, b);
   SIMR_LAUNCH_CHECK();
 }
 
 // ----------------------------------------------------------------- findk0 peak
-__global__ void k_cc_prep(const double2* __restrict__ P, float2* __restrict__ cc, long long n) {
+__global__ void k_cc_prep(PeakBatch b, float2* __restrict__ cc, long long n) {
+  const double2* __restrict__ P = b.P[blockIdx.y];
+  cc += (long long)blockIdx.y * n;
   long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) {
     const double2 v = P[e];
@@ -347,10 +384,11 @@ __device__ __forceinline__ float intensity(float2 a) {
   return a.x * a.x + a.y * a.y;
 }
 
-constexpr int kArgBlocks = 256;
-
 __global__ __launch_bounds__(256) void k_argmax_partial(const float2* __restrict__ cc, long long n,
                                                         float* __restrict__ pv, int* __restrict__ pi) {
+  cc += (long long)blockIdx.y * n;
+  pv += blockIdx.y * gridDim.x;
+  pi += blockIdx.y * gridDim.x;
   __shared__ float sv[256];
   __shared__ int si[256];
   const long long chunk = (n + gridDim.x - 1) / gridDim.x;
@@ -381,6 +419,11 @@ __global__ __launch_bounds__(256) void k_argmax_partial(const float2* __restrict
 __global__ __launch_bounds__(256) void k_argmax_final(const float2* __restrict__ cc, const float* __restrict__ pv,
                                                       const int* __restrict__ pi, int nparts, int nx, int ny,
                                                       float* __restrict__ vals, int* __restrict__ idx_out) {
+  cc += (long long)blockIdx.x * nx * ny;
+  pv += blockIdx.x * nparts;
+  pi += blockIdx.x * nparts;
+  vals += blockIdx.x * 8;
+  idx_out += blockIdx.x;
   __shared__ float sv[256];
   __shared__ int si[256];
   float v = -1e30f;
@@ -415,16 +458,17 @@ __global__ __launch_bounds__(256) void k_argmax_final(const float2* __restrict__
   idx_out[0] = idx;
 }
 
-void launch_findpeak(const double2* P, float2* cc, float* partial_val, int* partial_idx, float* peak_vals,
+void launch_findpeak(const PeakBatch& b, float2* cc, float* partial_val, int* partial_idx, float* peak_vals,
                      int* peak_idx, int nx, int ny, hipStream_t s) {
+  if (b.n < 1 || b.n > kMaxDirs) throw ArgError("peak batch: 1..8 slots");
   const long long n = (long long)nx * ny;
-  hipLaunchKernelGGL(k_cc_prep, dim3(ceil_div(n, 256)), dim3(256), 0, s, P, cc, n);
+  hipLaunchKernelGGL(k_cc_prep, dim3(ceil_div(n, 256), b.n), dim3(256), 0, s, b, cc, n);
   SIMR_LAUNCH_CHECK();
-  fft_rows(nx, cc, nx, ny, false, s);
-  fft_cols(ny, cc, nx, nx, 1, 0, false, s);
-  hipLaunchKernelGGL(k_argmax_partial, dim3(kArgBlocks), dim3(256), 0, s, cc, n, partial_val, partial_idx);
+  fft_rows(nx, cc, nx, (long long)ny * b.n, false, s);
+  fft_cols(ny, cc, nx, nx, b.n, n, false, s);
+  hipLaunchKernelGGL(k_argmax_partial, dim3(kArgBlocks, b.n), dim3(256), 0, s, cc, n, partial_val, partial_idx);
   SIMR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_argmax_final, dim3(1), dim3(256), 0, s, cc, partial_val, partial_idx, kArgBlocks, nx, ny,
+  hipLaunchKernelGGL(k_argmax_final, dim3(b.n), dim3(256), 0, s, cc, partial_val, partial_idx, kArgBlocks, nx, ny,
                      peak_vals, peak_idx);
   SIMR_LAUNCH_CHECK();
 }
@@ -436,12 +480,14 @@ constexpr int kModRows = 4;
 
 int modamp_partials_fp64(int nx, int ny) { return (ny + kModRows - 1) / kModRows; }
 
-__global__ __launch_bounds__(256) void k_modamp_fp64(const double2* __restrict__ P, const float2* __restrict__ kc,
-                                                     int nx, int ny, float dxy, double2* __restrict__ partials) {
+__global__ __launch_bounds__(256) void k_modamp_fp64(ModampSlots m, const float2* __restrict__ kc,
+                                                     const int* __restrict__ kslot, int nx, int ny, float dxy,
+                                                     double2* __restrict__ partials) {
   extern __shared__ double2 alpha[];  // nx phasors
   __shared__ double2 beta[kModRows];
   __shared__ double rx[256], ry[256];
   const int c = blockIdx.y;
+  const double2* __restrict__ P = m.P[kslot[c]];
   const double kx = (double)kc[c].x, ky = (double)kc[c].y;
   const double twopi_dxy = 2.0 * 3.14159265358979323846 * (double)dxy;
   for (int i = threadIdx.x; i < nx; i += 256) {
@@ -479,20 +525,29 @@ __global__ __launch_bounds__(256) void k_modamp_fp64(const double2* __restrict__
   if (threadIdx.x == 0) partials[(long long)c * gridDim.x + blockIdx.x] = make_double2(rx[0], ry[0]);
 }
 
-void launch_modamp_fp64(const double2* P, const float2* kcand, int ncand, int nx, int ny, float dxy,
-                        double2* partials, hipStream_t s) {
+void launch_modamp_fp64(const ModampSlots& m, const float2* kcand, const int* kslot, int ncand, int nx, int ny,
+                        float dxy, double2* partials, hipStream_t s) {
+  if (ncand < 1 || ncand > 65535) throw ArgError("modamp: 1..65535 candidates per launch");
   dim3 grid(modamp_partials_fp64(nx, ny), ncand);
-  hipLaunchKernelGGL(k_modamp_fp64, grid, dim3(256), sizeof(double2) * nx, s, P, kcand, nx, ny, dxy, partials);
+  hipLaunchKernelGGL(k_modamp_fp64, grid, dim3(256), sizeof(double2) * nx, s, m, kcand, kslot, nx, ny, dxy,
+                     partials);
   SIMR_LAUNCH_CHECK();
 }
 
 // The result arithmetic of one modamp candidate, in the host's order and
 // precision (one thread: the sums are sequential in block order).
-__global__ void k_modamp_finish(int fp32, const double2* __restrict__ part64, const float4* __restrict__ part32,
-                                int nparts, const double* __restrict__ totals, int zblocks, float2* __restrict__ amp,
-                                float* __restrict__ corr) {
+__global__ void k_modamp_finish(int fp32, ModampSlots m, const int* __restrict__ kslot,
+                                const double2* __restrict__ part64, const float4* __restrict__ part32, int nparts,
+                                int zblocks, float2* __restrict__ amp, float* __restrict__ corr, int stride,
+                                int order) {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (threadIdx.x != 0) return;
+  const int c = blockIdx.x, slot = kslot[c];
+  const double* __restrict__ totals = m.totals[slot];
+  part64 += (long long)c * nparts;
+  part32 += (long long)c * nparts;
+  amp += slot * stride + order;
+  corr += slot * stride + order;
   if (fp32) {
     float xr = 0.f, xi = 0.f, sx = 0.f, sy = 0.f;
     for (int b = 0; b < nparts; ++b) {
@@ -519,10 +574,62 @@ __global__ void k_modamp_finish(int fp32, const double2* __restrict__ part64, co
   }
 }
 
-void launch_modamp_finish(bool fp32, const double2* part64, const float4* part32, int nparts, const double* totals,
-                          int zblocks, float2* amp, float* corr, hipStream_t s) {
-  hipLaunchKernelGGL(k_modamp_finish, dim3(1), dim3(64), 0, s, fp32 ? 1 : 0, part64, part32, nparts, totals,
-                     zblocks, amp, corr);
+void launch_modamp_finish(bool fp32, const ModampSlots& m, const int* kslot, int ncand, const double2* part64,
+                          const float4* part32, int nparts, int zblocks, float2* amp, float* corr, int stride,
+                          int order, hipStream_t s) {
+  if (ncand < 1) return;
+  hipLaunchKernelGGL(k_modamp_finish, dim3(ncand), dim3(64), 0, s, fp32 ? 1 : 0, m, kslot, part64, part32, nparts,
+                     zblocks, amp, corr, stride, order);
+  SIMR_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(128) void k_modamp_evals(int fp32, ModampSlots m, const int* __restrict__ kslot,
+                                                      const double2* __restrict__ part64,
+                                                      const float4* __restrict__ part32, int nparts, int zblocks,
+                                                      double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x;
+  out += 4LL * c;
+  if (fp32) {
+    if (threadIdx.x != 0) return;
+    const float4* pp = part32 + (long long)c * nparts;
+    float xr = 0.f, xi = 0.f, sx = 0.f, sy = 0.f;
+    for (int b = 0; b < nparts; ++b) {
+      const float4 v = pp[b];
+      xr += v.x;
+      xi += v.y;
+      sx += v.z;
+      sy += v.w;
+    }
+    out[0] = xr;
+    out[1] = xi;
+    out[2] = sx;
+    out[3] = sy;
+    return;
+  }
+  if (threadIdx.x == 0) {
+    const double2* pp = part64 + (long long)c * nparts;
+    double xr = 0.0, xi = 0.0;
+    for (int b = 0; b < nparts; ++b) {
+      xr += pp[b].x;
+      xi += pp[b].y;
+    }
+    out[0] = xr;
+    out[1] = xi;
+  } else if (threadIdx.x == 64 || threadIdx.x == 65) {
+    // sx (lane 0) and sy (lane 1) of the second wavefront, side by side
+    const double* t = m.totals[kslot[c]] + (threadIdx.x == 64 ? 0 : zblocks);
+    double acc = 0.0;
+    for (int b = 0; b < zblocks; ++b) acc += t[b];
+    out[threadIdx.x == 64 ? 2 : 3] = acc;
+  }
+}
+
+void launch_modamp_evals(bool fp32, const ModampSlots& m, const int* kslot, int ncand, const double2* part64,
+                         const float4* part32, int nparts, int zblocks, double* out, hipStream_t s) {
+  if (ncand < 1) return;
+  hipLaunchKernelGGL(k_modamp_evals, dim3(ncand), dim3(128), 0, s, fp32 ? 1 : 0, m, kslot, part64, part32, nparts,
+                     zblocks, out);
   SIMR_LAUNCH_CHECK();
 }
 
@@ -570,12 +677,16 @@ void launch_amp_policy(const AmpPolicy& a, float2* amps, hipStream_t s) {
 // block-index order as cpuReduce does.
 int modamp_partials_fp32(int nx, int ny) { return ((nx + 63) / 64) * ((ny + 3) / 4); }
 
-__global__ __launch_bounds__(256) void k_modamp_fp32(const double2* __restrict__ P, const double* __restrict__ SX,
-                                                     const double* __restrict__ SY, const float2* __restrict__ kc,
-                                                     int nx, int ny, float dxy, float4* __restrict__ partials) {
+__global__ __launch_bounds__(256) void k_modamp_fp32(ModampSlots m, const float2* __restrict__ kc,
+                                                     const int* __restrict__ kslot, int nx, int ny, float dxy,
+                                                     float4* __restrict__ partials) {
 #pragma clang fp contract(off)
   __shared__ float bx[256], by[256], bs[256], bt[256];
   const int c = blockIdx.z;
+  const int slot = kslot[c];
+  const double2* __restrict__ P = m.P[slot];
+  const double* __restrict__ SX = m.SX[slot];
+  const double* __restrict__ SY = m.SY[slot];
   const float k0x = kc[c].x, k0y = kc[c].y;
   const int i = threadIdx.x + blockIdx.x * 64;
   const int j = threadIdx.y + blockIdx.y * 4;
@@ -615,10 +726,11 @@ __global__ __launch_bounds__(256) void k_modamp_fp32(const double2* __restrict__
   }
 }
 
-void launch_modamp_fp32(const double2* P, const double* SX, const double* SY, const float2* kcand, int ncand,
-                        int nx, int ny, float dxy, float4* partials, hipStream_t s) {
+void launch_modamp_fp32(const ModampSlots& m, const float2* kcand, const int* kslot, int ncand, int nx, int ny,
+                        float dxy, float4* partials, hipStream_t s) {
+  if (ncand < 1 || ncand > 65535) throw ArgError("modamp: 1..65535 candidates per launch");
   dim3 grid((nx + 63) / 64, (ny + 3) / 4, ncand);
-  hipLaunchKernelGGL(k_modamp_fp32, grid, dim3(64, 4), 0, s, P, SX, SY, kcand, nx, ny, dxy, partials);
+  hipLaunchKernelGGL(k_modamp_fp32, grid, dim3(64, 4), 0, s, m, kcand, kslot, nx, ny, dxy, partials);
   SIMR_LAUNCH_CHECK();
 }
 
@@ -628,13 +740,13 @@ void launch_modamp_fp32(const double2* P, const double* SX, const double* SY, co
 // kz planes (plane j holds kz = j - zd; every other kz is zero) of both
 // overlaps: vol[which][z][y][x].  A direct sum over the stored planes; this
 // runs once per dumped direction, not in the reconstruction.
-__global__ __launch_bounds__(256) void k_overlap_volumes(const float2* __restrict__ ovx, int zd, int nz, long long nxy,
-                                                         float2* __restrict__ vol) {
+__global__ __launch_bounds__(256) void k_overlap_volumes(const float2* __restrict__ ovx, int zd, int nzs_stride, int nz,
+                                                         long long nxy, float2* __restrict__ vol) {
   const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
   if (e >= nxy) return;
   const int z = blockIdx.y, which = blockIdx.z;
   const int nzs = 2 * zd + 1;
-  const float2* src = ovx + (long long)which * nzs * nxy + e;
+  const float2* src = ovx + (long long)which * nzs_stride * nxy + e;
   float re = 0.f, im = 0.f;
   for (int j = 0; j < nzs; ++j) {
     int m = ((j - zd) * z) % nz;  // exp(2 pi i kz z / nz) depends on kz*z mod nz only
@@ -649,11 +761,12 @@ __global__ __launch_bounds__(256) void k_overlap_volumes(const float2* __restric
   vol[((long long)which * nz + z) * nxy + e] = make_float2(re, im);
 }
 
-void launch_overlap_volumes(const float2* ovx, int zd, int nz, long long nxy, float2* vol, hipStream_t s) {
-  if (zd < 0 || 2 * zd + 1 > nz) throw ArgError("overlap z support wider than the stack");
+void launch_overlap_volumes(const float2* ovx, int zd, int nzs_stride, int nz, long long nxy, float2* vol,
+                            hipStream_t s) {
+  if (zd < 0 || 2 * zd + 1 > nz || 2 * zd + 1 > nzs_stride) throw ArgError("overlap z support wider than the stack");
   if (nz > 65535) throw ArgError("overlap volumes: too many z planes");
-  hipLaunchKernelGGL(k_overlap_volumes, dim3((unsigned)ceil_div(nxy, 256), nz, 2), dim3(256), 0, s, ovx, zd, nz, nxy,
-                     vol);
+  hipLaunchKernelGGL(k_overlap_volumes, dim3((unsigned)ceil_div(nxy, 256), nz, 2), dim3(256), 0, s, ovx, zd,
+                     nzs_stride, nz, nxy, vol);
   SIMR_LAUNCH_CHECK();
 }
 

@@ -74,11 +74,14 @@ def lib():
         "simr_stage_timings": (ip, [vp, C.POINTER(C.c_char_p), fp, ip]),
         "simr_fft3d_dev": (ip, [vp, ip, ip, ip, ip, vp]),
         "simr_fft3d_host": (ip, [fp, ip, ip, ip, ip]),
+        "simr_host_alloc": (C.c_void_p, [C.c_size_t]),
+        "simr_host_free": (ip, [vp]),
         "simr_fft_size_supported": (ip, [ip]),
         "simr_update_params": (ip, [vp, P]),
         "simr_otf_params_default": (None, [C.POINTER(SimrOtfParams)]),
         "simr_otf_elems": (C.c_size_t, [C.POINTER(SimrOtfParams), ip, ip, ip]),
         "simr_makeotf": (ip, [C.POINTER(SimrOtfParams), fp, ip, ip, ip, fp, fp, fp, C.POINTER(SimrOtfInfo)]),
+        "simr_makeotf_i2m": (ip, [C.POINTER(SimrOtfParams), fp, ip, ip, ip, fp, C.POINTER(SimrOtfInfo)]),
         "simr_get_result_dev": (ip, [vp, vp]),
         "simr_time0_state_size": (C.c_size_t, [vp]),
         "simr_get_time0_state": (ip, [vp, vp, C.c_size_t]),
@@ -140,6 +143,43 @@ def makeotf(params, psf, section_bg=None, want_band0=False):
                               None if band0 is None else _fp(band0), C.byref(info)))
     otf = out.view(np.complex64).reshape(info.nbands_out, info.nkr, info.nz)
     return otf, info, band0
+
+
+def makeotf_i2m(params, i2m):
+    """I2M sections [nz][ny][nx] -> (I2M OTF [nkr][nz] complex64, info) (makeotf -I2M)."""
+    i2m = np.ascontiguousarray(i2m, dtype=np.float32)
+    nz, ny, nx = i2m.shape
+    nkr = min(nx, ny) // 2 + 1
+    out = np.empty(2 * nkr * nz, np.float32)
+    info = SimrOtfInfo()
+    _check(lib().simr_makeotf_i2m(C.byref(params), _fp(i2m), nx, ny, nz, _fp(out), C.byref(info)))
+    return out.view(np.complex64).reshape(nkr, nz), info
+
+
+class _PinnedBlock:
+    """Page-locked host memory from simr_host_alloc, freed when the last array on it goes."""
+
+    def __init__(self, nbytes):
+        ptr = lib().simr_host_alloc(nbytes)
+        if not ptr:
+            raise SimrError(-1, "simr_host_alloc(%d): %s" % (nbytes, lib().simr_last_error().decode()))
+        self.ptr = ptr
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().simr_host_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype=np.float32):
+    """A numpy array in page-locked host memory: stacks and results in such arrays move
+    between host and GPU by DMA at the link's rate (simr_host_alloc)."""
+    dt = np.dtype(dtype)
+    n = max(int(np.prod(shape)) * dt.itemsize, 1)
+    block = _PinnedBlock(n)
+    buf = (C.c_char * n).from_address(block.ptr)
+    buf._block = block  # the buffer keeps the block, the array keeps the buffer
+    return np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
 
 
 def footprint(params):
@@ -234,8 +274,12 @@ class Reconstructor:
         return lib().simr_stream(self.h)
 
     # -- outputs
-    def result(self):
-        out = np.empty(output_shape(self.p), np.float32)
+    def result(self, out=None):
+        """The output volume; into `out` (e.g. a pinned_empty array) when given."""
+        if out is None:
+            out = np.empty(output_shape(self.p), np.float32)
+        elif out.dtype != np.float32 or out.size != int(np.prod(output_shape(self.p))) or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous float32 array of the output's size")
         _check(lib().simr_get_result_host(self.h, _fp(out)))
         return out
 

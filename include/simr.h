@@ -68,7 +68,12 @@ typedef struct simr_params {
   float wiener;
   int32_t nforceamp;
   float forceamp[SIMR_MAX_ORDERS];
-  /* OTF geometry (rotationally averaged: nyotf == 1, table laid out [kr][kz]) */
+  /* OTF geometry.  bRadAvgOTF: rotationally averaged, nyotf == 1, table
+   * [kr][kz] (nxotf radial x nzotf axial samples).  Otherwise the full OTF
+   * over kx >= 0, table [kz][ky][kx] (nxotf x nyotf x nzotf, ky and kz in FFT
+   * order, dkrotf the kx and ky step), read by trilinear interpolation with
+   * the Hermitian mirror for kx < 0 (the reference's GPU path loads such
+   * tables but evaluates them as zero, gpuFunctionsImpl.cu:1680-1714). */
   int32_t bRadAvgOTF, bOneOTFperAngle;
   int32_t nxotf, nyotf, nzotf;
   float dkrotf, dkzotf;
@@ -163,6 +168,16 @@ int simr_set_time_index(simr_ctx* ctx, int it);
 int simr_find_modulation(simr_ctx* ctx);             /* k0 search / fit and modamps for all directions */
 int simr_filter_assemble(simr_ctx* ctx);             /* Wiener filtering + zoomed assembly into the output volume */
 int simr_process_volume(simr_ctx* ctx);              /* find_modulation + filter_assemble */
+
+/*
+ * Page-locked host memory for stacks and results (no counterpart in the
+ * reference, whose loaders read into pageable buffers): the *_host calls
+ * above and below copy from and to such buffers by DMA at the link's rate,
+ * where pageable memory goes through the runtime's staging copies.  NULL on
+ * failure (simr_last_error tells why).
+ */
+void* simr_host_alloc(size_t bytes);
+int simr_host_free(void* p);
 
 /* Results. out: (zoom*nx)*(zoom*ny)*(z_zoom*nz) floats, layout [z][y][x]. */
 size_t simr_result_elems(const simr_ctx* ctx);
@@ -288,6 +303,21 @@ size_t simr_otf_elems(const simr_otf_params* p, int nx, int ny, int nsections);
  */
 int simr_makeotf(const simr_otf_params* p, const float* psf, int nx, int ny, int nsections, const float* section_bg,
                  float* otf, float* band0, simr_otf_info* info);
+/*
+ * The I2M OTF of makeotf -I2M (replaces src/otf/radialft.cpp:267-286 reading,
+ * 289-296 and determine_center's I2M search 624-664, 306-307, 345-357,
+ * 383-386 and 425-430): i2m holds the nz I2M sections [nz][ny][nx].  The
+ * last pixel of a section is replaced by its left neighbour when it is
+ * exactly -1; the sections are not apodized; the bead centre is the I2M
+ * volume's own peak with parabola refinement; one background, the 20-pixel
+ * border estimate of section nz/2, is subtracted from every section; no
+ * bead-size compensation, modify or cleanup; then p->interpkr's fixorigin
+ * and the order-0 rescale.  Only p->dr, dz and interpkr are read.
+ * otf: complex float [nkr][nz], nkr = min(nx, ny)/2 + 1; info as for
+ * simr_makeotf (nphases 1).
+ */
+int simr_makeotf_i2m(const simr_otf_params* p, const float* i2m, int nx, int ny, int nz, float* otf,
+                     simr_otf_info* info);
 
 /* Synchronise the context stream.  Returns the HIP stream as void*. */
 int simr_sync(simr_ctx* ctx);

@@ -57,6 +57,8 @@ def lib():
         L.orc_makeotf.restype = C.c_int
         L.orc_makeotf.argtypes = [C.POINTER(SimrOtfParams), fp, ip, ip, ip, fp, fp, fp, C.POINTER(SimrOtfInfo)]
         L.orc_makeotf_error.restype = C.c_char_p
+        L.orc_makeotf_i2m.restype = C.c_int
+        L.orc_makeotf_i2m.argtypes = [C.POINTER(SimrOtfParams), fp, ip, ip, ip, fp, C.POINTER(SimrOtfInfo)]
         L.orc_set_threads.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
@@ -257,6 +259,19 @@ def makeotf(params, psf, section_bg=None, want_band0=False):
     if rc != 0:
         raise RuntimeError(lib().orc_makeotf_error().decode())
     return out.view(np.complex64).reshape(nbo, nkr, nz), info, band0
+
+
+def makeotf_i2m(params, i2m):
+    """Oracle of makeotf -I2M's I2M OTF; same contract as engine.makeotf_i2m."""
+    i2m = np.ascontiguousarray(i2m, dtype=np.float32)
+    nz, ny, nx = i2m.shape
+    nkr = min(nx, ny) // 2 + 1
+    out = np.empty(2 * nkr * nz, np.float32)
+    info = SimrOtfInfo()
+    rc = lib().orc_makeotf_i2m(C.byref(params), _fp(i2m), nx, ny, nz, _fp(out), C.byref(info))
+    if rc != 0:
+        raise RuntimeError(lib().orc_makeotf_error().decode())
+    return out.view(np.complex64).reshape(nkr, nz), info
 
 
 def reconstruct(params, otf, raw, precision=32, fit64=False):

@@ -582,4 +582,84 @@ int orc_makeotf(const simr_otf_params* p, const float* psf, int nx, int ny, int 
   }
 }
 
+// The I2M branch of main() (radialft.cpp:267-286 reading, determine_center's
+// I2M search 624-664, 306-307 background, 345-357 FFT and centre shift,
+// 383-386 radial average, 425-430 fixorigin + rescale): i2m holds the nz I2M
+// sections [nz][ny][nx]; otf_out receives [nkr][nz] complex.
+int orc_makeotf_i2m(const simr_otf_params* p, const float* i2m, int nx, int ny, int nz, float* otf_out,
+                    simr_otf_info* info) {
+  try {
+    if (nz < 1 || nx < 2 || ny < 2) throw std::runtime_error("PSF too small");
+    const size_t nxy = (size_t)nx * ny;
+    const int nr = std::min(nx, ny), nkr = nr / 2 + 1;
+    const int border = 20;
+    std::vector<float> img(nxy * nz);
+    float background_i2m = 0.f;
+    for (int z = 0; z < nz; z++) {
+      float* sec = img.data() + nxy * z;
+      std::memcpy(sec, i2m + nxy * z, nxy * sizeof(float));
+      if (sec[nxy - 1] == -1.0) sec[nxy - 1] = sec[nxy - 2];  // fix camera error at even binnings
+      if (z == nz / 2) background_i2m = estimate_background(sec, nx, ny, border);
+    }
+    // the peak of the I2M volume (before the background subtraction), x
+    // neighbours wrapping with the padded width, padded columns reading 0
+    const int nxe = (nx % 2) ? nx + 1 : nx + 2;
+    auto at = [&](int x, int y, int z) -> float { return x >= nx ? 0.f : img[nxy * z + (size_t)y * nx + x]; };
+    float maxval = 0.0f;
+    int maxi = 0, maxj = 0, maxk = 0;
+    for (int z = 0; z < nz; z++)
+      for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+          const float v = at(x, y, z);
+          if (v > maxval) {
+            maxval = v;
+            maxi = y;
+            maxj = x;
+            maxk = z;
+          }
+        }
+    int iminus = maxi - 1, iplus = maxi + 1, jminus = maxj - 1, jplus = maxj + 1, kminus = maxk - 1, kplus = maxk + 1;
+    if (iminus < 0) iminus += ny;
+    if (iplus >= ny) iplus -= ny;
+    if (jminus < 0) jminus += nxe;
+    if (jplus >= nxe) jplus -= nxe;
+    if (kminus < 0) kminus += nz;
+    if (kplus >= nz) kplus -= nz;
+    const float zc = maxk + fitparabola(at(maxj, maxi, kminus), maxval, at(maxj, maxi, kplus));
+    const float yc = maxi + fitparabola(at(maxj, iminus, maxk), maxval, at(maxj, iplus, maxk));
+    const float xc = maxj + fitparabola(at(jminus, maxi, maxk), maxval, at(jplus, maxi, maxk));
+    for (float& v : img) v -= background_i2m;
+    const size_t H = (size_t)(nx / 2 + 1) * ny * nz;
+    std::vector<cf> spec(H);
+    rfft3d(img.data(), nx, ny, nz, spec.data());
+    shift_center(spec.data(), nx, ny, nz, xc, yc, zc);
+    std::vector<cf> avg((size_t)nkr * nz);
+    radialft(spec.data(), nx, ny, nz, p->dr, avg.data(), nr);
+    // unconditional in the reference; it changes nothing unless kx1 > 0
+    if (p->interpkr[0] > 0) fixorigin(avg.data(), nz, p->interpkr[0], p->interpkr[1]);
+    float scalefactor = 1.0f;
+    rescale(avg.data(), 0, nkr * nz, &scalefactor, p->dorescale);
+    std::memcpy(otf_out, avg.data(), avg.size() * sizeof(cf));
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->nx = nx;
+      info->ny = ny;
+      info->nz = nz;
+      info->norders = 1;
+      info->nbands_out = 1;
+      info->nkr = nkr;
+      info->dkr = 1 / (nx * p->dr);
+      info->dkz = 1 / (nz * p->dz);
+      info->xc = xc;
+      info->yc = yc;
+      info->zc = zc;
+      info->scalefactor = scalefactor;
+    }
+    return 0;
+  } catch (const std::exception& e) {
+    g_mk_err = e.what();
+    return -1;
+  }
+}
+
 }  // extern "C"

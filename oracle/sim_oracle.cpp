@@ -346,12 +346,53 @@ struct Oracle {
       orc::fft_lines(pz, band(dir, i), stride, stride, [](long long l) { return l; }, false);
   }
 
-  // dev_otfinterpolate (gpuFunctionsImpl.cu:1677-1715), RA tables only.
+  // A full (not rotationally averaged) OTF table [kz][ky][kx] over kx >= 0:
+  // trilinear lookup, Hermitian mirror conj(OTF(-k)) for kx < 0, ky and kz
+  // wrapping in FFT order, kx clamped at the edge.  The reference's
+  // dev_otfinterpolate returns 0 for such tables (gpuFunctionsImpl.cu:1682);
+  // this is the engine's extension of its (kr, kz) scheme, parity unpinned
+  // against the reference and pinned by its exactness on multilinear tables.
+  C2<T> otfinterp3d(const std::vector<C2<T>>& t, T kx, T ky, int kz, T kzscale) const {
+    const bool cj = kx < 0;
+    if (cj) {
+      kx = -kx;
+      ky = -ky;
+      kz = -kz;
+    }
+    const int nxo = P.nxotf, nyo = P.nyotf, nzo = P.nzotf;
+    const T fx = kx / (T)P.dkrotf;
+    T fy = ky / (T)P.dkrotf;
+    if (fy < 0) fy += (T)nyo;
+    T fz = (T)kz * kzscale;
+    if (fz < 0) fz += (T)nzo;
+    int ix = (int)std::floor(fx), iy = (int)std::floor(fy), iz = (int)std::floor(fz);
+    T ax = fx - (T)ix, ay = fy - (T)iy, az = fz - (T)iz;
+    if (ix > nxo - 2) { ix = nxo - 2; ax = (T)1; }
+    if (ix < 0) { ix = 0; ax = 0; }
+    if (iy > nyo - 1) iy = nyo - 1;
+    if (iy < 0) iy = 0;
+    if (iz > nzo - 1) iz = nzo - 1;
+    if (iz < 0) iz = 0;
+    const int iy1 = iy == nyo - 1 ? 0 : iy + 1, iz1 = iz == nzo - 1 ? 0 : iz + 1;
+    auto at = [&](int y, int z) {
+      const C2<T> a = t[((size_t)z * nyo + y) * nxo + ix], b = t[((size_t)z * nyo + y) * nxo + ix + 1];
+      return C2<T>{a.x * (1 - ax) + b.x * ax, a.y * (1 - ax) + b.y * ax};
+    };
+    const C2<T> v00 = at(iy, iz), v10 = at(iy1, iz), v01 = at(iy, iz1), v11 = at(iy1, iz1);
+    const C2<T> w0{v00.x * (1 - ay) + v10.x * ay, v00.y * (1 - ay) + v10.y * ay};
+    const C2<T> w1{v01.x * (1 - ay) + v11.x * ay, v01.y * (1 - ay) + v11.y * ay};
+    C2<T> v{w0.x * (1 - az) + w1.x * az, w0.y * (1 - az) + w1.y * az};
+    if (cj) v.y = -v.y;
+    return v;
+  }
+
+  // dev_otfinterpolate (gpuFunctionsImpl.cu:1677-1715) for RA tables; full
+  // tables go to otfinterp3d.
   // Indices are clamped to the table (the reference reads past it when the
   // support exceeds the table; both oracle and engine clamp).
   C2<T> otfinterp(const std::vector<C2<T>>& t, T kx, T ky, int kz, T kzscale) const {
     C2<T> v{0, 0};
-    if (!P.bRadAvgOTF) return v;
+    if (!P.bRadAvgOTF) return otfinterp3d(t, kx, ky, kz, kzscale);
     const int nzotf = P.nzotf, nxotf = P.nxotf;
     T krindex = Tsqrt(kx * kx + ky * ky) / (T)P.dkrotf;
     T kzindex = (T)kz * kzscale;

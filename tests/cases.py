@@ -45,6 +45,30 @@ def synthetic(nx, ny, nz, nphases=5, seed=11, photons=5000.0, **kw):
     return p, tab, raw, truth
 
 
+def full_otf_from_ra(p, tab):
+    """The full (not rotationally averaged) OTF table of an RA one: [order][kz][ky][kx] over
+    kx >= 0, ky in FFT order, each entry the RA profile linearly interpolated at
+    kr = |(kx, ky)| (clamped at the table's edge), with the same kr and kz steps.
+    Returns (params with bRadAvgOTF = 0 and the full geometry, table)."""
+    tab = np.asarray(tab)
+    norders, nxotf, nzotf = tab.shape
+    nyf = 2 * (nxotf - 1)
+    kx = np.arange(nxotf, dtype=np.float64)
+    ky = np.fft.fftfreq(nyf, 1.0 / nyf)
+    r = np.minimum(np.hypot(kx[None, :], ky[:, None]), nxotf - 1)  # [ky][kx]
+    i0 = np.minimum(np.floor(r).astype(int), nxotf - 2)
+    a = r - i0
+    full = np.empty((norders, nzotf, nyf, nxotf), np.complex64)
+    for o in range(norders):
+        t = tab[o].astype(np.complex128)  # [kr][kz]
+        v = t[i0] * (1 - a)[..., None] + t[i0 + 1] * a[..., None]  # [ky][kx][kz]
+        full[o] = np.transpose(v, (2, 0, 1))
+    q = p.copy()
+    q.bRadAvgOTF = 0
+    q.nxotf, q.nyotf, q.nzotf = nxotf, nyf, nzotf
+    return q, full
+
+
 def tiff_frames(path):
     """All pages of a TIFF as one array (Pillow's libtiff-backed reader)."""
     from PIL import Image

@@ -190,11 +190,20 @@ def test_deskew_to_an_unsupported_width_is_refused(tmp_path):
 
 
 def test_unsupported_branches_are_refused(tmp_path):
-    for extra in (["--bessel", "--noRecon"], ["--2lenses"], []):
+    for extra in (["--bessel", "--noRecon"], ["--2lenses"]):
         r = subprocess.run([EXE, os.path.join(GOLDEN, "raw.dv"), str(tmp_path / "o.dv"),
-                            os.path.join(GOLDEN, "otf.dv")] + extra + (["--otfRA"] if extra else []),
+                            os.path.join(GOLDEN, "otf.dv")] + extra + ["--otfRA"],
                            capture_output=True, text=True)
         assert "!!Error occurred:" in r.stderr and "not supported" in r.stderr, (extra, r.stderr)
+
+
+def test_ra_otf_file_without_otfra_is_refused(tmp_path):
+    # without --otfRA the file is read as a full 3D OTF (cudaSirecon.cpp:1505-1515): the RA
+    # file's zlen of 0 gives no kz step, which is an error here, not a reconstruction
+    r = subprocess.run([EXE, os.path.join(GOLDEN, "raw.dv"), str(tmp_path / "o.dv"), os.path.join(GOLDEN, "otf.dv")],
+                       capture_output=True, text=True)
+    assert "!!Error occurred: the OTF file does not hold a full 3D OTF" in r.stderr, r.stderr
+    assert "pass --otfRA" in r.stderr
 
 
 def test_tiff_folder_discovery_creates_output_folder(tmp_path):

@@ -13,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from cases import GOLDEN, load_testdata, load_testdata_tiff, rel_l2, tiff_frames
+from cases import GOLDEN, full_otf_from_ra, load_testdata, load_testdata_tiff, rel_l2, tiff_frames
 from cudasirecon_amd import mrc, sr
 from oracle import binding as ob
 
@@ -368,3 +368,27 @@ def test_cli_tiff_saveoverlaps_matches_oracle(tmp_path):
         err = rel_l2(got[sl], ref[key])
         log("tiff overlaps %s rel-L2 %.3e" % (key, err))
         assert err < TOL
+
+
+def test_cli_full_otf_mrc_matches_oracle(tmp_path):
+    """An OTF file that is not rotationally averaged (no otfRA): nzotf sections of
+    [ky][kx] complex per order, dkrotf = xlen and dkzotf = zlen (cudaSirecon.cpp:1505-1515,
+    1596-1603), read by trilinear lookups."""
+    p, otf, raw = load_testdata()
+    q, full = full_otf_from_ra(p, otf)
+    q.dkrotf = float(np.float32(q.dkrotf))
+    q.dkzotf = float(np.float32(q.dkzotf))
+    norders, nzotf, nyotf, nxotf = full.shape
+    h = mrc.make_header(nxotf, nyotf, norders * nzotf, 4, xlen=q.dkrotf, ylen=q.dkrotf, zlen=q.dkzotf)
+    otf_path = str(tmp_path / "full_otf.dv")
+    mrc.write(otf_path, full.reshape(norders * nzotf, nyotf, nxotf), h)
+    cfg = tmp_path / "config-full"
+    cfg.write_text("".join(l for l in open(os.path.join(GOLDEN, "config")) if not l.startswith("otfRA")))
+    out = str(tmp_path / "full_proc.dv")
+    r = run_exe([os.path.join(GOLDEN, "raw.dv"), out, otf_path, "-c", str(cfg)])
+    assert "nyotf=%d" % nyotf in r.stdout
+    _, vol = mrc.read(out)
+    ref = oracle_series(q, full, [raw])[0]
+    err = rel_l2(vol, ref)
+    log("full OTF (mrc) rel-L2 %.3e" % err)
+    assert err < TOL

@@ -141,3 +141,67 @@ def test_makeotf_cli_mrc(tmp_path):
     p = params(nphases=5, leavekz=[7, 11, 3])
     ref, _, _ = ob.makeotf(p, psf, section_bg=np.zeros(psf.shape[0], np.float32))
     assert rel_l2(data, ref) < 1e-5
+
+
+I2M_CASES = {
+    "i2m_64x16": ((64, 64, 16), dict(), 21),
+    "i2m_fixorigin": ((64, 64, 12), dict(interpkr=[3, 14]), 22),
+    "i2m_prime_nz": ((48, 48, 11), dict(), 23),
+    "i2m_2d": ((64, 64, 1), dict(), 24),
+}
+
+
+@pytest.mark.parametrize("name", list(I2M_CASES))
+def test_makeotf_i2m_matches_oracle(name):
+    (nx, ny, nz), kw, seed = I2M_CASES[name]
+    log("makeotf -I2M case", name)
+    p = params(**kw)
+    vol = synth.make_psf_stack(nx, ny, nz, 1, seed=seed, center=(nx / 2 - 1.3, ny / 2 + 0.6, nz / 2 - 0.3))
+    got, ginfo = engine.makeotf_i2m(p, vol)
+    ref, rinfo = ob.makeotf_i2m(p, vol)
+    err = rel_l2(got, ref)
+    log("  i2m otf rel-L2 %.3e  centre (%.4f %.4f %.4f)" % (err, ginfo.xc, ginfo.yc, ginfo.zc))
+    assert err < 1e-5
+    for a in ("xc", "yc", "zc"):
+        assert abs(getattr(ginfo, a) - getattr(rinfo, a)) < 1e-4
+
+
+def test_makeotf_cli_mrc_i2m(tmp_path):
+    """-I2M: each z holds the 5 phases and then one I2M section."""
+    nx = ny = 48
+    nz = 14
+    psf = synth.make_psf_stack(nx, ny, nz, 5, seed=31)
+    i2m = synth.make_psf_stack(nx, ny, nz, 1, seed=32, center=(22.6, 25.2, 6.4), background=80.0)
+    sec = np.concatenate([psf.reshape(nz, 5, ny, nx), i2m.reshape(nz, 1, ny, nx)], axis=1).reshape(nz * 6, ny, nx)
+    h = mrc.make_header(nx, ny, sec.shape[0], 2, xlen=0.08, ylen=0.08, zlen=0.125, wave=515, labels=["bead psf + i2m"])
+    src = str(tmp_path / "psf.dv")
+    mrc.write(src, sec, h)
+    out = str(tmp_path / "otf.dv")
+    out_i2m = str(tmp_path / "i2m_otf.dv")
+    log("run makeotf -I2M (mrc)")
+    r = subprocess.run([MAKEOTF, src, out, "-angle", "-0.8", "-ls", "0.2035", "-na", "1.42", "-fixorigin", "3", "12",
+                        "-I2M", out_i2m], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "I2M psf's center of mass" in r.stdout
+    ho, data = mrc.read(out)
+    assert (ho["nx"], ho["ny"], ho["nz"], ho["mode"]) == (nz, 25, 3, 4)
+    p = params(nphases=5, interpkr=[3, 12])
+    ref, _, _ = ob.makeotf(p, psf, section_bg=np.zeros(psf.shape[0], np.float32))
+    assert rel_l2(data, ref) < 1e-5
+    hi, di = mrc.read(out_i2m)
+    assert (hi["nx"], hi["ny"], hi["nz"], hi["mode"]) == (nz, 25, 1, 4)
+    assert hi["xlen"] == pytest.approx(1 / (nz * 0.125)) and hi["ylen"] == pytest.approx(1 / (nz * 0.125))
+    assert hi["zlen"] == pytest.approx(1 / (nx * 0.08))
+    ref_i2m, _ = ob.makeotf_i2m(p, i2m)
+    err = rel_l2(np.asarray(di).reshape(25, nz), ref_i2m)
+    log("  cli i2m otf rel-L2 %.3e" % err)
+    assert err < 1e-5
+
+
+def test_makeotf_cli_i2m_refuses_tiff(tmp_path):
+    psf = synth.make_psf_stack(48, 48, 6, 6, seed=2)
+    src = str(tmp_path / "psf.tif")
+    _write_tiff(src, psf)
+    r = subprocess.run([MAKEOTF, src, str(tmp_path / "o.tif"), "-I2M", str(tmp_path / "i.tif")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "-I2M needs MRC input" in r.stderr

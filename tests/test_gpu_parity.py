@@ -10,7 +10,7 @@ import sys
 import numpy as np
 import pytest
 
-from cases import load_testdata, rel_l2, synthetic
+from cases import full_otf_from_ra, load_testdata, rel_l2, synthetic
 from cudasirecon_amd import engine
 from oracle import binding as ob
 
@@ -61,7 +61,7 @@ def _oracle_preprocessed(p, otf, raw, prec=32, fit64=False):
 
 def _cases():
     return ["testdata", "syn3d", "syn2d", "bessel3d", "opts3d", "opts2d", "z14", "z21zz", "z11", "z13zz", "z61",
-            "z131zz"]
+            "z131zz", "full3d", "full2d"]
 
 
 def _case(name):
@@ -95,6 +95,14 @@ def _case(name):
         return p, otf, raw
     if name == "z131zz":  # prime depth 131 with z_zoom 2: assembly depth 262, above the transform kernels' 256
         p, otf, raw, _ = synthetic(32, 32, 131, nphases=5, z_zoom=2)
+        return p, otf, raw
+    if name == "full3d":  # a full (not rotationally averaged) 3D OTF table, trilinear lookups
+        p, otf, raw, _ = synthetic(64, 64, 16, nphases=5)
+        p, otf = full_otf_from_ra(p, otf)
+        return p, otf, raw
+    if name == "full2d":  # a full 2D OTF table [ky][kx]
+        p, otf, raw, _ = synthetic(96, 96, 1, nphases=3)
+        p, otf = full_otf_from_ra(p, otf)
         return p, otf, raw
     if name == "opts2d":  # non-square 2D, gamma apodization 0.8, suppression radius 5
         p, otf, raw, _ = synthetic(120, 100, 1, nphases=3, apodizeoutput=2, apoGamma=0.8, suppression_radius=5,
@@ -275,3 +283,25 @@ def test_nan_pixel_reports_error_and_context_recovers():
         vol = g.result()
     ref, _ = ob.reconstruct(p, otf, raw, 32, True)
     assert rel_l2(vol, ref) < 1e-5
+
+
+def test_pinned_host_buffers_give_the_same_result():
+    """Stacks and results in page-locked host memory (simr_host_alloc) take the same path
+    as pageable ones: bit-identical outputs."""
+    log("pinned host buffers")
+    p, otf, raw, _ = synthetic(64, 64, 8, nphases=5)
+    raw = raw.astype(np.uint16)
+    pin = engine.pinned_empty(raw.shape, np.uint16)
+    pin[...] = raw
+    with _gpu(p, otf) as g:
+        g.set_raw(raw)
+        g.load_and_rescale(0)
+        g.process_volume()
+        a = g.result()
+        g.set_raw(pin)
+        g.load_and_rescale(0)
+        g.process_volume()
+        out = engine.pinned_empty(a.shape, np.float32)
+        b = g.result(out)
+    assert b is out
+    np.testing.assert_array_equal(a, b)

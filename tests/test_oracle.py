@@ -164,6 +164,89 @@ def test_filter_reduces_to_textbook_wiener():
     assert rel_l2(got, exp) < 1e-6
 
 
+def _multilinear_full_otf(nxotf, nyotf, nzotf, dkr, dkz):
+    """A full OTF table [1][kz][ky][kx] (kx >= 0; ky, kz in FFT order) of the complex
+    function H = (1 - 0.01 kx/dkr)(1 + 0.004 ky/dkr)(1 + 0.05i kz/dkz), multilinear in
+    (kx, ky, kz), so trilinear interpolation reproduces it exactly inside the table."""
+    kx = np.arange(nxotf)
+    ky = np.fft.fftfreq(nyotf, 1.0 / nyotf)
+    kz = np.fft.fftfreq(nzotf, 1.0 / nzotf)
+    h = ((1 - 0.01 * kx)[None, None, :] * (1 + 0.004 * ky)[None, :, None] * (1 + 0.05j * kz)[:, None, None])
+    return h.astype(np.complex64)[None], lambda x, y, z: (1 - 0.01 * x / dkr) * (1 + 0.004 * y / dkr) * (1 + 0.05j * z / dkz)
+
+
+@pytest.mark.parametrize("nz", [4, 1])
+def test_filter_full_otf_reduces_to_textbook_wiener(nz):
+    """A full (not rotationally averaged) OTF: the same single-band Wiener case, its OTF
+    values read by trilinear interpolation at off-grid (kx, ky, kz) of a multilinear table."""
+    nx = ny = 32
+    dkr = 0.23
+    nxotf, nyotf = 40, 96
+    nzotf = 8 if nz > 1 else 1
+    dkzotf = 1.0 / (0.2 * 6.5) if nz > 1 else 1.0
+    p = default_params()
+    set_params(p, nx=nx, ny=ny, nz=nz, ndirs=1, nphases=1, dxy=0.1, dz=0.2, wave=500, na=1.2, nimm=1.33,
+               bRadAvgOTF=0, nzotf=nzotf, nxotf=nxotf, nyotf=nyotf, dkrotf=dkr, dkzotf=dkzotf,
+               bSuppress_singularities=0, apodizeoutput=0, wiener=0.05, napodize=0, k0angles=[0.0])
+    otf, H = _multilinear_full_otf(nxotf, nyotf, nzotf, dkr, dkzotf)
+    o = ob.Oracle(p, otf, 64)
+    rng = np.random.default_rng(2)
+    band = (rng.standard_normal((nz, ny, nx // 2 + 1)) + 1j * rng.standard_normal((nz, ny, nx // 2 + 1)))
+    o.set_band(0, 0, band.astype(np.complex64))
+    o.filter_dir(0)
+    got = o.band(0, 0, double=True)
+    dkx = 1.0 / (nx * 0.1)
+    dkz = 1.0 / (nz * 0.2)
+    rcut = min(2 * 1.2 / 0.5, 1 / (2 * 0.1))
+    zd = o.zdist()[0]
+    exp = np.zeros_like(band)
+    for iz in range(nz):
+        z0 = iz if iz <= zd else iz - nz
+        if not (iz <= zd or iz >= nz - zd):
+            continue
+        kz = z0 * dkz if nz > 1 else 0.0
+        for iy in range(ny):
+            y1 = iy if iy < ny // 2 else iy - ny
+            for ix in range(nx // 2 + 1):
+                if math.hypot(ix * dkx, y1 * dkx) > rcut:
+                    continue
+                h = H(ix * dkx, y1 * dkx, kz)
+                exp[iz, iy, ix] = band.astype(np.complex64)[iz, iy, ix] * np.conj(h) / (abs(h) ** 2 + 0.05 ** 2)
+    assert rel_l2(got, exp) < 1e-6
+
+
+@pytest.mark.parametrize("nz,nphases", [(8, 5), (1, 3)])
+def test_full_otf_sampled_from_ra_reproduces_ra_reconstruction(nz, nphases):
+    """A full OTF table that samples an RA OTF on the (kx, ky, kz) grid: the reconstruction
+    (fit, filter, assembly, every lookup through the Hermitian mirror for kx < 0 and the
+    ky / kz wraps) equals the RA one up to the interpolation difference (trilinear on the
+    grid versus linear in kr), and the fitted k0 agree.  In 3D the RA table carries the
+    axial phase of a PSF 0.37 planes off centre, so it is complex and Hermitian, and a
+    mirror that lost its conjugation would differ at every kx < 0."""
+    from cases import full_otf_from_ra
+
+    p, tab, raw, _ = synthetic(48, 48, nz, nphases=nphases)
+    tab = np.asarray(tab)
+    if nz > 1:
+        nzotf = tab.shape[2]
+        kz = np.fft.fftfreq(nzotf, 1.0 / nzotf)
+        tab = (tab * np.exp(-2j * np.pi * kz * 0.37 / nzotf)[None, None, :]).astype(np.complex64)
+    q, full = full_otf_from_ra(p, tab)
+    outs, fits = [], []
+    for par, t in ((p, tab), (q, full)):
+        o = ob.Oracle(par, t, 64)
+        o.load(raw.astype(np.float32))
+        o.rescale(0)
+        o.set_time(0)
+        o.process_volume()
+        outs.append(o.result(double=True))
+        fits.append([o.fit(d) for d in range(p.ndirs)])
+    assert rel_l2(outs[1], outs[0]) < 3e-3
+    dk = 1.0 / (p.nx * p.dxy)
+    for a, b in zip(*fits):
+        assert math.hypot(a.k0x - b.k0x, a.k0y - b.k0y) / dk < 1e-3
+
+
 # ------------------------------------------------------------------ assembly special case
 def _moved_grid(full, nx, ny, nz, Z):
     """move_kernel's scatter (gpuFunctionsImpl.cu:1853-1913) of a full spectrum into the zoomed grid."""
