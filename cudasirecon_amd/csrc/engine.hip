@@ -18,11 +18,12 @@ constexpr float K0_WARNING_THRESH = 0.05f;  // cudaSireconImpl.h:54
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 constexpr int kMaxCand = 128;  // modamp candidates (of every direction) reduced per launch
 // the fit's zero-copy host region: candidate vectors, their directions, the
-// modamp sums (4 doubles each), the peak values (8 floats per direction) and
-// peak indices
-constexpr size_t kZcSlot = ((size_t)kMaxCand * 8 + 255) & ~(size_t)255;
-constexpr size_t kZcEval = kZcSlot + (((size_t)kMaxCand * 4 + 255) & ~(size_t)255);
-constexpr size_t kZcPeakV = kZcEval + (size_t)kMaxCand * 32;
+// modamp sums (4 doubles each) of two queued batches of kMaxCand, then the
+// peak values (8 floats per direction) and peak indices
+constexpr int kZcCand = 2 * kMaxCand;
+constexpr size_t kZcSlot = ((size_t)kZcCand * 8 + 255) & ~(size_t)255;
+constexpr size_t kZcEval = kZcSlot + (((size_t)kZcCand * 4 + 255) & ~(size_t)255);
+constexpr size_t kZcPeakV = kZcEval + (size_t)kZcCand * 32;
 constexpr size_t kZcPeakI = kZcPeakV + (size_t)kMaxDirs * 32;
 constexpr size_t kZcBytes = kZcPeakI + 256;
 
@@ -129,7 +130,7 @@ size_t fit_block_bytes(const Layout& L) {
   size_t b = align256((size_t)L.ndirs * ovx_elems(L) * 8);
   b += align256((size_t)L.ndirs * L.nxy * 8);
   b += (size_t)L.ndirs * (align256((size_t)L.nxy * 16) + 2 * align256((size_t)L.nxy * 8) +
-                          align256((size_t)2 * kZsumMaxBlocks * 8));
+                          align256((size_t)kZsumTotals * 8));
   b += align256((size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16);
   b += align256((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
   b += 2 * align256((size_t)kMaxDirs * kArgBlocks * 4);
@@ -140,7 +141,7 @@ size_t fit_block_bytes(const Layout& L) {
 // SY and the z-sum partials.
 size_t dbg_set_bytes(const Layout& L) {
   return align256(ovx_elems(L) * 8) + align256((size_t)L.nxy * 16) + 2 * align256((size_t)L.nxy * 8) +
-         align256((size_t)2 * kZsumMaxBlocks * 8);
+         align256((size_t)kZsumTotals * 8);
 }
 }  // namespace
 
@@ -261,7 +262,7 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
       w.P = (double2*)take((size_t)L.nxy * 16);
       w.SX = (double*)take((size_t)L.nxy * 8);
       w.SY = (double*)take((size_t)L.nxy * 8);
-      w.sumparts = (double*)take((size_t)2 * kZsumMaxBlocks * 8);
+      w.sumparts = (double*)take((size_t)kZsumTotals * 8);
     }
     part64_ = (double2*)take((size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16);
     part32_ = (float4*)take((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
@@ -281,7 +282,7 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
     dbg_.P = (double2*)take((size_t)L.nxy * 16);
     dbg_.SX = (double*)take((size_t)L.nxy * 8);
     dbg_.SY = (double*)take((size_t)L.nxy * 8);
-    dbg_.sumparts = (double*)take((size_t)2 * kZsumMaxBlocks * 8);
+    dbg_.sumparts = (double*)take((size_t)kZsumTotals * 8);
   }
   {
     void* h = nullptr;
@@ -720,50 +721,60 @@ ModampSlots Engine::modamp_slots() const {
   return m;
 }
 
-// The modamp reductions of every direction's candidates (req: direction and
-// its candidate vectors), in launches of up to kMaxCand candidates.  The
-// candidates reach the device in one copy, and the partials are summed per
-// candidate in block order on the device as the host does (cpuReduce,
-// gpuFunctionsImpl_hh.cu:162-180), straight into the zero-copy region: a
-// round is one copy, two kernels and one wait.
-std::vector<std::vector<Engine::Eval>> Engine::eval_multi(
-    const std::vector<std::pair<int, std::vector<float2>>>& req) {
-  std::vector<std::vector<Eval>> res(req.size());
-  std::vector<std::pair<int, int>> flat;  // (request, candidate)
-  for (size_t r = 0; r < req.size(); ++r) {
-    SIMR_REQUIRE(req[r].first >= 0 && req[r].first < L.ndirs, "direction out of range");
-    for (size_t c = 0; c < req[r].second.size(); ++c) flat.push_back({(int)r, (int)c});
-    res[r].resize(req[r].second.size());
+// Queues the modamp reductions of n <= kMaxCand candidates (direction,
+// vector) at entries [at, at + n) of the zero-copy region: the candidates
+// reach the device in one copy, and the partials are summed per candidate in
+// block order on the device as the host does (cpuReduce,
+// gpuFunctionsImpl_hh.cu:162-180), straight into the region.  The sums are
+// read (eval_read) after the next wait_fit().
+void Engine::eval_enqueue(const Cand* c, int n, int at) {
+  SIMR_REQUIRE(n >= 1 && n <= kMaxCand && at >= 0 && at + n <= kZcCand, "modamp batch out of range");
+  float2* kv = (float2*)hzc_;
+  int* ks = (int*)(hzc_ + kZcSlot);
+  for (int i = 0; i < n; ++i) {
+    SIMR_REQUIRE(c[i].first >= 0 && c[i].first < L.ndirs, "direction out of range");
+    kv[at + i] = c[i].second;
+    ks[at + i] = c[i].first;
   }
   const ModampSlots m = modamp_slots();
   const int nb = fit_fp32_ ? modamp_partials_fp32(L.nx, L.ny) : modamp_partials_fp64(L.nx, L.ny);
-  float2* kv = (float2*)hzc_;
-  int* ks = (int*)(hzc_ + kZcSlot);
-  const double* ev = (const double*)(hzc_ + kZcEval);
-  const float2* dkv = (const float2*)kdev_;
-  const int* dks = (const int*)(kdev_ + kZcSlot);
-  double* dev = (double*)(dzc_ + kZcEval);
+  const float2* dkv = (const float2*)kdev_ + at;
+  const int* dks = (const int*)(kdev_ + kZcSlot) + at;
+  double* dev = (double*)(dzc_ + kZcEval) + 4 * (size_t)at;
+  SIMR_HIP(hipMemcpyAsync(kdev_, hzc_, kZcEval, hipMemcpyHostToDevice, stream_));
+  const int st = stage_begin("modamp", stream_);
+  if (fit_fp32_)
+    launch_modamp_fp32(m, dkv, dks, n, L.nx, L.ny, P.dxy, part32_, stream_);
+  else
+    launch_modamp_fp64(m, dkv, dks, n, L.nx, L.ny, P.dxy, part64_, stream_);
+  launch_modamp_evals(fit_fp32_, m, dks, n, part64_, part32_, nb, zsum_blocks(L.nxy), dev, stream_);
+  stage_end(st, stream_);
+}
+
+void Engine::eval_read(int n, int at, Eval* out) const {
+  const double* ev = (const double*)(hzc_ + kZcEval) + 4 * (size_t)at;
+  for (int i = 0; i < n; ++i) out[i] = {ev[4 * i], ev[4 * i + 1], ev[4 * i + 2], ev[4 * i + 3]};
+}
+
+// The modamp reductions of every direction's candidates (req: direction and
+// its candidate vectors), in rounds of up to kMaxCand candidates: one copy,
+// two kernels and one wait each.
+std::vector<std::vector<Engine::Eval>> Engine::eval_multi(
+    const std::vector<std::pair<int, std::vector<float2>>>& req) {
+  std::vector<Cand> flat;
+  for (auto& r : req)
+    for (auto& k : r.second) flat.push_back({r.first, k});
+  std::vector<Eval> ev(flat.size());
   for (size_t c0 = 0; c0 < flat.size(); c0 += kMaxCand) {
     const int nc = (int)std::min<size_t>(kMaxCand, flat.size() - c0);
-    for (int c = 0; c < nc; ++c) {
-      const auto& f = flat[c0 + c];
-      kv[c] = req[f.first].second[f.second];
-      ks[c] = req[f.first].first;
-    }
-    SIMR_HIP(hipMemcpyAsync(kdev_, hzc_, kZcSlot + (size_t)nc * 4, hipMemcpyHostToDevice, stream_));
-    const int st = stage_begin("modamp", stream_);
-    if (fit_fp32_)
-      launch_modamp_fp32(m, dkv, dks, nc, L.nx, L.ny, P.dxy, part32_, stream_);
-    else
-      launch_modamp_fp64(m, dkv, dks, nc, L.nx, L.ny, P.dxy, part64_, stream_);
-    launch_modamp_evals(fit_fp32_, m, dks, nc, part64_, part32_, nb, zsum_blocks(L.nxy), dev, stream_);
-    stage_end(st, stream_);
+    eval_enqueue(flat.data() + c0, nc, 0);
     wait_fit();
-    for (int c = 0; c < nc; ++c) {
-      const auto& f = flat[c0 + c];
-      res[f.first][f.second] = {ev[4 * c], ev[4 * c + 1], ev[4 * c + 2], ev[4 * c + 3]};
-    }
+    eval_read(nc, 0, ev.data() + c0);
   }
+  std::vector<std::vector<Eval>> res(req.size());
+  size_t i = 0;
+  for (size_t r = 0; r < req.size(); ++r)
+    for (size_t c = 0; c < req[r].second.size(); ++c) res[r].push_back(ev[i++]);
   return res;
 }
 
@@ -825,23 +836,35 @@ void Engine::prefetch_multi(const std::vector<std::pair<int, Walk>>& req, int o1
 
 // findrealspacemodamp at each direction's k0_ with overlaps of orders
 // (o1, o2) built there (redoarrays): amp_[dir][o2] and corr_[dir][o2].
-void Engine::realspacemodamp_multi(const std::vector<int>& dirs, int o1, int o2) {
-  if (dirs.empty()) return;
-  std::vector<OvReq> reqs;
-  std::vector<std::pair<int, std::vector<float2>>> kv;
-  for (int d : dirs) {
-    reqs.push_back({d, o1, o2, k0_[d].x, k0_[d].y});
-    kv.push_back({d, {make_float2(k0_[d].x * (float)(o2 - o1), k0_[d].y * (float)(o2 - o1))}});
+// npend evaluations already queued at entries [0, npend) come back in *pend,
+// read after the same wait.
+void Engine::realspacemodamp_multi(const std::vector<int>& dirs, int o1, int o2, int npend,
+                                   std::vector<Eval>* pend) {
+  if (dirs.empty() && npend == 0) return;
+  std::vector<Cand> c;
+  if (!dirs.empty()) {
+    std::vector<OvReq> reqs;
+    for (int d : dirs) {
+      reqs.push_back({d, o1, o2, k0_[d].x, k0_[d].y});
+      c.push_back({d, make_float2(k0_[d].x * (float)(o2 - o1), k0_[d].y * (float)(o2 - o1))});
+    }
+    make_overlaps(reqs);
+    eval_enqueue(c.data(), (int)c.size(), npend);
   }
-  make_overlaps(reqs);
-  auto ev = eval_multi(kv);
+  wait_fit();
+  if (npend > 0) {
+    pend->resize(npend);
+    eval_read(npend, 0, pend->data());
+  }
+  std::vector<Eval> ev(c.size());
+  if (!c.empty()) eval_read((int)c.size(), npend, ev.data());
   for (size_t i = 0; i < dirs.size(); ++i) {
     const int d = dirs[i];
     stats_[d].n_reductions++;
     double a2;
     float2 m;
     float cc;
-    modamp_of(ev[i][0], &m, &a2, &cc);
+    modamp_of(ev[i], &m, &a2, &cc);
     amp_[d][o2] = m;
     corr_[d][o2] = cc;
   }
@@ -1057,28 +1080,47 @@ void Engine::fitk0_all(const std::vector<int>& dirs) {
   };
   run_walks(true);   // angle walk at the start magnitude
   run_walks(false);  // magnitude walk at the fitted angle
-  {
-    std::vector<std::pair<int, Walk>> fin;
-    for (int d : dirs) fin.push_back({d, {{angle[d], mag[d]}}});
-    prefetch_multi(fin, fo1, fo2);
+  // The last getmodamp at the fitted (angle, mag) is queued behind the walks
+  // and read, with the other orders' modamps at the fitted k0, after one wait.
+  std::vector<Cand> fin;
+  std::vector<int> fin_dir;
+  std::vector<Eval> final_ev(L.ndirs);  // the next overlaps clear the caches
+  for (int d : dirs) {
+    k0_[d] = make_float2(mag[d] * cosf(angle[d]), mag[d] * sinf(angle[d]));
+    auto it = fw_[d].cache.find(key_of(angle[d], mag[d]));
+    if (it != fw_[d].cache.end()) {
+      final_ev[d] = it->second;
+    } else {
+      fin.push_back({d, kvec_of(angle[d], mag[d], fo1, fo2)});
+      fin_dir.push_back(d);
+    }
   }
+  if (!fin.empty()) eval_enqueue(fin.data(), (int)fin.size(), 0);
+  std::vector<std::pair<int, int>> others;  // (order1, order2) at the fitted k0
+  if (L.nz == 1) {
+    for (int order = 2; order < L.norders; ++order) others.push_back({order - 1, order});
+  } else {
+    for (int order = 1; order < L.norders; ++order)
+      if (order != fo2) others.push_back({0, order});
+  }
+  std::vector<Eval> fin_ev;
+  // the first other order's overlaps reuse the fit's scratch after the queued
+  // evaluation has read it (stream order)
+  if (others.empty())
+    realspacemodamp_multi({}, 0, 0, (int)fin.size(), &fin_ev);
+  else
+    realspacemodamp_multi(dirs, others[0].first, others[0].second, (int)fin.size(), &fin_ev);
+  for (size_t i = 0; i < fin.size(); ++i) final_ev[fin_dir[i]] = fin_ev[i];
   for (int d : dirs) {
     float2 modamp;
     double a2;
     float cc = 0;
-    modamp_of(fw_[d].cache.at(key_of(angle[d], mag[d])), &modamp, &a2, &cc);
+    modamp_of(final_ev[d], &modamp, &a2, &cc);
     stats_[d].n_reductions++;
     corr_[d][fo2] = cc;
-    k0_[d] = make_float2(mag[d] * cosf(angle[d]), mag[d] * sinf(angle[d]));
     amp_[d][fo2] = modamp;
   }
-  // the other orders at the fitted k0
-  if (L.nz == 1) {
-    for (int order = 2; order < L.norders; ++order) realspacemodamp_multi(dirs, order - 1, order);
-  } else {
-    for (int order = 1; order < L.norders; ++order)
-      if (order != fo2) realspacemodamp_multi(dirs, 0, order);
-  }
+  for (size_t k = 1; k < others.size(); ++k) realspacemodamp_multi(dirs, others[k].first, others[k].second);
 }
 
 // A non-finite k0 (the reference carries it on into NaN output) only comes

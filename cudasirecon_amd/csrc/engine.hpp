@@ -119,13 +119,17 @@ class Engine {
   // builds the requested overlap pairs (each in its direction's slot, or
   // every one in `only` when given) and their z sums
   void make_overlaps(const std::vector<OvReq>& reqs, bool per_elem = false, FitWork* only = nullptr);
+  using Cand = std::pair<int, float2>;  // (direction, candidate vector)
   ModampSlots modamp_slots() const;
+  void eval_enqueue(const Cand* c, int n, int at);
+  void eval_read(int n, int at, Eval* out) const;
   std::vector<std::vector<Eval>> eval_multi(const std::vector<std::pair<int, std::vector<float2>>>& req);
   void modamp_of(const Eval& e, float2* modamp, double* amp2, float* corr) const;
   void prefetch_multi(const std::vector<std::pair<int, Walk>>& req, int o1, int o2);
   bool replay_walk(int dir, bool is_angle, float start, float other, float delta, int o1, int o2, float* out,
                    int* nevals, Walk* need);
-  void realspacemodamp_multi(const std::vector<int>& dirs, int o1, int o2);
+  void realspacemodamp_multi(const std::vector<int>& dirs, int o1, int o2, int npend = 0,
+                             std::vector<Eval>* pend = nullptr);
   void findk0_all(const std::vector<int>& dirs);
   void fitk0_all(const std::vector<int>& dirs);
   void require_finite_k0(int dir, const char* where) const;

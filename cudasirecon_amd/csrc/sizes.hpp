@@ -6,14 +6,15 @@
 #define SIMR_XY_SIZES(X)                                                                         \
   X(16) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(100) X(120) X(128) X(144) X(160) X(192)      \
   X(200) X(240) X(256) X(288) X(300) X(320) X(360) X(384) X(400) X(480) X(500) X(512) X(576)     \
-  X(600) X(640) X(720) X(768) X(800) X(960) X(1000) X(1024)
+  X(600) X(640) X(720) X(768) X(800) X(960) X(1000) X(1024) X(1152) X(1200) X(1280) X(1440) X(1536)   \
+  X(1600) X(1920) X(2000) X(2048)
 
 // Lateral sizes of the zoomed output grid (assembly x pass).
 #define SIMR_ZOOM_SIZES(X)                                                                        \
   X(32) X(48) X(64) X(80) X(96) X(100) X(120) X(128) X(144) X(160) X(192) X(200) X(240) X(256)   \
   X(288) X(300) X(320) X(360) X(384) X(400) X(480) X(500) X(512) X(576) X(600) X(640) X(720)     \
   X(768) X(800) X(960) X(1000) X(1024) X(1152) X(1200) X(1280) X(1440) X(1536) X(1600) X(1920)   \
-  X(2000) X(2048)
+  X(2000) X(2048) X(2304) X(2400) X(2560) X(2880) X(3072) X(3200) X(3840) X(4000) X(4096)
 
 // Axial sizes (z passes); 1 = 2D data.
 #define SIMR_Z_SIZES(X)                                                                           \

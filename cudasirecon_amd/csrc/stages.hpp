@@ -86,8 +86,13 @@ void launch_overlaps_y(int nx, int ny, int nzs_stride, float2* ovx, int nslots, 
 // fp32: accumulated in float as the reference's reductionKernel threads, P,
 // SX, SY stored per (x, y).  Otherwise in double, P per (x, y) and the totals
 // of SX and SY as zsum_blocks(nx*ny) partials each: totals[b], totals[nb + b]
-// (per_elem: SX and SY per (x, y) as well).
+// (per_elem: SX and SY per (x, y) as well), and their sums in block order at
+// totals[kZsumFinal], totals[kZsumFinal + 1].
 constexpr int kZsumMaxBlocks = 2048;
+// totals buffer of one slot: [2][kZsumMaxBlocks] block partials, then the two
+// totals summed in block order (fp64, written by the z-sum pass)
+constexpr int kZsumFinal = 2 * kZsumMaxBlocks;
+constexpr int kZsumTotals = kZsumFinal + 2;
 int zsum_blocks(long long nxy);
 struct ZsumBatch {
   const float2* ovx[kMaxDirs];
@@ -133,8 +138,8 @@ void launch_modamp_fp32(const ModampSlots& m, const float2* kcand, const int* ks
                         float dxy, float4* partials, hipStream_t s);
 // The sums the host makes of the modamp partials (cpuReduce,
 // gpuFunctionsImpl_hh.cu:162-180), on the device: for candidate c, its
-// partials summed in block order and, fp64, its slot's z-sum totals summed in
-// block order; out[4c..4c+3] = (xr, xi, sx, sy) (fp32: the float sums).
+// partials summed in block order and, fp64, its slot's z-sum totals;
+// out[4c..4c+3] = (xr, xi, sx, sy) (fp32: the float sums).
 void launch_modamp_evals(bool fp32, const ModampSlots& m, const int* kslot, int ncand, const double2* part64,
                          const float4* part32, int nparts, int zblocks, double* out, hipStream_t s);
 // findrealspacemodamp's result (gpuFunctionsImpl.cu:1128-1147) on the device,

@@ -289,11 +289,14 @@ __device__ __forceinline__ int asm_y_pos(int Y) {
   else
     return lds_swz<N>(Y);
 }
+// Two-pass tiles hold at most 80 KB, so that two workgroups share a CU;
+// their row segments (8 columns, 64 bytes) pair up in one L2 through the
+// XCD-contiguous tile order.
 template <int N>
 constexpr int asm_y_w() {
   if (asm_y_fft2<N>()) {
     int w = 16;
-    while (w > 1 && w * asm_y_pitch<N>() * 8 > 150 * 1024) w /= 2;
+    while (w > 1 && w * asm_y_pitch<N>() * 8 > 80 * 1024) w /= 2;
     while (w * asm_y_tpt<N>() < 64) w *= 2;
     return w;
   }
@@ -312,6 +315,7 @@ __global__ __launch_bounds__(asm_y_w<N>() * asm_y_tpt<N>()) void k_asm_y(AsmPara
   constexpr int NT = W * TPT;
   constexpr int LP = asm_y_pitch<N>();
   static_assert(!F2 || NT % 64 == 0, "two-pass tiles are whole wavefronts");
+  static_assert(NT <= 1024 && (W * LP + ((N > 1 && !F2) ? N : 1)) * 8 <= 160 * 1024, "the tile fits a workgroup");
   __shared__ float2 lds[W * LP];
   FftTw<F2 ? 1 : N> twr;
   __shared__ float2 twstage[(N > 1 && !F2) ? N : 1];
@@ -320,8 +324,10 @@ __global__ __launch_bounds__(asm_y_w<N>() * asm_y_tpt<N>()) void k_asm_y(AsmPara
     tw2.load(threadIdx.x % TPT, tw);
   else
     load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
-  const int a = blockIdx.z, zout = blockIdx.y;
-  const int xc0 = blockIdx.x * W;
+  unsigned bx, by, bz;
+  xcd_tile3(bx, by, bz);
+  const int a = (int)bz, zout = (int)by;
+  const int xc0 = (int)bx * W;
   const float2* src = p.stageZ + ((long long)a * p.zdim + zout) * p.ny * p.nx;
   // Source row sr goes to Y = sr - hy (sr >= hy) or Y = sr + N - hy (sr < hy);
   // the rows between are zero.  Loads run over source rows in batches of B
@@ -387,9 +393,12 @@ constexpr int asm_x_tpt() {
   if (asm_x_fft2<N>()) return fft2_tpt(N);
   return fft_tpt(N) >= 128 ? 128 : fft_tpt(N);
 }
+// The radix-stage core takes 512 threads (four rows of 128), or 256 (two
+// rows) from 3072-point rows on, so that the rows and the twiddle table fit
+// the 160 KB of LDS.
 template <int N>
 constexpr int asm_x_threads() {
-  return asm_x_fft2<N>() ? kAsmXBands * fft2_tpt(N) : 512;
+  return asm_x_fft2<N>() ? kAsmXBands * fft2_tpt(N) : (N >= 3072 ? 256 : 512);
 }
 
 // modulation phasor of assembly band a (>= 1) at output column X of row Y
@@ -473,14 +482,15 @@ __device__ __forceinline__ void asm_x_table_bands(const AsmParams& p, const floa
 }
 
 // Adds the G transformed bands of round R (bands R*G + sl in LDS slot sl),
-// in band order.  Bands 0..LB come with compile-time table entries; the
-// rest compute their phasors.
-template <int N, int NT, int G, int E, int LB, int R>
+// in band order.  Band 0 and side bands 1..NTAB come with compile-time table
+// entries (NTAB = LB with the phasor table, 0 without it); the rest compute
+// their phasors.
+template <int N, int NT, int G, int E, int LB, int NTAB, int R>
 __device__ __forceinline__ void asm_x_round(const AsmParams& p, const float2* lds, int a0, int Y, float (&acc)[E],
                                             const float2 (&lut)[E][LB]) {
   constexpr int LP = asm_x_pitch<N>();
   constexpr int A0 = R * G;
-  constexpr int AT = (R < 0) ? 0 : ((A0 + G < LB + 1) ? A0 + G : LB + 1);  // first band without a table entry
+  constexpr int AT = (R < 0) ? 0 : ((A0 + G < NTAB + 1) ? A0 + G : NTAB + 1);  // first band without a table entry
   int sl = 0;
   if constexpr (R >= 0 && AT > A0) {
     asm_x_table_bands<N, NT, G, E, LB, R, A0, AT>(p, lds, Y, acc, lut);
@@ -590,6 +600,7 @@ __global__ __launch_bounds__(asm_x_threads<N>(), 2) void k_asm_x(AsmParams p,
   constexpr bool LUT = E <= 4;
   constexpr int LB = LUT ? kAsmLutBands : 1;
   static_assert(!asm_x_fft2<N>() || (64 % TPT == 0 && NT % 64 == 0), "two-pass transforms must not straddle waves");
+  static_assert((G * LP + N) * 8 <= 160 * 1024, "band rows and twiddles fit the LDS");
   __shared__ float2 lds[G * LP];
   __shared__ float2 twt[N];
   for (int i = threadIdx.x; i < N; i += NT) twt[i] = tw[i];
@@ -629,11 +640,11 @@ __global__ __launch_bounds__(asm_x_threads<N>(), 2) void k_asm_x(AsmParams p,
           fft2_lds<N, true, true>(lds + g * LP, t, twt);
           __syncthreads();
           if (a0 == 0)
-            asm_x_round<N, NT, G, E, LB, 0>(p, lds, a0, Y, acc, lut);
+            asm_x_round<N, NT, G, E, LB, LUT ? LB : 0, 0>(p, lds, a0, Y, acc, lut);
           else if (a0 == G)
-            asm_x_round<N, NT, G, E, LB, 1>(p, lds, a0, Y, acc, lut);
+            asm_x_round<N, NT, G, E, LB, LUT ? LB : 0, 1>(p, lds, a0, Y, acc, lut);
           else
-            asm_x_round<N, NT, G, E, LB, -1>(p, lds, a0, Y, acc, lut);
+            asm_x_round<N, NT, G, E, LB, LUT ? LB : 0, -1>(p, lds, a0, Y, acc, lut);
           __syncthreads();
         }
         float* orow = p.out + ((long long)zout * p.ydim + Y) * N;
@@ -686,11 +697,11 @@ __global__ __launch_bounds__(asm_x_threads<N>(), 2) void k_asm_x(AsmParams p,
         fft_lds_t<N, true, TPT>(lds + g * LP, t, twt);
       }
       if (a0 == 0)
-        asm_x_round<N, NT, G, E, LB, 0>(p, lds, a0, Y, acc, lut);
+        asm_x_round<N, NT, G, E, LB, LUT ? LB : 0, 0>(p, lds, a0, Y, acc, lut);
       else if (a0 == G)
-        asm_x_round<N, NT, G, E, LB, 1>(p, lds, a0, Y, acc, lut);
+        asm_x_round<N, NT, G, E, LB, LUT ? LB : 0, 1>(p, lds, a0, Y, acc, lut);
       else
-        asm_x_round<N, NT, G, E, LB, -1>(p, lds, a0, Y, acc, lut);
+        asm_x_round<N, NT, G, E, LB, LUT ? LB : 0, -1>(p, lds, a0, Y, acc, lut);
       __syncthreads();
     }
     float* orow = p.out + ((long long)zout * p.ydim + Y) * N;

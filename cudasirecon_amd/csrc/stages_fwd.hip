@@ -180,8 +180,16 @@ void launch_ratios(const double* partials, double* sdp, float* ratio, int ndirs,
 //   X0[k] = (Z[k] + conj Z[N-k]) / 2,   X1[k] = (Z[k] - conj Z[N-k]) / 2i.
 // All raw loads of a workgroup are 4-pixel vector loads with compile-time
 // trip counts, so they are all in flight before the first use.
+// Threads per row transform: the FFT core's count, halved until the five
+// band rows fit one workgroup (2048-point rows take 128 threads, not 256).
+constexpr int rows_tpt(int N) {
+  int t = fft_tpt(N);
+  while (kMaxBands * t > 1024) t /= 2;
+  return t;
+}
+
 template <int N, typename T>
-__global__ __launch_bounds__(kMaxBands* fft_tpt(N)) void k_rows_fwd(FwdParams p, const T* __restrict__ raw,
+__global__ __launch_bounds__(kMaxBands* rows_tpt(N)) void k_rows_fwd(FwdParams p, const T* __restrict__ raw,
                                                                      const float* __restrict__ bg,
                                                                      const float* __restrict__ slope,
                                                                      const float* __restrict__ ratio,
@@ -189,10 +197,11 @@ __global__ __launch_bounds__(kMaxBands* fft_tpt(N)) void k_rows_fwd(FwdParams p,
                                                                      float2* __restrict__ bands,
                                                                      const float2* __restrict__ tw) {
 #pragma clang fp contract(off)
-  constexpr int TPT = fft_tpt(N);
+  constexpr int TPT = rows_tpt(N);
   constexpr int NT = kMaxBands * TPT;
   constexpr int P = lds_pitch(N);
   constexpr int Q = N / 4;
+  static_assert((kMaxBands * P + N) * 8 <= 160 * 1024, "band rows and twiddles fit the LDS");
   constexpr int NQ = kMaxBands * 2 * Q;
   constexpr int QPT = (NQ + NT - 1) / NT;
   // the flat-fielded phase rows (rv, 2 kMaxBands rows of N floats) share
@@ -203,7 +212,7 @@ __global__ __launch_bounds__(kMaxBands* fft_tpt(N)) void k_rows_fwd(FwdParams p,
   float* const rv = reinterpret_cast<float*>(cz);
   __shared__ float dy[kMaxBands * 2];
   const int tid = threadIdx.x;
-  FftTw<N> twr;
+  FftTw<N, TPT> twr;
   __shared__ float2 twstage[N > 1 ? N : 1];
   load_twiddles(twr, twstage, tid % TPT, tw, threadIdx.x, blockDim.x);
   const int y0 = 2 * blockIdx.x;
@@ -304,7 +313,7 @@ __global__ __launch_bounds__(kMaxBands* fft_tpt(N)) void k_rows_fwd(FwdParams p,
     }
   }
   __syncthreads();
-  fft_lds<N, false>(cz + (tid / TPT) * P, tid % TPT, twr);
+  fft_lds<N, false, TPT>(cz + (tid / TPT) * P, tid % TPT, twr);
   // 4. split the packed transforms into the two rows' half spectra
   constexpr int nxh = N / 2 + 1;
   constexpr int UPT = (kMaxBands * nxh + NT - 1) / NT;
@@ -329,7 +338,7 @@ template <int N>
 static void launch_rows_fwd_n(const FwdParams& p, const void* raw, bool u16, const float* bg, const float* slope,
                               const float* ratio, const float* apofact, float2* bands, hipStream_t s) {
   dim3 grid((p.ny + 1) / 2, p.nz, p.ndirs);
-  dim3 block(kMaxBands * fft_tpt(N));
+  dim3 block(kMaxBands * rows_tpt(N));
   if (u16)
     hipLaunchKernelGGL((k_rows_fwd<N, uint16_t>), grid, block, 0, s, p, (const uint16_t*)raw, bg, slope, ratio,
                        apofact, bands, twiddles(N));

@@ -28,6 +28,38 @@
 
 namespace simr {
 
+// Sequential sums in block order, as the host's loops make them, of K
+// interleaved components at once: the workgroup stages a chunk of the
+// entries in LDS with every thread (coalesced, all loads in flight), then
+// lane k adds up component k in order, eight staged values loaded ahead of
+// each eight additions.  Returns lane k's sum (k < K).
+constexpr int kSeqChunk = 1024;  // entries per LDS chunk
+
+template <typename T, int K, typename Get>
+__device__ __forceinline__ T seq_sum_lanes(int n, double* stage, Get get) {
+#pragma clang fp contract(off)
+  T acc = 0;
+  const int lane = threadIdx.x;
+  for (int c0 = 0; c0 < n; c0 += kSeqChunk) {
+    const int m = n - c0 < kSeqChunk ? n - c0 : kSeqChunk;
+    for (int i = threadIdx.x; i < m * K; i += blockDim.x) stage[i] = get(c0 + i / K, i % K);
+    __syncthreads();
+    if (lane < K) {
+      int i = 0;
+      for (; i + 8 <= m; i += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = stage[(i + u) * K + lane];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += (T)v[u];
+      }
+      for (; i < m; ++i) acc += (T)stage[i * K + lane];
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
 // Where one element of overlap0 (which == 0) or overlap1 (which == 1) at
 // column j, row i, plane z0 comes from, before the inverse transform: its
 // band index, the conjugation of the Hermitian mirror, and the OTF factor
@@ -141,13 +173,21 @@ __device__ __forceinline__ float2 ov_combine(const OvSrc& o, float2 vre, float2 
   return make_float2(vre.x - vim.y, vre.y + vim.x);
 }
 
-// x pass: rows [kOvRows*blockIdx.x, +kOvRows) of plane zb (blockIdx.y) of both
-// overlaps of slot blockIdx.z; the plane's two radial OTF profiles are built
-// once in LDS.
-constexpr int kOvRows = 4;
+// x pass: rows [R*blockIdx.x, +R) of plane zb (blockIdx.y) of both overlaps
+// of slot blockIdx.z, R = ov_rows(N) (4, fewer when 2R transforms would
+// exceed a workgroup or 96 KB of LDS, which leaves 64 KB for the profiles);
+// the plane's two radial OTF profiles are built once in LDS.
+constexpr int kOvStaticLds = 96 * 1024;
+constexpr int ov_rows(int N) {
+  int r = 4;
+  while (r > 1 && (2 * r * fft_tpt(N) > 1024 || (2 * r * lds_pitch(N) + N) * 8 > kOvStaticLds)) r /= 2;
+  return r;
+}
 
 template <int N, bool RA>
-__global__ __launch_bounds__(2 * kOvRows * fft_tpt(N)) void k_overlaps_x(OvBatch b, const float2* __restrict__ tw) {
+__global__ __launch_bounds__(2 * ov_rows(N) * fft_tpt(N)) void k_overlaps_x(OvBatch b, const float2* __restrict__ tw) {
+  constexpr int kOvRows = ov_rows(N);
+  static_assert((2 * kOvRows * lds_pitch(N) + N) * 8 <= kOvStaticLds, "rows and twiddles leave room for the profiles");
   const OvParams& p = b.p[blockIdx.z];
   const int zb = blockIdx.y;
   if (zb > 2 * p.zd) return;  // a slot with a narrower z support (whole block)
@@ -222,9 +262,10 @@ __global__ __launch_bounds__(2 * kOvRows * fft_tpt(N)) void k_overlaps_x(OvBatch
 template <int N>
 static void launch_ov_x(const OvBatch& b, int nzs_max, hipStream_t s) {
   const OvParams& p = b.p[0];
-  const dim3 grid(ceil_div(p.ny, kOvRows), nzs_max, b.n), block(2 * kOvRows * fft_tpt(N));
+  const dim3 grid(ceil_div(p.ny, ov_rows(N)), nzs_max, b.n), block(2 * ov_rows(N) * fft_tpt(N));
   if (p.otf1.ra) {
     const size_t lds = (size_t)2 * p.otf1.nxotf * sizeof(float2);
+    if (lds > 160 * 1024 - kOvStaticLds) throw ArgError("OTF table too long for the overlap pass's LDS profiles");
     hipLaunchKernelGGL((k_overlaps_x<N, true>), grid, block, lds, s, b, twiddles(N));
   } else {
     hipLaunchKernelGGL((k_overlaps_x<N, false>), grid, block, 0, s, b, twiddles(N));
@@ -346,6 +387,14 @@ __global__ __launch_bounds__(256) void k_overlaps_zsum(ZsumBatch b) {
   }
 }
 
+// The two z-sum totals of each slot (blockIdx.x), summed in block order.
+__global__ __launch_bounds__(256) void k_zsum_totals(ZsumBatch b, int zblocks) {
+  __shared__ double stage[2 * kSeqChunk];
+  double* t = b.totals[blockIdx.x];
+  const double v = seq_sum_lanes<double, 2>(zblocks, stage, [&](int i, int k) { return t[k * zblocks + i]; });
+  if (threadIdx.x < 2) t[kZsumFinal + threadIdx.x] = v;
+}
+
 void launch_overlaps_z_reduce(const ZsumBatch& b, bool fp32, bool per_elem, hipStream_t s) {
   if (b.n < 1 || b.n > kMaxDirs) throw ArgError("z-sum batch: 1..8 slots");
   for (int k = 0; k < b.n; ++k)
@@ -359,6 +408,10 @@ void launch_overlaps_z_reduce(const ZsumBatch& b, bool fp32, bool per_elem, hipS
   else
     hipLaunchKernelGGL((k_overlaps_zsum<false, false>), grid, dim3(256), 0, s, b);
   SIMR_LAUNCH_CHECK();
+  if (!fp32) {
+    hipLaunchKernelGGL(k_zsum_totals, dim3(b.n), dim3(256), 0, s, b, (int)grid.x);
+    SIMR_LAUNCH_CHECK();
+  }
 }
 
 // ----------------------------------------------------------------- findk0 peak
@@ -536,12 +589,14 @@ void launch_modamp_fp64(const ModampSlots& m, const float2* kcand, const int* ks
 
 // The result arithmetic of one modamp candidate, in the host's order and
 // precision (one thread: the sums are sequential in block order).
-__global__ void k_modamp_finish(int fp32, ModampSlots m, const int* __restrict__ kslot,
-                                const double2* __restrict__ part64, const float4* __restrict__ part32, int nparts,
-                                int zblocks, float2* __restrict__ amp, float* __restrict__ corr, int stride,
-                                int order) {
+__global__ __launch_bounds__(256) void k_modamp_finish(int fp32, ModampSlots m, const int* __restrict__ kslot,
+                                                       const double2* __restrict__ part64,
+                                                       const float4* __restrict__ part32, int nparts, int zblocks,
+                                                       float2* __restrict__ amp, float* __restrict__ corr, int stride,
+                                                       int order) {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0) return;
+  __shared__ double stage[4 * kSeqChunk];
+  __shared__ double sums[4];
   const int c = blockIdx.x, slot = kslot[c];
   const double* __restrict__ totals = m.totals[slot];
   part64 += (long long)c * nparts;
@@ -549,26 +604,25 @@ __global__ void k_modamp_finish(int fp32, ModampSlots m, const int* __restrict__
   amp += slot * stride + order;
   corr += slot * stride + order;
   if (fp32) {
-    float xr = 0.f, xi = 0.f, sx = 0.f, sy = 0.f;
-    for (int b = 0; b < nparts; ++b) {
-      const float4 v = part32[b];
-      xr += v.x;
-      xi += v.y;
-      sx += v.z;
-      sy += v.w;
-    }
+    const float v = seq_sum_lanes<float, 4>(nparts, stage, [&](int i, int k) {
+      const float4 q = part32[i];
+      return (double)(k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w);
+    });
+    if (threadIdx.x < 4) sums[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const float xr = (float)sums[0], xi = (float)sums[1], sx = (float)sums[2], sy = (float)sums[3];
     const float mx = xr / sx, my = xi / sx;
     *amp = make_float2(mx, my);
     *corr = sqrtf((xr * xr + xi * xi) / (sx * sy));
   } else {
-    double sx = 0.0, sy = 0.0;
-    for (int b = 0; b < zblocks; ++b) sx += totals[b];
-    for (int b = 0; b < zblocks; ++b) sy += totals[zblocks + b];
-    double xr = 0.0, xi = 0.0;
-    for (int b = 0; b < nparts; ++b) {
-      xr += part64[b].x;
-      xi += part64[b].y;
-    }
+    const double v = seq_sum_lanes<double, 2>(nparts, stage, [&](int i, int k) {
+      return k == 0 ? part64[i].x : part64[i].y;
+    });
+    if (threadIdx.x < 2) sums[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double xr = sums[0], xi = sums[1], sx = totals[kZsumFinal], sy = totals[kZsumFinal + 1];
     *amp = make_float2((float)(xr / sx), (float)(xi / sx));
     *corr = (float)sqrt((xr * xr + xi * xi) / (sx * sy));
   }
@@ -578,57 +632,38 @@ void launch_modamp_finish(bool fp32, const ModampSlots& m, const int* kslot, int
                           const float4* part32, int nparts, int zblocks, float2* amp, float* corr, int stride,
                           int order, hipStream_t s) {
   if (ncand < 1) return;
-  hipLaunchKernelGGL(k_modamp_finish, dim3(ncand), dim3(64), 0, s, fp32 ? 1 : 0, m, kslot, part64, part32, nparts,
+  hipLaunchKernelGGL(k_modamp_finish, dim3(ncand), dim3(256), 0, s, fp32 ? 1 : 0, m, kslot, part64, part32, nparts,
                      zblocks, amp, corr, stride, order);
   SIMR_LAUNCH_CHECK();
 }
 
-__global__ __launch_bounds__(128) void k_modamp_evals(int fp32, ModampSlots m, const int* __restrict__ kslot,
+__global__ __launch_bounds__(256) void k_modamp_evals(int fp32, ModampSlots m, const int* __restrict__ kslot,
                                                       const double2* __restrict__ part64,
                                                       const float4* __restrict__ part32, int nparts, int zblocks,
                                                       double* __restrict__ out) {
 #pragma clang fp contract(off)
+  __shared__ double stage[4 * kSeqChunk];
   const int c = blockIdx.x;
   out += 4LL * c;
   if (fp32) {
-    if (threadIdx.x != 0) return;
     const float4* pp = part32 + (long long)c * nparts;
-    float xr = 0.f, xi = 0.f, sx = 0.f, sy = 0.f;
-    for (int b = 0; b < nparts; ++b) {
-      const float4 v = pp[b];
-      xr += v.x;
-      xi += v.y;
-      sx += v.z;
-      sy += v.w;
-    }
-    out[0] = xr;
-    out[1] = xi;
-    out[2] = sx;
-    out[3] = sy;
+    const float v = seq_sum_lanes<float, 4>(nparts, stage, [&](int i, int k) {
+      const float4 q = pp[i];
+      return (double)(k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w);
+    });
+    if (threadIdx.x < 4) out[threadIdx.x] = v;
     return;
   }
-  if (threadIdx.x == 0) {
-    const double2* pp = part64 + (long long)c * nparts;
-    double xr = 0.0, xi = 0.0;
-    for (int b = 0; b < nparts; ++b) {
-      xr += pp[b].x;
-      xi += pp[b].y;
-    }
-    out[0] = xr;
-    out[1] = xi;
-  } else if (threadIdx.x == 64 || threadIdx.x == 65) {
-    // sx (lane 0) and sy (lane 1) of the second wavefront, side by side
-    const double* t = m.totals[kslot[c]] + (threadIdx.x == 64 ? 0 : zblocks);
-    double acc = 0.0;
-    for (int b = 0; b < zblocks; ++b) acc += t[b];
-    out[threadIdx.x == 64 ? 2 : 3] = acc;
-  }
+  const double2* pp = part64 + (long long)c * nparts;
+  const double v = seq_sum_lanes<double, 2>(nparts, stage, [&](int i, int k) { return k == 0 ? pp[i].x : pp[i].y; });
+  if (threadIdx.x < 2) out[threadIdx.x] = v;
+  if (threadIdx.x == 2 || threadIdx.x == 3) out[threadIdx.x] = m.totals[kslot[c]][kZsumFinal + threadIdx.x - 2];
 }
 
 void launch_modamp_evals(bool fp32, const ModampSlots& m, const int* kslot, int ncand, const double2* part64,
                          const float4* part32, int nparts, int zblocks, double* out, hipStream_t s) {
   if (ncand < 1) return;
-  hipLaunchKernelGGL(k_modamp_evals, dim3(ncand), dim3(128), 0, s, fp32 ? 1 : 0, m, kslot, part64, part32, nparts,
+  hipLaunchKernelGGL(k_modamp_evals, dim3(ncand), dim3(256), 0, s, fp32 ? 1 : 0, m, kslot, part64, part32, nparts,
                      zblocks, out);
   SIMR_LAUNCH_CHECK();
 }

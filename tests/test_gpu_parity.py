@@ -61,7 +61,7 @@ def _oracle_preprocessed(p, otf, raw, prec=32, fit64=False):
 
 def _cases():
     return ["testdata", "syn3d", "syn2d", "bessel3d", "opts3d", "opts2d", "z14", "z21zz", "z11", "z13zz", "z61",
-            "z131zz", "full3d", "full2d"]
+            "z131zz", "full3d", "full2d", "w2048", "w1536_2d"]
 
 
 def _case(name):
@@ -95,6 +95,12 @@ def _case(name):
         return p, otf, raw
     if name == "z131zz":  # prime depth 131 with z_zoom 2: assembly depth 262, above the transform kernels' 256
         p, otf, raw, _ = synthetic(32, 32, 131, nphases=5, z_zoom=2)
+        return p, otf, raw
+    if name == "w2048":  # a 2048-wide camera: 4096-point assembly rows, 128-thread row transforms
+        p, otf, raw, _ = synthetic(2048, 48, 6, nphases=5)
+        return p, otf, raw
+    if name == "w1536_2d":  # 1536-wide 2D
+        p, otf, raw, _ = synthetic(1536, 64, 1, nphases=3)
         return p, otf, raw
     if name == "full3d":  # a full (not rotationally averaged) 3D OTF table, trilinear lookups
         p, otf, raw, _ = synthetic(64, 64, 16, nphases=5)
