@@ -1,5 +1,6 @@
 // Host orchestration of one reconstruction context.  See engine.hpp.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -16,16 +17,16 @@ constexpr double kPI = 3.14159265358979323846;
 constexpr float K0_WARNING_THRESH = 0.05f;  // cudaSireconImpl.h:54
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
-constexpr int kMaxCand = 128;  // modamp candidates (of every direction) reduced per launch
-// the fit's zero-copy host region: candidate vectors, their directions, the
-// modamp sums (4 doubles each) of two queued batches of kMaxCand, then the
-// peak values (8 floats per direction) and peak indices
+constexpr int kMaxCand = kArgCand;  // modamp candidates (of every direction) reduced per launch
+// the fit's zero-copy host region: the modamp sums (4 doubles each) of two
+// queued batches of kMaxCand, the peak values (8 floats per direction) and
+// peak indices, and the completion flag
 constexpr int kZcCand = 2 * kMaxCand;
-constexpr size_t kZcSlot = ((size_t)kZcCand * 8 + 255) & ~(size_t)255;
-constexpr size_t kZcEval = kZcSlot + (((size_t)kZcCand * 4 + 255) & ~(size_t)255);
+constexpr size_t kZcEval = 0;
 constexpr size_t kZcPeakV = kZcEval + (size_t)kZcCand * 32;
 constexpr size_t kZcPeakI = kZcPeakV + (size_t)kMaxDirs * 32;
-constexpr size_t kZcBytes = kZcPeakI + 256;
+constexpr size_t kZcFlag = kZcPeakI + 256;
+constexpr size_t kZcBytes = kZcFlag + 256;
 
 int norders_of(const simr_params& p) { return p.norders_output != 0 ? p.norders_output : p.nphases / 2 + 1; }
 
@@ -134,7 +135,6 @@ size_t fit_block_bytes(const Layout& L) {
   b += align256((size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16);
   b += align256((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
   b += 2 * align256((size_t)kMaxDirs * kArgBlocks * 4);
-  b += align256(kZcEval);  // the candidates' device copy
   return b;
 }
 // The overlap dumps' scratch, in its own region: one overlap stage, P, SX,
@@ -268,7 +268,6 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
     part32_ = (float4*)take((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
     pv_ = (float*)take((size_t)kMaxDirs * kArgBlocks * 4);
     pi_ = (int*)take((size_t)kMaxDirs * kArgBlocks * 4);
-    kdev_ = take(kZcEval);
     SIMR_REQUIRE((size_t)(at - stage) <= stage_bytes_, "fit scratch larger than the staging region");
   }
   {
@@ -291,8 +290,8 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
     hzc_ = (char*)h;
     SIMR_HIP(hipHostGetDevicePointer(&d, h, 0));
     dzc_ = (char*)d;
+    std::memset(hzc_, 0, kZcBytes);
   }
-  SIMR_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
   {
     int ids[kMaxDirs];
     for (int k = 0; k < kMaxDirs; ++k) ids[k] = k;
@@ -394,7 +393,6 @@ Engine::~Engine() {
   if (graph_) (void)hipGraphDestroy(graph_);
   if (fit_host_) (void)hipHostFree(fit_host_);
   if (hzc_) (void)hipHostFree(hzc_);
-  if (done_) (void)hipEventDestroy(done_);
   for (auto& e : events_) {
     (void)hipEventDestroy(e.second.first);
     (void)hipEventDestroy(e.second.second);
@@ -407,21 +405,28 @@ Engine::~Engine() {
 void Engine::sync() { SIMR_HIP(hipStreamSynchronize(stream_)); }
 
 // Waits for the work queued on the context stream.  The fit's read-backs are
-// small and on its critical path, so the event is polled: the wait ends
-// within microseconds of the copy, where a blocking synchronize adds tens of
-// them.  After 2 ms of polling (a busy device, e.g. other contexts' work) it
-// falls back to the blocking wait.
+// small and on its critical path: a one-thread kernel queued behind the work
+// writes a sequence number into the zero-copy region, and the host polls that
+// word, which ends the wait within a few microseconds of the device's last
+// write (an event's completion signal takes tens of them to be seen).  After
+// 2 ms of polling (a busy device, e.g. other contexts' work) it falls back to
+// the blocking synchronize, which also reports a failed kernel.
 void Engine::wait_fit() {
-  SIMR_HIP(hipEventRecord(done_, stream_));
+  const unsigned seq = ++fit_seq_;
+  launch_fit_signal((unsigned*)(dzc_ + kZcFlag), seq, stream_);
+  const volatile unsigned* flag = (const volatile unsigned*)(hzc_ + kZcFlag);
   const auto t0 = std::chrono::steady_clock::now();
   for (int i = 0;; ++i) {
-    const hipError_t e = hipEventQuery(done_);
-    if (e == hipSuccess) return;
-    if (e != hipErrorNotReady) SIMR_HIP(e);
-    if ((i & 15) == 15 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
-    for (int k = 0; k < 32; ++k) __builtin_ia32_pause();  // a few microseconds between queries
+    if (*flag == seq) {
+      std::atomic_thread_fence(std::memory_order_acquire);
+      return;
+    }
+    if ((i & 63) == 63 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+    __builtin_ia32_pause();
   }
-  SIMR_HIP(hipEventSynchronize(done_));
+  SIMR_HIP(hipStreamSynchronize(stream_));
+  SIMR_REQUIRE(*flag == seq, "fit completion flag not written");
+  std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 void Engine::enable_timing(bool on) {
@@ -729,25 +734,27 @@ ModampSlots Engine::modamp_slots() const {
 // read (eval_read) after the next wait_fit().
 void Engine::eval_enqueue(const Cand* c, int n, int at) {
   SIMR_REQUIRE(n >= 1 && n <= kMaxCand && at >= 0 && at + n <= kZcCand, "modamp batch out of range");
-  float2* kv = (float2*)hzc_;
-  int* ks = (int*)(hzc_ + kZcSlot);
+  CandSet cs;
+  cs.kc = nullptr;
+  cs.ks = nullptr;
   for (int i = 0; i < n; ++i) {
     SIMR_REQUIRE(c[i].first >= 0 && c[i].first < L.ndirs, "direction out of range");
-    kv[at + i] = c[i].second;
-    ks[at + i] = c[i].first;
+    cs.k[i] = c[i].second;
+    cs.slot[i] = c[i].first;
+  }
+  for (int i = n; i < kArgCand; ++i) {
+    cs.k[i] = make_float2(0.f, 0.f);
+    cs.slot[i] = 0;
   }
   const ModampSlots m = modamp_slots();
   const int nb = fit_fp32_ ? modamp_partials_fp32(L.nx, L.ny) : modamp_partials_fp64(L.nx, L.ny);
-  const float2* dkv = (const float2*)kdev_ + at;
-  const int* dks = (const int*)(kdev_ + kZcSlot) + at;
   double* dev = (double*)(dzc_ + kZcEval) + 4 * (size_t)at;
-  SIMR_HIP(hipMemcpyAsync(kdev_, hzc_, kZcEval, hipMemcpyHostToDevice, stream_));
   const int st = stage_begin("modamp", stream_);
   if (fit_fp32_)
-    launch_modamp_fp32(m, dkv, dks, n, L.nx, L.ny, P.dxy, part32_, stream_);
+    launch_modamp_fp32(m, cs, n, L.nx, L.ny, P.dxy, part32_, stream_);
   else
-    launch_modamp_fp64(m, dkv, dks, n, L.nx, L.ny, P.dxy, part64_, stream_);
-  launch_modamp_evals(fit_fp32_, m, dks, n, part64_, part32_, nb, zsum_blocks(L.nxy), dev, stream_);
+    launch_modamp_fp64(m, cs, n, L.nx, L.ny, P.dxy, part64_, stream_);
+  launch_modamp_evals(fit_fp32_, m, cs, n, part64_, part32_, nb, zsum_blocks(L.nxy), dev, stream_);
   stage_end(st, stream_);
 }
 
@@ -1319,6 +1326,13 @@ void Engine::assemble() {
   a.fact = (float)((double)P.explodefact / 0.5);
   a.dxyo = P.dxy / P.zoomfact;
   for (int d = 0; d < L.ndirs; ++d) a.k0[d] = k0_[d];
+  {
+    // the filter zeroes every band element beyond rdistcutoff (filter_params)
+    const Geo g = geometry();
+    a.dkx = g.dkx;
+    a.dky = g.dky;
+    a.rc2_zero = (float)((double)g.rdistcutoff * (double)g.rdistcutoff * 1.0002);
+  }
   cur_stage_ = stage_begin("asm_z", stream_);
   launch_asm_z(a, stream_);
   stage_end(cur_stage_, stream_);
@@ -1382,11 +1396,13 @@ void Engine::build_graph() {
       std::vector<OvReq> reqs;
       for (int d = 0; d < L.ndirs; ++d) reqs.push_back({d, 0, o, k0_[d].x, k0_[d].y});
       make_overlaps(reqs);
-      const float2* kc_o = kc_dev_ + (size_t)o * L.ndirs;
+      CandSet cs{};
+      cs.kc = kc_dev_ + (size_t)o * L.ndirs;
+      cs.ks = slot_ids_;
       if (fit_fp32_)
-        launch_modamp_fp32(m, kc_o, slot_ids_, L.ndirs, L.nx, L.ny, P.dxy, part32_, stream_);
+        launch_modamp_fp32(m, cs, L.ndirs, L.nx, L.ny, P.dxy, part32_, stream_);
       else
-        launch_modamp_fp64(m, kc_o, slot_ids_, L.ndirs, L.nx, L.ny, P.dxy, part64_, stream_);
+        launch_modamp_fp64(m, cs, L.ndirs, L.nx, L.ny, P.dxy, part64_, stream_);
       launch_modamp_finish(fit_fp32_, m, slot_ids_, L.ndirs, part64_, part32_, nparts, zsum_blocks(L.nxy),
                            amps_dev_, corr_dev_, L.norders, o, stream_);
     }

@@ -168,15 +168,14 @@ class Engine {
   float4* part32_ = nullptr;
   float* pv_ = nullptr;
   int* pi_ = nullptr;
-  // Coherent pinned host memory: the candidate vectors and their directions
-  // are written there and reach the device in one copy (kdev_, same layout);
-  // the modamp sums and the cross-correlation peaks are written there by the
-  // kernels directly, so a line-search round has one copy and one wait.
-  // Host and device views of the same bytes.
+  // Coherent pinned host memory: the modamp sums, the cross-correlation
+  // peaks and the round's completion flag are written there by the kernels
+  // directly (the candidates travel in the kernel arguments), so a
+  // line-search round has no copy and one wait.  Host and device views of
+  // the same bytes.
   char* hzc_ = nullptr;
   char* dzc_ = nullptr;
-  char* kdev_ = nullptr;  // device copy of the candidate part of hzc_
-  hipEvent_t done_ = nullptr;  // round completion (polled by wait_fit)
+  unsigned fit_seq_ = 0;  // the last completion flag value queued
   float2 *stageZ_ = nullptr, *stageY_ = nullptr;
   float* resampled_ = nullptr;  // [nimg][ny][nx] resampled raw planes (aliases the staging region)
   float* out_ = nullptr;

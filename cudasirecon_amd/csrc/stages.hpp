@@ -130,18 +130,32 @@ struct ModampSlots {
   const double* SY[kMaxDirs];
   const double* totals[kMaxDirs];  // the z-sum totals (fp64 finish)
 };
+// The candidates of one batch: device arrays (kc, ks; the graph's persistent
+// ones), or, with kc == nullptr, the values themselves in the kernel
+// arguments, so that a line-search round needs no copy.
+constexpr int kArgCand = 128;
+struct CandSet {
+  const float2* kc;
+  const int* ks;
+  float2 k[kArgCand];
+  int slot[kArgCand];
+};
 int modamp_partials_fp64(int nx, int ny);
 int modamp_partials_fp32(int nx, int ny);
-void launch_modamp_fp64(const ModampSlots& m, const float2* kcand, const int* kslot, int ncand, int nx, int ny,
-                        float dxy, double2* partials, hipStream_t s);
-void launch_modamp_fp32(const ModampSlots& m, const float2* kcand, const int* kslot, int ncand, int nx, int ny,
-                        float dxy, float4* partials, hipStream_t s);
+void launch_modamp_fp64(const ModampSlots& m, const CandSet& c, int ncand, int nx, int ny, float dxy,
+                        double2* partials, hipStream_t s);
+void launch_modamp_fp32(const ModampSlots& m, const CandSet& c, int ncand, int nx, int ny, float dxy,
+                        float4* partials, hipStream_t s);
 // The sums the host makes of the modamp partials (cpuReduce,
 // gpuFunctionsImpl_hh.cu:162-180), on the device: for candidate c, its
 // partials summed in block order and, fp64, its slot's z-sum totals;
-// out[4c..4c+3] = (xr, xi, sx, sy) (fp32: the float sums).
-void launch_modamp_evals(bool fp32, const ModampSlots& m, const int* kslot, int ncand, const double2* part64,
+// out[4c..4c+3] = (xr, xi, sx, sy) (fp32: the float sums).  out may be
+// host memory: the writes are made visible system-wide before the kernel ends.
+void launch_modamp_evals(bool fp32, const ModampSlots& m, const CandSet& c, int ncand, const double2* part64,
                          const float4* part32, int nparts, int zblocks, double* out, hipStream_t s);
+// Writes seq to *flag (host memory) once the work queued before it is done,
+// with system-scope ordering: the host polls the flag instead of an event.
+void launch_fit_signal(unsigned* flag, unsigned seq, hipStream_t s);
 // findrealspacemodamp's result (gpuFunctionsImpl.cu:1128-1147) on the device,
 // for each of ncand candidates: the partials summed in block order as the
 // host does (cpuReduce, gpuFunctionsImpl_hh.cu:162-180), then
@@ -208,6 +222,10 @@ struct AsmParams {
   float* out;               // [zdim][ydim][xdim]
   float fact, dxyo;
   float2 k0[kMaxDirs];
+  // the filter's lateral support: it leaves zeros wherever
+  // (x dkx)^2 + (y dky)^2 exceeds rdistcutoff^2; rc2_zero is that square with
+  // a margin (0: unknown, nothing is skipped)
+  float dkx, dky, rc2_zero;
 };
 void launch_asm_z(const AsmParams& p, hipStream_t s);
 void launch_asm_y(const AsmParams& p, hipStream_t s);

@@ -169,13 +169,39 @@ __global__ __launch_bounds__(asm_zp_threads<B0>()) void k_asm_z_pair(AsmParams p
   __shared__ float2 lds[S * W * LP];
   FftTw<N> twr;
   __shared__ float2 twstage[N > 1 ? N : 1];
-  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
   unsigned bx, by, bz;
   xcd_tile3(bx, by, bz);
   const int a = B0 ? 0 : 1 + (int)bz;
   const int s = (int)by, sp = s == 0 ? 0 : p.ny - s;  // stored rows s and -s
   const int x0 = (int)bx * W;
   const int xmax = p.nx / 2;  // stored columns 0 .. nx/2 feed the output
+  const int ycs = s + p.ny / 2 - 1;                  // output row of centred y = s
+  const int ycp = (s == 0 ? 0 : -s) + p.ny / 2 - 1;  // output row of centred y = -s
+  const int ycp2 = s == p.ny / 2 ? ycs : ycp;        // s = ny/2 is its own mirror
+  {
+    // The tile's columns lie at |kx| >= x0 dkx on rows |ky| = s dky: when
+    // even the nearest is beyond the filter's support, every input is zero
+    // and so is every output (written without reading or transforming).
+    const float kx = (float)x0 * p.dkx, ky = (float)s * p.dky;
+    if (p.rc2_zero > 0.f && kx * kx + ky * ky > p.rc2_zero) {
+      for (int idx = threadIdx.x; idx < W * N; idx += NT) {
+        const int w = idx % W, zout = idx / W;
+        const int xs = x0 + w;
+        if (xs > xmax) continue;
+        float2* plane = p.stageZ + ((long long)a * N + zout) * p.ny * p.nx;
+        const int xd = xs + p.nx / 2 - 1;
+        plane[(long long)ycs * p.nx + xd] = make_float2(0.f, 0.f);
+        plane[(long long)ycp2 * p.nx + xd] = make_float2(0.f, 0.f);
+        if (xs >= 1 && xs <= xmax - 1) {
+          const int xm = p.nx / 2 - 1 - xs;
+          plane[(long long)ycp2 * p.nx + xm] = make_float2(0.f, 0.f);
+          plane[(long long)ycs * p.nx + xm] = make_float2(0.f, 0.f);
+        }
+      }
+      return;  // uniform over the workgroup
+    }
+  }
+  load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
   const long long dirstride = (long long)p.nbands * p.band_stride;
   const float2* b0;
   const float2* b1;
@@ -231,9 +257,6 @@ __global__ __launch_bounds__(asm_zp_threads<B0>()) void k_asm_z_pair(AsmParams p
     if (threadIdx.x + k * NT < E) lds[slot[k]] = v[k];
   __syncthreads();
   fft_lds<N, true>(lds + (threadIdx.x / TPT) * LP, threadIdx.x % TPT, twr);
-  const int ycs = s + p.ny / 2 - 1;                       // output row of centred y = s
-  const int ycp = (s == 0 ? 0 : -s) + p.ny / 2 - 1;       // output row of centred y = -s
-  const int ycp2 = s == p.ny / 2 ? ycs : ycp;             // s = ny/2 is its own mirror
   for (int idx = threadIdx.x; idx < W * N; idx += NT) {
     const int w = idx % W, zout = idx / W;
     const int xs = x0 + w;

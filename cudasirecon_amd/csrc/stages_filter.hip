@@ -206,19 +206,20 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
   }
 }
 
-// Workgroup (blockIdx.x, iz = blockIdx.y): kFiltPer*256 stored elements
-// (jin, iin) of plane iz of the band pair.  The plane's radial OTF profiles
-// at kz = +z0 and kz = -z0 (the conjugate visit) are built once in LDS.
-// Planes whose z lies outside +-zd are zeroed (kernel3).
+// Workgroup (blockIdx.x, iz = blockIdx.y): per*256 stored elements
+// (jin, iin) of plane iz of the band pair (per = 4, or 1 when the band is too
+// small to fill the device that way).  The plane's radial OTF profiles at
+// kz = +z0 and kz = -z0 (the conjugate visit) are built once in LDS.  Planes
+// whose z lies outside +-zd are zeroed (kernel3).
 constexpr int kFiltPer = 4;
 
 template <int ND, int NO, bool UNV, bool RA>
-__global__ __launch_bounds__(256) void k_filter(FilterParams p) {
+__global__ __launch_bounds__(256) void k_filter(FilterParams p, int per) {
 #pragma clang fp contract(off)
   extern __shared__ float2 prof[];  // [2][norders][nxotf]
   const long long nplane = (long long)p.nxh * p.ny;
   const int iz = blockIdx.y;
-  const long long ep0 = (long long)blockIdx.x * (256 * kFiltPer) + threadIdx.x;
+  const long long ep0 = (long long)blockIdx.x * (256 * per) + threadIdx.x;
   const int zd = p.zd[p.order];
   float2* bp = p.bp;
   float2* bp2 = p.bp2;
@@ -229,8 +230,7 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
   else if (iz >= p.nz - zd)
     z0n = iz - p.nz;
   else {
-#pragma unroll
-    for (int k = 0; k < kFiltPer; ++k) {
+    for (int k = 0; k < per; ++k) {
       const long long ep = ep0 + k * 256;
       if (ep < nplane) {
         const long long e = (long long)iz * nplane + ep;
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
     }
     __syncthreads();
   }
-  for (int k = 0; k < kFiltPer; ++k) {
+  for (int k = 0; k < per; ++k) {
     const long long ep = ep0 + k * 256;
     if (ep >= nplane) break;
     const long long e = (long long)iz * nplane + ep;
@@ -289,32 +289,35 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p) {
 }
 
 template <bool RA>
-static void launch_filter_kind(const FilterParams& p, size_t lds, dim3 grid, hipStream_t s) {
+static void launch_filter_kind(const FilterParams& p, size_t lds, dim3 grid, int per, hipStream_t s) {
   const bool unv = p.unit_nv != 0;
   if (p.ndirs == 3 && p.norders == 3) {  // the 3-beam 3D case
     if (unv)
-      hipLaunchKernelGGL((k_filter<3, 3, true, RA>), grid, dim3(256), lds, s, p);
+      hipLaunchKernelGGL((k_filter<3, 3, true, RA>), grid, dim3(256), lds, s, p, per);
     else
-      hipLaunchKernelGGL((k_filter<3, 3, false, RA>), grid, dim3(256), lds, s, p);
+      hipLaunchKernelGGL((k_filter<3, 3, false, RA>), grid, dim3(256), lds, s, p, per);
   } else if (p.ndirs == 3 && p.norders == 2) {  // 2D, 3 phases
     if (unv)
-      hipLaunchKernelGGL((k_filter<3, 2, true, RA>), grid, dim3(256), lds, s, p);
+      hipLaunchKernelGGL((k_filter<3, 2, true, RA>), grid, dim3(256), lds, s, p, per);
     else
-      hipLaunchKernelGGL((k_filter<3, 2, false, RA>), grid, dim3(256), lds, s, p);
+      hipLaunchKernelGGL((k_filter<3, 2, false, RA>), grid, dim3(256), lds, s, p, per);
   } else {
-    hipLaunchKernelGGL((k_filter<0, 0, false, RA>), grid, dim3(256), lds, s, p);
+    hipLaunchKernelGGL((k_filter<0, 0, false, RA>), grid, dim3(256), lds, s, p, per);
   }
 }
 
 void launch_filter(const FilterParams& p, hipStream_t s) {
   const long long nplane = (long long)p.nxh * p.ny;
-  const dim3 grid(ceil_div(nplane, 256 * kFiltPer), p.nz);
+  // four elements per thread amortise the plane's profile build; a band
+  // that would then give fewer than 2048 workgroups takes one per thread
+  const int per = ceil_div(nplane, 256 * kFiltPer) * p.nz >= 2048 ? kFiltPer : 1;
+  const dim3 grid(ceil_div(nplane, 256 * per), p.nz);
   if (p.otf[0].ra) {
     const size_t lds = (size_t)2 * p.norders * p.otf[0].nxotf * sizeof(float2);
     if (lds > 64 * 1024) throw ArgError("OTF table too long for the filter's LDS profile");
-    launch_filter_kind<true>(p, lds, grid, s);
+    launch_filter_kind<true>(p, lds, grid, per, s);
   } else {
-    launch_filter_kind<false>(p, 0, grid, s);
+    launch_filter_kind<false>(p, 0, grid, per, s);
   }
   SIMR_LAUNCH_CHECK();
 }

@@ -173,10 +173,31 @@ __device__ __forceinline__ float2 ov_combine(const OvSrc& o, float2 vre, float2 
   return make_float2(vre.x - vim.y, vre.y + vim.x);
 }
 
-// x pass: rows [R*blockIdx.x, +R) of plane zb (blockIdx.y) of both overlaps
-// of slot blockIdx.z, R = ov_rows(N) (4, fewer when 2R transforms would
-// exceed a workgroup or 96 KB of LDS, which leaves 64 KB for the profiles);
-// the plane's two radial OTF profiles are built once in LDS.
+// Whether row i of overlap `which` can hold a nonzero element: an element at
+// lateral frequency (x, y) is nonzero only if |(x, y)| and |(x, y) - s| are
+// both within rdistcutoff (s = (kx, ky) for overlap0, -(kx, ky) for overlap1;
+// no wrap-around), so the row meets both discs iff |sx| <= a + b with
+// a^2 = rc^2 - y^2 and b^2 = rc^2 - (y - sy)^2.  The radii carry a relative
+// margin of 1e-4, far beyond the rounding of the per-element tests, so a row
+// this rejects is zero.
+__device__ __forceinline__ bool ov_row_live(const OvParams& p, int which, int i, int z0) {
+  if (i >= p.ny || (z0 == 0 && p.bNoKz0)) return false;
+  float y = (float)i;
+  if (y > (float)(p.ny / 2)) y -= (float)p.ny;
+  y *= p.dky;
+  const float sx = which == 0 ? p.kx : -p.kx, sy = which == 0 ? p.ky : -p.ky;
+  const float r2 = p.rc2 * 1.0002f;
+  const float a2 = r2 - y * y, b2 = r2 - (y - sy) * (y - sy);
+  if (a2 < 0.f || b2 < 0.f) return false;
+  return fabsf(sx) <= (sqrtf(a2) + sqrtf(b2)) * 1.0001f;
+}
+
+// x pass: rows [R*(blockIdx.x/2), +R) of overlap blockIdx.x%2 on plane zb
+// (blockIdx.y) of slot blockIdx.z, R = ov_rows(N) (4, fewer when 2R
+// transforms would exceed a workgroup or 96 KB of LDS); the plane's two
+// radial OTF profiles are built once in LDS.
+// A workgroup whose rows all lie outside the overlap's support (about a third
+// of them at 3D-SIM geometries) writes zeros and returns.
 constexpr int kOvStaticLds = 96 * 1024;
 constexpr int ov_rows(int N) {
   int r = 4;
@@ -185,30 +206,46 @@ constexpr int ov_rows(int N) {
 }
 
 template <int N, bool RA>
-__global__ __launch_bounds__(2 * ov_rows(N) * fft_tpt(N)) void k_overlaps_x(OvBatch b, const float2* __restrict__ tw) {
+__global__ __launch_bounds__(ov_rows(N) * fft_tpt(N)) void k_overlaps_x(OvBatch b, const float2* __restrict__ tw) {
   constexpr int kOvRows = ov_rows(N);
-  static_assert((2 * kOvRows * lds_pitch(N) + N) * 8 <= kOvStaticLds, "rows and twiddles leave room for the profiles");
+  static_assert((kOvRows * lds_pitch(N) + N) * 8 <= kOvStaticLds, "rows and twiddles leave room for the profiles");
   const OvParams& p = b.p[blockIdx.z];
   const int zb = blockIdx.y;
   if (zb > 2 * p.zd) return;  // a slot with a narrower z support (whole block)
   float2* __restrict__ ovx = p.ovx;
   const int nzs = b.nzs_stride;
   constexpr int TPT = fft_tpt(N);
-  constexpr int NT = 2 * kOvRows * TPT;
+  constexpr int RW = kOvRows;  // rows per workgroup
+  constexpr int NT = RW * TPT;
   constexpr int P = lds_pitch(N);
-  __shared__ float2 lds[2 * kOvRows * P];
+  constexpr int EPT = (RW * N + NT - 1) / NT;
+  const int which = blockIdx.x & 1;
+  const int i0 = (blockIdx.x >> 1) * RW;
+  const int z0 = zb - p.zd;
+  float2* __restrict__ orow = ovx + (((long long)which * nzs + zb) * p.ny + i0) * p.nx;
+  {
+    bool live = false;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) live = live || ov_row_live(p, which, i0 + r, z0);
+    if (!live) {
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        if (idx < RW * N && i0 + idx / N < p.ny) orow[idx] = make_float2(0.f, 0.f);
+      }
+      return;  // uniform over the workgroup
+    }
+  }
+  __shared__ float2 lds[RW * P];
   extern __shared__ float2 prof[];  // [2][nxotf]
   FftTw<N> twr;
   __shared__ float2 twstage[N > 1 ? N : 1];
   load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
-  const int z0 = zb - p.zd;
   const int nxotf = p.otf1.nxotf;
   if constexpr (RA)
     for (int k = threadIdx.x; k < 2 * nxotf; k += NT)
       prof[k] = otf_profile_entry(k < nxotf ? p.otf1 : p.otf2, k < nxotf ? k : k - nxotf, z0, p.kzscale);
   __syncthreads();
-  const int i0 = blockIdx.x * kOvRows;
-  constexpr int EPT = (2 * kOvRows * N + NT - 1) / NT;
   // In chunks of CH elements per thread: the sources of a chunk first, then
   // all its band loads (clamped to a valid index for zero elements) at
   // once, then the products.  Chunks keep the registers (and so the
@@ -220,28 +257,25 @@ __global__ __launch_bounds__(2 * ov_rows(N) * fft_tpt(N)) void k_overlaps_x(OvBa
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int idx = threadIdx.x + (q0 + c) * NT;
-      const int t = idx / N, j = idx - (idx / N) * N;  // t = which * kOvRows + row
-      const int which = t / kOvRows, i = i0 + (t - which * kOvRows);
-      src[c] = (q0 + c < EPT && idx < 2 * kOvRows * N && i < p.ny)
-                   ? ov_source<RA>(p, prof, prof + nxotf, which, j, i, z0)
-                   : OvSrc{0, false, false, make_float2(0.f, 0.f)};
+      const int t = idx / N, j = idx - (idx / N) * N;  // t = row in the workgroup
+      const int i = i0 + t;
+      src[c] = (q0 + c < EPT && idx < RW * N && i < p.ny) ? ov_source<RA>(p, prof, prof + nxotf, which, j, i, z0)
+                                                          : OvSrc{0, false, false, make_float2(0.f, 0.f)};
     }
+    const float2* bre = which == 0 ? p.b1re : p.b2re;
+    const float2* bim = which == 0 ? (p.order1 > 0 ? p.b1im : p.b1re) : p.b2im;
     float2 vre[CH], vim[CH];
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      const int idx = threadIdx.x + (q0 + c) * NT;
-      const int which = (idx / N) / kOvRows < 2 ? (idx / N) / kOvRows : 1;
-      const float2* bre = which == 0 ? p.b1re : p.b2re;
-      const float2* bim = which == 0 ? (p.order1 > 0 ? p.b1im : p.b1re) : p.b2im;
       vre[c] = bre[src[c].ind];
       vim[c] = bim[src[c].ind];
     }
+    const bool has_im = which == 1 || p.order1 > 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const int idx = threadIdx.x + (q0 + c) * NT;
-      if (q0 + c < EPT && idx < 2 * kOvRows * N) {
+      if (q0 + c < EPT && idx < RW * N) {
         const int t = idx / N, j = idx - (idx / N) * N;
-        const bool has_im = t >= kOvRows || p.order1 > 0;
         lds[t * P + lds_swz<N>(j)] = ov_combine(src[c], vre[c], vim[c], has_im);
       }
     }
@@ -251,10 +285,9 @@ __global__ __launch_bounds__(2 * ov_rows(N) * fft_tpt(N)) void k_overlaps_x(OvBa
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
     const int idx = threadIdx.x + q * NT;
-    if (idx < 2 * kOvRows * N) {
+    if (idx < RW * N) {
       const int t = idx / N, j = idx - (idx / N) * N;
-      const int which = t / kOvRows, i = i0 + (t - which * kOvRows);
-      if (i < p.ny) ovx[(((long long)which * nzs + zb) * p.ny + i) * p.nx + j] = lds[t * P + lds_swz<N>(j)];
+      if (i0 + t < p.ny) orow[idx] = lds[t * P + lds_swz<N>(j)];
     }
   }
 }
@@ -262,7 +295,7 @@ __global__ __launch_bounds__(2 * ov_rows(N) * fft_tpt(N)) void k_overlaps_x(OvBa
 template <int N>
 static void launch_ov_x(const OvBatch& b, int nzs_max, hipStream_t s) {
   const OvParams& p = b.p[0];
-  const dim3 grid(ceil_div(p.ny, ov_rows(N)), nzs_max, b.n), block(2 * ov_rows(N) * fft_tpt(N));
+  const dim3 grid(2 * ceil_div(p.ny, ov_rows(N)), nzs_max, b.n), block(ov_rows(N) * fft_tpt(N));
   if (p.otf1.ra) {
     const size_t lds = (size_t)2 * p.otf1.nxotf * sizeof(float2);
     if (lds > 160 * 1024 - kOvStaticLds) throw ArgError("OTF table too long for the overlap pass's LDS profiles");
@@ -509,6 +542,7 @@ __global__ __launch_bounds__(256) void k_argmax_final(const float2* __restrict__
   vals[3] = I(yc == 0 ? ny - 1 : yc - 1, xc);
   vals[4] = I(yc == ny - 1 ? 0 : yc + 1, xc);
   idx_out[0] = idx;
+  __threadfence_system();  // the outputs may be host memory
 }
 
 void launch_findpeak(const PeakBatch& b, float2* cc, float* partial_val, int* partial_idx, float* peak_vals,
@@ -533,15 +567,18 @@ constexpr int kModRows = 4;
 
 int modamp_partials_fp64(int nx, int ny) { return (ny + kModRows - 1) / kModRows; }
 
-__global__ __launch_bounds__(256) void k_modamp_fp64(ModampSlots m, const float2* __restrict__ kc,
-                                                     const int* __restrict__ kslot, int nx, int ny, float dxy,
+__device__ __forceinline__ float2 cand_k(const CandSet& s, int c) { return s.kc ? s.kc[c] : s.k[c]; }
+__device__ __forceinline__ int cand_slot(const CandSet& s, int c) { return s.ks ? s.ks[c] : s.slot[c]; }
+
+__global__ __launch_bounds__(256) void k_modamp_fp64(ModampSlots m, CandSet cs, int nx, int ny, float dxy,
                                                      double2* __restrict__ partials) {
   extern __shared__ double2 alpha[];  // nx phasors
   __shared__ double2 beta[kModRows];
   __shared__ double rx[256], ry[256];
   const int c = blockIdx.y;
-  const double2* __restrict__ P = m.P[kslot[c]];
-  const double kx = (double)kc[c].x, ky = (double)kc[c].y;
+  const double2* __restrict__ P = m.P[cand_slot(cs, c)];
+  const float2 kc = cand_k(cs, c);
+  const double kx = (double)kc.x, ky = (double)kc.y;
   const double twopi_dxy = 2.0 * 3.14159265358979323846 * (double)dxy;
   for (int i = threadIdx.x; i < nx; i += 256) {
     double sn, cs;
@@ -578,12 +615,17 @@ __global__ __launch_bounds__(256) void k_modamp_fp64(ModampSlots m, const float2
   if (threadIdx.x == 0) partials[(long long)c * gridDim.x + blockIdx.x] = make_double2(rx[0], ry[0]);
 }
 
-void launch_modamp_fp64(const ModampSlots& m, const float2* kcand, const int* kslot, int ncand, int nx, int ny,
-                        float dxy, double2* partials, hipStream_t s) {
-  if (ncand < 1 || ncand > 65535) throw ArgError("modamp: 1..65535 candidates per launch");
+static void check_cands(const CandSet& c, int ncand) {
+  if (ncand < 1 || (c.kc == nullptr && ncand > kArgCand) || ncand > 65535)
+    throw ArgError("modamp: 1..128 candidates per launch (by value), 1..65535 from device arrays");
+  if ((c.kc == nullptr) != (c.ks == nullptr)) throw ArgError("modamp: candidates and slots from the same source");
+}
+
+void launch_modamp_fp64(const ModampSlots& m, const CandSet& c, int ncand, int nx, int ny, float dxy,
+                        double2* partials, hipStream_t s) {
+  check_cands(c, ncand);
   dim3 grid(modamp_partials_fp64(nx, ny), ncand);
-  hipLaunchKernelGGL(k_modamp_fp64, grid, dim3(256), sizeof(double2) * nx, s, m, kcand, kslot, nx, ny, dxy,
-                     partials);
+  hipLaunchKernelGGL(k_modamp_fp64, grid, dim3(256), sizeof(double2) * nx, s, m, c, nx, ny, dxy, partials);
   SIMR_LAUNCH_CHECK();
 }
 
@@ -637,7 +679,7 @@ void launch_modamp_finish(bool fp32, const ModampSlots& m, const int* kslot, int
   SIMR_LAUNCH_CHECK();
 }
 
-__global__ __launch_bounds__(256) void k_modamp_evals(int fp32, ModampSlots m, const int* __restrict__ kslot,
+__global__ __launch_bounds__(256) void k_modamp_evals(int fp32, ModampSlots m, CandSet cs,
                                                       const double2* __restrict__ part64,
                                                       const float4* __restrict__ part32, int nparts, int zblocks,
                                                       double* __restrict__ out) {
@@ -652,19 +694,32 @@ __global__ __launch_bounds__(256) void k_modamp_evals(int fp32, ModampSlots m, c
       return (double)(k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w);
     });
     if (threadIdx.x < 4) out[threadIdx.x] = v;
+    __threadfence_system();
     return;
   }
   const double2* pp = part64 + (long long)c * nparts;
   const double v = seq_sum_lanes<double, 2>(nparts, stage, [&](int i, int k) { return k == 0 ? pp[i].x : pp[i].y; });
   if (threadIdx.x < 2) out[threadIdx.x] = v;
-  if (threadIdx.x == 2 || threadIdx.x == 3) out[threadIdx.x] = m.totals[kslot[c]][kZsumFinal + threadIdx.x - 2];
+  if (threadIdx.x == 2 || threadIdx.x == 3)
+    out[threadIdx.x] = m.totals[cand_slot(cs, c)][kZsumFinal + threadIdx.x - 2];
+  __threadfence_system();
 }
 
-void launch_modamp_evals(bool fp32, const ModampSlots& m, const int* kslot, int ncand, const double2* part64,
+void launch_modamp_evals(bool fp32, const ModampSlots& m, const CandSet& c, int ncand, const double2* part64,
                          const float4* part32, int nparts, int zblocks, double* out, hipStream_t s) {
-  if (ncand < 1) return;
-  hipLaunchKernelGGL(k_modamp_evals, dim3(ncand), dim3(256), 0, s, fp32 ? 1 : 0, m, kslot, part64, part32, nparts,
+  check_cands(c, ncand);
+  hipLaunchKernelGGL(k_modamp_evals, dim3(ncand), dim3(256), 0, s, fp32 ? 1 : 0, m, c, part64, part32, nparts,
                      zblocks, out);
+  SIMR_LAUNCH_CHECK();
+}
+
+__global__ void k_fit_signal(unsigned* flag, unsigned seq) {
+  __threadfence_system();
+  __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void launch_fit_signal(unsigned* flag, unsigned seq, hipStream_t s) {
+  hipLaunchKernelGGL(k_fit_signal, dim3(1), dim3(1), 0, s, flag, seq);
   SIMR_LAUNCH_CHECK();
 }
 
@@ -712,17 +767,17 @@ void launch_amp_policy(const AmpPolicy& a, float2* amps, hipStream_t s) {
 // block-index order as cpuReduce does.
 int modamp_partials_fp32(int nx, int ny) { return ((nx + 63) / 64) * ((ny + 3) / 4); }
 
-__global__ __launch_bounds__(256) void k_modamp_fp32(ModampSlots m, const float2* __restrict__ kc,
-                                                     const int* __restrict__ kslot, int nx, int ny, float dxy,
+__global__ __launch_bounds__(256) void k_modamp_fp32(ModampSlots m, CandSet cand, int nx, int ny, float dxy,
                                                      float4* __restrict__ partials) {
 #pragma clang fp contract(off)
   __shared__ float bx[256], by[256], bs[256], bt[256];
   const int c = blockIdx.z;
-  const int slot = kslot[c];
+  const int slot = cand_slot(cand, c);
   const double2* __restrict__ P = m.P[slot];
   const double* __restrict__ SX = m.SX[slot];
   const double* __restrict__ SY = m.SY[slot];
-  const float k0x = kc[c].x, k0y = kc[c].y;
+  const float2 kc = cand_k(cand, c);
+  const float k0x = kc.x, k0y = kc.y;
   const int i = threadIdx.x + blockIdx.x * 64;
   const int j = threadIdx.y + blockIdx.y * 4;
   const float a = ((float)i - 0.5f * (float)nx) * k0x;
@@ -761,11 +816,11 @@ __global__ __launch_bounds__(256) void k_modamp_fp32(ModampSlots m, const float2
   }
 }
 
-void launch_modamp_fp32(const ModampSlots& m, const float2* kcand, const int* kslot, int ncand, int nx, int ny,
-                        float dxy, float4* partials, hipStream_t s) {
-  if (ncand < 1 || ncand > 65535) throw ArgError("modamp: 1..65535 candidates per launch");
+void launch_modamp_fp32(const ModampSlots& m, const CandSet& c, int ncand, int nx, int ny, float dxy,
+                        float4* partials, hipStream_t s) {
+  check_cands(c, ncand);
   dim3 grid((nx + 63) / 64, (ny + 3) / 4, ncand);
-  hipLaunchKernelGGL(k_modamp_fp32, grid, dim3(64, 4), 0, s, m, kcand, kslot, nx, ny, dxy, partials);
+  hipLaunchKernelGGL(k_modamp_fp32, grid, dim3(64, 4), 0, s, m, c, nx, ny, dxy, partials);
   SIMR_LAUNCH_CHECK();
 }
 
