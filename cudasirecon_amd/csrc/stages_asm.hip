@@ -24,6 +24,15 @@
 
 namespace simr {
 
+// Whether centred column xc (lateral frequency (xc - (nx/2 - 1)) dkx) lies
+// beyond the filter's support for every y, so that every value of it, in
+// every stage of the assembly, is zero.  The y pass writes zeros for tiles of
+// such columns without reading or transforming them.
+__device__ __forceinline__ bool asm_col_dead(const AsmParams& p, int xc) {
+  const float kx = (float)(xc - (p.nx / 2 - 1)) * p.dkx;
+  return p.rc2_zero > 0.f && kx * kx > p.rc2_zero;
+}
+
 // move_kernel's map: occupied output coordinate x = xc - (nx/2 - 1), xc < nx.
 __device__ __forceinline__ int z_of_zout(int zout, int nz, int zdim, bool& valid) {
   valid = true;
@@ -351,6 +360,19 @@ __global__ __launch_bounds__(asm_y_w<N>() * asm_y_tpt<N>()) void k_asm_y(AsmPara
   xcd_tile3(bx, by, bz);
   const int a = (int)bz, zout = (int)by;
   const int xc0 = (int)bx * W;
+  {
+    // the tile's column nearest to the centre decides: the dead columns lie
+    // at both ends of the row (uniform over the workgroup)
+    const int c = p.nx / 2 - 1 < xc0 ? xc0 : (p.nx / 2 - 1 > xc0 + W - 1 ? xc0 + W - 1 : p.nx / 2 - 1);
+    if (asm_col_dead(p, c)) {
+      float2* dst = p.stageY + ((long long)a * p.zdim + zout) * (long long)N * p.nx;
+      for (int idx = threadIdx.x; idx < N * W; idx += NT) {
+        const int Y = idx / W, w = idx - (idx / W) * W;
+        if (xc0 + w < p.nx) dst[(long long)Y * p.nx + xc0 + w] = make_float2(0.f, 0.f);
+      }
+      return;
+    }
+  }
   const float2* src = p.stageZ + ((long long)a * p.zdim + zout) * p.ny * p.nx;
   // Source row sr goes to Y = sr - hy (sr >= hy) or Y = sr + N - hy (sr < hy);
   // the rows between are zero.  Loads run over source rows in batches of B
