@@ -3,9 +3,11 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 
+#include "dtrig.hpp"
 #include "engine.hpp"
 #include "fft_api.hpp"
 #include "sizes.hpp"
@@ -26,7 +28,8 @@ constexpr size_t kZcEval = 0;
 constexpr size_t kZcPeakV = kZcEval + (size_t)kZcCand * 32;
 constexpr size_t kZcPeakI = kZcPeakV + (size_t)kMaxDirs * 32;
 constexpr size_t kZcFlag = kZcPeakI + 256;
-constexpr size_t kZcBytes = kZcFlag + 256;
+constexpr size_t kZcFit = kZcFlag + 256;  // FitOut of the device-driven fit
+constexpr size_t kZcBytes = kZcFit + ((sizeof(FitOut) + 255) & ~(size_t)255);
 
 int norders_of(const simr_params& p) { return p.norders_output != 0 ? p.norders_output : p.nphases / 2 + 1; }
 
@@ -135,6 +138,7 @@ size_t fit_block_bytes(const Layout& L) {
   b += align256((size_t)kMaxCand * modamp_partials_fp64(L.nx, L.ny) * 16);
   b += align256((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
   b += 2 * align256((size_t)kMaxDirs * kArgBlocks * 4);
+  b += align256(sizeof(FitState));  // device-driven fit
   return b;
 }
 // The overlap dumps' scratch, in its own region: one overlap stage, P, SX,
@@ -214,6 +218,12 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   SIMR_HIP(hipSetDevice(device_));
   SIMR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   fit_fp32_ = P.fit_fp64 == 0;
+  // SIMR_FIT_ON_HOST=1: the line searches replay on the host with a read-back
+  // per round (the device-driven fit's reference in tests)
+  if (const char* e = std::getenv("SIMR_FIT_ON_HOST")) device_fit_ = !(e[0] == '1' && e[1] == 0);
+  // SIMR_FIT_ROUNDS=n: the device-driven fit's first rounds (tests of the
+  // fall-back to the host walk when they are too few)
+  if (const char* e = std::getenv("SIMR_FIT_ROUNDS")) fit_rounds_ = std::min(kFitMaxRounds, std::max(1, std::atoi(e)));
   // every twiddle table this context's transforms use (rows, columns, z, the
   // zoomed assembly lengths), so that no launch path allocates one
   for (int n : {L.nx, L.ny, L.nz, L.xdim, L.ydim, L.zdim})
@@ -268,6 +278,7 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
     part32_ = (float4*)take((size_t)kMaxCand * modamp_partials_fp32(L.nx, L.ny) * 16);
     pv_ = (float*)take((size_t)kMaxDirs * kArgBlocks * 4);
     pi_ = (int*)take((size_t)kMaxDirs * kArgBlocks * 4);
+    fitst_ = (FitState*)take(sizeof(FitState));
     SIMR_REQUIRE((size_t)(at - stage) <= stage_bytes_, "fit scratch larger than the staging region");
   }
   {
@@ -686,8 +697,11 @@ void Engine::make_overlaps(const std::vector<OvReq>& reqs, bool per_elem, FitWor
     SIMR_REQUIRE(r.dir >= 0 && r.dir < L.ndirs, "direction out of range");
     FitWork& w = only ? *only : fw_[r.dir];
     for (int j = 0; j < k; ++j) SIMR_REQUIRE(reqs[j].dir != r.dir, "overlap batch: one request per direction");
+    SIMR_REQUIRE(r.k0d == nullptr || !P.bBessel, "device-side overlap shifts need a fixed z support");
     b.p[k] = ov_params(r.dir, r.o1, r.o2, r.k0x, r.k0y);
     b.p[k].ovx = w.ovx;
+    b.p[k].k0d = r.k0d;
+    b.p[k].kmul = r.kmul;
     z.ovx[k] = w.ovx;
     z.P[k] = w.P;
     z.SX[k] = w.SX;
@@ -728,15 +742,13 @@ ModampSlots Engine::modamp_slots() const {
 
 // Queues the modamp reductions of n <= kMaxCand candidates (direction,
 // vector) at entries [at, at + n) of the zero-copy region: the candidates
-// reach the device in one copy, and the partials are summed per candidate in
+// travel in the kernel arguments, and the partials are summed per candidate in
 // block order on the device as the host does (cpuReduce,
 // gpuFunctionsImpl_hh.cu:162-180), straight into the region.  The sums are
 // read (eval_read) after the next wait_fit().
 void Engine::eval_enqueue(const Cand* c, int n, int at) {
   SIMR_REQUIRE(n >= 1 && n <= kMaxCand && at >= 0 && at + n <= kZcCand, "modamp batch out of range");
-  CandSet cs;
-  cs.kc = nullptr;
-  cs.ks = nullptr;
+  CandSet cs{};  // every pointer null: candidates by value, sums by k_modamp_evals
   for (int i = 0; i < n; ++i) {
     SIMR_REQUIRE(c[i].first >= 0 && c[i].first < L.ndirs, "direction out of range");
     cs.k[i] = c[i].second;
@@ -754,7 +766,7 @@ void Engine::eval_enqueue(const Cand* c, int n, int at) {
     launch_modamp_fp32(m, cs, n, L.nx, L.ny, P.dxy, part32_, stream_);
   else
     launch_modamp_fp64(m, cs, n, L.nx, L.ny, P.dxy, part64_, stream_);
-  launch_modamp_evals(fit_fp32_, m, cs, n, part64_, part32_, nb, zsum_blocks(L.nxy), dev, stream_);
+  launch_modamp_evals(fit_fp32_, m, cs, n, part64_, part32_, nb, zsum_blocks(L.nxy), dev, true, stream_);
   stage_end(st, stream_);
 }
 
@@ -812,7 +824,7 @@ static std::pair<uint32_t, uint32_t> key_of(float a, float b) {
 
 static float2 kvec_of(float angle, float mag, int o1, int o2) {
   // getmodamp (gpuFunctionsImpl.cu:1058-1059) then findrealspacemodamp (1093-1094)
-  const float kx = mag * cosf(angle), ky = mag * sinf(angle);
+  const float kx = mag * dtrig::fcos(angle), ky = mag * dtrig::fsin(angle);
   return make_float2(kx * (float)(o2 - o1), ky * (float)(o2 - o1));
 }
 
@@ -1042,8 +1054,8 @@ void Engine::fitk0_all(const std::vector<int>& dirs) {
     for (int d : dirs) {
       const float2 kv = k0_[d];
       k0mag[d] = sqrtf(kv.x * kv.x + kv.y * kv.y);
-      k0angle[d] = atan2f(kv.y, kv.x);
-      reqs.push_back({d, fo1, fo2, k0mag[d] * cosf(k0angle[d]), k0mag[d] * sinf(k0angle[d])});
+      k0angle[d] = dtrig::fatan2(kv.y, kv.x);
+      reqs.push_back({d, fo1, fo2, k0mag[d] * dtrig::fcos(k0angle[d]), k0mag[d] * dtrig::fsin(k0angle[d])});
     }
     make_overlaps(reqs);
   }
@@ -1093,7 +1105,7 @@ void Engine::fitk0_all(const std::vector<int>& dirs) {
   std::vector<int> fin_dir;
   std::vector<Eval> final_ev(L.ndirs);  // the next overlaps clear the caches
   for (int d : dirs) {
-    k0_[d] = make_float2(mag[d] * cosf(angle[d]), mag[d] * sinf(angle[d]));
+    k0_[d] = make_float2(mag[d] * dtrig::fcos(angle[d]), mag[d] * dtrig::fsin(angle[d]));
     auto it = fw_[d].cache.find(key_of(angle[d], mag[d]));
     if (it != fw_[d].cache.end()) {
       final_ev[d] = it->second;
@@ -1130,6 +1142,116 @@ void Engine::fitk0_all(const std::vector<int>& dirs) {
   for (size_t k = 1; k < others.size(); ++k) realspacemodamp_multi(dirs, others[k].first, others[k].second);
 }
 
+// findk0_all + fitk0_all + the other orders' modamps with the decisions on
+// the device: the cross-correlation peaks are refined there, the fit's
+// overlaps read their shift from device memory, fit_rounds_ (evaluate, walk)
+// rounds run the line searches, and the other orders' overlaps and modamps
+// follow at the fitted k0; the host waits once, for the results.  false (the
+// host path then runs instead) when the walks needed more rounds than were
+// queued.
+bool Engine::fit_on_device() {
+  const int fo2 = fit_order2();
+  const Geo g = geometry();
+  FitArgs a{};
+  a.ndirs = L.ndirs;
+  a.norders = L.norders;
+  a.fo2 = fo2;
+  a.fp32 = fit_fp32_ ? 1 : 0;
+  a.nx = L.nx;
+  a.ny = L.ny;
+  a.dkx = g.dkx;
+  a.dky = g.dky;
+  a.deltaangle = 0.001f;
+  a.deltamag = (float)(0.1 / (double)((float)std::max(L.nx, L.ny) * P.dxy));
+  for (int d = 0; d < L.ndirs; ++d) a.guess[d] = k0_[d];
+  // findk0's overlaps at the guesses and their cross-correlation peaks
+  std::vector<OvReq> reqs;
+  for (int d = 0; d < L.ndirs; ++d) reqs.push_back({d, 0, fo2, k0_[d].x, k0_[d].y});
+  make_overlaps(reqs);
+  PeakBatch pb{};
+  pb.n = L.ndirs;
+  for (int k = 0; k < pb.n; ++k) pb.P[k] = fw_[k].P;
+  int st = stage_begin("findk0_peak", stream_);
+  launch_findpeak(pb, fw_[0].cc, pv_, pi_, fitst_->peak_vals, fitst_->peak_idx, L.nx, L.ny, stream_);
+  launch_fit_start(fitst_, a, stream_);
+  stage_end(st, stream_);
+  // the fit's overlaps at every direction's start vector, then the rounds
+  reqs.clear();
+  for (int d = 0; d < L.ndirs; ++d) reqs.push_back({d, 0, fo2, 0.f, 0.f, &fitst_->d[d].kov, (float)fo2});
+  make_overlaps(reqs);
+  const ModampSlots ms = modamp_slots();
+  launch_fit_walk(fitst_, a, amps_dev_, corr_dev_, stream_);
+  CandSet cs{};
+  cs.kc = fitst_->kc;
+  cs.ks = fitst_->ks;
+  cs.nd = &fitst_->ncand;
+  const int np = fit_fp32_ ? modamp_partials_fp32(L.nx, L.ny) : modamp_partials_fp64(L.nx, L.ny);
+  const int rounds = fit_rounds_;
+  for (int r = 0; r < rounds; ++r) {
+    st = stage_begin("modamp", stream_);
+    if (fit_fp32_)
+      launch_modamp_fp32(ms, cs, kFitReq * L.ndirs, L.nx, L.ny, P.dxy, part32_, stream_);
+    else
+      launch_modamp_fp64(ms, cs, kFitReq * L.ndirs, L.nx, L.ny, P.dxy, part64_, stream_);
+    launch_modamp_evals(fit_fp32_, ms, cs, kFitReq * L.ndirs, part64_, part32_, np, zsum_blocks(L.nxy),
+                        &fitst_->ev[0][0], false, stream_);
+    launch_fit_walk(fitst_, a, amps_dev_, corr_dev_, stream_);
+    stage_end(st, stream_);
+  }
+  // the other orders' modamps at the fitted k0 (as fitk0_all)
+  std::vector<std::pair<int, int>> others;
+  if (L.nz == 1) {
+    for (int order = 2; order < L.norders; ++order) others.push_back({order - 1, order});
+  } else {
+    for (int order = 1; order < L.norders; ++order)
+      if (order != fo2) others.push_back({0, order});
+  }
+  const int nparts = fit_fp32_ ? modamp_partials_fp32(L.nx, L.ny) : modamp_partials_fp64(L.nx, L.ny);
+  for (auto& oo : others) {
+    const float mult = (float)(oo.second - oo.first);
+    reqs.clear();
+    for (int d = 0; d < L.ndirs; ++d) reqs.push_back({d, oo.first, oo.second, 0.f, 0.f, &fitst_->d[d].kov, mult});
+    make_overlaps(reqs);
+    float2* kc = kc_dev_ + (size_t)oo.second * L.ndirs;
+    launch_fit_kc(fitst_, L.ndirs, mult, kc, stream_);
+    CandSet c2{};
+    c2.kc = kc;
+    c2.ks = slot_ids_;
+    st = stage_begin("modamp", stream_);
+    const ModampSlots m = modamp_slots();
+    if (fit_fp32_)
+      launch_modamp_fp32(m, c2, L.ndirs, L.nx, L.ny, P.dxy, part32_, stream_);
+    else
+      launch_modamp_fp64(m, c2, L.ndirs, L.nx, L.ny, P.dxy, part64_, stream_);
+    launch_modamp_finish(fit_fp32_, m, slot_ids_, L.ndirs, part64_, part32_, nparts, zsum_blocks(L.nxy), amps_dev_,
+                         corr_dev_, L.norders, oo.second, stream_);
+    stage_end(st, stream_);
+  }
+  FitOut* out = (FitOut*)(dzc_ + kZcFit);
+  launch_fit_publish(fitst_, a, amps_dev_, corr_dev_, out, stream_);
+  wait_fit();
+  const FitOut& o = *(const FitOut*)(hzc_ + kZcFit);
+  if (o.overflow || o.ndone < L.ndirs) {
+    fit_rounds_ = std::min(kFitMaxRounds, rounds + 2);
+    return false;
+  }
+  // queue what this fit needed, and one round more
+  fit_rounds_ = std::min(kFitMaxRounds, std::max(2, o.rounds + 1));
+  for (int d = 0; d < L.ndirs; ++d) {
+    k0_[d] = o.k0[d];
+    require_finite_k0(d, "the k0 search");
+    k0_[d] = o.kov[d];
+    stats_[d].n_angle_evals = o.n_angle_evals[d];
+    stats_[d].n_mag_evals = o.n_mag_evals[d];
+    stats_[d].n_reductions = o.n_reductions[d] + (int)others.size();
+    for (int order = 1; order < L.norders; ++order) {
+      amp_[d][order] = o.amps[d * L.norders + order];
+      corr_[d][order] = o.corr[d * L.norders + order];
+    }
+  }
+  return true;
+}
+
 // A non-finite k0 (the reference carries it on into NaN output) only comes
 // from non-finite or empty input: a NaN/Inf pixel spreads through the
 // transforms to every overlap product, and a stack with no modulated signal
@@ -1159,9 +1281,19 @@ void Engine::find_modulation() {
     if (!reuse) k0_[d] = k0guess_[d];
   }
   if (P.bSearchforvector && !reuse) {
-    findk0_all(all);
-    for (int d : all) require_finite_k0(d, "the k0 search");
-    fitk0_all(all);
+    // the device-driven fit covers every case but a later time point's search
+    // (whose fallback to time 0's k0 needs the host) and Bessel SIM (whose
+    // overlap z support follows |k0|)
+    const bool on_device = device_fit_ && !P.bBessel && !(P.ntimes > 1 && cur_t_ > 0) && fit_on_device();
+    if (!on_device) {
+      for (int d : all) {
+        stats_[d] = simr_fit{};
+        k0_[d] = k0guess_[d];
+      }
+      findk0_all(all);
+      for (int d : all) require_finite_k0(d, "the k0 search");
+      fitk0_all(all);
+    }
     std::vector<int> fallback;
     for (int d : all) {
       require_finite_k0(d, "the k0 fit");

@@ -106,6 +106,8 @@ class Engine {
   struct OvReq {
     int dir, o1, o2;
     float k0x, k0y;
+    const float2* k0d = nullptr;  // non-null: the shift is *k0d * kmul, read on the device
+    float kmul = 0.f;
   };
   using Walk = std::vector<std::pair<float, float>>;  // (angle, mag) candidates
 
@@ -132,6 +134,7 @@ class Engine {
                              std::vector<Eval>* pend = nullptr);
   void findk0_all(const std::vector<int>& dirs);
   void fitk0_all(const std::vector<int>& dirs);
+  bool fit_on_device();
   void require_finite_k0(int dir, const char* where) const;
   bool graph_eligible() const;
   void build_graph();
@@ -176,6 +179,12 @@ class Engine {
   char* hzc_ = nullptr;
   char* dzc_ = nullptr;
   unsigned fit_seq_ = 0;  // the last completion flag value queued
+  // device-driven fit (fit_on_device): its state, in the fit scratch; the
+  // (evaluate, walk) rounds queued per fit, adapted to the rounds the last
+  // fit needed
+  FitState* fitst_ = nullptr;
+  int fit_rounds_ = 4;
+  bool device_fit_ = true;
   float2 *stageZ_ = nullptr, *stageY_ = nullptr;
   float* resampled_ = nullptr;  // [nimg][ny][nx] resampled raw planes (aliases the staging region)
   float* out_ = nullptr;

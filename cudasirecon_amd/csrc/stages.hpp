@@ -70,6 +70,9 @@ struct OvParams {
   // host-derived: rdistcutoff^2, otfcutoff^2, order0_2_factor^2, 1/dkrotf^2 of otf1 and otf2
   float rc2, cut2, f2, inv_dkr2_1, inv_dkr2_2;
   float2* ovx;  // [2][nzs_stride][ny][nx]: overlap0 planes, then overlap1 planes
+  // non-null: (kx, ky) = *k0d * kmul, read on the device (a k0 fitted there)
+  const float2* k0d;
+  float kmul;
 };
 struct OvBatch {
   OvParams p[kMaxDirs];
@@ -139,6 +142,10 @@ struct CandSet {
   const int* ks;
   float2 k[kArgCand];
   int slot[kArgCand];
+  // non-null: the number of candidates, read on the device (blocks of
+  // candidates at or past it do nothing); the launch covers the most there
+  // can be
+  const int* nd;
 };
 int modamp_partials_fp64(int nx, int ny);
 int modamp_partials_fp32(int nx, int ny);
@@ -149,13 +156,77 @@ void launch_modamp_fp32(const ModampSlots& m, const CandSet& c, int ncand, int n
 // The sums the host makes of the modamp partials (cpuReduce,
 // gpuFunctionsImpl_hh.cu:162-180), on the device: for candidate c, its
 // partials summed in block order and, fp64, its slot's z-sum totals;
-// out[4c..4c+3] = (xr, xi, sx, sy) (fp32: the float sums).  out may be
-// host memory: the writes are made visible system-wide before the kernel ends.
+// out[4c..4c+3] = (xr, xi, sx, sy) (fp32: the float sums).  host_out: out is
+// host memory, and the writes are made visible system-wide before the kernel
+// ends.
 void launch_modamp_evals(bool fp32, const ModampSlots& m, const CandSet& c, int ncand, const double2* part64,
-                         const float4* part32, int nparts, int zblocks, double* out, hipStream_t s);
+                         const float4* part32, int nparts, int zblocks, double* out, bool host_out, hipStream_t s);
 // Writes seq to *flag (host memory) once the work queued before it is done,
 // with system-scope ordering: the host polls the flag instead of an event.
 void launch_fit_signal(unsigned* flag, unsigned seq, hipStream_t s);
+
+// ------------------------------------------------------------ device-driven fit
+// findk0's peak refinement and fitk0andmodamps' two line searches
+// (gpuFunctionsImpl.cu:874-1020) run on the device, every direction at once:
+// a walker kernel replays each direction's walk over its cache of evaluated
+// (angle, mag) candidates, exactly as the host does, and writes the next
+// candidates for the modamp kernels; the host queues a fixed number of
+// (evaluate, walk) rounds and reads the results once.
+constexpr int kFitLook = 6;                  // steps evaluated ahead in each direction
+constexpr int kFitReq = 2 * kFitLook + 1;    // candidates a direction requests in one round
+constexpr int kFitMaxRounds = 8;
+constexpr int kFitCache = kFitReq * (kFitMaxRounds + 1);  // evaluated candidates kept per direction
+enum { kFitAngle = 0, kFitMag = 1, kFitFinal = 2, kFitDone = 3 };
+struct FitDir {
+  float2 k0;             // findk0's result
+  float2 kov;            // vector of the next overlap construction: the fit's start, then the fitted k0
+  float k0mag, k0angle;  // the walks' start
+  float angle, mag;      // fitted
+  int phase, started;    // walk phase; whether its first batch was requested
+  int nreq, cand0;       // candidates requested in the last round, their first index
+  float req_a[kFitReq], req_m[kFitReq];
+  int n_angle_evals, n_mag_evals, n_reductions;
+  int ncache;
+  float ca[kFitCache], cm[kFitCache];
+  double cev[kFitCache][4];
+};
+struct FitState {
+  FitDir d[kMaxDirs];
+  float2 kc[kArgCand];  // the round's candidate vectors and their directions
+  int ks[kArgCand];
+  int ncand;
+  int ndone, rounds, overflow;
+  float peak_vals[kMaxDirs * 8];  // findk0's cross-correlation peaks
+  int peak_idx[kMaxDirs];
+  double ev[kArgCand][4];  // the round's sums (xr, xi, sx, sy) per candidate
+};
+struct FitArgs {
+  int ndirs, norders, fo2, fp32, nx, ny;
+  float dkx, dky, deltaangle, deltamag;
+  float2 guess[kMaxDirs];  // the k0 findk0's overlaps were built at
+};
+// Refines every direction's cross-correlation peak into k0 (findk0,
+// gpuFunctionsImpl.cu:316-370) and sets up the walks.
+void launch_fit_start(FitState* fs, const FitArgs& a, hipStream_t s);
+// One walker step: merges the last round's sums (fs->ev) into the caches,
+// advances every direction's walks, and writes the next round's candidates.
+// A direction that finishes gets its fitted k0 (kov) and its order-fo2
+// modamp and correlation (amps / corr [dir][norders]).
+void launch_fit_walk(FitState* fs, const FitArgs& a, float2* amps, float* corr, hipStream_t s);
+// kc[d] = kov[d] * mult for every direction
+void launch_fit_kc(const FitState* fs, int ndirs, float mult, float2* kc, hipStream_t s);
+// The results for the host, into (host-mapped) out: per direction k0 (the
+// search's), kov (the fitted k0) and the counters, then amps and corr
+// [dir][norders], then ndone, rounds and overflow.
+struct FitOut {
+  float2 k0[kMaxDirs], kov[kMaxDirs];
+  int n_angle_evals[kMaxDirs], n_mag_evals[kMaxDirs], n_reductions[kMaxDirs];
+  float2 amps[kMaxDirs * 5];
+  float corr[kMaxDirs * 5];
+  int ndone, rounds, overflow;
+};
+void launch_fit_publish(const FitState* fs, const FitArgs& a, const float2* amps, const float* corr, FitOut* out,
+                        hipStream_t s);
 // findrealspacemodamp's result (gpuFunctionsImpl.cu:1128-1147) on the device,
 // for each of ncand candidates: the partials summed in block order as the
 // host does (cpuReduce, gpuFunctionsImpl_hh.cu:162-180), then

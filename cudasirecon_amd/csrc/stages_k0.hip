@@ -96,8 +96,8 @@ __device__ __forceinline__ float2 prof_lerp(const float2* prof, int nxotf, float
 }
 
 template <bool RA>
-__device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof1, const float2* prof2, int which,
-                                           int j, int i, int z0) {
+__device__ __forceinline__ OvSrc ov_source(const OvParams& p, float kx, float ky, const float2* prof1,
+                                           const float2* prof2, int which, int j, int i, int z0) {
 #pragma clang fp contract(off)
   OvSrc o{0, false, false, make_float2(0.f, 0.f)};
   float x1f = (float)j, y1f = (float)i;
@@ -107,9 +107,9 @@ __device__ __forceinline__ OvSrc ov_source(const OvParams& p, const float2* prof
   y1f *= p.dky;
   const float d1 = __builtin_fmaf(x1f, x1f, y1f * y1f);
   if (which == 0 ? !(d1 <= p.rc2) : !(d1 < p.rc2)) return o;
-  const float x12 = x1f - p.kx, y12 = y1f - p.ky;
+  const float x12 = x1f - kx, y12 = y1f - ky;
   const float d12 = __builtin_fmaf(x12, x12, y12 * y12);
-  const float x21 = x1f + p.kx, y21 = y1f + p.ky;
+  const float x21 = x1f + kx, y21 = y1f + ky;
   const float d21 = __builtin_fmaf(x21, x21, y21 * y21);
   if (!(d12 <= p.rc2 || d21 <= p.rc2)) return o;
   if (z0 == 0 && p.bNoKz0) return o;
@@ -180,12 +180,12 @@ __device__ __forceinline__ float2 ov_combine(const OvSrc& o, float2 vre, float2 
 // a^2 = rc^2 - y^2 and b^2 = rc^2 - (y - sy)^2.  The radii carry a relative
 // margin of 1e-4, far beyond the rounding of the per-element tests, so a row
 // this rejects is zero.
-__device__ __forceinline__ bool ov_row_live(const OvParams& p, int which, int i, int z0) {
+__device__ __forceinline__ bool ov_row_live(const OvParams& p, float kx, float ky, int which, int i, int z0) {
   if (i >= p.ny || (z0 == 0 && p.bNoKz0)) return false;
   float y = (float)i;
   if (y > (float)(p.ny / 2)) y -= (float)p.ny;
   y *= p.dky;
-  const float sx = which == 0 ? p.kx : -p.kx, sy = which == 0 ? p.ky : -p.ky;
+  const float sx = which == 0 ? kx : -kx, sy = which == 0 ? ky : -ky;
   const float r2 = p.rc2 * 1.0002f;
   const float a2 = r2 - y * y, b2 = r2 - (y - sy) * (y - sy);
   if (a2 < 0.f || b2 < 0.f) return false;
@@ -222,11 +222,18 @@ __global__ __launch_bounds__(ov_rows(N) * fft_tpt(N)) void k_overlaps_x(OvBatch 
   const int which = blockIdx.x & 1;
   const int i0 = (blockIdx.x >> 1) * RW;
   const int z0 = zb - p.zd;
+  float kx = p.kx, ky = p.ky;
+  if (p.k0d != nullptr) {
+#pragma clang fp contract(off)
+    const float2 k = *p.k0d;
+    kx = k.x * p.kmul;
+    ky = k.y * p.kmul;
+  }
   float2* __restrict__ orow = ovx + (((long long)which * nzs + zb) * p.ny + i0) * p.nx;
   {
     bool live = false;
 #pragma unroll
-    for (int r = 0; r < RW; ++r) live = live || ov_row_live(p, which, i0 + r, z0);
+    for (int r = 0; r < RW; ++r) live = live || ov_row_live(p, kx, ky, which, i0 + r, z0);
     if (!live) {
 #pragma unroll
       for (int q = 0; q < EPT; ++q) {
@@ -259,8 +266,9 @@ __global__ __launch_bounds__(ov_rows(N) * fft_tpt(N)) void k_overlaps_x(OvBatch 
       const int idx = threadIdx.x + (q0 + c) * NT;
       const int t = idx / N, j = idx - (idx / N) * N;  // t = row in the workgroup
       const int i = i0 + t;
-      src[c] = (q0 + c < EPT && idx < RW * N && i < p.ny) ? ov_source<RA>(p, prof, prof + nxotf, which, j, i, z0)
-                                                          : OvSrc{0, false, false, make_float2(0.f, 0.f)};
+      src[c] = (q0 + c < EPT && idx < RW * N && i < p.ny)
+                   ? ov_source<RA>(p, kx, ky, prof, prof + nxotf, which, j, i, z0)
+                   : OvSrc{0, false, false, make_float2(0.f, 0.f)};
     }
     const float2* bre = which == 0 ? p.b1re : p.b2re;
     const float2* bim = which == 0 ? (p.order1 > 0 ? p.b1im : p.b1re) : p.b2im;
@@ -576,6 +584,7 @@ __global__ __launch_bounds__(256) void k_modamp_fp64(ModampSlots m, CandSet cs, 
   __shared__ double2 beta[kModRows];
   __shared__ double rx[256], ry[256];
   const int c = blockIdx.y;
+  if (cs.nd != nullptr && c >= *cs.nd) return;  // whole block
   const double2* __restrict__ P = m.P[cand_slot(cs, c)];
   const float2 kc = cand_k(cs, c);
   const double kx = (double)kc.x, ky = (double)kc.y;
@@ -682,10 +691,11 @@ void launch_modamp_finish(bool fp32, const ModampSlots& m, const int* kslot, int
 __global__ __launch_bounds__(256) void k_modamp_evals(int fp32, ModampSlots m, CandSet cs,
                                                       const double2* __restrict__ part64,
                                                       const float4* __restrict__ part32, int nparts, int zblocks,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out, int host_out) {
 #pragma clang fp contract(off)
   __shared__ double stage[4 * kSeqChunk];
   const int c = blockIdx.x;
+  if (cs.nd != nullptr && c >= *cs.nd) return;  // whole block
   out += 4LL * c;
   if (fp32) {
     const float4* pp = part32 + (long long)c * nparts;
@@ -694,7 +704,7 @@ __global__ __launch_bounds__(256) void k_modamp_evals(int fp32, ModampSlots m, C
       return (double)(k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : q.w);
     });
     if (threadIdx.x < 4) out[threadIdx.x] = v;
-    __threadfence_system();
+    if (host_out) __threadfence_system();
     return;
   }
   const double2* pp = part64 + (long long)c * nparts;
@@ -702,14 +712,14 @@ __global__ __launch_bounds__(256) void k_modamp_evals(int fp32, ModampSlots m, C
   if (threadIdx.x < 2) out[threadIdx.x] = v;
   if (threadIdx.x == 2 || threadIdx.x == 3)
     out[threadIdx.x] = m.totals[cand_slot(cs, c)][kZsumFinal + threadIdx.x - 2];
-  __threadfence_system();
+  if (host_out) __threadfence_system();
 }
 
 void launch_modamp_evals(bool fp32, const ModampSlots& m, const CandSet& c, int ncand, const double2* part64,
-                         const float4* part32, int nparts, int zblocks, double* out, hipStream_t s) {
+                         const float4* part32, int nparts, int zblocks, double* out, bool host_out, hipStream_t s) {
   check_cands(c, ncand);
   hipLaunchKernelGGL(k_modamp_evals, dim3(ncand), dim3(256), 0, s, fp32 ? 1 : 0, m, c, part64, part32, nparts,
-                     zblocks, out);
+                     zblocks, out, host_out ? 1 : 0);
   SIMR_LAUNCH_CHECK();
 }
 
@@ -772,6 +782,7 @@ __global__ __launch_bounds__(256) void k_modamp_fp32(ModampSlots m, CandSet cand
 #pragma clang fp contract(off)
   __shared__ float bx[256], by[256], bs[256], bt[256];
   const int c = blockIdx.z;
+  if (cand.nd != nullptr && c >= *cand.nd) return;  // whole block
   const int slot = cand_slot(cand, c);
   const double2* __restrict__ P = m.P[slot];
   const double* __restrict__ SX = m.SX[slot];

@@ -48,6 +48,13 @@ inline T mag2(C2<T> a) { return a.x * a.x + a.y * a.y; }
 inline float Tsqrt(float v) { return sqrtf(v); }
 inline double Tsqrt(double v) { return std::sqrt(v); }
 inline float Tcos(float v) { return cosf(v); }
+// The line search's angles and candidate vectors: cos, sin and atan2 in double
+// rounded once to float, where the reference calls cosf/sinf/atan2f (at most
+// one ulp apart, in a small fraction of arguments).  The engine forms them the
+// same way, bit-reproducibly on host and device.
+inline float fcos1(float v) { return (float)std::cos((double)v); }
+inline float fsin1(float v) { return (float)std::sin((double)v); }
+inline float fatan2_1(float y, float x) { return (float)std::atan2((double)y, (double)x); }
 inline double Tcos(double v) { return std::cos(v); }
 inline float Tsin(float v) { return sinf(v); }
 inline double Tsin(double v) { return std::sin(v); }
@@ -737,8 +744,8 @@ struct Oracle {
   double getmodamp(int dir, T kangle, T klength, int o1, int o2, C2<T>* modamp, int redo, float* cc = nullptr) {
     C2<T> k1;
     if (sizeof(T) == sizeof(float)) {
-      k1.x = (T)((float)klength * cosf((float)kangle));
-      k1.y = (T)((float)klength * sinf((float)kangle));
+      k1.x = (T)((float)klength * fcos1((float)kangle));
+      k1.y = (T)((float)klength * fsin1((float)kangle));
     } else {
       k1.x = klength * Tcos(kangle);
       k1.y = klength * Tsin(kangle);
@@ -771,7 +778,7 @@ struct Oracle {
     const int fo1 = 0, fo2 = fitorder2();
     C2<T> kv = k0[dir];
     T k0mag = Tsqrt(kv.x * kv.x + kv.y * kv.y);
-    T k0angle = Tatan2(kv.y, kv.x);
+    T k0angle = sizeof(T) == sizeof(float) ? (T)fatan2_1((float)kv.y, (float)kv.x) : Tatan2(kv.y, kv.x);
     const bool dparab = fit64 || sizeof(T) == sizeof(double);
     auto parab = [&](T x1, double y1, T x2, double y2, T x3, double y3) -> T {
       if (dparab) return (T)fitxyparabola<double>((double)x1, y1, (double)x2, y2, (double)x3, y3);
@@ -847,8 +854,8 @@ struct Oracle {
     getmodamp(dir, angle, mag, fo1, fo2, &modamp, redo, &cc);
     corr[dir][fo2] = cc;
     if (sizeof(T) == sizeof(float)) {
-      k0[dir].x = (T)((float)mag * cosf((float)angle));
-      k0[dir].y = (T)((float)mag * sinf((float)angle));
+      k0[dir].x = (T)((float)mag * fcos1((float)angle));
+      k0[dir].y = (T)((float)mag * fsin1((float)angle));
     } else {
       k0[dir].x = mag * Tcos(angle);
       k0[dir].y = mag * Tsin(angle);

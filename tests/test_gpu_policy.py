@@ -153,3 +153,47 @@ def test_2d_five_phases_chains_order2_modamp():
     p, otf, raw, _ = synthetic(128, 128, 1, nphases=5, seed=26)
     gv, gf, ov, of = series(p, otf, [raw])
     check("2D nphases 5", p, gv, gf, ov, of)
+
+
+def _run(p, otf, raw, env):
+    os.environ.update(env)
+    try:
+        with engine.Reconstructor(p, otf) as g:
+            g.set_raw(raw)
+            g.load_and_rescale(0)
+            g.set_time(0)
+            g.process_volume()
+            return g.result(), [g.fit(d) for d in range(p.ndirs)]
+    finally:
+        for k in env:
+            os.environ.pop(k, None)
+
+
+@pytest.mark.parametrize("case", ["3d_fp64", "3d_fp32", "2d_3ph", "2d_5ph", "testdata", "fallback"])
+def test_device_fit_matches_host_fit(case):
+    """The line searches driven on the device (the default) against the same
+    walks replayed on the host with a read-back per round (SIMR_FIT_ON_HOST=1):
+    the same candidates are evaluated in the same order (so the evaluation
+    counts agree), and k0, modamps and the output agree bit for bit (the
+    angles and candidate vectors come from the same reproducible cos, sin and
+    atan2 on both sides).  "fallback": one device round only
+    (SIMR_FIT_ROUNDS=1), so the device fit stops short and the host walk
+    takes over."""
+    log("device fit vs host fit:", case)
+    if case in ("testdata", "fallback"):
+        p, otf, raw = load_testdata()
+    else:
+        nz = 1 if case.startswith("2d") else 12
+        nph = 5 if case in ("3d_fp64", "3d_fp32", "2d_5ph") else 3
+        p, otf, raw, _ = synthetic(96, 96, nz, nphases=nph)
+        if case == "3d_fp32":
+            p.fit_fp64 = 0
+    vd, fd = _run(p, otf, raw, {"SIMR_FIT_ROUNDS": "1"} if case == "fallback" else {})
+    vh, fh = _run(p, otf, raw, {"SIMR_FIT_ON_HOST": "1"})
+    k0e, ame = fit_errors(p, fd, fh)
+    e = rel_l2(vd, vh)
+    log("  rel-L2 %.2e, k0 %.2e px, modamp %.2e" % (e, k0e, ame))
+    for a, b in zip(fd, fh):
+        assert (a.n_angle_evals, a.n_mag_evals, a.n_reductions) == (b.n_angle_evals, b.n_mag_evals, b.n_reductions)
+    assert k0e == 0.0 and ame == 0.0, (k0e, ame)
+    assert e == 0.0, e
