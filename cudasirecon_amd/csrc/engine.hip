@@ -224,6 +224,8 @@ Engine::Engine(const simr_params& p, const float* otf, int device) : P(p), L(Lay
   // SIMR_FIT_ROUNDS=n: the device-driven fit's first rounds (tests of the
   // fall-back to the host walk when they are too few)
   if (const char* e = std::getenv("SIMR_FIT_ROUNDS")) fit_rounds_ = std::min(kFitMaxRounds, std::max(1, std::atoi(e)));
+  // SIMR_OVERLAPS_3PASS=1: the three-pass overlap construction everywhere
+  if (const char* e = std::getenv("SIMR_OVERLAPS_3PASS")) fused_ov_ = !(e[0] == '1' && e[1] == 0);
   // every twiddle table this context's transforms use (rows, columns, z, the
   // zoomed assembly lengths), so that no launch path allocates one
   for (int n : {L.nx, L.ny, L.nz, L.xdim, L.ydim, L.zdim})
@@ -712,6 +714,13 @@ void Engine::make_overlaps(const std::vector<OvReq>& reqs, bool per_elem, FitWor
     w.cache.clear();
     dmin = std::min(dmin, r.dir);
     dmax = std::max(dmax, r.dir);
+  }
+  // the FP64 fit's overlaps in two passes where the sizes allow
+  if (!only && !per_elem && !fit_fp32_ && fused_ov_) {
+    const int st = stage_begin("ov_fused", stream_);
+    const bool done = launch_overlaps_fused(b, z, stream_);
+    stage_end(st, stream_);
+    if (done) return;
   }
   int st = stage_begin("ov_x", stream_);
   launch_overlaps_x(b, stream_);

@@ -22,6 +22,7 @@
 
 #include "device_math.hpp"
 #include "fft.hpp"
+#include "fft2.hpp"
 #include "fft_api.hpp"
 #include "sizes.hpp"
 #include "stages.hpp"
@@ -334,6 +335,281 @@ void launch_overlaps_x(const OvBatch& b, hipStream_t s) {
 #undef SIMR_CASE
     default:
       throw ArgError("image width " + std::to_string(b.p[0].nx) + " is not a supported size");
+  }
+}
+
+__global__ __launch_bounds__(256) void k_zsum_totals(ZsumBatch b, int zblocks);
+
+// ---------------------------------------------------------------------------
+// The overlaps' inverse transforms with the z sums fused, for the FP64 fit:
+// the formation runs with the y transform, by columns (k_ovf_cols), and the
+// x transform with the z sums, by rows (k_ovf_rows): a workgroup takes row y
+// of every stored plane of both overlaps, so P(x, y) and the SX / SY totals
+// accumulate in registers plane after plane.  The transformed planes are
+// written once and read once (the three-pass form writes them twice and
+// reads them twice).  The order of the two 1D transforms is the other one
+// than the three-pass form's, so P agrees with it to rounding, not bit for
+// bit.
+
+// Whether column j of overlap `which` can hold a nonzero element (the row
+// test of ov_row_live with x and y exchanged).
+__device__ __forceinline__ bool ov_col_live(const OvParams& p, float kx, float ky, int which, int j, int z0) {
+  if (j >= p.nx || (z0 == 0 && p.bNoKz0)) return false;
+  float x = (float)j;
+  if (x > (float)(p.nx / 2)) x -= (float)p.nx;
+  x *= p.dkx;
+  const float sx = which == 0 ? kx : -kx, sy = which == 0 ? ky : -ky;
+  const float r2 = p.rc2 * 1.0002f;
+  const float a2 = r2 - x * x, b2 = r2 - (x - sx) * (x - sx);
+  if (a2 < 0.f || b2 < 0.f) return false;
+  return fabsf(sy) <= (sqrtf(a2) + sqrtf(b2)) * 1.0001f;
+}
+
+// columns per workgroup of the column pass (radix-stage core: its rows are
+// a quarter of the two-pass core's LDS per thread, which keeps the occupancy
+// the formation's gathers need)
+template <int NY>
+constexpr int ovf_cols_w() {
+  int w = 8;
+  while (w > 1 && w * fft_tpt(NY) > 512) w /= 2;
+  return w;
+}
+template <int NY>
+constexpr bool ovf_cols_ok() {
+  return fft_supported(NY) && NY > 1 && (ovf_cols_w<NY>() * lds_pitch(NY) + NY) * 8 <= 64 * 1024 &&
+         (NY * ovf_cols_w<NY>()) % (ovf_cols_w<NY>() * fft_tpt(NY)) == 0;
+}
+
+// Columns [W*(blockIdx.x/2), +W) of overlap blockIdx.x%2 on plane blockIdx.y
+// of slot blockIdx.z: formation (as k_overlaps_x) and the inverse y transform.
+template <int NY, bool RA>
+__global__ __launch_bounds__(ovf_cols_w<NY>() * fft_tpt(NY)) void k_ovf_cols(OvBatch b,
+                                                                          const float2* __restrict__ tw) {
+  constexpr int W = ovf_cols_w<NY>();
+  constexpr int TPT = fft_tpt(NY);
+  constexpr int NT = W * TPT;
+  constexpr int P = lds_pitch(NY);
+  constexpr int EPT = NY * W / NT;
+  constexpr int CH = EPT < 4 ? EPT : 4;
+  static_assert(EPT % CH == 0, "chunks cover the elements");
+  const OvParams& p = b.p[blockIdx.z];
+  const int zb = blockIdx.y;
+  if (zb > 2 * p.zd) return;  // a slot with a narrower z support (whole block)
+  const int which = blockIdx.x & 1;
+  const int c0 = (blockIdx.x >> 1) * W;
+  const int z0 = zb - p.zd;
+  float kx = p.kx, ky = p.ky;
+  if (p.k0d != nullptr) {
+#pragma clang fp contract(off)
+    const float2 k = *p.k0d;
+    kx = k.x * p.kmul;
+    ky = k.y * p.kmul;
+  }
+  float2* __restrict__ plane = p.ovx + ((long long)which * b.nzs_stride + zb) * NY * p.nx;
+  {
+    bool live = false;
+    for (int w = 0; w < W; ++w) live = live || ov_col_live(p, kx, ky, which, c0 + w, z0);
+    if (!live) {
+      for (int idx = threadIdx.x; idx < NY * W; idx += NT) {
+        const int i = idx / W, w = idx % W;
+        if (c0 + w < p.nx) plane[(long long)i * p.nx + c0 + w] = make_float2(0.f, 0.f);
+      }
+      return;  // uniform over the workgroup
+    }
+  }
+  __shared__ float2 lds[W * P];
+  extern __shared__ float2 prof[];  // [2][nxotf]
+  const int tid = threadIdx.x;
+  FftTw<NY> twr;
+  __shared__ float2 twstage[NY];
+  load_twiddles(twr, twstage, tid % TPT, tw, tid, NT);
+  const int nxotf = p.otf1.nxotf;
+  if constexpr (RA)
+    for (int k = tid; k < 2 * nxotf; k += NT)
+      prof[k] = otf_profile_entry(k < nxotf ? p.otf1 : p.otf2, k < nxotf ? k : k - nxotf, z0, p.kzscale);
+  __syncthreads();
+  const float2* bre = which == 0 ? p.b1re : p.b2re;
+  const float2* bim = which == 0 ? (p.order1 > 0 ? p.b1im : p.b1re) : p.b2im;
+  const bool has_im = which == 1 || p.order1 > 0;
+#pragma unroll
+  for (int q0 = 0; q0 < EPT; q0 += CH) {
+    OvSrc src[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int idx = tid + (q0 + c) * NT;
+      const int i = idx / W, j = c0 + idx % W;
+      src[c] = j < p.nx ? ov_source<RA>(p, kx, ky, prof, prof + nxotf, which, j, i, z0)
+                        : OvSrc{0, false, false, make_float2(0.f, 0.f)};
+    }
+    float2 vre[CH], vim[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      vre[c] = bre[src[c].ind];
+      vim[c] = bim[src[c].ind];
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int idx = tid + (q0 + c) * NT;
+      lds[(idx % W) * P + lds_swz<NY>(idx / W)] = ov_combine(src[c], vre[c], vim[c], has_im);
+    }
+  }
+  __syncthreads();
+  fft_lds<NY, true>(lds + (tid / TPT) * P, tid % TPT, twr);
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int idx = tid + k * NT;
+    const int i = idx / W, w = idx % W;
+    if (c0 + w < p.nx) plane[(long long)i * p.nx + c0 + w] = lds[w * P + lds_swz<NY>(i)];
+  }
+}
+
+// planes per round of the row pass: G of each overlap, 2G transforms
+constexpr int kOvfG = 8;
+template <int NX>
+constexpr bool ovf_rows_ok() {
+  return use_fft2<NX>() && (2 * kOvfG * fft2_tpt(NX)) % 64 == 0 && 2 * kOvfG * fft2_tpt(NX) <= 1024 &&
+         2 * kOvfG * fft2_pitch<NX>() * 8 <= 160 * 1024;
+}
+
+// Row y = blockIdx.x of every stored plane of both overlaps of slot
+// blockIdx.y: the inverse x transforms, kOvfG planes at a time, then
+// P(x, y) = nz sum_kz conj(X) Y in plane order (as k_overlaps_zsum), and the
+// block's share of the SX and SY totals.
+template <int NX>
+__global__ __launch_bounds__(2 * kOvfG * fft2_tpt(NX)) void k_ovf_rows(ZsumBatch b, const float2* __restrict__ tw) {
+#pragma clang fp contract(off)
+  constexpr int TPT = fft2_tpt(NX);
+  constexpr int NT = 2 * kOvfG * TPT;
+  constexpr int SP = fft2_pitch<NX>();
+  constexpr int EPT = (NX + NT - 1) / NT;  // output elements per thread
+  constexpr int LPT = 2 * kOvfG * NX / NT;  // loads per thread per round
+  __shared__ float2 lds[2 * kOvfG * SP];
+  __shared__ double red[NT];
+  const int tid = threadIdx.x;
+  const int y = blockIdx.x, slot = blockIdx.y;
+  const int nzs = b.nzs[slot];
+  const long long nxy = b.nxy;
+  const float2* X = b.ovx[slot] + (long long)y * NX;
+  const float2* Y = X + (long long)b.nzs_stride * nxy;
+  Fft2Tw<NX> tw2;
+  tw2.load(tid % TPT, tw);
+  double sxr[EPT], sxi[EPT], sx[EPT], sy[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) sxr[e] = sxi[e] = sx[e] = sy[e] = 0.0;
+  const double n = (double)b.nz;
+  for (int g0 = 0; g0 < nzs; g0 += kOvfG) {
+    const int cnt = nzs - g0 < kOvfG ? nzs - g0 : kOvfG;  // uniform
+    // transform t < kOvfG: X of plane g0 + t; t >= kOvfG: Y of plane g0 + t - kOvfG
+    constexpr int LB = LPT < 8 ? LPT : 8;  // loads in flight per batch
+    static_assert(LPT % LB == 0, "load batches cover the rows");
+#pragma unroll
+    for (int k0 = 0; k0 < LPT; k0 += LB) {
+      float2 v[LB];
+#pragma unroll
+      for (int k = 0; k < LB; ++k) {
+        const int idx = tid + (k0 + k) * NT;
+        const int t = idx / NX, x = idx % NX;
+        const int pl = t < kOvfG ? t : t - kOvfG;
+        const float2* src = t < kOvfG ? X : Y;
+        v[k] = src[(long long)(g0 + (pl < cnt ? pl : 0)) * nxy + x];
+      }
+#pragma unroll
+      for (int k = 0; k < LB; ++k) {
+        const int idx = tid + (k0 + k) * NT;
+        lds[(idx / NX) * SP + idx % NX] = v[k];
+      }
+    }
+    __syncthreads();
+    fft2_lds<NX, true, true>(lds + (tid / TPT) * SP, tid % TPT, tw2);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int x = tid + e * NT;
+      if (x < NX) {
+        for (int k = 0; k < cnt; ++k) {
+          const float2 xa = lds[k * SP + x], yb = lds[(kOvfG + k) * SP + x];
+          const double ar = xa.x, ai = xa.y, br = yb.x, bi = yb.y;
+          // float x float products are exact in double: a product plus a
+          // fused multiply-add is the correctly rounded a*b + c*d
+          sxr[e] += __builtin_fma(ar, br, ai * bi);
+          sxi[e] += __builtin_fma(ar, bi, -(ai * br));
+          sx[e] += __builtin_fma(ar, ar, ai * ai);
+          sy[e] += __builtin_fma(br, br, bi * bi);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double2* P = b.P[slot] + (long long)y * NX;
+  double tsx = 0.0, tsy = 0.0;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int x = tid + e * NT;
+    if (x < NX) {
+      P[x] = make_double2(sxr[e] * n, sxi[e] * n);
+      tsx += sx[e] * n;
+      tsy += sy[e] * n;
+    }
+  }
+  const double a = block_sum_d<NT>(tsx, red);
+  const double c = block_sum_d<NT>(tsy, red);
+  if (tid == 0) {
+    b.totals[slot][y] = a;
+    b.totals[slot][gridDim.x + y] = c;
+  }
+}
+
+template <int NX, int NY>
+static bool launch_ovf_nn(const OvBatch& ob, const ZsumBatch& zb, int nzs_max, hipStream_t s) {
+  if constexpr (ovf_cols_ok<NY>() && ovf_rows_ok<NX>()) {
+    const OvParams& p = ob.p[0];
+    const dim3 g1(2 * ceil_div(NX, ovf_cols_w<NY>()), nzs_max, ob.n), b1(ovf_cols_w<NY>() * fft_tpt(NY));
+    if (p.otf1.ra) {
+      const size_t lds = (size_t)2 * p.otf1.nxotf * sizeof(float2);
+      if (lds > 32 * 1024) return false;
+      hipLaunchKernelGGL((k_ovf_cols<NY, true>), g1, b1, lds, s, ob, twiddles(NY));
+    } else {
+      hipLaunchKernelGGL((k_ovf_cols<NY, false>), g1, b1, 0, s, ob, twiddles(NY));
+    }
+    SIMR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_ovf_rows<NX>, dim3(NY, zb.n), dim3(2 * kOvfG * fft2_tpt(NX)), 0, s, zb, twiddles(NX));
+    SIMR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_zsum_totals, dim3(zb.n), dim3(256), 0, s, zb, NY);
+    SIMR_LAUNCH_CHECK();
+    return true;
+  } else {
+    return false;
+  }
+}
+
+template <int NX>
+static bool launch_ovf_n(const OvBatch& ob, const ZsumBatch& zb, int nzs_max, int ny, hipStream_t s) {
+  switch (ny) {
+    case 256: return launch_ovf_nn<NX, 256>(ob, zb, nzs_max, s);
+    case 512: return launch_ovf_nn<NX, 512>(ob, zb, nzs_max, s);
+    case 1024: return launch_ovf_nn<NX, 1024>(ob, zb, nzs_max, s);
+    default: return false;
+  }
+}
+
+bool launch_overlaps_fused(const OvBatch& ob, const ZsumBatch& zb, hipStream_t s) {
+  if (ob.n < 1 || ob.n > kMaxDirs || zb.n != ob.n) return false;
+  const OvParams& p0 = ob.p[0];
+  if (p0.ny > kZsumMaxBlocks) return false;
+  int nzs_max = 0;
+  for (int k = 0; k < ob.n; ++k) {
+    const OvParams& p = ob.p[k];
+    if (p.nx != p0.nx || p.ny != p0.ny || p.otf1.nxotf != p0.otf1.nxotf || p.otf2.nxotf != p0.otf1.nxotf ||
+        p.otf1.ra != p0.otf1.ra || p.otf2.ra != p0.otf1.ra)
+      return false;
+    if (p.zd < 0 || 2 * p.zd + 1 > ob.nzs_stride || zb.nzs[k] != 2 * p.zd + 1 || zb.ovx[k] != p.ovx) return false;
+    nzs_max = std::max(nzs_max, 2 * p.zd + 1);
+  }
+  switch (p0.nx) {
+    case 256: return launch_ovf_n<256>(ob, zb, nzs_max, p0.ny, s);
+    case 512: return launch_ovf_n<512>(ob, zb, nzs_max, p0.ny, s);
+    case 1024: return launch_ovf_n<1024>(ob, zb, nzs_max, p0.ny, s);
+    default: return false;
   }
 }
 

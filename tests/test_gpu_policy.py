@@ -197,3 +197,27 @@ def test_device_fit_matches_host_fit(case):
         assert (a.n_angle_evals, a.n_mag_evals, a.n_reductions) == (b.n_angle_evals, b.n_mag_evals, b.n_reductions)
     assert k0e == 0.0 and ame == 0.0, (k0e, ame)
     assert e == 0.0, e
+
+
+@pytest.mark.parametrize("case", ["3d_512", "3d_256", "2d_256", "testdata"])
+def test_two_pass_overlaps_match_three_pass(case):
+    """The FP64 fit's overlaps built in two passes (formation with the y
+    transform by columns, then the x transform with the z sums by rows)
+    against the three-pass form (SIMR_OVERLAPS_3PASS=1): the 2D inverse
+    transform is taken in the other order, so P differs by rounding only, and
+    the fitted k0, the modamps and the output agree to that level."""
+    log("two-pass vs three-pass overlaps:", case)
+    if case == "testdata":
+        p, otf, raw = load_testdata()
+    elif case == "2d_256":
+        p, otf, raw, _ = synthetic(256, 256, 1, nphases=3)
+    else:
+        n = 512 if case == "3d_512" else 256
+        p, otf, raw, _ = synthetic(n, n, 16, nphases=5)
+    v2, f2 = _run(p, otf, raw, {})
+    v3, f3 = _run(p, otf, raw, {"SIMR_OVERLAPS_3PASS": "1"})
+    k0e, ame = fit_errors(p, f2, f3)
+    e = rel_l2(v2, v3)
+    log("  rel-L2 %.2e, k0 %.2e px, modamp %.2e" % (e, k0e, ame))
+    assert k0e < 1e-4 and ame < 1e-5, (k0e, ame)
+    assert e < 2e-6, e
