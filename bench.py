@@ -40,6 +40,8 @@ KERNEL_OF_STAGE = {
     "ov_x": "simr::k_overlaps_x<{nx}>",
     "ov_y": "simr::k_fft2_cols<{ny}, true, ...>",
     "ov_zsum": "simr::k_overlaps_zsum<false, false>",
+    "ov_cols": "simr::k_ovf_cols<{ny}, true>",
+    "ov_rows": "simr::k_ovf_rows<{nx}> + simr::k_zsum_totals",
     "modamp": "simr::k_modamp_fp64",
     "findk0_peak": "simr::k_cc_prep + 2D FFT + simr::k_argmax_*",
     "filter0": "simr::k_filter<...> (order 0)",
@@ -84,6 +86,10 @@ def stage_bytes(p, L):
         "ov_x": p.ndirs * (nread * nzs * L["nxh"] * p.ny * 8 + 2 * nzs * nxy * 8),
         "ov_y": p.ndirs * 2 * 2 * nzs * nxy * 8,
         "ov_zsum": p.ndirs * (2 * nzs * nxy * 8 + 16 * nxy),
+        # the two-pass form: formation and y transform (bands in, both
+        # overlaps' planes out), then x transform and z sums (planes in, P out)
+        "ov_cols": p.ndirs * (nread * nzs * L["nxh"] * p.ny * 8 + 2 * nzs * nxy * 8),
+        "ov_rows": p.ndirs * (2 * nzs * nxy * 8 + 16 * nxy),
         "filter0": 2 * H * 8,
         "filter": 4 * H * 8,
         "asm_z": p.ndirs * nb * H * 8 + L["nasm"] * L["zdim"] * nxy * 8,
@@ -107,6 +113,8 @@ def stage_fft_flops(p, L):
         "cols_fwd_z": p.ndirs * nb * p.ny * L["nxh"] * c(p.nz),
         "ov_x": p.ndirs * 2 * nzs * p.ny * c(p.nx),
         "ov_y": p.ndirs * 2 * nzs * p.nx * c(p.ny),
+        "ov_cols": p.ndirs * 2 * nzs * p.nx * c(p.ny),
+        "ov_rows": p.ndirs * 2 * nzs * p.ny * c(p.nx),
         "asm_z": L["nasm"] * p.nx * p.ny * c(L["zdim"]),
         "asm_y": L["nasm"] * L["zdim"] * p.nx * c(L["ydim"]),
         "asm_x": L["nasm"] * L["zdim"] * L["ydim"] * c(L["xdim"]),

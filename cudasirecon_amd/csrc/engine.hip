@@ -716,11 +716,14 @@ void Engine::make_overlaps(const std::vector<OvReq>& reqs, bool per_elem, FitWor
     dmax = std::max(dmax, r.dir);
   }
   // the FP64 fit's overlaps in two passes where the sizes allow
-  if (!only && !per_elem && !fit_fp32_ && fused_ov_) {
-    const int st = stage_begin("ov_fused", stream_);
-    const bool done = launch_overlaps_fused(b, z, stream_);
+  if (!only && !per_elem && !fit_fp32_ && fused_ov_ && overlaps_fused_ok(b, z)) {
+    int st = stage_begin("ov_cols", stream_);
+    launch_overlaps_fused_cols(b, stream_);
     stage_end(st, stream_);
-    if (done) return;
+    st = stage_begin("ov_rows", stream_);
+    launch_overlaps_fused_rows(z, L.nx, L.ny, stream_);
+    stage_end(st, stream_);
+    return;
   }
   int st = stage_begin("ov_x", stream_);
   launch_overlaps_x(b, stream_);

@@ -185,7 +185,7 @@ class Engine {
   FitState* fitst_ = nullptr;
   int fit_rounds_ = 4;
   bool device_fit_ = true;
-  bool fused_ov_ = true;  // two-pass overlap construction (launch_overlaps_fused)
+  bool fused_ov_ = true;  // two-pass overlap construction (launch_overlaps_fused_cols/_rows)
   float2 *stageZ_ = nullptr, *stageY_ = nullptr;
   float* resampled_ = nullptr;  // [nimg][ny][nx] resampled raw planes (aliases the staging region)
   float* out_ = nullptr;

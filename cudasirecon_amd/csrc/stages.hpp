@@ -109,11 +109,13 @@ struct ZsumBatch {
 };
 void launch_overlaps_z_reduce(const ZsumBatch& b, bool fp32, bool per_elem, hipStream_t s);
 // The FP64 fit's overlap construction in two passes instead of three: the
-// formation with the inverse y transform (by columns), then the inverse x
-// transform with the z sums (by rows); P and the totals as
-// launch_overlaps_z_reduce(fp32 = false, per_elem = false) leaves them.
-// false (nothing launched) for the sizes it does not cover.
-bool launch_overlaps_fused(const OvBatch& ob, const ZsumBatch& zb, hipStream_t s);
+// formation with the inverse y transform (by columns, ..._cols), then the
+// inverse x transform with the z sums (by rows, ..._rows); P and the totals
+// as launch_overlaps_z_reduce(fp32 = false, per_elem = false) leaves them.
+// overlaps_fused_ok: whether the geometry is covered.
+bool overlaps_fused_ok(const OvBatch& ob, const ZsumBatch& zb);
+void launch_overlaps_fused_cols(const OvBatch& ob, hipStream_t s);
+void launch_overlaps_fused_rows(const ZsumBatch& zb, int nx, int ny, hipStream_t s);
 // Inverse z DFT of the stored kz planes of both overlaps into real-space
 // volumes vol[2][nz][nx*ny] (debug dump only).
 void launch_overlap_volumes(const float2* ovx, int zd, int nzs_stride, int nz, long long nxy, float2* vol,

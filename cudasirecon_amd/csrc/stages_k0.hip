@@ -463,12 +463,13 @@ __global__ __launch_bounds__(ovf_cols_w<NY>() * fft_tpt(NY)) void k_ovf_cols(OvB
   }
 }
 
-// planes per round of the row pass: G of each overlap, 2G transforms
-constexpr int kOvfG = 8;
+// planes per round of the row pass: G of each overlap, 2G transforms on the
+// radix-stage core
+constexpr int kOvfG = 4;
 template <int NX>
 constexpr bool ovf_rows_ok() {
-  return use_fft2<NX>() && (2 * kOvfG * fft2_tpt(NX)) % 64 == 0 && 2 * kOvfG * fft2_tpt(NX) <= 1024 &&
-         2 * kOvfG * fft2_pitch<NX>() * 8 <= 160 * 1024;
+  return fft_supported(NX) && NX > 1 && 2 * kOvfG * fft_tpt(NX) <= 1024 && 2 * kOvfG * fft_tpt(NX) >= 64 &&
+         (2 * kOvfG * lds_pitch(NX) + NX) * 8 <= 96 * 1024 && (2 * kOvfG * NX) % (2 * kOvfG * fft_tpt(NX)) == 0;
 }
 
 // Row y = blockIdx.x of every stored plane of both overlaps of slot
@@ -476,14 +477,14 @@ constexpr bool ovf_rows_ok() {
 // P(x, y) = nz sum_kz conj(X) Y in plane order (as k_overlaps_zsum), and the
 // block's share of the SX and SY totals.
 template <int NX>
-__global__ __launch_bounds__(2 * kOvfG * fft2_tpt(NX)) void k_ovf_rows(ZsumBatch b, const float2* __restrict__ tw) {
+__global__ __launch_bounds__(2 * kOvfG * fft_tpt(NX)) void k_ovf_rows(ZsumBatch b, const float2* __restrict__ tw) {
 #pragma clang fp contract(off)
-  constexpr int TPT = fft2_tpt(NX);
+  constexpr int TPT = fft_tpt(NX);
   constexpr int NT = 2 * kOvfG * TPT;
-  constexpr int SP = fft2_pitch<NX>();
+  constexpr int P = lds_pitch(NX);
   constexpr int EPT = (NX + NT - 1) / NT;  // output elements per thread
   constexpr int LPT = 2 * kOvfG * NX / NT;  // loads per thread per round
-  __shared__ float2 lds[2 * kOvfG * SP];
+  __shared__ float2 lds[2 * kOvfG * P];
   __shared__ double red[NT];
   const int tid = threadIdx.x;
   const int y = blockIdx.x, slot = blockIdx.y;
@@ -491,8 +492,9 @@ __global__ __launch_bounds__(2 * kOvfG * fft2_tpt(NX)) void k_ovf_rows(ZsumBatch
   const long long nxy = b.nxy;
   const float2* X = b.ovx[slot] + (long long)y * NX;
   const float2* Y = X + (long long)b.nzs_stride * nxy;
-  Fft2Tw<NX> tw2;
-  tw2.load(tid % TPT, tw);
+  FftTw<NX> twr;
+  __shared__ float2 twstage[NX];
+  load_twiddles(twr, twstage, tid % TPT, tw, tid, NT);
   double sxr[EPT], sxi[EPT], sx[EPT], sy[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) sxr[e] = sxi[e] = sx[e] = sy[e] = 0.0;
@@ -500,34 +502,30 @@ __global__ __launch_bounds__(2 * kOvfG * fft2_tpt(NX)) void k_ovf_rows(ZsumBatch
   for (int g0 = 0; g0 < nzs; g0 += kOvfG) {
     const int cnt = nzs - g0 < kOvfG ? nzs - g0 : kOvfG;  // uniform
     // transform t < kOvfG: X of plane g0 + t; t >= kOvfG: Y of plane g0 + t - kOvfG
-    constexpr int LB = LPT < 8 ? LPT : 8;  // loads in flight per batch
-    static_assert(LPT % LB == 0, "load batches cover the rows");
+    float2 v[LPT];
 #pragma unroll
-    for (int k0 = 0; k0 < LPT; k0 += LB) {
-      float2 v[LB];
+    for (int k = 0; k < LPT; ++k) {
+      const int idx = tid + k * NT;
+      const int t = idx / NX, x = idx % NX;
+      const int pl = t < kOvfG ? t : t - kOvfG;
+      const float2* src = t < kOvfG ? X : Y;
+      v[k] = src[(long long)(g0 + (pl < cnt ? pl : 0)) * nxy + x];
+    }
+    __syncthreads();  // the previous round's reads of lds are done
 #pragma unroll
-      for (int k = 0; k < LB; ++k) {
-        const int idx = tid + (k0 + k) * NT;
-        const int t = idx / NX, x = idx % NX;
-        const int pl = t < kOvfG ? t : t - kOvfG;
-        const float2* src = t < kOvfG ? X : Y;
-        v[k] = src[(long long)(g0 + (pl < cnt ? pl : 0)) * nxy + x];
-      }
-#pragma unroll
-      for (int k = 0; k < LB; ++k) {
-        const int idx = tid + (k0 + k) * NT;
-        lds[(idx / NX) * SP + idx % NX] = v[k];
-      }
+    for (int k = 0; k < LPT; ++k) {
+      const int idx = tid + k * NT;
+      lds[(idx / NX) * P + lds_swz<NX>(idx % NX)] = v[k];
     }
     __syncthreads();
-    fft2_lds<NX, true, true>(lds + (tid / TPT) * SP, tid % TPT, tw2);
-    __syncthreads();
+    fft_lds<NX, true>(lds + (tid / TPT) * P, tid % TPT, twr);
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int x = tid + e * NT;
       if (x < NX) {
+        const int xs = lds_swz<NX>(x);
         for (int k = 0; k < cnt; ++k) {
-          const float2 xa = lds[k * SP + x], yb = lds[(kOvfG + k) * SP + x];
+          const float2 xa = lds[k * P + xs], yb = lds[(kOvfG + k) * P + xs];
           const double ar = xa.x, ai = xa.y, br = yb.x, bi = yb.y;
           // float x float products are exact in double: a product plus a
           // fused multiply-add is the correctly rounded a*b + c*d
@@ -538,15 +536,14 @@ __global__ __launch_bounds__(2 * kOvfG * fft2_tpt(NX)) void k_ovf_rows(ZsumBatch
         }
       }
     }
-    __syncthreads();
   }
-  double2* P = b.P[slot] + (long long)y * NX;
+  double2* Pout = b.P[slot] + (long long)y * NX;
   double tsx = 0.0, tsy = 0.0;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int x = tid + e * NT;
     if (x < NX) {
-      P[x] = make_double2(sxr[e] * n, sxi[e] * n);
+      Pout[x] = make_double2(sxr[e] * n, sxi[e] * n);
       tsx += sx[e] * n;
       tsy += sy[e] * n;
     }
@@ -559,22 +556,18 @@ __global__ __launch_bounds__(2 * kOvfG * fft2_tpt(NX)) void k_ovf_rows(ZsumBatch
   }
 }
 
-template <int NX, int NY>
-static bool launch_ovf_nn(const OvBatch& ob, const ZsumBatch& zb, int nzs_max, hipStream_t s) {
-  if constexpr (ovf_cols_ok<NY>() && ovf_rows_ok<NX>()) {
+template <int NY>
+static bool ovf_cols_launch(const OvBatch& ob, int nzs_max, bool run, hipStream_t s) {
+  if constexpr (ovf_cols_ok<NY>()) {
     const OvParams& p = ob.p[0];
-    const dim3 g1(2 * ceil_div(NX, ovf_cols_w<NY>()), nzs_max, ob.n), b1(ovf_cols_w<NY>() * fft_tpt(NY));
-    if (p.otf1.ra) {
-      const size_t lds = (size_t)2 * p.otf1.nxotf * sizeof(float2);
-      if (lds > 32 * 1024) return false;
+    const size_t lds = p.otf1.ra ? (size_t)2 * p.otf1.nxotf * sizeof(float2) : 0;
+    if (lds > 32 * 1024) return false;
+    if (!run) return true;
+    const dim3 g1(2 * ceil_div(p.nx, ovf_cols_w<NY>()), nzs_max, ob.n), b1(ovf_cols_w<NY>() * fft_tpt(NY));
+    if (p.otf1.ra)
       hipLaunchKernelGGL((k_ovf_cols<NY, true>), g1, b1, lds, s, ob, twiddles(NY));
-    } else {
+    else
       hipLaunchKernelGGL((k_ovf_cols<NY, false>), g1, b1, 0, s, ob, twiddles(NY));
-    }
-    SIMR_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_ovf_rows<NX>, dim3(NY, zb.n), dim3(2 * kOvfG * fft2_tpt(NX)), 0, s, zb, twiddles(NX));
-    SIMR_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_zsum_totals, dim3(zb.n), dim3(256), 0, s, zb, NY);
     SIMR_LAUNCH_CHECK();
     return true;
   } else {
@@ -583,34 +576,63 @@ static bool launch_ovf_nn(const OvBatch& ob, const ZsumBatch& zb, int nzs_max, h
 }
 
 template <int NX>
-static bool launch_ovf_n(const OvBatch& ob, const ZsumBatch& zb, int nzs_max, int ny, hipStream_t s) {
-  switch (ny) {
-    case 256: return launch_ovf_nn<NX, 256>(ob, zb, nzs_max, s);
-    case 512: return launch_ovf_nn<NX, 512>(ob, zb, nzs_max, s);
-    case 1024: return launch_ovf_nn<NX, 1024>(ob, zb, nzs_max, s);
+static bool ovf_rows_launch(const ZsumBatch& zb, int ny, bool run, hipStream_t s) {
+  if constexpr (ovf_rows_ok<NX>()) {
+    if (!run) return true;
+    hipLaunchKernelGGL(k_ovf_rows<NX>, dim3(ny, zb.n), dim3(2 * kOvfG * fft_tpt(NX)), 0, s, zb, twiddles(NX));
+    SIMR_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_zsum_totals, dim3(zb.n), dim3(256), 0, s, zb, ny);
+    SIMR_LAUNCH_CHECK();
+    return true;
+  } else {
+    return false;
+  }
+}
+
+static int ovf_nzs_max(const OvBatch& ob) {
+  int m = 0;
+  for (int k = 0; k < ob.n; ++k) m = std::max(m, 2 * ob.p[k].zd + 1);
+  return m;
+}
+
+static bool ovf_cols_dispatch(const OvBatch& ob, bool run, hipStream_t s) {
+  switch (ob.p[0].ny) {
+    case 256: return ovf_cols_launch<256>(ob, ovf_nzs_max(ob), run, s);
+    case 512: return ovf_cols_launch<512>(ob, ovf_nzs_max(ob), run, s);
+    case 1024: return ovf_cols_launch<1024>(ob, ovf_nzs_max(ob), run, s);
     default: return false;
   }
 }
 
-bool launch_overlaps_fused(const OvBatch& ob, const ZsumBatch& zb, hipStream_t s) {
+static bool ovf_rows_dispatch(const ZsumBatch& zb, int nx, int ny, bool run, hipStream_t s) {
+  switch (nx) {
+    case 256: return ovf_rows_launch<256>(zb, ny, run, s);
+    case 512: return ovf_rows_launch<512>(zb, ny, run, s);
+    case 1024: return ovf_rows_launch<1024>(zb, ny, run, s);
+    default: return false;
+  }
+}
+
+bool overlaps_fused_ok(const OvBatch& ob, const ZsumBatch& zb) {
   if (ob.n < 1 || ob.n > kMaxDirs || zb.n != ob.n) return false;
   const OvParams& p0 = ob.p[0];
   if (p0.ny > kZsumMaxBlocks) return false;
-  int nzs_max = 0;
   for (int k = 0; k < ob.n; ++k) {
     const OvParams& p = ob.p[k];
     if (p.nx != p0.nx || p.ny != p0.ny || p.otf1.nxotf != p0.otf1.nxotf || p.otf2.nxotf != p0.otf1.nxotf ||
         p.otf1.ra != p0.otf1.ra || p.otf2.ra != p0.otf1.ra)
       return false;
     if (p.zd < 0 || 2 * p.zd + 1 > ob.nzs_stride || zb.nzs[k] != 2 * p.zd + 1 || zb.ovx[k] != p.ovx) return false;
-    nzs_max = std::max(nzs_max, 2 * p.zd + 1);
   }
-  switch (p0.nx) {
-    case 256: return launch_ovf_n<256>(ob, zb, nzs_max, p0.ny, s);
-    case 512: return launch_ovf_n<512>(ob, zb, nzs_max, p0.ny, s);
-    case 1024: return launch_ovf_n<1024>(ob, zb, nzs_max, p0.ny, s);
-    default: return false;
-  }
+  return ovf_cols_dispatch(ob, false, nullptr) && ovf_rows_dispatch(zb, p0.nx, p0.ny, false, nullptr);
+}
+
+void launch_overlaps_fused_cols(const OvBatch& ob, hipStream_t s) {
+  if (!ovf_cols_dispatch(ob, true, s)) throw ArgError("two-pass overlaps: unsupported geometry");
+}
+
+void launch_overlaps_fused_rows(const ZsumBatch& zb, int nx, int ny, hipStream_t s) {
+  if (!ovf_rows_dispatch(zb, nx, ny, true, s)) throw ArgError("two-pass overlaps: unsupported geometry");
 }
 
 // Every reserved plane is transformed (planes beyond a slot's own 2zd+1 hold
