@@ -44,8 +44,7 @@ KERNEL_OF_STAGE = {
     "ov_rows": "simr::k_ovf_rows<{nx}> + simr::k_zsum_totals",
     "modamp": "simr::k_modamp_fp64",
     "findk0_peak": "simr::k_cc_prep + 2D FFT + simr::k_argmax_*",
-    "filter0": "simr::k_filter<...> (order 0)",
-    "filter": "simr::k_filter<...> (side band pair)",
+    "filter": "simr::k_filter<...> (every band of the stack)",
     "asm_z": "simr::k_asm_z_pair<{zdim}, true> + simr::k_asm_z_pair<{zdim}, false>",
     "asm_y": "simr::k_asm_y<{ydim}>",
     "asm_x": "simr::k_asm_x<{xdim}>",
@@ -90,8 +89,8 @@ def stage_bytes(p, L):
         # overlaps' planes out), then x transform and z sums (planes in, P out)
         "ov_cols": p.ndirs * (nread * nzs * L["nxh"] * p.ny * 8 + 2 * nzs * nxy * 8),
         "ov_rows": p.ndirs * (2 * nzs * nxy * 8 + 16 * nxy),
-        "filter0": 2 * H * 8,
-        "filter": 4 * H * 8,
+        # every direction's order-0 band (read and written) and side-band pairs
+        "filter": p.ndirs * (2 + 4 * (L["norders"] - 1)) * H * 8,
         "asm_z": p.ndirs * nb * H * 8 + L["nasm"] * L["zdim"] * nxy * 8,
         "asm_y": L["nasm"] * L["zdim"] * nxy * 8 + L["nasm"] * L["zdim"] * L["ydim"] * p.nx * 8,
         "asm_x": L["nasm"] * L["zdim"] * L["ydim"] * p.nx * 8 + L["zdim"] * L["ydim"] * L["xdim"] * 4,

@@ -427,6 +427,11 @@ void Engine::sync() { SIMR_HIP(hipStreamSynchronize(stream_)); }
 void Engine::wait_fit() {
   const unsigned seq = ++fit_seq_;
   launch_fit_signal((unsigned*)(dzc_ + kZcFlag), seq, stream_);
+  wait_flag(seq);
+}
+
+// Waits until the flag word holds seq (written by the last kernel queued).
+void Engine::wait_flag(unsigned seq) {
   const volatile unsigned* flag = (const volatile unsigned*)(hzc_ + kZcFlag);
   const auto t0 = std::chrono::steady_clock::now();
   for (int i = 0;; ++i) {
@@ -1240,8 +1245,9 @@ bool Engine::fit_on_device() {
     stage_end(st, stream_);
   }
   FitOut* out = (FitOut*)(dzc_ + kZcFit);
-  launch_fit_publish(fitst_, a, amps_dev_, corr_dev_, out, stream_);
-  wait_fit();
+  const unsigned seq = ++fit_seq_;
+  launch_fit_publish(fitst_, a, amps_dev_, corr_dev_, out, (unsigned*)(dzc_ + kZcFlag), seq, stream_);
+  wait_flag(seq);
   const FitOut& o = *(const FitOut*)(hzc_ + kZcFit);
   if (o.overflow || o.ndone < L.ndirs) {
     fit_rounds_ = std::min(kFitMaxRounds, rounds + 2);
@@ -1440,14 +1446,25 @@ FilterParams Engine::filter_params(int dir, int order) const {
   return f;
 }
 
-void Engine::filter_dir(int dir) {
-  refresh_fit();
-  for (int order = 0; order < L.norders; ++order) {
-    const int st = stage_begin(order == 0 ? "filter0" : "filter", stream_);
-    launch_filter(filter_params(dir, order), stream_);
-    stage_end(st, stream_);
-  }
+// Every order of directions d0 .. d1-1 in one launch; amps: the modamps on
+// the device (graphed time points) or null (the host's).
+void Engine::filter_dirs(int d0, int d1, const float2* amps) {
+  if (amps == nullptr) refresh_fit();  // (a graph capture passes the device modamps)
+  FilterParams f = filter_params(d0, 0);
+  f.amps = amps;
+  const float2* otf0 = f.otf[0].t;
+  FilterBatch b{};
+  for (int d = d0; d < d1; ++d)
+    for (int o = 0; o < L.norders; ++o) {
+      const FilterParams g = filter_params(d, o);
+      b.b[b.n++] = FilterBand{d, o, g.conjamp, g.bp, g.bp2, (long long)(g.otf[0].t - otf0)};
+    }
+  const int st = stage_begin("filter", stream_);
+  launch_filter_batch(f, b, stream_);
+  stage_end(st, stream_);
 }
+
+void Engine::filter_dir(int dir) { filter_dirs(dir, dir + 1, nullptr); }
 
 void Engine::assemble() {
   AsmParams a{};
@@ -1490,7 +1507,7 @@ void Engine::assemble() {
 
 void Engine::filter_assemble() {
   if (!fitted_) throw StateError("filter_assemble before find_modulation");
-  for (int dir = 0; dir < L.ndirs; ++dir) filter_dir(dir);
+  filter_dirs(0, L.ndirs, nullptr);
   assemble();
   processed_ = true;
 }
@@ -1558,12 +1575,7 @@ void Engine::build_graph() {
     for (int o = 0; o < SIMR_MAX_ORDERS; ++o) a.forceamp[o] = P.forceamp[o];
     a.fit_all_phases = P.bFitallphases;
     launch_amp_policy(a, amps_dev_, stream_);
-    for (int d = 0; d < L.ndirs; ++d)
-      for (int o = 0; o < L.norders; ++o) {
-        FilterParams f = filter_params(d, o);
-        f.amps = amps_dev_;
-        launch_filter(f, stream_);
-      }
+    filter_dirs(0, L.ndirs, amps_dev_);
     assemble();
   } catch (...) {
     hipGraph_t g = nullptr;

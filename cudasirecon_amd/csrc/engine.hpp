@@ -60,6 +60,7 @@ class Engine {
   // runs as one captured graph (use_graphs)
   void process_volume();
   void filter_dir(int dir);
+  void filter_dirs(int d0, int d1, const float2* amps);
   void get_result(float* host);
   const float* result_dev() const { return out_; }
   size_t result_elems() const { return (size_t)L.zdim * L.ydim * L.xdim; }
@@ -117,6 +118,7 @@ class Engine {
   int fit_order2() const;
   OtfTab otf_table(int dir, int order) const;
   void wait_fit();
+  void wait_flag(unsigned seq);
   OvParams ov_params(int dir, int o1, int o2, float k0x, float k0y) const;
   // builds the requested overlap pairs (each in its direction's slot, or
   // every one in `only` when given) and their z sums

@@ -233,8 +233,10 @@ struct FitOut {
   float corr[kMaxDirs * 5];
   int ndone, rounds, overflow;
 };
+// ... then writes seq into *flag (host memory) once every result is visible
+// to the host, as launch_fit_signal does.
 void launch_fit_publish(const FitState* fs, const FitArgs& a, const float2* amps, const float* corr, FitOut* out,
-                        hipStream_t s);
+                        unsigned* flag, unsigned seq, hipStream_t s);
 // findrealspacemodamp's result (gpuFunctionsImpl.cu:1128-1147) on the device,
 // for each of ncand candidates: the partials summed in block order as the
 // host does (cpuReduce, gpuFunctionsImpl_hh.cu:162-180), then
@@ -283,6 +285,22 @@ struct FilterParams {
   float inv_min_dkr2, inv_zd0, inv_rcut, inv_apo, inv_apo2, inv_zapo;  // reciprocals of the divisors
   float inv_nxh;    // 1 / nxh
 };
+// The bands one filter launch covers: every field of FilterParams that
+// differs between (dir, order) comes from the entry (blockIdx.z), the rest
+// from the shared FilterParams; the bands are independent, so one launch
+// filters a whole stack.
+struct FilterBand {
+  int dir, order;
+  float2 conjamp;
+  float2 *bp, *bp2;
+  long long otf_off;  // elements added to every otf[o].t (per-direction OTFs)
+};
+struct FilterBatch {
+  int n;
+  FilterBand b[kMaxDirs * kMaxOrders];
+};
+void launch_filter_batch(const FilterParams& p, const FilterBatch& b, hipStream_t s);
+// One band (p.dir, p.order, p.conjamp, p.bp, p.bp2, p.otf).
 void launch_filter(const FilterParams& p, hipStream_t s);
 // Fill n float2 with zero.
 void launch_zero(float2* p, long long n, hipStream_t s);

@@ -46,6 +46,13 @@ __device__ __forceinline__ float2 prof_pos(const float2* prof, int nxotf, float 
   return make_float2(__builtin_fmaf(ar, t1.x - t0.x, t0.x), __builtin_fmaf(ar, t1.y - t0.y, t0.y));
 }
 
+// OTF table of order o for the band being filtered
+__device__ __forceinline__ OtfTab otf_at(const FilterParams& p, int o, long long off) {
+  OtfTab t = p.otf[o];
+  t.t += off;
+  return t;
+}
+
 // One visit of filterbands_kernel1 at band coordinates (x1, y1, z0) whose
 // rdist1 <= rdistcutoff; updates (re, im) of the stored element.
 // prof: this plane's radial OTF profiles at kz = z0, [order][nxotf].
@@ -68,10 +75,11 @@ __device__ __forceinline__ float amp_mag2(const FilterParams& p, int dir, int or
 }
 
 template <bool UNV, bool RA>
-__device__ __forceinline__ float filter_term(const FilterParams& p, const float2* prof, int nxotf, int order, int dir2,
-                                             int order2, float xabs, float yabs, int z0, float zf3) {
+__device__ __forceinline__ float filter_term(const FilterParams& p, const float2* prof, int nxotf, int dir, int order,
+                                             long long otf_off, int dir2, int order2, float xabs, float yabs, int z0,
+                                             float zf3) {
 #pragma clang fp contract(off)
-  if (dir2 == p.dir && order2 == order) return 0.f;
+  if (dir2 == dir && order2 == order) return 0.f;
   if (!p.bFilteroverlaps && !(order2 == 0 && order == 0)) return 0.f;
   // x2 = xabs - order2 k0x: order2 k0x is exact for |order2| <= 2, so the
   // fused form rounds exactly as the reference's subtraction (k0 is read from
@@ -90,7 +98,7 @@ __device__ __forceinline__ float filter_term(const FilterParams& p, const float2
   if constexpr (RA)
     otf2 = prof_pos(prof + ao2 * nxotf, nxotf, __builtin_amdgcn_sqrtf(d2 * p.inv_dkr2));
   else
-    otf2 = otf3d_interp(p.otf[ao2], x2, y2, z0, p.kzscale);
+    otf2 = otf3d_interp(otf_at(p, ao2, otf_off), x2, y2, z0, p.kzscale);
   float w = __builtin_fmaf(otf2.x, otf2.x, otf2.y * otf2.y);
   if (!UNV && !p.unit_nv) w = w / p.noiseVar[dir2 * p.norders + ao2];
   if (order2 != 0) w *= amp_mag2(p, dir2, ao2);
@@ -108,12 +116,13 @@ __device__ __forceinline__ float filter_term(const FilterParams& p, const float2
 // (dir2, order2) loop is unrolled, so its per-term constants are read once
 // and the OTF lookups of different terms overlap); 0: taken from p.
 template <int ND, int NO, bool UNV, bool RA>
-__device__ __forceinline__ void filter_visit(const FilterParams& p, const float2* prof, int x1, int y1, int z0,
-                                             bool conj, float2& re, float2& im) {
+__device__ __forceinline__ void filter_visit(const FilterParams& p, const FilterBand& fb, const float2* prof, int x1,
+                                             int y1, int z0, bool conj, float2& re, float2& im) {
 #pragma clang fp contract(off)
-  const int order = p.order;
-  const float kx = (float)order * p.k0[p.dir].x;
-  const float ky = (float)order * p.k0[p.dir].y;
+  const int dir = fb.dir, order = fb.order;
+  const long long otf_off = fb.otf_off;
+  const float kx = (float)order * p.k0[dir].x;
+  const float ky = (float)order * p.k0[dir].y;
   const float x1f = (float)x1 * p.dkx, y1f = (float)y1 * p.dky;
   // divisions by per-launch constants are products with their reciprocals and
   // square roots are the hardware's (each within an ulp or two of the
@@ -131,10 +140,10 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
   if constexpr (RA)
     otf1 = prof_pos(prof + order * nxotf, nxotf, __builtin_amdgcn_sqrtf(d1 * p.inv_dkr2));
   else
-    otf1 = otf3d_interp(p.otf[order], x1f, y1f, z0, p.kzscale);
+    otf1 = otf3d_interp(otf_at(p, order, otf_off), x1f, y1f, z0, p.kzscale);
   float weight = otf1.x * otf1.x + otf1.y * otf1.y;
-  if (order != 0) weight *= amp_mag2(p, p.dir, order);
-  float dampfact = p.inv_nv[p.dir * p.norders + order];
+  if (order != 0) weight *= amp_mag2(p, dir, order);
+  float dampfact = p.inv_nv[dir * p.norders + order];
   if (p.bSuppress && order != 0 && d1 <= p.supp2)
     dampfact *= dev_suppress(__builtin_amdgcn_sqrtf(d1 * p.inv_min_dkr2));
   else if (!p.bDampenOrder0 && p.bSuppress && order == 0 && d1 <= p.supp2)
@@ -149,11 +158,11 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
     for (int dir2 = 0; dir2 < ND; dir2++)
 #pragma unroll
       for (int order2 = -(NO - 1); order2 < NO; order2++)
-        sumweight += filter_term<UNV, RA>(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
+        sumweight += filter_term<UNV, RA>(p, prof, nxotf, dir, order, otf_off, dir2, order2, xabs, yabs, z0, zf3);
   } else {
     for (int dir2 = 0; dir2 < p.ndirs; dir2++)
       for (int order2 = -(p.norders - 1); order2 < p.norders; order2++)
-        sumweight += filter_term<false, RA>(p, prof, nxotf, order, dir2, order2, xabs, yabs, z0, zf3);
+        sumweight += filter_term<false, RA>(p, prof, nxotf, dir, order, otf_off, dir2, order2, xabs, yabs, z0, zf3);
   }
   sumweight += p.wiener;
   const float rs = __builtin_amdgcn_rcpf(sumweight);
@@ -182,9 +191,9 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
   if (order == 0) {
     re = cmul_nc(re, scale);
   } else {
-    float2 ca = p.conjamp;
+    float2 ca = fb.conjamp;
     if (p.amps != nullptr) {
-      const float2 a = p.amps[p.dir * p.norders + order];
+      const float2 a = p.amps[dir * p.norders + order];
       ca = make_float2(a.x, -a.y);
     }
     scale = cmul_nc(scale, ca);
@@ -206,24 +215,25 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const float2
   }
 }
 
-// Workgroup (blockIdx.x, iz = blockIdx.y): per*256 stored elements
-// (jin, iin) of plane iz of the band pair (per = 4, or 1 when the band is too
-// small to fill the device that way).  The plane's radial OTF profiles at
+// Workgroup (blockIdx.x, iz = blockIdx.y, band blockIdx.z of the batch):
+// per*256 stored elements (jin, iin) of plane iz of the band pair (per = 4,
+// or 1 when the batch is too small to fill the device that way).  The plane's radial OTF profiles at
 // kz = +z0 and kz = -z0 (the conjugate visit) are built once in LDS.  Planes
 // whose z lies outside +-zd are zeroed (kernel3).
 constexpr int kFiltPer = 4;
 
 template <int ND, int NO, bool UNV, bool RA>
-__global__ __launch_bounds__(256) void k_filter(FilterParams p, int per) {
+__global__ __launch_bounds__(256) void k_filter(FilterParams p, FilterBatch batch, int per) {
 #pragma clang fp contract(off)
   extern __shared__ float2 prof[];  // [2][norders][nxotf]
+  const FilterBand fb = batch.b[blockIdx.z];
   const long long nplane = (long long)p.nxh * p.ny;
   const int iz = blockIdx.y;
   const long long ep0 = (long long)blockIdx.x * (256 * per) + threadIdx.x;
-  const int zd = p.zd[p.order];
-  float2* bp = p.bp;
-  float2* bp2 = p.bp2;
-  const bool two = p.order != 0;
+  const int zd = p.zd[fb.order];
+  float2* bp = fb.bp;
+  float2* bp2 = fb.bp2;
+  const bool two = fb.order != 0;
   int z0n = 0;
   if (iz <= zd)
     z0n = iz;
@@ -246,7 +256,7 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p, int per) {
     for (int i = threadIdx.x; i < 2 * nprof; i += 256) {
       const int side = i / nprof, rem = i - side * nprof;
       const int o = rem / nxotf, ir = rem - o * nxotf;
-      prof[i] = otf_profile_entry(p.otf[o], ir, side ? -z0n : z0n, p.kzscale);
+      prof[i] = otf_profile_entry(otf_at(p, o, fb.otf_off), ir, side ? -z0n : z0n, p.kzscale);
     }
     __syncthreads();
   }
@@ -281,7 +291,7 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p, int per) {
     const int nvis = (two && jin >= 1 && jin <= p.nx / 2 - 1) ? 2 : 1;
     const int yc = -((iin <= p.ny / 2) ? iin : iin - p.ny);
     for (int v = 0; v < nvis; ++v)
-      filter_visit<ND, NO, UNV, RA>(p, v ? prof + nprof : prof, v ? -jin : jin, v ? yc : y1n, v ? -z0n : z0n,
+      filter_visit<ND, NO, UNV, RA>(p, fb, v ? prof + nprof : prof, v ? -jin : jin, v ? yc : y1n, v ? -z0n : z0n,
                                     v != 0, re, im);
     bp[e] = re;
     if (two) bp2[e] = im;
@@ -289,37 +299,52 @@ __global__ __launch_bounds__(256) void k_filter(FilterParams p, int per) {
 }
 
 template <bool RA>
-static void launch_filter_kind(const FilterParams& p, size_t lds, dim3 grid, int per, hipStream_t s) {
+static void launch_filter_kind(const FilterParams& p, const FilterBatch& b, size_t lds, dim3 grid, int per,
+                               hipStream_t s) {
   const bool unv = p.unit_nv != 0;
   if (p.ndirs == 3 && p.norders == 3) {  // the 3-beam 3D case
     if (unv)
-      hipLaunchKernelGGL((k_filter<3, 3, true, RA>), grid, dim3(256), lds, s, p, per);
+      hipLaunchKernelGGL((k_filter<3, 3, true, RA>), grid, dim3(256), lds, s, p, b, per);
     else
-      hipLaunchKernelGGL((k_filter<3, 3, false, RA>), grid, dim3(256), lds, s, p, per);
+      hipLaunchKernelGGL((k_filter<3, 3, false, RA>), grid, dim3(256), lds, s, p, b, per);
   } else if (p.ndirs == 3 && p.norders == 2) {  // 2D, 3 phases
     if (unv)
-      hipLaunchKernelGGL((k_filter<3, 2, true, RA>), grid, dim3(256), lds, s, p, per);
+      hipLaunchKernelGGL((k_filter<3, 2, true, RA>), grid, dim3(256), lds, s, p, b, per);
     else
-      hipLaunchKernelGGL((k_filter<3, 2, false, RA>), grid, dim3(256), lds, s, p, per);
+      hipLaunchKernelGGL((k_filter<3, 2, false, RA>), grid, dim3(256), lds, s, p, b, per);
   } else {
-    hipLaunchKernelGGL((k_filter<0, 0, false, RA>), grid, dim3(256), lds, s, p, per);
+    hipLaunchKernelGGL((k_filter<0, 0, false, RA>), grid, dim3(256), lds, s, p, b, per);
   }
 }
 
-void launch_filter(const FilterParams& p, hipStream_t s) {
+void launch_filter_batch(const FilterParams& p, const FilterBatch& b, hipStream_t s) {
+  if (b.n < 1 || b.n > kMaxDirs * kMaxOrders) throw ArgError("filter: batch of " + std::to_string(b.n) + " bands");
+  for (int k = 0; k < b.n; ++k) {
+    const FilterBand& e = b.b[k];
+    if (e.dir < 0 || e.dir >= p.ndirs || e.order < 0 || e.order >= p.norders || e.bp == nullptr ||
+        (e.order != 0 && e.bp2 == nullptr))
+      throw ArgError("filter: band entry out of range");
+  }
   const long long nplane = (long long)p.nxh * p.ny;
-  // four elements per thread amortise the plane's profile build; a band
+  // four elements per thread amortise the plane's profile build; a batch
   // that would then give fewer than 2048 workgroups takes one per thread
-  const int per = ceil_div(nplane, 256 * kFiltPer) * p.nz >= 2048 ? kFiltPer : 1;
-  const dim3 grid(ceil_div(nplane, 256 * per), p.nz);
+  const int per = ceil_div(nplane, 256 * kFiltPer) * p.nz * b.n >= 2048 ? kFiltPer : 1;
+  const dim3 grid(ceil_div(nplane, 256 * per), p.nz, b.n);
   if (p.otf[0].ra) {
     const size_t lds = (size_t)2 * p.norders * p.otf[0].nxotf * sizeof(float2);
     if (lds > 64 * 1024) throw ArgError("OTF table too long for the filter's LDS profile");
-    launch_filter_kind<true>(p, lds, grid, per, s);
+    launch_filter_kind<true>(p, b, lds, grid, per, s);
   } else {
-    launch_filter_kind<false>(p, 0, grid, per, s);
+    launch_filter_kind<false>(p, b, 0, grid, per, s);
   }
   SIMR_LAUNCH_CHECK();
+}
+
+void launch_filter(const FilterParams& p, hipStream_t s) {
+  FilterBatch b{};
+  b.n = 1;
+  b.b[0] = FilterBand{p.dir, p.order, p.conjamp, p.bp, p.bp2, 0};
+  launch_filter_batch(p, b, s);
 }
 
 __global__ void k_zero(float2* __restrict__ p, long long n) {

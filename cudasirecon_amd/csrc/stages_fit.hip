@@ -384,7 +384,8 @@ __global__ void k_fit_kc(const FitState* fs, int ndirs, float mult, float2* kc) 
   kc[d] = make_float2(k.x * mult, k.y * mult);
 }
 
-__global__ void k_fit_publish(const FitState* fs, FitArgs a, const float2* amps, const float* corr, FitOut* out) {
+__global__ void k_fit_publish(const FitState* fs, FitArgs a, const float2* amps, const float* corr, FitOut* out,
+                              unsigned* flag, unsigned seq) {
   const int t = threadIdx.x;
   if (t < a.ndirs) {
     const FitDir& f = fs->d[t];
@@ -404,6 +405,8 @@ __global__ void k_fit_publish(const FitState* fs, FitArgs a, const float2* amps,
     out->overflow = fs->overflow;
   }
   __threadfence_system();  // out is host memory
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 void check_fit_args(const FitArgs& a) {
@@ -432,9 +435,9 @@ void launch_fit_kc(const FitState* fs, int ndirs, float mult, float2* kc, hipStr
 }
 
 void launch_fit_publish(const FitState* fs, const FitArgs& a, const float2* amps, const float* corr, FitOut* out,
-                        hipStream_t s) {
+                        unsigned* flag, unsigned seq, hipStream_t s) {
   check_fit_args(a);
-  hipLaunchKernelGGL(k_fit_publish, dim3(1), dim3(64), 0, s, fs, a, amps, corr, out);
+  hipLaunchKernelGGL(k_fit_publish, dim3(1), dim3(64), 0, s, fs, a, amps, corr, out, flag, seq);
   SIMR_LAUNCH_CHECK();
 }
 
