@@ -331,6 +331,43 @@ __device__ __forceinline__ void fft2_lds(float2* buf, int t, const float2* __res
   fft2_sync<WAVE>();
 }
 
+// As fft2_lds with table twiddles, with pass 1's inputs already in the
+// thread's registers, v[c N1 + n1] = x[N2 n1 + N1 c + t] (the row is not read
+// from buf; buf only has to be free for this transform when it is called).
+template <int N, bool INV, bool WAVE = false>
+__device__ __forceinline__ void fft2_lds_from_regs(float2* buf, int t, const float2* __restrict__ tab,
+                                                   float2 (&v)[fft2_plan(N).n2]) {
+#pragma clang fp contract(off)
+  constexpr F2Plan P = fft2_plan(N);
+  constexpr int N1 = P.n1, N2 = P.n2, MC = N2 / N1, RP = N2 + 1;
+  static_assert(N1 > 0, "no two-pass plan for this length");
+  float2* tcol = buf + t;
+#pragma unroll
+  for (int c = 0; c < MC; ++c) {
+    const int n2 = t + N1 * c;
+    reg_dft<N1, INV>(v + c * N1);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+      float2 w = tab[n2 * k1];
+      if (INV) w.y = -w.y;
+      v[c * N1 + k1] = cmul(v[c * N1 + k1], w);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) tcol[RP * k1 + N1 * c] = v[c * N1 + k1];
+  }
+  fft2_sync<WAVE>();
+  const float2* row = buf + RP * t;
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) v[n2] = row[n2];
+  fft2_sync<WAVE>();
+  reg_dft<N2, INV>(v);
+  float2* out = buf + t;
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) out[N1 * k2] = v[k2];
+  fft2_sync<WAVE>();
+}
+
 template <int N, bool INV, bool WAVE = false>
 __device__ __forceinline__ void fft2_lds(float2* buf, int t, const Fft2Tw<N>& tw) {
 #pragma clang fp contract(off)

@@ -633,6 +633,41 @@ __device__ __forceinline__ void asm_x_put(const AsmParams& p, float2* dst, const
     for (int X = p.nx - half + 1 + lane; X < N - (half - 1); X += 64) dst[r * LP + asm_x_pos<N>(X)] = make_float2(0.f, 0.f);
 }
 
+// Exact 2x zoom (nx = N/2, half = N/4): thread t's pass-1 inputs of row Y
+// of band a, plane zout, loaded straight into registers, slot c N1 + n1
+// holding X = N2 n1 + N1 c + t of the zero-padded row (asm_x_put's layout:
+// X <= N/4 from source column X + N/4 - 1, X >= 3N/4 + 1 from X - 3N/4 - 1,
+// zero between).  The slots' cases are compile-time after unrolling; only a
+// slot that straddles an edge of the zero band tests per thread.
+template <int N>
+__device__ __forceinline__ void asm_x_fetch_cols(const AsmParams& p, float2 (&v)[fft2_plan(N).n2], int a, int zout,
+                                                 int Y, int t) {
+  constexpr int N1 = fft2_plan(N).n1, N2 = fft2_plan(N).n2, MC = N2 / N1;
+  constexpr int H = N / 4, XHI = N - H + 1;
+  const float2* src = p.stageY + (((long long)a * p.zdim + zout) * p.ydim + Y) * p.nx;
+#pragma unroll
+  for (int c = 0; c < MC; ++c)
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      const int X0 = N2 * n1 + N1 * c;
+      float2& d = v[c * N1 + n1];
+      if (X0 + N1 - 1 <= H) {
+        d = src[X0 + t + H - 1];
+      } else if (X0 >= XHI) {
+        d = src[X0 + t - XHI];
+      } else if (X0 > H && X0 + N1 - 1 < XHI) {
+        d = make_float2(0.f, 0.f);
+      } else {
+        const int X = X0 + t;
+        const bool lo = X <= H, hi = X >= XHI;
+        int sc = lo ? X + H - 1 : X - XHI;
+        sc = sc < 0 ? 0 : (sc > p.nx - 1 ? p.nx - 1 : sc);
+        const float2 w = src[sc];
+        d = (lo || hi) ? w : make_float2(0.f, 0.f);
+      }
+    }
+}
+
 template <int N>
 __global__ __launch_bounds__(asm_x_threads<N>(), 2) void k_asm_x(AsmParams p,
                                                                                    const float2* __restrict__ tw) {
@@ -662,6 +697,47 @@ __global__ __launch_bounds__(asm_x_threads<N>(), 2) void k_asm_x(AsmParams p,
     }
   __syncthreads();
   if constexpr (asm_x_fft2<N>()) {
+    if (2 * p.nx == N) {
+      // exact 2x zoom: each thread's pass-1 inputs come from global memory
+      // straight into registers (no LDS staging of the padded rows), and the
+      // next (plane, round) is fetched while this one is transformed
+      constexpr int N2 = fft2_plan(N).n2;
+      float2 nxt[N2];
+      asm_x_fetch_cols<N>(p, nxt, g < p.nasm ? g : p.nasm - 1, 0, Y, t);
+      for (int zout = 0; zout < p.zdim; ++zout) {
+        float acc[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) acc[e] = 0.f;
+#pragma nounroll
+        for (int a0 = 0; a0 < p.nasm; a0 += G) {
+          float2 v[N2];
+#pragma unroll
+          for (int i = 0; i < N2; ++i) v[i] = nxt[i];
+          int na0 = a0 + G, nz = zout;
+          if (na0 >= p.nasm) {
+            na0 = 0;
+            ++nz;
+          }
+          if (nz < p.zdim) asm_x_fetch_cols<N>(p, nxt, na0 + g < p.nasm ? na0 + g : p.nasm - 1, nz, Y, t);
+          fft2_lds_from_regs<N, true, true>(lds + g * LP, t, twt, v);
+          __syncthreads();
+          if (a0 == 0)
+            asm_x_round<N, NT, G, E, LB, LUT ? LB : 0, 0>(p, lds, a0, Y, acc, lut);
+          else if (a0 == G)
+            asm_x_round<N, NT, G, E, LB, LUT ? LB : 0, 1>(p, lds, a0, Y, acc, lut);
+          else
+            asm_x_round<N, NT, G, E, LB, LUT ? LB : 0, -1>(p, lds, a0, Y, acc, lut);
+          __syncthreads();
+        }
+        float* orow = p.out + ((long long)zout * p.ydim + Y) * N;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int X = threadIdx.x + e * NT;
+          if (X < N) orow[X] = acc[e];
+        }
+      }
+      return;
+    }
     if (p.nx <= 64 * asm_x_batch<N>()) {
       constexpr int TPW = 64 / TPT;
       const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
