@@ -403,6 +403,32 @@ __device__ __forceinline__ void fft2_lds(float2* buf, int t, const Fft2Tw<N>& tw
   fft2_sync<WAVE>();
 }
 
+// Register in, register out: pass 1 from the thread's registers
+// (v[c N1 + n1] = x[N2 n1 + N1 c + t]), register twiddles, the transposed
+// intermediate through buf (fft2_span(N) elements), pass 2 into the
+// registers: v[k2] = X[t + N1 k2].  buf is last read by the row reads, so
+// the caller passes a barrier before writing it again.
+template <int N, bool INV, bool WAVE = false>
+__device__ __forceinline__ void fft2_regs(float2* buf, int t, const Fft2Tw<N>& tw, float2 (&v)[fft2_plan(N).n2]) {
+#pragma clang fp contract(off)
+  constexpr F2Plan P = fft2_plan(N);
+  constexpr int N1 = P.n1, N2 = P.n2, MC = N2 / N1, RP = N2 + 1;
+  static_assert(N1 > 0, "no two-pass plan for this length");
+#pragma unroll
+  for (int c = 0; c < MC; ++c) reg_dft<N1, INV>(v + c * N1);
+  fft2_twiddle_all<N, INV>(v, tw);
+  float2* tcol = buf + t;
+#pragma unroll
+  for (int c = 0; c < MC; ++c)
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) tcol[RP * k1 + N1 * c] = v[c * N1 + k1];
+  fft2_sync<WAVE>();
+  const float2* row = buf + RP * t;
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) v[n2] = row[n2];
+  reg_dft<N2, INV>(v);
+}
+
 // As fft2_lds with register twiddles, but the result stays in the caller's
 // registers: out[k2] = X[t + N1 k2], k2 < N2 (nothing is written back; buf
 // is free again when it returns).

@@ -339,6 +339,47 @@ constexpr int asm_y_w() {
   return w;
 }
 
+// Column pitch of the register-in, register-out tiles (exact 2x zoom): a
+// wavefront holds 8 columns x 8 threads, so the pitch is 2 mod 16 (the 16
+// lanes of a 64-bit LDS access land on distinct bank pairs).
+template <int N>
+constexpr int asm_y_pitch_regs() {
+  return asm_y_fft2<N>() ? fft2_span(N) + (18 - fft2_span(N) % 16) % 16 : 1;
+}
+
+// Exact 2x zoom (ny = N/2): slot c N1 + n1 of thread t holds row
+// Y = N2 n1 + N1 c + t of the zero-padded column (source row Y + N/4 - 1 for
+// Y <= N/4, Y - 3N/4 - 1 for Y >= 3N/4 + 1, zero between), read from scol
+// (the column's first element, row stride nx).  The cases are compile-time
+// after unrolling except for a slot that straddles an edge of the zero band.
+template <int N>
+__device__ __forceinline__ void asm_y_fetch_col(const float2* scol, int nx, int ny, float2 (&v)[fft2_plan(N).n2],
+                                                int t) {
+  constexpr int N1 = fft2_plan(N).n1, N2 = fft2_plan(N).n2, MC = N2 / N1;
+  constexpr int H = N / 4, YHI = N - H + 1;
+#pragma unroll
+  for (int c = 0; c < MC; ++c)
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      const int Y0 = N2 * n1 + N1 * c;
+      float2& d = v[c * N1 + n1];
+      if (Y0 + N1 - 1 <= H) {
+        d = scol[(long long)(Y0 + t + H - 1) * nx];
+      } else if (Y0 >= YHI) {
+        d = scol[(long long)(Y0 + t - YHI) * nx];
+      } else if (Y0 > H && Y0 + N1 - 1 < YHI) {
+        d = make_float2(0.f, 0.f);
+      } else {
+        const int Y = Y0 + t;
+        const bool lo = Y <= H, hi = Y >= YHI;
+        int sr = lo ? Y + H - 1 : Y - YHI;
+        sr = sr < 0 ? 0 : (sr > ny - 1 ? ny - 1 : sr);
+        const float2 w = scol[(long long)sr * nx];
+        d = (lo || hi) ? w : make_float2(0.f, 0.f);
+      }
+    }
+}
+
 template <int N>
 __global__ __launch_bounds__(asm_y_w<N>() * asm_y_tpt<N>()) void k_asm_y(AsmParams p, const float2* __restrict__ tw) {
   constexpr bool F2 = asm_y_fft2<N>();
@@ -347,13 +388,16 @@ __global__ __launch_bounds__(asm_y_w<N>() * asm_y_tpt<N>()) void k_asm_y(AsmPara
   constexpr int NT = W * TPT;
   constexpr int LP = asm_y_pitch<N>();
   static_assert(!F2 || NT % 64 == 0, "two-pass tiles are whole wavefronts");
-  static_assert(NT <= 1024 && (W * LP + ((N > 1 && !F2) ? N : 1)) * 8 <= 160 * 1024, "the tile fits a workgroup");
-  __shared__ float2 lds[W * LP];
+  constexpr int LPR = asm_y_pitch_regs<N>();
+  constexpr int LDSN = W * (LP > LPR ? LP : LPR);
+  static_assert(NT <= 1024 && (LDSN + ((N > 1 && !F2) ? N : 1)) * 8 <= 160 * 1024, "the tile fits a workgroup");
+  __shared__ float2 lds[LDSN];
   FftTw<F2 ? 1 : N> twr;
   __shared__ float2 twstage[(N > 1 && !F2) ? N : 1];
   Fft2Tw<F2 ? N : 64> tw2;
+  const bool zoom2 = F2 && 2 * p.ny == N;
   if constexpr (F2)
-    tw2.load(threadIdx.x % TPT, tw);
+    tw2.load(zoom2 ? threadIdx.x / W : threadIdx.x % TPT, tw);
   else
     load_twiddles(twr, twstage, threadIdx.x % TPT, tw, threadIdx.x, blockDim.x);
   unsigned bx, by, bz;
@@ -374,6 +418,27 @@ __global__ __launch_bounds__(asm_y_w<N>() * asm_y_tpt<N>()) void k_asm_y(AsmPara
     }
   }
   const float2* src = p.stageZ + ((long long)a * p.zdim + zout) * p.ny * p.nx;
+  if constexpr (F2) {
+    if (zoom2) {
+      // thread (t, w) = (threadIdx.x / W, threadIdx.x % W): column xc0 + w,
+      // pass-1 inputs from global memory into registers, outputs from
+      // registers to global memory; each wavefront's loads and stores cover
+      // whole W-column row segments of 8 consecutive rows, as the staged
+      // copies do, and the padded column never passes through LDS
+      constexpr int N1 = fft2_plan(N).n1, N2 = fft2_plan(N).n2;
+      const int w = threadIdx.x % W, t = threadIdx.x / W;
+      const int xc = xc0 + w < p.nx ? xc0 + w : p.nx - 1;
+      float2 v[N2];
+      asm_y_fetch_col<N>(src + xc, p.nx, p.ny, v, t);
+      fft2_regs<N, true, false>(lds + w * LPR, t, tw2, v);
+      if (xc0 + w < p.nx) {
+        float2* dcol = p.stageY + ((long long)a * p.zdim + zout) * (long long)N * p.nx + xc0 + w;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) dcol[(long long)(t + N1 * k2) * p.nx] = v[k2];
+      }
+      return;
+    }
+  }
   // Source row sr goes to Y = sr - hy (sr >= hy) or Y = sr + N - hy (sr < hy);
   // the rows between are zero.  Loads run over source rows in batches of B
   // per thread with clamped addresses and no branches, so a batch is in

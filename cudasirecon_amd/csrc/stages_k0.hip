@@ -392,11 +392,15 @@ __global__ __launch_bounds__(ovf_cols_w<NY>() * fft_tpt(NY)) void k_ovf_cols(OvB
   constexpr int EPT = NY * W / NT;
   constexpr int CH = EPT < 4 ? EPT : 4;
   static_assert(EPT % CH == 0, "chunks cover the elements");
-  const OvParams& p = b.p[blockIdx.z];
-  const int zb = blockIdx.y;
+  // XCD-contiguous tile order: neighbouring column tiles, whose 64-byte
+  // segments of the 257-element band rows share cache lines, run in one L2
+  unsigned bx, by, bz;
+  xcd_tile3(bx, by, bz);
+  const OvParams& p = b.p[bz];
+  const int zb = by;
   if (zb > 2 * p.zd) return;  // a slot with a narrower z support (whole block)
-  const int which = blockIdx.x & 1;
-  const int c0 = (blockIdx.x >> 1) * W;
+  const int which = bx & 1;
+  const int c0 = (bx >> 1) * W;
   const int z0 = zb - p.zd;
   float kx = p.kx, ky = p.ky;
   if (p.k0d != nullptr) {
