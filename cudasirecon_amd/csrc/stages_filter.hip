@@ -154,11 +154,21 @@ __device__ __forceinline__ void filter_visit(const FilterParams& p, const Filter
   weight *= dampfact;
   float sumweight = weight;
   if constexpr (ND > 0) {
+    // With a radially averaged OTF and unit noise variances the order-0 term
+    // is the same for every direction (x2 = xabs, the band's own profile,
+    // the same damping): it is formed once and added at each direction's
+    // place in the sum, so the summation order stays the reference's
+    float w0 = 0.f;
+    if constexpr (RA && UNV) w0 = filter_term<UNV, RA>(p, prof, nxotf, -1, order, otf_off, 0, 0, xabs, yabs, z0, zf3);
 #pragma unroll
     for (int dir2 = 0; dir2 < ND; dir2++)
 #pragma unroll
-      for (int order2 = -(NO - 1); order2 < NO; order2++)
-        sumweight += filter_term<UNV, RA>(p, prof, nxotf, dir, order, otf_off, dir2, order2, xabs, yabs, z0, zf3);
+      for (int order2 = -(NO - 1); order2 < NO; order2++) {
+        if (RA && UNV && order2 == 0)
+          sumweight += (dir2 == dir && order == 0) ? 0.f : w0;
+        else
+          sumweight += filter_term<UNV, RA>(p, prof, nxotf, dir, order, otf_off, dir2, order2, xabs, yabs, z0, zf3);
+      }
   } else {
     for (int dir2 = 0; dir2 < p.ndirs; dir2++)
       for (int order2 = -(p.norders - 1); order2 < p.norders; order2++)
