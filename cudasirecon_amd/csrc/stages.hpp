@@ -260,6 +260,8 @@ void launch_amp_policy(const AmpPolicy& a, float2* amps, hipStream_t s);
 // ---- Wiener filter ------------------------------------------------------------
 struct FilterParams {
   int nx, ny, nz, nxh;
+  // dir, order, conjamp, bp, bp2 describe one band: the host builds the
+  // launch's FilterBand entries from them; the kernel reads the entry
   int dir, ndirs, order, norders;
   int zd[kMaxOrders];
   float rdistcutoff, apocutoff, zapocutoff, kzscale, wiener;
@@ -300,8 +302,6 @@ struct FilterBatch {
   FilterBand b[kMaxDirs * kMaxOrders];
 };
 void launch_filter_batch(const FilterParams& p, const FilterBatch& b, hipStream_t s);
-// One band (p.dir, p.order, p.conjamp, p.bp, p.bp2, p.otf).
-void launch_filter(const FilterParams& p, hipStream_t s);
 // Fill n float2 with zero.
 void launch_zero(float2* p, long long n, hipStream_t s);
 

@@ -300,8 +300,7 @@ __global__ __launch_bounds__(asm_zp_threads<B0>()) void k_asm_z_pair(AsmParams p
 
 // ---- y pass: W adjacent xc columns of plane zout (blockIdx.y) of band a (blockIdx.z)
 // Lengths with a wave-local two-pass plan use that core (natural-order LDS
-// columns, register twiddles) with 16 columns per workgroup, so every global
-// row segment is a full 128-byte line; the others the radix-stage core.
+// columns, register twiddles); the others the radix-stage core.
 template <int N>
 constexpr bool asm_y_fft2() {
   return use_fft2<N>() && fft2_tpt(N) >= 4;

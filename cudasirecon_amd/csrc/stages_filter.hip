@@ -350,13 +350,6 @@ void launch_filter_batch(const FilterParams& p, const FilterBatch& b, hipStream_
   SIMR_LAUNCH_CHECK();
 }
 
-void launch_filter(const FilterParams& p, hipStream_t s) {
-  FilterBatch b{};
-  b.n = 1;
-  b.b[0] = FilterBand{p.dir, p.order, p.conjamp, p.bp, p.bp2, 0};
-  launch_filter_batch(p, b, s);
-}
-
 __global__ void k_zero(float2* __restrict__ p, long long n) {
   const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
   if (e < n) p[e] = make_float2(0.f, 0.f);
