@@ -69,3 +69,14 @@ def test_plain_stack_bytes_closed_forms():
     three = sb["ov_x"] + sb["ov_y"] + sb["ov_zsum"]
     two = sb["ov_cols"] + sb["ov_rows"]
     assert three - two == p.ndirs * 2 * 2 * nzs * nxy * 8
+
+
+def test_profiled_traffic_covers_the_default_path():
+    # the FP64 fit (the default) builds its overlaps in two passes; the
+    # three-pass stages run only with the FP32 fit and the dumps
+    three_pass = {"ov_x", "ov_y", "ov_zsum"}
+    p = plain_stack()
+    sb = bench.stage_bytes(p, bench.geometry(p))
+    for stage in sorted(set(sb) - three_pass):
+        t = bench.profiled_traffic(stage)
+        assert t is not None and t > 0, stage
